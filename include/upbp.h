@@ -1,0 +1,320 @@
+/*
+ * upbp.h -- C ABI of the MI355X volumetric merging estimators.
+ *
+ * Two estimators of SmallUPBP are served through this boundary:
+ *   - the Beam Radiance Estimate (PB2D): a query beam gathers photon POINTS
+ *     with the 2D Epanechnikov kernel (reference: SmallUPBP/src/Bre/Bre.hxx:40-184,
+ *     SmallUPBP/src/Bre/Bre.cxx:199-277,812-964);
+ *   - the photon-beam lookup (BB1D): a query beam gathers photon BEAMS with the
+ *     1D kernel (reference: SmallUPBP/src/Beams/PhBeams.hxx:46-160,
+ *     SmallUPBP/src/Beams/PhotonBeam.hxx:238-367, SmallUPBP/src/Beams/PhBeams.cxx:177-324).
+ *
+ * The reference interface is the pair of C++ classes EmbreeBre and
+ * PhotonBeamsEvaluator, called once per camera ray. Here the same contract is
+ * exposed as plain C over opaque handles: structures are built once per
+ * iteration (build), evaluated for a BATCH of query rays (eval), and released
+ * (destroy). Every array argument is a plain pointer plus a count; the
+ * `memory` field of each input struct says whether its pointers are host or
+ * device (HIP) pointers. No function exits the process: every failure returns
+ * an upbp_status and leaves a message in upbp_ctx_last_error().
+ *
+ * Threading: a upbp_ctx is externally synchronised (one per host thread or a
+ * caller-held mutex), like one EmbreeBre per OpenMP thread in the reference
+ * (SmallUPBP/src/SmallUPBP.cxx:66-79). Handles built on a context belong to it.
+ *
+ * The library runs only on an AMD GPU (gfx950). There is no CPU fallback:
+ * upbp_ctx_create fails with UPBP_EDEVICE when no device is usable.
+ */
+#ifndef UPBP_H_
+#define UPBP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UPBP_ABI_VERSION 1
+
+/* ---------------------------------------------------------------------------
+ * Status codes. The reference has none: its assertions print and exit(2)
+ * (SmallUPBP/src/Misc/Defs.hxx:96-97). */
+typedef enum upbp_status {
+    UPBP_OK = 0,
+    UPBP_EINVAL = 1,        /* bad argument: null pointer, negative count, r <= 0, tmin > tmax, bad medium id */
+    UPBP_ENOMEM = 2,        /* device or host allocation failed */
+    UPBP_EDEVICE = 3,       /* HIP runtime error, no device, or a kernel reported an internal overflow */
+    UPBP_ENONFINITE = 4,    /* NaN/Inf in an input position, direction, radius, power or segment */
+    UPBP_EUNSUPPORTED = 5,  /* valid request this build does not serve (e.g. KNN radius, SHORT query in the adjoint) */
+    UPBP_ENOTBUILT = 6      /* handle is empty (build found no photons/beams) where data is required */
+} upbp_status;
+
+/* ---------------------------------------------------------------------------
+ * Constants, numerically identical to the reference enums. */
+/* BeamType, SmallUPBP/src/Misc/Defs.hxx:146-150 */
+#define UPBP_SHORT_BEAM 1u
+#define UPBP_LONG_BEAM 2u
+/* EstimatorTechnique and OtherSettings, SmallUPBP/src/Misc/Defs.hxx:175-194 */
+#define UPBP_BPT 4u
+#define UPBP_SURF 8u
+#define UPBP_PP3D 16u
+#define UPBP_PB2D 32u
+#define UPBP_BB1D 64u
+#define UPBP_NO_SINE_IN_WEIGHTS 128u
+#define UPBP_PREVIOUS 256u
+#define UPBP_COMPATIBLE 512u
+#define UPBP_BB1D_PREVIOUS 1024u
+#define UPBP_SPECULAR_ONLY 2048u
+/* AbstractMedium::RaySamplingFlags, SmallUPBP/src/Scene/Media.hxx:45-49 */
+#define UPBP_ORIGIN_IN_MEDIUM 1u
+#define UPBP_END_IN_MEDIUM 2u
+/* RadiusCalculation, SmallUPBP/src/Misc/Defs.hxx:155-159 */
+#define UPBP_CONSTANT_RADIUS 0
+#define UPBP_KNN_RADIUS 1
+
+typedef enum upbp_memory {
+    UPBP_MEM_HOST = 0,   /* pointers are ordinary host memory; the library copies over PCIe */
+    UPBP_MEM_DEVICE = 1  /* pointers are device memory on the context's GPU */
+} upbp_memory;
+
+/* Query ordering inside one eval call (the result order never changes). */
+#define UPBP_ORDER_SORTED 0   /* default: queries are regrouped by a Morton key of their entry point so
+                                 each 64-lane wave carries a spatially coherent bundle */
+#define UPBP_ORDER_AS_GIVEN 1 /* lanes take queries in input order (caller already tiled them) */
+
+typedef struct upbp_ctx upbp_ctx;
+typedef struct upbp_bre upbp_bre;       /* photon-point structure: replaces EmbreeBre's Embree BVH4 */
+typedef struct upbp_beams upbp_beams;   /* photon-beam structure: replaces PhotonBeamsEvaluator's grid */
+
+/* ---------------------------------------------------------------------------
+ * Homogeneous medium. Replaces HomogeneousMedium's constructor
+ * (SmallUPBP/src/Scene/Media.hxx:129-159): the library takes absolute values,
+ * forms sigma_t = sigma_a + sigma_s, picks the minimum positive sigma_t
+ * component (first index on ties), and sets the has-scattering flag.
+ * Emission is not used by either estimator and is not carried. */
+typedef struct upbp_medium {
+    float absorption[3];     /* sigma_a, RGB */
+    float scattering[3];     /* sigma_s, RGB */
+    float mean_cosine;       /* Henyey-Greenstein g; |g| < 1e-3 is isotropic (Media.hxx:111) */
+    float continuation_prob; /* used by the UPBP MIS weights only (Media.hxx:117) */
+} upbp_medium;
+
+/* One medium interval of a query ray. Same fields as VolumeSegment
+ * (SmallUPBP/src/Path/Ray.hxx:131-140) minus mEmission, which the estimators
+ * never read. With lite segments (VolumeSegmentLite, Ray.hxx:222-227) only
+ * dist_min, dist_max and medium_id are read. 32 bytes. */
+typedef struct upbp_volume_segment {
+    float dist_min;
+    float dist_max;
+    float ray_sample_pdf;
+    float ray_sample_rev_pdf;
+    float attenuation[3];
+    int32_t medium_id;
+} upbp_volume_segment;
+
+/* ---------------------------------------------------------------------------
+ * Build inputs. */
+typedef struct upbp_build_params {
+    int32_t radius_calculation; /* UPBP_CONSTANT_RADIUS; UPBP_KNN_RADIUS returns UPBP_EUNSUPPORTED */
+    float radius;               /* photon radius / beam radius, > 0, used when no per-item radius array */
+    int32_t knn;                /* reserved for UPBP_KNN_RADIUS */
+    int32_t leaf_size;          /* max items per LBVH leaf, 1..32; 0 = library default */
+    float beam_piece_factor;    /* beams only: target piece length in units of the max beam radius; 0 = default */
+    int32_t max_pieces_per_beam;/* beams only: cap on pieces per beam, 1..256; 0 = default */
+} upbp_build_params;
+
+/* Photons: light vertices in media. Replaces the inputs of EmbreeBre::build
+ * (SmallUPBP/src/Bre/Bre.cxx:526-632): position = mHitpoint,
+ * inc_dir = mBSDF.WorldDirFix(), power = mThroughput. */
+typedef struct upbp_photon_input {
+    int64_t count;
+    const float* position;    /* [count][3] */
+    const float* inc_dir;     /* [count][3]; may be NULL when every medium is isotropic */
+    const float* power;       /* [count][3] */
+    const float* radius;      /* [count] per-photon radius, or NULL for params->radius */
+    const uint8_t* in_medium; /* [count] filter (mIsInMedium, Bre.cxx:538-547), or NULL = all */
+    upbp_memory memory;
+} upbp_photon_input;
+
+/* Photon beams. Replaces PhotonBeam's MC-filled fields
+ * (SmallUPBP/src/Beams/PhotonBeam.hxx:44-54) and PhotonBeamsEvaluator::build's
+ * radius assignment (SmallUPBP/src/Beams/PhBeams.cxx:92-117). */
+typedef struct upbp_beam_input {
+    int64_t count;
+    const float* origin;       /* [count][3] mRay.origin */
+    const float* direction;    /* [count][3] mRay.direction, unit */
+    const float* length;       /* [count] mLength, > 0 */
+    const float* throughput;   /* [count][3] mThroughputAtOrigin */
+    const uint32_t* flags;     /* [count] UPBP_SHORT_BEAM or UPBP_LONG_BEAM (mFlags) */
+    const int32_t* medium_id;  /* [count] mMedium as an index into the context's media */
+    const float* start_radius; /* [count] or NULL for params->radius */
+    const float* end_radius;   /* [count] or NULL for start radius (constant cone) */
+    upbp_memory memory;
+} upbp_beam_input;
+
+/* ---------------------------------------------------------------------------
+ * A batch of query rays. Each query is one call of EmbreeBre::evalBre /
+ * PhotonBeamsEvaluator::evalBeamBeamEstimate with (beamType, ray, segments). */
+typedef struct upbp_query_batch {
+    int64_t count;
+    const float* origin;                  /* [count][3] */
+    const float* direction;               /* [count][3], unit */
+    const int64_t* segment_offsets;       /* [count+1] CSR offsets into segments */
+    const upbp_volume_segment* segments;  /* [segment_offsets[count]] */
+    uint32_t beam_type;                   /* UPBP_SHORT_BEAM or UPBP_LONG_BEAM for every query */
+    uint32_t lite_segments;               /* 1 = VolumeSegmentLite semantics (LONG only, Bre.cxx:895-964) */
+    uint32_t order;                       /* UPBP_ORDER_SORTED or UPBP_ORDER_AS_GIVEN */
+    upbp_memory memory;
+} upbp_query_batch;
+
+/* Counters of one eval/adjoint call. Hits are accepted (query, photon) and
+ * (query, beam) pairs; candidates are items tested against a lane's ray. */
+typedef struct upbp_eval_stats {
+    uint64_t photon_hits;
+    uint64_t beam_hits;
+    uint64_t photon_candidates;
+    uint64_t beam_candidates;
+    uint64_t nodes_visited;       /* LBVH nodes fetched, summed over waves */
+    float kernel_ms_photons;      /* HIP-event time of the photon traversal kernel(s) on the context stream */
+    float kernel_ms_beams;        /* HIP-event time of the beam traversal kernel(s) */
+    float kernel_ms_other;        /* validation, query sort, clears */
+    uint32_t launches;            /* traversal kernel launches in the call */
+} upbp_eval_stats;
+
+/* ---------------------------------------------------------------------------
+ * Context. */
+upbp_status upbp_ctx_create(int device, upbp_ctx** out_ctx);
+upbp_status upbp_ctx_destroy(upbp_ctx* ctx);
+const char* upbp_ctx_last_error(const upbp_ctx* ctx);
+/* Use an existing HIP stream (hipStream_t passed as void*); NULL restores the
+ * context's own stream. Every kernel of the context runs on this stream. */
+upbp_status upbp_ctx_set_stream(upbp_ctx* ctx, void* hip_stream);
+void* upbp_ctx_get_stream(const upbp_ctx* ctx);
+/* sync = 1 (default): calls return after the GPU work finished. sync = 0:
+ * calls that write device outputs return after enqueueing (stats still wait). */
+upbp_status upbp_ctx_set_sync(upbp_ctx* ctx, int sync);
+/* Media table, indexed by medium_id. Replaces Scene::mMedia lookups
+ * (Bre.cxx:836, PhBeams.cxx:201). Copied to the device. */
+upbp_status upbp_set_media(upbp_ctx* ctx, const upbp_medium* media, int32_t count);
+
+/* ---------------------------------------------------------------------------
+ * Photon points (PB2D). */
+/* Replaces EmbreeBre::build (SmallUPBP/src/Bre/Bre.cxx:526-632). Builds a GPU
+ * LBVH over photon cubes pos +- r (Bre.cxx:174-177). *out_num_photons receives
+ * the number of photons kept after the in_medium filter (the reference return
+ * value). A build with zero photons succeeds with an empty handle whose
+ * evaluation returns zeros (Bre.cxx:825-826). */
+upbp_status upbp_bre_build(upbp_ctx* ctx, const upbp_photon_input* photons,
+                           const upbp_build_params* params, upbp_bre** out_bre,
+                           int64_t* out_num_photons);
+/* Replaces EmbreeBre::destroy (Bre.cxx:762-777). NULL is accepted. */
+upbp_status upbp_bre_destroy(upbp_bre* bre);
+/* Replace the photon powers ([count][3], original input order of the kept
+ * photons) without rebuilding the tree. Used by differentiable callers. */
+upbp_status upbp_bre_set_power(upbp_bre* bre, const float* power, upbp_memory memory);
+int64_t upbp_bre_count(const upbp_bre* bre);
+
+/* ---------------------------------------------------------------------------
+ * Photon beams (BB1D). */
+/* Replaces PhotonBeamsEvaluator::build (SmallUPBP/src/Beams/PhBeams.cxx:66-148):
+ * assigns radii like the reference (constant radius: start = end = r,
+ * mRadiusChange = 0), then cuts every beam into pieces and builds a GPU LBVH
+ * over the pieces' cone bounding boxes (PhotonBeam.hxx:186-198). Each
+ * (query, beam) pair is counted once: a piece accepts only closest-approach
+ * parameters t2 in its own half-open range. */
+upbp_status upbp_beams_build(upbp_ctx* ctx, const upbp_beam_input* beams,
+                             const upbp_build_params* params, upbp_beams** out_beams);
+/* Replaces PhotonBeamsEvaluator::destroy (PhBeams.cxx:155-159). */
+upbp_status upbp_beams_destroy(upbp_beams* beams);
+int64_t upbp_beams_count(const upbp_beams* beams);
+int64_t upbp_beams_piece_count(const upbp_beams* beams);
+/* Replaces PhotonBeamsEvaluator::getBeamSelectionPdf (PhBeams.hxx, Grid.hxx:409-415).
+ * Without beam reduction (sMaxBeamsInCell = 0, the reference default) it is 1. */
+float upbp_beams_selection_pdf(const upbp_beams* beams, const float position[3]);
+
+/* ---------------------------------------------------------------------------
+ * Evaluation. out_photons receives, per query, EmbreeBre::evalBre
+ * (Bre.cxx:812-879 full segments, :895-964 lite); out_beams receives
+ * PhotonBeamsEvaluator::evalBeamBeamEstimate (PhBeams.cxx:177-241, :257-324),
+ * both with techniques = PB2D / BB1D and no MIS data (the plain VolLightTracer
+ * path, SmallUPBP/src/Renderers/VolLightTracer.hxx:378,384). Normalisation by
+ * the light-path count stays with the caller, as in the reference.
+ * Either structure may be NULL, then its output is not written.
+ * Outputs are [count][3] floats in the query order of the batch. */
+upbp_status upbp_eval(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams* beams,
+                      const upbp_query_batch* queries, float* out_photons, float* out_beams,
+                      upbp_memory out_memory, upbp_eval_stats* stats);
+
+/* Hit-set signatures per query: sig[q] = {number of hits, sum over hits of
+ * hash32(original item index)} (wrapping uint32). Identical hit sets give
+ * identical signatures; used to compare against a CPU evaluation. Either
+ * output may be NULL. Outputs are [count][2] uint32 in batch order. */
+upbp_status upbp_hit_signatures(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams* beams,
+                                const upbp_query_batch* queries, uint32_t* photon_sig,
+                                uint32_t* beam_sig, upbp_memory out_memory);
+
+/* ---------------------------------------------------------------------------
+ * Adjoint of upbp_eval for a scalar loss L(out_photons, out_beams). Inputs are
+ * the upstream gradients dL/dout ([count][3], NULL = zero). Outputs (each may
+ * be NULL) are ACCUMULATED into (+=), so clear them first:
+ *   grad_scattering, grad_absorption: [media][3]  dL/dsigma_s, dL/dsigma_a
+ *   grad_photon_power: [photons][3] dL/dpower, original order of the kept photons
+ *   grad_beam_power:   [beams][3]   dL/dthroughput
+ * Segment attenuations are taken as exp(-sigma_t * (dist_max - dist_min)) of the
+ * segment's medium (what Scene::Intersect stores, Scene.hxx:542), so their
+ * derivative is -(dist_max - dist_min) * attenuation. Transmittance is
+ * recomputed, never stored. LONG query beams only (UPBP_EUNSUPPORTED for SHORT). */
+upbp_status upbp_adjoint(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams* beams,
+                         const upbp_query_batch* queries, const float* grad_out_photons,
+                         const float* grad_out_beams, float* grad_scattering,
+                         float* grad_absorption, float* grad_photon_power,
+                         float* grad_beam_power, upbp_memory memory, upbp_eval_stats* stats);
+
+/* ---------------------------------------------------------------------------
+ * Batched evaluation of many independent instances in one persistent launch.
+ * Each instance names its structures, a device-resident query batch, its
+ * media (a range of the context's media table: medium_id + media_base) and
+ * device outputs. Waves take (instance, query tile) items from a device work
+ * queue until it drains. */
+typedef struct upbp_instance {
+    const upbp_bre* bre;          /* may be NULL */
+    const upbp_beams* beams;      /* may be NULL */
+    upbp_query_batch queries;     /* memory must be UPBP_MEM_DEVICE */
+    int32_t media_base;           /* added to every medium id of this instance */
+    float* out_photons;           /* device [count][3] or NULL */
+    float* out_beams;             /* device [count][3] or NULL */
+} upbp_instance;
+
+typedef struct upbp_batch_plan upbp_batch_plan;
+/* Prepare a reusable plan: validates the instances, uploads the descriptor
+ * table and sizes the work queue. The instances' structures and buffers must
+ * outlive the plan. */
+upbp_status upbp_batch_plan_create(upbp_ctx* ctx, const upbp_instance* instances,
+                                   int32_t count, upbp_batch_plan** out_plan);
+upbp_status upbp_batch_plan_destroy(upbp_batch_plan* plan);
+/* Run the plan: one fill kernel that resets the queue counters, then one
+ * persistent photon kernel and one persistent beam kernel. With
+ * use_graph = 1 the three launches are captured once into a hipGraph and
+ * replayed on later calls. */
+upbp_status upbp_batch_run(upbp_ctx* ctx, upbp_batch_plan* plan, int use_graph,
+                           upbp_eval_stats* stats);
+
+/* ---------------------------------------------------------------------------
+ * Launch tuning. The autotuner times the traversal kernels over a grid of
+ * wave-per-block and waves-per-CU settings and keeps the fastest per
+ * (structure size, query count) bucket. */
+typedef struct upbp_tuning {
+    int32_t waves_per_block;  /* 1..16 */
+    int32_t blocks_per_cu;    /* persistent kernels: resident blocks per CU; 0 = occupancy-derived */
+} upbp_tuning;
+upbp_status upbp_set_tuning(upbp_ctx* ctx, const upbp_tuning* tuning);
+upbp_status upbp_get_tuning(const upbp_ctx* ctx, upbp_tuning* tuning);
+upbp_status upbp_autotune(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams* beams,
+                          const upbp_query_batch* queries, upbp_tuning* best, float* best_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* UPBP_H_ */

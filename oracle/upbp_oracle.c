@@ -1,0 +1,854 @@
+/*
+ * upbp_oracle.c -- CPU restatement of SmallUPBP's PB2D (BRE) and BB1D (photon
+ * beam) estimators. TEST INFRASTRUCTURE: the checker of the GPU library and the
+ * timed CPU baseline; never part of the product. See upbp_oracle.h.
+ *
+ * Build: oracle/Makefile (gcc -O2 -ffp-contract=off -fopenmp). The FP32 path
+ * must not be compiled with FMA contraction or -ffast-math: it reproduces the
+ * reference's SSE operation order (SmallUPBP/SmallUPBP.vcxproj:160-162 builds
+ * with /fp:precise and SSE2).
+ */
+#include "upbp_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* SmallUPBP/src/Misc/Defs.hxx:74,77 */
+#define PI_F 3.141592653589793238462643383279f
+#define INV_PI_F (1.f / PI_F)
+
+/* ------------------------------------------------------------------------- */
+/* Media, SmallUPBP/src/Scene/Media.hxx:129-159 */
+void or_prepare_media(const upbp_medium* in, int32_t n, or_medium* out) {
+    for (int32_t i = 0; i < n; ++i) {
+        or_medium* m = &out[i];
+        memset(m, 0, sizeof(*m));
+        for (int c = 0; c < 3; ++c) {
+            m->sigma_a[c] = fabsf(in[i].absorption[c]);
+            m->sigma_s[c] = fabsf(in[i].scattering[c]);
+            m->sigma_t[c] = m->sigma_a[c] + m->sigma_s[c];
+        }
+        m->mean_cosine = in[i].mean_cosine;
+        m->continuation_prob = in[i].continuation_prob;
+        /* min positive component, first index on ties, zero components skipped (Media.hxx:142-145) */
+        m->min_comp = m->sigma_t[0];
+        m->min_idx = 0;
+        if (m->sigma_t[1] > 0 && (m->min_comp == 0 || m->sigma_t[1] < m->min_comp)) {
+            m->min_comp = m->sigma_t[1];
+            m->min_idx = 1;
+        }
+        if (m->sigma_t[2] > 0 && (m->min_comp == 0 || m->sigma_t[2] < m->min_comp)) {
+            m->min_comp = m->sigma_t[2];
+            m->min_idx = 2;
+        }
+        m->has_scattering = (m->sigma_s[0] > 0 || m->sigma_s[1] > 0 || m->sigma_s[2] > 0);
+        m->has_attenuation = (m->sigma_t[0] > 0 || m->sigma_t[1] > 0 || m->sigma_t[2] > 0);
+        float mx = fmaxf(m->sigma_t[0], fmaxf(m->sigma_t[1], m->sigma_t[2]));
+        m->max_beam_length = m->has_attenuation ? (float)(-log(1e-9) / mx) : INFINITY;
+        m->mfp = m->has_attenuation ? 1.0f / mx : INFINITY;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Single-precision exp by the Cephes method (S. Moshier, Cephes Math Library
+ * expf), as vectorised by J. Pommier's sse_mathfun exp_ps, which the reference
+ * uses for RGB exponentials (Rgb.hxx:334-336 -> FastFloat.h:85-87 ->
+ * sse_mathfun.h:250-300): clamp to +-88.3762626647949, split
+ * x = n*ln2 + r with a two-part ln2 (0.693359375, -2.12194440e-4), a degree-5
+ * minimax polynomial on r, then scale by 2^n. */
+float or_exp_cephes(float x) {
+    const float hi = 88.3762626647949f, lo = -88.3762626647949f;
+    if (x > hi) x = hi;
+    if (x < lo) x = lo;
+    float fx = x * 1.44269504088896341f;
+    fx = fx + 0.5f;
+    float fl = floorf(fx);
+    float tmp = fl * 0.693359375f;
+    float z = fl * -2.12194440e-4f;
+    x = x - tmp;
+    x = x - z;
+    z = x * x;
+    float y = 1.9875691500E-4f;
+    y = y * x;
+    y = y + 1.3981999507E-3f;
+    y = y * x;
+    y = y + 8.3334519073E-3f;
+    y = y * x;
+    y = y + 4.1665795894E-2f;
+    y = y * x;
+    y = y + 1.6666665459E-1f;
+    y = y * x;
+    y = y + 5.0000001201E-1f;
+    y = y * z;
+    y = y + x;
+    y = y + 1.0f;
+    int n = (int)fl + 127;
+    uint32_t bits = (uint32_t)n << 23;
+    float p2;
+    memcpy(&p2, &bits, 4);
+    return y * p2;
+}
+
+uint32_t or_hash32(uint32_t i) {
+    /* "lowbias32" integer hash (C. Wellons); i + 1 so that index 0 hashes non-trivially */
+    uint32_t x = i + 1u;
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Acceleration structures (culling only). */
+typedef struct or_bvh_node {
+    float lo[3], hi[3];
+    int32_t first; /* leaf: first item; inner: index of the left child (right = first + 1) */
+    int32_t count; /* leaf: item count > 0; inner: 0 */
+} or_bvh_node;
+
+struct or_photons {
+    int64_t n;    /* kept photons */
+    int64_t n_in; /* photons given to create (before the in_medium filter) */
+    float* pos;
+    float* r2;
+    float* inc;
+    float* power;
+    int64_t* orig;
+    or_bvh_node* nodes;
+    int64_t n_nodes;
+};
+
+typedef struct or_grid {
+    float lo[3], hi[3];
+    float cell[3], inv_cell[3];
+    int res[3];
+    int64_t shift[3];
+    uint32_t* cells;
+    int64_t n_cells;
+    uint32_t* ptrs;
+    int64_t n_ptrs;
+} or_grid;
+
+struct or_beams {
+    int64_t n;
+    float* o;
+    float* d;
+    float* len;
+    float* flux;
+    uint32_t* flags;
+    int32_t* medium;
+    float* r0;
+    float* r1;
+    float* dr;
+    float* r2max;
+    float* margins;
+    or_grid grid;
+};
+
+/* Slab test of a ray parameter interval against a box (culling only). */
+static inline int or_slab(const float* lo, const float* hi, const float* o, const float* inv,
+                          float tn, float tf) {
+    for (int a = 0; a < 3; ++a) {
+        float t0 = (lo[a] - o[a]) * inv[a];
+        float t1 = (hi[a] - o[a]) * inv[a];
+        if (t0 != t0) t0 = -INFINITY;
+        if (t1 != t1) t1 = INFINITY;
+        float a0 = t0 < t1 ? t0 : t1, a1 = t0 < t1 ? t1 : t0;
+        if (a0 > tn) tn = a0;
+        if (a1 < tf) tf = a1;
+    }
+    return tn <= tf;
+}
+
+/* BoundingBox3::intersect, SmallUPBP/src/Structs/BoundingBox.hxx:218-271
+ * (flipcode SSE ray-box test, NaN-filtered by min/max against +-inf). */
+static int or_sse_box_intersect(const float* lo, const float* hi, const float* o, const float* inv,
+                                float* tmin, float* tmax) {
+    float lmax = INFINITY, lmin = -INFINITY;
+    float mx[3], mn[3];
+    for (int a = 0; a < 3; ++a) {
+        float l1 = (lo[a] - o[a]) * inv[a];
+        float l2 = (hi[a] - o[a]) * inv[a];
+        /* _mm_min_ps(l, +inf) maps NaN to +inf; _mm_max_ps(l, -inf) maps NaN to -inf */
+        float l1a = (l1 < INFINITY) ? l1 : INFINITY, l2a = (l2 < INFINITY) ? l2 : INFINITY;
+        float l1b = (l1 > -INFINITY) ? l1 : -INFINITY, l2b = (l2 > -INFINITY) ? l2 : -INFINITY;
+        mx[a] = (l1a > l2a) ? l1a : l2a;
+        mn[a] = (l1b < l2b) ? l1b : l2b;
+    }
+    lmax = mx[0];
+    lmin = mn[0];
+    if (mx[1] < lmax) lmax = mx[1];
+    if (mn[1] > lmin) lmin = mn[1];
+    if (mx[2] < lmax) lmax = mx[2];
+    if (mn[2] > lmin) lmin = mn[2];
+    if (lmax >= lmin) {
+        *tmin = lmin;
+        *tmax = lmax;
+        return 1;
+    }
+    return 0;
+}
+
+/* ---- binned-SAH BVH over photon cubes ----------------------------------- */
+typedef struct bvh_build {
+    const float* blo; /* [n][3] item boxes */
+    const float* bhi;
+    const float* cen; /* [n][3] centroids */
+    int32_t* idx;
+    or_bvh_node* nodes;
+    int64_t n_nodes, cap;
+    int leaf_size;
+} bvh_build;
+
+static int64_t bvh_alloc(bvh_build* b, int k) {
+    if (b->n_nodes + k > b->cap) {
+        b->cap = (b->cap * 2 > b->n_nodes + k) ? b->cap * 2 : b->n_nodes + k;
+        b->nodes = (or_bvh_node*)realloc(b->nodes, (size_t)b->cap * sizeof(or_bvh_node));
+    }
+    int64_t r = b->n_nodes;
+    b->n_nodes += k;
+    return r;
+}
+
+static float box_area(const float* lo, const float* hi) {
+    float e0 = hi[0] - lo[0], e1 = hi[1] - lo[1], e2 = hi[2] - lo[2];
+    if (e0 < 0 || e1 < 0 || e2 < 0) return 0;
+    return e0 * e1 + e1 * e2 + e2 * e0;
+}
+
+static void bvh_recurse(bvh_build* b, int64_t node, int32_t first, int32_t count, float eps) {
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int32_t k = first; k < first + count; ++k) {
+        int32_t i = b->idx[k];
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fminf(lo[a], b->blo[3 * i + a]);
+            hi[a] = fmaxf(hi[a], b->bhi[3 * i + a]);
+            clo[a] = fminf(clo[a], b->cen[3 * i + a]);
+            chi[a] = fmaxf(chi[a], b->cen[3 * i + a]);
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        b->nodes[node].lo[a] = lo[a] - eps;
+        b->nodes[node].hi[a] = hi[a] + eps;
+    }
+    if (count <= b->leaf_size) {
+        b->nodes[node].first = first;
+        b->nodes[node].count = count;
+        return;
+    }
+    /* 16-bin SAH on the widest centroid axis; median split when degenerate */
+    int axis = 0;
+    float ext = chi[0] - clo[0];
+    for (int a = 1; a < 3; ++a)
+        if (chi[a] - clo[a] > ext) {
+            ext = chi[a] - clo[a];
+            axis = a;
+        }
+    int32_t mid = first + count / 2;
+    if (ext > 0) {
+        enum { NB = 16 };
+        int32_t cnt[NB] = {0};
+        float blo[NB][3], bhi[NB][3];
+        for (int k = 0; k < NB; ++k)
+            for (int a = 0; a < 3; ++a) {
+                blo[k][a] = INFINITY;
+                bhi[k][a] = -INFINITY;
+            }
+        float scale = NB / ext;
+        for (int32_t k = first; k < first + count; ++k) {
+            int32_t i = b->idx[k];
+            int bin = (int)((b->cen[3 * i + axis] - clo[axis]) * scale);
+            if (bin >= NB) bin = NB - 1;
+            if (bin < 0) bin = 0;
+            cnt[bin]++;
+            for (int a = 0; a < 3; ++a) {
+                blo[bin][a] = fminf(blo[bin][a], b->blo[3 * i + a]);
+                bhi[bin][a] = fmaxf(bhi[bin][a], b->bhi[3 * i + a]);
+            }
+        }
+        float best = INFINITY;
+        int best_split = -1;
+        for (int s = 1; s < NB; ++s) {
+            float llo[3] = {INFINITY, INFINITY, INFINITY}, lhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            float rlo[3] = {INFINITY, INFINITY, INFINITY}, rhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            int32_t nl = 0, nr = 0;
+            for (int k = 0; k < s; ++k) {
+                nl += cnt[k];
+                for (int a = 0; a < 3; ++a) {
+                    llo[a] = fminf(llo[a], blo[k][a]);
+                    lhi[a] = fmaxf(lhi[a], bhi[k][a]);
+                }
+            }
+            for (int k = s; k < NB; ++k) {
+                nr += cnt[k];
+                for (int a = 0; a < 3; ++a) {
+                    rlo[a] = fminf(rlo[a], blo[k][a]);
+                    rhi[a] = fmaxf(rhi[a], bhi[k][a]);
+                }
+            }
+            if (nl == 0 || nr == 0) continue;
+            float cost = box_area(llo, lhi) * nl + box_area(rlo, rhi) * nr;
+            if (cost < best) {
+                best = cost;
+                best_split = s;
+            }
+        }
+        if (best_split > 0) {
+            int32_t i0 = first, i1 = first + count - 1;
+            while (i0 <= i1) {
+                int32_t i = b->idx[i0];
+                int bin = (int)((b->cen[3 * i + axis] - clo[axis]) * scale);
+                if (bin >= NB) bin = NB - 1;
+                if (bin < 0) bin = 0;
+                if (bin < best_split) {
+                    i0++;
+                } else {
+                    int32_t t = b->idx[i0];
+                    b->idx[i0] = b->idx[i1];
+                    b->idx[i1] = t;
+                    i1--;
+                }
+            }
+            mid = i0;
+        }
+    }
+    if (mid <= first || mid >= first + count) mid = first + count / 2;
+    int64_t c = bvh_alloc(b, 2);
+    b->nodes[node].first = (int32_t)c;
+    b->nodes[node].count = 0;
+    bvh_recurse(b, c, first, mid - first, eps);
+    bvh_recurse(b, c + 1, mid, first + count - mid, eps);
+}
+
+static void photons_build_bvh(or_photons* ph, int leaf_size) {
+    int64_t n = ph->n;
+    if (n <= 0) return;
+    float* blo = (float*)malloc((size_t)n * 12);
+    float* bhi = (float*)malloc((size_t)n * 12);
+    float* cen = (float*)malloc((size_t)n * 12);
+    float maxc = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        float r = sqrtf(ph->r2[i]);
+        for (int a = 0; a < 3; ++a) {
+            float p = ph->pos[3 * i + a];
+            blo[3 * i + a] = p - r;
+            bhi[3 * i + a] = p + r;
+            cen[3 * i + a] = p;
+            if (fabsf(p) + r > maxc) maxc = fabsf(p) + r;
+        }
+    }
+    bvh_build b;
+    memset(&b, 0, sizeof(b));
+    b.blo = blo;
+    b.bhi = bhi;
+    b.cen = cen;
+    b.leaf_size = leaf_size > 0 ? leaf_size : 4;
+    b.idx = (int32_t*)malloc((size_t)n * 4);
+    for (int64_t i = 0; i < n; ++i) b.idx[i] = (int32_t)i;
+    bvh_alloc(&b, 1);
+    bvh_recurse(&b, 0, 0, (int32_t)n, 1e-5f * (maxc > 1 ? maxc : 1));
+    /* reorder photon arrays into leaf order so leaves are contiguous */
+    float* np = (float*)malloc((size_t)n * 12);
+    float* nr = (float*)malloc((size_t)n * 4);
+    float* ni = (float*)malloc((size_t)n * 12);
+    float* nw = (float*)malloc((size_t)n * 12);
+    int64_t* no = (int64_t*)malloc((size_t)n * 8);
+    for (int64_t k = 0; k < n; ++k) {
+        int32_t i = b.idx[k];
+        memcpy(&np[3 * k], &ph->pos[3 * i], 12);
+        nr[k] = ph->r2[i];
+        memcpy(&ni[3 * k], &ph->inc[3 * i], 12);
+        memcpy(&nw[3 * k], &ph->power[3 * i], 12);
+        no[k] = ph->orig[i];
+    }
+    free(ph->pos);
+    free(ph->r2);
+    free(ph->inc);
+    free(ph->power);
+    free(ph->orig);
+    ph->pos = np;
+    ph->r2 = nr;
+    ph->inc = ni;
+    ph->power = nw;
+    ph->orig = no;
+    ph->nodes = b.nodes;
+    ph->n_nodes = b.n_nodes;
+    free(b.idx);
+    free(blo);
+    free(bhi);
+    free(cen);
+}
+
+or_photons* or_photons_create(int64_t n, const float* pos, const float* inc_dir, const float* power,
+                              const float* radius, float const_radius, const uint8_t* in_medium,
+                              int32_t leaf_size) {
+    or_photons* ph = (or_photons*)calloc(1, sizeof(or_photons));
+    int64_t m = 0;
+    for (int64_t i = 0; i < n; ++i)
+        if (!in_medium || in_medium[i]) m++;
+    ph->n = m;
+    ph->n_in = n;
+    ph->pos = (float*)malloc((size_t)(m ? m : 1) * 12);
+    ph->r2 = (float*)malloc((size_t)(m ? m : 1) * 4);
+    ph->inc = (float*)malloc((size_t)(m ? m : 1) * 12);
+    ph->power = (float*)malloc((size_t)(m ? m : 1) * 12);
+    ph->orig = (int64_t*)malloc((size_t)(m ? m : 1) * 8);
+    int64_t k = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        if (in_medium && !in_medium[i]) continue;
+        float r = radius ? radius[i] : const_radius;
+        memcpy(&ph->pos[3 * k], &pos[3 * i], 12);
+        ph->r2[k] = r * r; /* radiusSqr = aRadius*aRadius, Bre.cxx:148 */
+        if (inc_dir)
+            memcpy(&ph->inc[3 * k], &inc_dir[3 * i], 12);
+        else
+            ph->inc[3 * k] = ph->inc[3 * k + 1] = ph->inc[3 * k + 2] = 0.0f;
+        memcpy(&ph->power[3 * k], &power[3 * i], 12);
+        ph->orig[k] = i;
+        k++;
+    }
+    if (leaf_size >= 0) photons_build_bvh(ph, leaf_size);
+    return ph;
+}
+
+void or_photons_free(or_photons* ph) {
+    if (!ph) return;
+    free(ph->pos);
+    free(ph->r2);
+    free(ph->inc);
+    free(ph->power);
+    free(ph->orig);
+    free(ph->nodes);
+    free(ph);
+}
+
+int64_t or_photons_count(const or_photons* ph) { return ph->n; }
+
+void or_photons_orig_index(const or_photons* ph, int64_t* out) {
+    for (int64_t i = 0; i < ph->n; ++i) out[i] = ph->orig[i];
+}
+
+/* ---- beams: radii and the reference grid -------------------------------- */
+/* PhotonBeam::getSegmentAABB, SmallUPBP/src/Beams/PhotonBeam.hxx:186-198 */
+static void beam_segment_aabb(const or_beams* bm, int64_t i, float s0, float s1, float* lo, float* hi) {
+    s0 *= bm->len[i];
+    s1 *= bm->len[i];
+    float rs = bm->r0[i] + s0 * bm->dr[i];
+    float re = bm->r0[i] + s1 * bm->dr[i];
+    for (int a = 0; a < 3; ++a) {
+        float bs = bm->o[3 * i + a] + bm->d[3 * i + a] * s0;
+        float be = bm->o[3 * i + a] + bm->d[3 * i + a] * s1;
+        float ms = rs * bm->margins[3 * i + a], me = re * bm->margins[3 * i + a];
+        float l0 = bs - ms, l1 = be - me, h0 = bs + ms, h1 = be + me;
+        lo[a] = l0 < l1 ? l0 : l1;
+        hi[a] = h0 > h1 ? h0 : h1;
+    }
+}
+
+/* Grid::build + buildloop, SmallUPBP/src/Beams/Grid.hxx:167-279, and
+ * AccelStruct::build (PhGrid.hxx:75-87) for the grid AABB. */
+static void beams_build_grid(or_beams* bm, int res) {
+    or_grid* g = &bm->grid;
+    for (int a = 0; a < 3; ++a) {
+        g->lo[a] = INFINITY;
+        g->hi[a] = -INFINITY;
+    }
+    for (int64_t i = 0; i < bm->n; ++i) {
+        float lo[3], hi[3];
+        beam_segment_aabb(bm, i, 0.0f, 1.0f, lo, hi);
+        for (int a = 0; a < 3; ++a) {
+            g->lo[a] = fminf(g->lo[a], lo[a]);
+            g->hi[a] = fmaxf(g->hi[a], hi[a]);
+        }
+    }
+    float ext[3] = {g->hi[0] - g->lo[0], g->hi[1] - g->lo[1], g->hi[2] - g->lo[2]};
+    float emax = ext[0] > ext[1] ? ext[0] : ext[1];
+    if (ext[2] > emax) emax = ext[2];
+    float size = emax / (float)res;
+    for (int a = 0; a < 3; ++a) g->res[a] = (int)(unsigned)(ext[a] / size) + 1;
+    g->shift[0] = 1;
+    g->shift[1] = g->res[0];
+    g->shift[2] = (int64_t)g->res[0] * g->res[1];
+    for (int a = 0; a < 3; ++a) {
+        g->cell[a] = ext[a] / (float)g->res[a];
+        g->inv_cell[a] = 1.0f / g->cell[a];
+    }
+    g->n_cells = (int64_t)(g->res[0] + 1) * (g->res[1] + 1) * (g->res[2] + 1) + 1;
+    g->cells = (uint32_t*)calloc((size_t)g->n_cells, 4);
+    uint32_t* dup = (uint32_t*)malloc((size_t)g->n_cells * 4);
+    for (int64_t c = 0; c < g->n_cells; ++c) dup[c] = (uint32_t)bm->n;
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1) {
+            /* inclusive prefix sum (Grid.hxx:256-264), then Store decrements back */
+            uint64_t acc = 0;
+            for (int64_t c = 0; c < g->n_cells; ++c) {
+                acc += g->cells[c];
+                g->cells[c] = (uint32_t)acc;
+                dup[c] = (uint32_t)bm->n;
+            }
+            g->n_ptrs = (int64_t)acc;
+            g->ptrs = (uint32_t*)malloc((size_t)(acc ? acc : 1) * 4);
+        }
+        for (int64_t i = 0; i < bm->n; ++i) {
+            float blo[3], bhi[3];
+            beam_segment_aabb(bm, i, 0.0f, 1.0f, blo, bhi);
+            float ext_b[3] = {bhi[0] - blo[0], bhi[1] - blo[1], bhi[2] - blo[2]};
+            float ad[3] = {fabsf(bm->d[3 * i]), fabsf(bm->d[3 * i + 1]), fabsf(bm->d[3 * i + 2])};
+            int max_axis = (ad[0] > ad[1]) ? ((ad[0] > ad[2]) ? 0 : 2) : ((ad[1] > ad[2]) ? 1 : 2);
+            int chops = (int)(ext_b[max_axis] * g->inv_cell[max_axis]) + 1;
+            float inv_chops = 1.0f / (float)chops;
+            for (int ch = 0; ch < chops; ++ch) {
+                float lo[3], hi[3];
+                beam_segment_aabb(bm, i, (float)ch * inv_chops, (float)(ch + 1) * inv_chops, lo, hi);
+                int is[3], ie[3];
+                for (int a = 0; a < 3; ++a) {
+                    is[a] = (int)((lo[a] - g->lo[a]) * g->inv_cell[a]);
+                    ie[a] = (int)((hi[a] - g->lo[a]) * g->inv_cell[a]);
+                }
+                int64_t id = (int64_t)is[0] + (int64_t)g->res[0] * ((int64_t)is[1] + (int64_t)g->res[1] * is[2]);
+                for (int z = is[2]; z <= ie[2]; ++z, id += g->shift[2]) {
+                    int64_t idy = id;
+                    for (int y = is[1]; y <= ie[1]; ++y, idy += g->shift[1]) {
+                        int64_t idx = idy;
+                        for (int x = is[0]; x <= ie[0]; ++x, ++idx) {
+                            if (idx < 0) continue;
+                            if (idx >= g->n_cells) break;
+                            if (dup[idx] != (uint32_t)i) {
+                                dup[idx] = (uint32_t)i;
+                                if (pass == 0)
+                                    g->cells[idx]++;
+                                else
+                                    g->ptrs[--g->cells[idx]] = (uint32_t)i;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+    free(dup);
+}
+
+or_beams* or_beams_create(int64_t n, const float* origin, const float* dir, const float* length,
+                          const float* throughput, const uint32_t* flags, const int32_t* medium,
+                          const float* start_radius, const float* end_radius, float const_radius,
+                          int32_t grid_res) {
+    or_beams* bm = (or_beams*)calloc(1, sizeof(or_beams));
+    int64_t m = n > 0 ? n : 1;
+    bm->n = n;
+    bm->o = (float*)malloc((size_t)m * 12);
+    bm->d = (float*)malloc((size_t)m * 12);
+    bm->len = (float*)malloc((size_t)m * 4);
+    bm->flux = (float*)malloc((size_t)m * 12);
+    bm->flags = (uint32_t*)malloc((size_t)m * 4);
+    bm->medium = (int32_t*)malloc((size_t)m * 4);
+    bm->r0 = (float*)malloc((size_t)m * 4);
+    bm->r1 = (float*)malloc((size_t)m * 4);
+    bm->dr = (float*)malloc((size_t)m * 4);
+    bm->r2max = (float*)malloc((size_t)m * 4);
+    bm->margins = (float*)malloc((size_t)m * 12);
+    for (int64_t i = 0; i < n; ++i) {
+        memcpy(&bm->o[3 * i], &origin[3 * i], 12);
+        memcpy(&bm->d[3 * i], &dir[3 * i], 12);
+        bm->len[i] = length[i];
+        memcpy(&bm->flux[3 * i], &throughput[3 * i], 12);
+        bm->flags[i] = flags[i];
+        bm->medium[i] = medium[i];
+        /* PhBeams.cxx:92-117 */
+        if (start_radius) {
+            float r0 = start_radius[i];
+            float r1 = end_radius ? end_radius[i] : r0;
+            float mx = r0 > r1 ? r0 : r1;
+            bm->r0[i] = r0;
+            bm->r1[i] = r1;
+            bm->r2max[i] = mx * mx;
+            bm->dr[i] = (r1 - r0) / length[i];
+        } else {
+            bm->r0[i] = bm->r1[i] = const_radius;
+            bm->r2max[i] = const_radius * const_radius;
+            bm->dr[i] = 0.0f;
+        }
+        /* margins of the cone AABB, PhotonBeam.hxx:74-80 */
+        float dx = bm->d[3 * i] * bm->d[3 * i], dy = bm->d[3 * i + 1] * bm->d[3 * i + 1],
+              dz = bm->d[3 * i + 2] * bm->d[3 * i + 2];
+        bm->margins[3 * i] = sqrtf(dy + dz);
+        bm->margins[3 * i + 1] = sqrtf(dx + dz);
+        bm->margins[3 * i + 2] = sqrtf(dx + dy);
+    }
+    if (grid_res > 0 && n > 0) beams_build_grid(bm, grid_res);
+    return bm;
+}
+
+void or_beams_free(or_beams* bm) {
+    if (!bm) return;
+    free(bm->o);
+    free(bm->d);
+    free(bm->len);
+    free(bm->flux);
+    free(bm->flags);
+    free(bm->medium);
+    free(bm->r0);
+    free(bm->r1);
+    free(bm->dr);
+    free(bm->r2max);
+    free(bm->margins);
+    free(bm->grid.cells);
+    free(bm->grid.ptrs);
+    free(bm);
+}
+
+int64_t or_beams_count(const or_beams* bm) { return bm->n; }
+
+void or_beams_grid_info(const or_beams* bm, int64_t* info6) {
+    info6[0] = bm->grid.res[0];
+    info6[1] = bm->grid.res[1];
+    info6[2] = bm->grid.res[2];
+    info6[3] = bm->grid.n_cells;
+    info6[4] = bm->grid.n_ptrs;
+    info6[5] = bm->n;
+}
+
+/* ------------------------------------------------------------------------- */
+/* The estimator mathematics, in FP32 and FP64. */
+#define REAL float
+#define SUF(name) name##_f32
+#define EXPRGB(x) or_exp_cephes(x)
+#define EXP1(x) expf(x)
+#define SQRT(x) sqrtf(x)
+#define FABS(x) fabsf(x)
+#include "oracle_core.inc"
+#undef REAL
+#undef SUF
+#undef EXPRGB
+#undef EXP1
+#undef SQRT
+#undef FABS
+
+#define REAL double
+#define SUF(name) name##_f64
+#define EXPRGB(x) exp(x)
+#define EXP1(x) exp(x)
+#define SQRT(x) sqrt(x)
+#define FABS(x) fabs(x)
+#include "oracle_core.inc"
+#undef REAL
+#undef SUF
+#undef EXPRGB
+#undef EXP1
+#undef SQRT
+#undef FABS
+
+float or_phase_pdf(const float fix[3], const float gen[3], float g, float* sin_theta) {
+    if (sin_theta) {
+        float c = -dot3_f32(gen, fix);
+        *sin_theta = sqrtf(fmaxf(0.f, 1.f - c * c));
+    }
+    return phase_f32(fix, gen, g);
+}
+
+int or_test_bre(const float o[3], const float d[3], float min_t, float max_t, const float p[3],
+                float r2, float* t, float* s) {
+    return test_bre_f32(o, d, min_t, max_t, p, r2, t, s);
+}
+
+int or_test_beam_beam(const float o1[3], const float d1[3], float min_t1, float max_t1,
+                      const float o2[3], const float d2[3], float min_t2, float max_t2,
+                      float max_dist_sqr, float* dist, float* sin_theta, float* t1, float* t2) {
+    return test_beam_beam_f32(o1, d1, min_t1, max_t1, o2, d2, min_t2, max_t2, max_dist_sqr, dist,
+                              sin_theta, t1, t2);
+}
+
+static int check_batch(const upbp_query_batch* q, int32_t n_media) {
+    if (!q || q->count < 0) return UPBP_EINVAL;
+    if (q->count > 0 && (!q->origin || !q->direction || !q->segment_offsets)) return UPBP_EINVAL;
+    if (q->lite_segments && q->beam_type != UPBP_LONG_BEAM) return UPBP_EINVAL;
+    if (n_media <= 0) return UPBP_EINVAL;
+    return 0;
+}
+
+int or_bre_eval(const or_photons* ph, const upbp_medium* media, int32_t n_media,
+                const upbp_query_batch* q, int accel, int precision, int threads, float* out32,
+                double* out64, uint32_t* sig, uint64_t* counters) {
+    int e = check_batch(q, n_media);
+    if (e) return e;
+    or_medium* m = (or_medium*)malloc(sizeof(or_medium) * (size_t)n_media);
+    or_prepare_media(media, n_media, m);
+    if (threads <= 0) threads = 1;
+    int r;
+    if (precision == 64)
+        r = bre_eval_all_f64(ph, m, n_media, q, accel, threads, out64, sig, counters);
+    else
+        r = bre_eval_all_f32(ph, m, n_media, q, accel, threads, out32, sig, counters);
+    free(m);
+    return r;
+}
+
+int or_beams_eval(const or_beams* bm, const upbp_medium* media, int32_t n_media,
+                  const upbp_query_batch* q, int accel, int precision, int threads, float* out32,
+                  double* out64, uint32_t* sig, uint64_t* counters) {
+    int e = check_batch(q, n_media);
+    if (e) return e;
+    or_medium* m = (or_medium*)malloc(sizeof(or_medium) * (size_t)n_media);
+    or_prepare_media(media, n_media, m);
+    if (threads <= 0) threads = 1;
+    int r;
+    if (precision == 64)
+        r = beams_eval_all_f64(bm, m, n_media, q, accel == 1 ? 0 : accel, threads, out64, sig, counters);
+    else
+        r = beams_eval_all_f32(bm, m, n_media, q, accel, threads, out32, sig, counters);
+    free(m);
+    return r;
+}
+
+/* ------------------------------------------------------------------------- */
+/* FP64 analytic gradient of the plain estimators (LONG queries).
+ * Forward (per query q, segment s of medium m, prefix P_s = prod_{j<s} A_j with
+ * A_j = exp(-sigma_t[m_j] * (dist_max_j - dist_min_j))):
+ *   photon hit:  C_c = P_c * flux_c * exp(-st_c * (t - tmin)) * ss_c * rho * K
+ *   beam hit:    C_c = P_c * flux_c * ss_c * exp(-st_c * (t1 - tmin)) * Eb_c * rho * K1
+ *                Eb_c = exp(-st_c t2), divided by exp(-st_k t2) for SHORT beams.
+ * d/d sigma_s = direct + d/d sigma_t; d/d sigma_a = d/d sigma_t. */
+typedef struct adj_acc {
+    double* gs; /* [n_media][3] d/d sigma_s direct */
+    double* gt; /* [n_media][3] d/d sigma_t */
+} adj_acc;
+
+int or_adjoint(const or_photons* ph, const or_beams* bm, const upbp_medium* media, int32_t n_media,
+               const upbp_query_batch* q, const double* w_ph, const double* w_bm, int threads,
+               double* grad_sigma_s, double* grad_sigma_a, double* grad_photon_power,
+               double* grad_beam_power) {
+    int e = check_batch(q, n_media);
+    if (e) return e;
+    if (q->beam_type != UPBP_LONG_BEAM) return UPBP_EUNSUPPORTED;
+    or_medium* md = (or_medium*)malloc(sizeof(or_medium) * (size_t)n_media);
+    or_prepare_media(media, n_media, md);
+    if (threads <= 0) threads = 1;
+    int64_t nph = ph ? ph->n : 0, nbm = bm ? bm->n : 0;
+    double* gs = (double*)calloc((size_t)n_media * 3, 8);
+    double* gt = (double*)calloc((size_t)n_media * 3, 8);
+    double* gph = (double*)calloc((size_t)(nph ? nph : 1) * 3, 8);
+    double* gbm = (double*)calloc((size_t)(nbm ? nbm : 1) * 3, 8);
+    int err = 0;
+#pragma omp parallel num_threads(threads)
+    {
+        double* ls = (double*)calloc((size_t)n_media * 3, 8);
+        double* lt = (double*)calloc((size_t)n_media * 3, 8);
+#pragma omp for schedule(dynamic, 16)
+        for (int64_t qi = 0; qi < q->count; ++qi) {
+            double o[3], d[3];
+            for (int a = 0; a < 3; ++a) {
+                o[a] = q->origin[3 * qi + a];
+                d[a] = q->direction[3 * qi + a];
+            }
+            double P[3] = {1, 1, 1};
+            int64_t s0 = q->segment_offsets[qi], s1 = q->segment_offsets[qi + 1];
+            for (int64_t si = s0; si < s1; ++si) {
+                const upbp_volume_segment* sg = &q->segments[si];
+                if (sg->medium_id < 0 || sg->medium_id >= n_media) {
+                    err = 1;
+                    break;
+                }
+                const or_medium* m = &md[sg->medium_id];
+                double tmin = sg->dist_min, tmax = sg->dist_max;
+                double g[3] = {0, 0, 0}; /* sum over hits of w_c * C_c in this segment */
+                if (m->has_scattering) {
+                    const double* w = w_ph ? &w_ph[3 * qi] : NULL;
+                    for (int64_t i = 0; w && i < nph; ++i) {
+                        double p[3] = {ph->pos[3 * i], ph->pos[3 * i + 1], ph->pos[3 * i + 2]};
+                        double t, s;
+                        if (!test_bre_f64(o, d, tmin, tmax, p, (double)ph->r2[i], &t, &s)) continue;
+                        double r2 = ph->r2[i];
+                        double inc[3] = {-ph->inc[3 * i], -ph->inc[3 * i + 1], -ph->inc[3 * i + 2]};
+                        double rho = phase_f64(d, inc, m->mean_cosine);
+                        double K = (1.0 - s / r2) / (r2 * (double)PI_F * 0.5);
+                        for (int c = 0; c < 3; ++c) {
+                            double E = exp(-(double)m->sigma_t[c] * (t - tmin));
+                            double base = P[c] * E * rho * K; /* C_c = base * flux_c * ss_c */
+                            double Cc = base * ph->power[3 * i + c] * m->sigma_s[c];
+#pragma omp atomic
+                            gph[3 * i + c] += w[c] * base * m->sigma_s[c];
+                            ls[3 * sg->medium_id + c] += w[c] * base * ph->power[3 * i + c];
+                            lt[3 * sg->medium_id + c] += w[c] * Cc * (-(t - tmin));
+                            g[c] += w[c] * Cc;
+                        }
+                    }
+                    const double* wb = w_bm ? &w_bm[3 * qi] : NULL;
+                    for (int64_t i = 0; wb && i < nbm; ++i) {
+                        if (bm->medium[i] != sg->medium_id) continue;
+                        double o2[3] = {bm->o[3 * i], bm->o[3 * i + 1], bm->o[3 * i + 2]};
+                        double d2[3] = {bm->d[3 * i], bm->d[3 * i + 1], bm->d[3 * i + 2]};
+                        double dist, st, t1, t2;
+                        if (!test_beam_beam_f64(o, d, tmin, tmax, o2, d2, 0.0, bm->len[i],
+                                                bm->r2max[i], &dist, &st, &t1, &t2))
+                            continue;
+                        double radius = (double)bm->r0[i] + t2 * (double)bm->dr[i];
+                        if (radius < dist) continue;
+                        double rho = phase_f64(d, d2, m->mean_cosine);
+                        double K1 = (1.0 - dist * dist / (radius * radius)) * 3.0 / (4.0 * radius * st);
+                        int k = m->min_idx;
+                        int sb = (bm->flags[i] & UPBP_SHORT_BEAM) != 0;
+                        double Ebk = exp(-(double)m->sigma_t[k] * t2);
+                        double Cs[3];
+                        for (int c = 0; c < 3; ++c) {
+                            double Eq = exp(-(double)m->sigma_t[c] * (t1 - tmin));
+                            double Eb = exp(-(double)m->sigma_t[c] * t2);
+                            if (sb) Eb /= Ebk;
+                            double base = P[c] * Eq * Eb * rho * K1;
+                            double Cc = base * bm->flux[3 * i + c] * m->sigma_s[c];
+                            Cs[c] = wb[c] * Cc;
+#pragma omp atomic
+                            gbm[3 * i + c] += wb[c] * base * m->sigma_s[c];
+                            ls[3 * sg->medium_id + c] += wb[c] * base * bm->flux[3 * i + c];
+                            double diag = -(t1 - tmin) - ((sb && c == k) ? 0.0 : t2);
+                            lt[3 * sg->medium_id + c] += Cs[c] * diag;
+                            g[c] += Cs[c];
+                        }
+                        if (sb)
+                            for (int c = 0; c < 3; ++c)
+                                if (c != k) lt[3 * sg->medium_id + k] += Cs[c] * t2;
+                    }
+                }
+                /* prefix transmittance of the earlier segments */
+                for (int64_t sj = s0; sj < si; ++sj) {
+                    const upbp_volume_segment* sp = &q->segments[sj];
+                    double len = (double)sp->dist_max - (double)sp->dist_min;
+                    for (int c = 0; c < 3; ++c) lt[3 * sp->medium_id + c] += g[c] * (-len);
+                }
+                for (int c = 0; c < 3; ++c) P[c] *= (double)sg->attenuation[c];
+                if (!(P[0] > 1e-20 && P[1] > 1e-20 && P[2] > 1e-20)) break;
+            }
+        }
+#pragma omp critical
+        {
+            for (int64_t k = 0; k < (int64_t)n_media * 3; ++k) {
+                gs[k] += ls[k];
+                gt[k] += lt[k];
+            }
+        }
+        free(ls);
+        free(lt);
+    }
+    for (int64_t k = 0; k < (int64_t)n_media * 3; ++k) {
+        if (grad_sigma_s) grad_sigma_s[k] = gs[k] + gt[k];
+        if (grad_sigma_a) grad_sigma_a[k] = gt[k];
+    }
+    if (grad_photon_power && ph) {
+        memset(grad_photon_power, 0, (size_t)ph->n_in * 3 * sizeof(double));
+        for (int64_t i = 0; i < nph; ++i)
+            for (int c = 0; c < 3; ++c) grad_photon_power[3 * ph->orig[i] + c] = gph[3 * i + c];
+    }
+    if (grad_beam_power)
+        for (int64_t i = 0; i < nbm; ++i)
+            for (int c = 0; c < 3; ++c) grad_beam_power[3 * i + c] = gbm[3 * i + c];
+    free(gs);
+    free(gt);
+    free(gph);
+    free(gbm);
+    free(md);
+    return err ? UPBP_EINVAL : 0;
+}

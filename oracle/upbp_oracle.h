@@ -1,0 +1,117 @@
+/*
+ * upbp_oracle.h -- CPU restatement of SmallUPBP's volumetric merging estimators.
+ *
+ * TEST INFRASTRUCTURE. This library is the checker of the GPU library and the
+ * timed CPU baseline. Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load it. The product (smallupbp_amd, libupbp_hip.so) never
+ * links, loads or calls it.
+ *
+ * It is a clean-room restatement of the reference algorithm in the reference's
+ * floating-point operation order (FP32, no FMA contraction), with an FP64
+ * re-evaluation of the same mathematics. Every function cites the reference
+ * file:line it follows. Accelerations:
+ *   photons: brute force over all photons, or a binned-SAH binary BVH over the
+ *            photon cubes (the role of Embree's BVH4, SmallUPBP/src/Bre/Bre.cxx:492-507);
+ *   beams:   brute force over all beams (SmallUPBP/src/Beams/PhBrute.hxx:80-88),
+ *            or the reference's uniform grid with 3D-DDA (SmallUPBP/src/Beams/Grid.hxx:167-399).
+ * Culling never changes a hit decision: the predicate that decides a hit is
+ * the same function on every path.
+ */
+#ifndef UPBP_ORACLE_H_
+#define UPBP_ORACLE_H_
+
+#include <stdint.h>
+#include "../include/upbp.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Derived medium constants (SmallUPBP/src/Scene/Media.hxx:129-159). */
+typedef struct or_medium {
+    float sigma_a[3], sigma_s[3], sigma_t[3];
+    float mean_cosine, continuation_prob;
+    float min_comp;   /* minimum positive sigma_t component (Media.hxx:142-145) */
+    int32_t min_idx;  /* its index; 0 when no component is positive */
+    int32_t has_scattering, has_attenuation;
+    float mfp, max_beam_length;
+} or_medium;
+
+void or_prepare_media(const upbp_medium* in, int32_t n, or_medium* out);
+
+/* Cephes/Pommier single-precision exp (the method behind exp_ps,
+ * SmallUPBP/src/PrecompiledLibs/FastFloat/sse_mathfun.h:250-300). */
+float or_exp_cephes(float x);
+
+/* Phase function pdf (SmallUPBP/src/Path/PhaseFunction.hxx:52-80). */
+float or_phase_pdf(const float fix[3], const float gen[3], float g, float* sin_theta);
+
+/* Reference predicates, exposed for closed-form tests. */
+int or_test_bre(const float o[3], const float d[3], float min_t, float max_t, const float p[3],
+                float r2, float* t, float* s);
+int or_test_beam_beam(const float o1[3], const float d1[3], float min_t1, float max_t1,
+                      const float o2[3], const float d2[3], float min_t2, float max_t2,
+                      float max_dist_sqr, float* dist, float* sin_theta, float* t1, float* t2);
+
+/* 32-bit hash of an item index used by hit signatures (same as the GPU library). */
+uint32_t or_hash32(uint32_t i);
+
+/* ---- photon points (PB2D) ------------------------------------------------ */
+typedef struct or_photons or_photons;
+/* EmbreeBre::build semantics (Bre.cxx:526-632): keep photons with in_medium != 0
+ * (all when NULL), radius from the array or const_radius, r2 = r * r. Builds a BVH
+ * (leaf_size items per leaf) for accel = 1. */
+or_photons* or_photons_create(int64_t n, const float* pos, const float* inc_dir, const float* power,
+                              const float* radius, float const_radius, const uint8_t* in_medium,
+                              int32_t leaf_size);
+void or_photons_free(or_photons* ph);
+int64_t or_photons_count(const or_photons* ph);
+/* Copy of the kept photons' original indices (int64 [count]). */
+void or_photons_orig_index(const or_photons* ph, int64_t* out);
+
+/* EmbreeBre::evalBre (Bre.cxx:812-879 full, :895-964 lite) with the plain
+ * PB2D callback breIntersectFuncHomogeneous (Bre.cxx:239-277), per query.
+ * accel: 0 brute force, 1 BVH. precision: 32 (reference order, Cephes exp) or
+ * 64 (double). Writes out32 ([Q][3]) for 32, out64 for 64. sig ([Q][2]) and
+ * counters ([2]: hits, candidates) are optional. Returns 0 or an upbp_status. */
+int or_bre_eval(const or_photons* ph, const upbp_medium* media, int32_t n_media,
+                const upbp_query_batch* q, int accel, int precision, int threads,
+                float* out32, double* out64, uint32_t* sig, uint64_t* counters);
+
+/* ---- photon beams (BB1D) ------------------------------------------------- */
+typedef struct or_beams or_beams;
+/* PhotonBeamsEvaluator::build semantics (PhBeams.cxx:92-117): constant radius
+ * gives start = end = r, mMaxRadiusSqr = r*r, mRadiusChange = 0; per-beam radii
+ * give mRadiusChange = (end - start) / length. grid_res > 0 builds the
+ * reference uniform grid (Grid.hxx:223-279) with that many cells on the
+ * longest axis. */
+or_beams* or_beams_create(int64_t n, const float* origin, const float* dir, const float* length,
+                          const float* throughput, const uint32_t* flags, const int32_t* medium,
+                          const float* start_radius, const float* end_radius, float const_radius,
+                          int32_t grid_res);
+void or_beams_free(or_beams* bm);
+int64_t or_beams_count(const or_beams* bm);
+/* Grid statistics: res[3], cells, pointers (registered beam references). */
+void or_beams_grid_info(const or_beams* bm, int64_t* info6);
+
+/* PhotonBeamsEvaluator::evalBeamBeamEstimate (PhBeams.cxx:177-241, :257-324)
+ * with PhotonBeam::accumulate (PhotonBeam.hxx:315-367).
+ * accel: 0 brute force (PhBrute.hxx:80-88), 1 grid DDA (Grid.hxx:318-399). */
+int or_beams_eval(const or_beams* bm, const upbp_medium* media, int32_t n_media,
+                  const upbp_query_batch* q, int accel, int precision, int threads,
+                  float* out32, double* out64, uint32_t* sig, uint64_t* counters);
+
+/* ---- adjoint (FP64 analytic gradient of the plain estimators) ----------- */
+/* Gradient of L = sum_q <w_photons[q], out_photons[q]> + <w_beams[q], out_beams[q]>
+ * (w are the upstream gradients). Brute-force traversal, double arithmetic,
+ * LONG queries only. Outputs are overwritten (not accumulated). */
+int or_adjoint(const or_photons* ph, const or_beams* bm, const upbp_medium* media, int32_t n_media,
+               const upbp_query_batch* q, const double* w_photons, const double* w_beams,
+               int threads, double* grad_sigma_s, double* grad_sigma_a,
+               double* grad_photon_power, double* grad_beam_power);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* UPBP_ORACLE_H_ */

@@ -1,0 +1,747 @@
+// capi.hip -- the extern "C" boundary declared in include/upbp.h.
+//
+// Every entry point catches all errors and returns an upbp_status; the message
+// is kept in the context (upbp_ctx_last_error). Nothing here exits the process.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "upbp_internal.hpp"
+
+using namespace upbp;
+
+struct upbp_ctx : Ctx {};
+struct upbp_bre : Bre {};
+struct upbp_beams : Beams {};
+
+struct upbp_batch_plan {
+    Ctx* ctx = nullptr;
+    int n = 0;
+    int64_t total_tiles = 0;
+    std::vector<InstanceDev> host;
+    DevBuf table;
+    DevBuf counters;  // [2] queue counters (photons, beams)
+    std::vector<DevBuf*> perms;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int graph_blocks = 0;
+    ~upbp_batch_plan() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (DevBuf* p : perms) delete p;
+    }
+};
+
+namespace {
+
+thread_local std::string g_no_ctx_error;
+
+upbp_status fail(Ctx* ctx, upbp_status st, const std::string& msg) {
+    if (ctx)
+        ctx->last_error = msg;
+    else
+        g_no_ctx_error = msg;
+    return st;
+}
+
+template <class F>
+upbp_status guarded(Ctx* ctx, F&& f) {
+    try {
+        if (ctx) UPBP_HIP(hipSetDevice(ctx->device));
+        f();
+        if (ctx) ctx->last_error.clear();
+        return UPBP_OK;
+    } catch (const Error& e) {
+        return fail(ctx, e.status, e.message);
+    } catch (const std::bad_alloc&) {
+        return fail(ctx, UPBP_ENOMEM, "host allocation failed");
+    } catch (...) {
+        return fail(ctx, UPBP_EDEVICE, "unexpected exception");
+    }
+}
+
+MediumDev derive_medium(const upbp_medium& in) {
+    // HomogeneousMedium constructor, SmallUPBP/src/Scene/Media.hxx:129-159
+    MediumDev m = {};
+    for (int c = 0; c < 3; ++c) {
+        m.sa[c] = std::fabs(in.absorption[c]);
+        m.ss[c] = std::fabs(in.scattering[c]);
+        m.st[c] = m.sa[c] + m.ss[c];
+    }
+    m.min_comp = m.st[0];
+    m.min_idx = 0;
+    if (m.st[1] > 0 && (m.min_comp == 0 || m.st[1] < m.min_comp)) {
+        m.min_comp = m.st[1];
+        m.min_idx = 1;
+    }
+    if (m.st[2] > 0 && (m.min_comp == 0 || m.st[2] < m.min_comp)) {
+        m.min_comp = m.st[2];
+        m.min_idx = 2;
+    }
+    m.g = in.mean_cosine;
+    m.cont = in.continuation_prob;
+    m.has_scattering = (m.ss[0] > 0 || m.ss[1] > 0 || m.ss[2] > 0) ? 1 : 0;
+    m.isotropic = std::fabs(in.mean_cosine) < 1e-3f ? 1 : 0;
+    return m;
+}
+
+template <class T>
+const T* stage_q(Ctx& ctx, int slot, const T* src, int64_t elems, upbp_memory mem) {
+    if (!src || mem == UPBP_MEM_DEVICE) return src;
+    ctx.staging[slot].ensure((size_t)std::max<int64_t>(elems, 1) * sizeof(T));
+    UPBP_HIP(hipMemcpyAsync(ctx.staging[slot].p, src, (size_t)elems * sizeof(T), hipMemcpyHostToDevice, ctx.stream));
+    return ctx.staging[slot].as<T>();
+}
+
+void check_batch(const Ctx& ctx, const upbp_query_batch* q) {
+    UPBP_REQUIRE(q != nullptr, UPBP_EINVAL, "query batch is NULL");
+    UPBP_REQUIRE(q->count >= 0, UPBP_EINVAL, "negative query count");
+    UPBP_REQUIRE(q->count < ((int64_t)1 << 32), UPBP_EINVAL, "query count out of range");
+    UPBP_REQUIRE(q->count == 0 || (q->origin && q->direction && q->segment_offsets), UPBP_EINVAL,
+                 "query origin, direction and segment_offsets are required");
+    UPBP_REQUIRE(q->beam_type == UPBP_SHORT_BEAM || q->beam_type == UPBP_LONG_BEAM, UPBP_EINVAL,
+                 "query beam_type must be UPBP_SHORT_BEAM or UPBP_LONG_BEAM");
+    UPBP_REQUIRE(!q->lite_segments || q->beam_type == UPBP_LONG_BEAM, UPBP_EINVAL,
+                 "lite segments require LONG query beams (Bre.cxx:903)");
+    UPBP_REQUIRE(q->order == UPBP_ORDER_SORTED || q->order == UPBP_ORDER_AS_GIVEN, UPBP_EINVAL, "bad query order");
+    UPBP_REQUIRE(ctx.n_media > 0, UPBP_EINVAL, "upbp_set_media must be called before evaluation");
+}
+
+// Device view of a batch (copies host batches), with the sorted permutation when requested.
+QueryView make_view(Ctx& ctx, const upbp_query_batch* q, const float* lo, const float* hi) {
+    QueryView v;
+    v.count = q->count;
+    v.beam_type = q->beam_type;
+    v.lite = q->lite_segments;
+    if (q->count == 0) return v;
+    int64_t nseg = 0;
+    if (q->memory == UPBP_MEM_HOST) {
+        nseg = q->segment_offsets[q->count];
+        UPBP_REQUIRE(nseg >= 0, UPBP_EINVAL, "segment_offsets[count] is negative");
+        UPBP_REQUIRE(nseg == 0 || q->segments, UPBP_EINVAL, "segments pointer is NULL");
+    }
+    v.origin = stage_q(ctx, 0, q->origin, 3 * q->count, q->memory);
+    v.direction = stage_q(ctx, 1, q->direction, 3 * q->count, q->memory);
+    v.offsets = stage_q(ctx, 2, q->segment_offsets, q->count + 1, q->memory);
+    v.segments = q->memory == UPBP_MEM_HOST ? stage_q(ctx, 3, q->segments, nseg, q->memory) : q->segments;
+    if (q->order == UPBP_ORDER_SORTED && q->count > 64) {
+        ctx.qkeys.ensure(q->count * 4);
+        ctx.qkeys2.ensure(q->count * 4);
+        ctx.qperm.ensure(q->count * 4);
+        ctx.qperm2.ensure(q->count * 4);
+        launch_query_keys(ctx, v, lo, hi, ctx.qkeys.as<uint32_t>(), ctx.qperm.as<uint32_t>());
+        sort_pairs_u32(ctx, ctx.qkeys.as<uint32_t>(), ctx.qkeys2.as<uint32_t>(), ctx.qperm.as<uint32_t>(),
+                       ctx.qperm2.as<uint32_t>(), q->count, 30);
+        v.perm = ctx.qperm2.as<uint32_t>();
+    }
+    return v;
+}
+
+void scene_bounds(const Bre* bre, const Beams* bm, float* lo, float* hi) {
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = INFINITY;
+        hi[a] = -INFINITY;
+    }
+    if (bre && bre->n > 0)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], bre->bvh.lo[a]);
+            hi[a] = std::max(hi[a], bre->bvh.hi[a]);
+        }
+    if (bm && bm->pieces > 0)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], bm->bvh.lo[a]);
+            hi[a] = std::max(hi[a], bm->bvh.hi[a]);
+        }
+    for (int a = 0; a < 3; ++a)
+        if (!(lo[a] <= hi[a])) {
+            lo[a] = 0.f;
+            hi[a] = 1.f;
+        }
+}
+
+void check_stats_errors(Ctx& ctx, const uint64_t* st) {
+    const uint64_t e = st[kStatError];
+    UPBP_REQUIRE(!(e & kErrBadOffsets), UPBP_EINVAL, "segment_offsets not monotone or outside [0, offsets[count]]");
+    UPBP_REQUIRE(!(e & kErrBadMedium), UPBP_EINVAL, "segment medium_id outside the media table");
+    UPBP_REQUIRE(!(e & kErrNonFinite), UPBP_ENONFINITE, "non-finite query origin, direction or segment value");
+    UPBP_REQUIRE(!(e & kErrBadSegment), UPBP_EINVAL, "segment with dist_min > dist_max");
+    UPBP_REQUIRE(st[kStatOverflow] == 0, UPBP_EDEVICE, "LBVH traversal stack overflow (tree deeper than 128)");
+    (void)ctx;
+}
+
+void fill_stats(upbp_eval_stats* out, const uint64_t* st, float ms_ph, float ms_bm, float ms_other, uint32_t launches) {
+    if (!out) return;
+    out->photon_hits = st[kStatPhotonHits];
+    out->beam_hits = st[kStatBeamHits];
+    out->photon_candidates = st[kStatPhotonCands];
+    out->beam_candidates = st[kStatBeamCands];
+    out->nodes_visited = st[kStatNodes];
+    out->kernel_ms_photons = ms_ph;
+    out->kernel_ms_beams = ms_bm;
+    out->kernel_ms_other = ms_other;
+    out->launches = launches;
+}
+
+PhotonArgs photon_args(const Bre* b) {
+    PhotonArgs a;
+    a.nodes = b->bvh.nodes.as<NodeDev>();
+    a.pos_r2 = b->pos_r2.as<float4>();
+    a.power = b->power.as<float4>();
+    a.inc = b->inc.as<float4>();
+    a.orig = b->orig.as<uint32_t>();
+    a.n = b->n;
+    return a;
+}
+
+BeamArgs beam_args(const Beams* b) {
+    BeamArgs a;
+    a.nodes = b->bvh.nodes.as<NodeDev>();
+    a.pieces = b->piece.as<PieceDev>();
+    a.piece_beam = b->piece_beam.as<uint32_t>();
+    a.pieces_n = b->pieces;
+    return a;
+}
+
+// Shared driver of eval / signatures / adjoint.
+struct RunOut {
+    float* out_ph = nullptr;
+    float* out_bm = nullptr;
+    uint32_t* sig_ph = nullptr;
+    uint32_t* sig_bm = nullptr;
+};
+
+void run_eval(Ctx& ctx, const Bre* bre, const Beams* bm, const upbp_query_batch* q, Mode mode, RunOut o,
+              upbp_memory out_mem, upbp_eval_stats* stats, const AdjointOut* adj_ph, const AdjointOut* adj_bm,
+              bool force_sync) {
+    check_batch(ctx, q);
+    if (bre) UPBP_REQUIRE(bre->ctx == &ctx, UPBP_EINVAL, "photon structure belongs to another context");
+    if (bm) UPBP_REQUIRE(bm->ctx == &ctx, UPBP_EINVAL, "beam structure belongs to another context");
+    const int64_t Q = q->count;
+    const bool want_ph = bre && (mode == Mode::Adjoint ? adj_ph != nullptr : (o.out_ph || o.sig_ph));
+    const bool want_bm = bm && (mode == Mode::Adjoint ? adj_bm != nullptr : (o.out_bm || o.sig_bm));
+    const bool timing = stats != nullptr;
+    if (timing) UPBP_HIP(hipEventRecord(ctx.ev[0], ctx.stream));
+    float lo[3], hi[3];
+    scene_bounds(bre, bm, lo, hi);
+    QueryView v = make_view(ctx, q, lo, hi);
+    launch_clear_u64(ctx, ctx.stats.as<uint64_t>(), kStatSlots);
+    // device outputs (temporary for host outputs)
+    float* dph = o.out_ph;
+    float* dbm = o.out_bm;
+    uint32_t* sph = o.sig_ph;
+    uint32_t* sbm = o.sig_bm;
+    if (out_mem == UPBP_MEM_HOST && Q > 0) {
+        if (dph) { ctx.staging[4].ensure(Q * 12); dph = ctx.staging[4].as<float>(); }
+        if (dbm) { ctx.staging[5].ensure(Q * 12); dbm = ctx.staging[5].as<float>(); }
+        if (sph) { ctx.staging[6].ensure(Q * 8); sph = ctx.staging[6].as<uint32_t>(); }
+        if (sbm) { ctx.staging[7].ensure(Q * 8); sbm = ctx.staging[7].as<uint32_t>(); }
+    }
+    if (timing) UPBP_HIP(hipEventRecord(ctx.ev[1], ctx.stream));
+    uint32_t launches = 0;
+    if (want_ph) {
+        if (bre->n > 0) {
+            launch_photons(ctx, mode, photon_args(bre), v, dph, sph, adj_ph);
+            launches++;
+        } else if (mode != Mode::Adjoint) {
+            if (dph) launch_clear_f32(ctx, dph, 3 * Q);
+            if (sph) launch_clear_f32(ctx, (float*)sph, 2 * Q);
+        }
+    }
+    if (timing) UPBP_HIP(hipEventRecord(ctx.ev[2], ctx.stream));
+    if (want_bm) {
+        if (bm->pieces > 0) {
+            launch_beams(ctx, mode, beam_args(bm), v, dbm, sbm, adj_bm);
+            launches++;
+        } else if (mode != Mode::Adjoint) {
+            if (dbm) launch_clear_f32(ctx, dbm, 3 * Q);
+            if (sbm) launch_clear_f32(ctx, (float*)sbm, 2 * Q);
+        }
+    }
+    if (timing) UPBP_HIP(hipEventRecord(ctx.ev[3], ctx.stream));
+    if (out_mem == UPBP_MEM_HOST && Q > 0) {
+        if (o.out_ph && want_ph) UPBP_HIP(hipMemcpyAsync(o.out_ph, dph, Q * 12, hipMemcpyDeviceToHost, ctx.stream));
+        if (o.out_bm && want_bm) UPBP_HIP(hipMemcpyAsync(o.out_bm, dbm, Q * 12, hipMemcpyDeviceToHost, ctx.stream));
+        if (o.sig_ph && want_ph) UPBP_HIP(hipMemcpyAsync(o.sig_ph, sph, Q * 8, hipMemcpyDeviceToHost, ctx.stream));
+        if (o.sig_bm && want_bm) UPBP_HIP(hipMemcpyAsync(o.sig_bm, sbm, Q * 8, hipMemcpyDeviceToHost, ctx.stream));
+    }
+    const bool do_sync = ctx.sync || timing || force_sync || out_mem == UPBP_MEM_HOST || q->memory == UPBP_MEM_HOST;
+    if (!do_sync) return;
+    uint64_t st[kStatSlots];
+    UPBP_HIP(hipMemcpyAsync(st, ctx.stats.p, sizeof(st), hipMemcpyDeviceToHost, ctx.stream));
+    UPBP_HIP(hipStreamSynchronize(ctx.stream));
+    float ms_other = 0.f, ms_ph = 0.f, ms_bm = 0.f;
+    if (timing) {
+        UPBP_HIP(hipEventElapsedTime(&ms_other, ctx.ev[0], ctx.ev[1]));
+        UPBP_HIP(hipEventElapsedTime(&ms_ph, ctx.ev[1], ctx.ev[2]));
+        UPBP_HIP(hipEventElapsedTime(&ms_bm, ctx.ev[2], ctx.ev[3]));
+    }
+    fill_stats(stats, st, ms_ph, ms_bm, ms_other, launches);
+    check_stats_errors(ctx, st);
+}
+
+}  // namespace
+
+extern "C" {
+
+upbp_status upbp_ctx_create(int device, upbp_ctx** out_ctx) {
+    if (!out_ctx) return fail(nullptr, UPBP_EINVAL, "out_ctx is NULL");
+    *out_ctx = nullptr;
+    upbp_ctx* ctx = nullptr;
+    try {
+        int n = 0;
+        hipError_t e = hipGetDeviceCount(&n);
+        if (e != hipSuccess || n == 0) return fail(nullptr, UPBP_EDEVICE, "no HIP device available");
+        if (device < 0 || device >= n) return fail(nullptr, UPBP_EINVAL, "device index out of range");
+        UPBP_HIP(hipSetDevice(device));
+        hipDeviceProp_t prop;
+        UPBP_HIP(hipGetDeviceProperties(&prop, device));
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+            return fail(nullptr, UPBP_EDEVICE, std::string("this library is built for gfx950 only, device is ") +
+                                                   prop.gcnArchName);
+        ctx = new upbp_ctx();
+        ctx->device = device;
+        ctx->num_cus = prop.multiProcessorCount;
+        UPBP_HIP(hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking));
+        ctx->stream = ctx->own_stream;
+        for (auto& ev : ctx->ev) UPBP_HIP(hipEventCreate(&ev));
+        ctx->stats.ensure(kStatSlots * 8);
+        *out_ctx = ctx;
+        return UPBP_OK;
+    } catch (const Error& e) {
+        delete ctx;
+        return fail(nullptr, e.status, e.message);
+    } catch (...) {
+        delete ctx;
+        return fail(nullptr, UPBP_EDEVICE, "context creation failed");
+    }
+}
+
+upbp_status upbp_ctx_destroy(upbp_ctx* ctx) {
+    if (!ctx) return UPBP_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto& ev : ctx->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+    return UPBP_OK;
+}
+
+const char* upbp_ctx_last_error(const upbp_ctx* ctx) {
+    return ctx ? ctx->last_error.c_str() : g_no_ctx_error.c_str();
+}
+
+upbp_status upbp_ctx_set_stream(upbp_ctx* ctx, void* hip_stream) {
+    if (!ctx) return fail(nullptr, UPBP_EINVAL, "ctx is NULL");
+    ctx->stream = hip_stream ? (hipStream_t)hip_stream : ctx->own_stream;
+    return UPBP_OK;
+}
+
+void* upbp_ctx_get_stream(const upbp_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+upbp_status upbp_ctx_set_sync(upbp_ctx* ctx, int sync) {
+    if (!ctx) return fail(nullptr, UPBP_EINVAL, "ctx is NULL");
+    ctx->sync = sync != 0;
+    return UPBP_OK;
+}
+
+upbp_status upbp_set_media(upbp_ctx* ctx, const upbp_medium* media, int32_t count) {
+    if (!ctx) return fail(nullptr, UPBP_EINVAL, "ctx is NULL");
+    return guarded(ctx, [&] {
+        UPBP_REQUIRE(count > 0 && count <= (1 << 24) && media, UPBP_EINVAL, "media must be a non-empty array");
+        std::vector<MediumDev> m(count);
+        for (int i = 0; i < count; ++i) {
+            for (int c = 0; c < 3; ++c)
+                UPBP_REQUIRE(std::isfinite(media[i].absorption[c]) && std::isfinite(media[i].scattering[c]),
+                             UPBP_ENONFINITE, "non-finite medium coefficient");
+            UPBP_REQUIRE(media[i].mean_cosine >= -1.f && media[i].mean_cosine <= 1.f, UPBP_EINVAL,
+                         "mean_cosine outside [-1, 1]");
+            m[i] = derive_medium(media[i]);
+        }
+        ctx->media.ensure(count * sizeof(MediumDev));
+        UPBP_HIP(hipMemcpyAsync(ctx->media.p, m.data(), count * sizeof(MediumDev), hipMemcpyHostToDevice, ctx->stream));
+        UPBP_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->media_host = m;
+        ctx->n_media = count;
+    });
+}
+
+upbp_status upbp_bre_build(upbp_ctx* ctx, const upbp_photon_input* photons, const upbp_build_params* params,
+                           upbp_bre** out_bre, int64_t* out_num_photons) {
+    if (!ctx) return fail(nullptr, UPBP_EINVAL, "ctx is NULL");
+    if (!photons || !params || !out_bre) return fail(ctx, UPBP_EINVAL, "photons, params and out_bre are required");
+    *out_bre = nullptr;
+    upbp_bre* b = new (std::nothrow) upbp_bre();
+    if (!b) return fail(ctx, UPBP_ENOMEM, "host allocation failed");
+    upbp_status st = guarded(ctx, [&] { bre_build(*ctx, *photons, *params, *b); });
+    if (st != UPBP_OK) {
+        delete b;
+        return st;
+    }
+    *out_bre = b;
+    if (out_num_photons) *out_num_photons = b->n;
+    return UPBP_OK;
+}
+
+upbp_status upbp_bre_destroy(upbp_bre* bre) {
+    if (!bre) return UPBP_OK;
+    if (bre->ctx) (void)hipSetDevice(bre->ctx->device);
+    delete bre;
+    return UPBP_OK;
+}
+
+upbp_status upbp_bre_set_power(upbp_bre* bre, const float* power, upbp_memory memory) {
+    if (!bre) return fail(nullptr, UPBP_EINVAL, "bre is NULL");
+    Ctx* ctx = bre->ctx;
+    return guarded(ctx, [&] {
+        if (bre->n == 0) return;
+        UPBP_REQUIRE(power, UPBP_EINVAL, "power is NULL");
+        const float* p = stage_q(*ctx, 0, power, 3 * bre->n_in, memory);
+        bre_set_power(*bre, p);
+        UPBP_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int64_t upbp_bre_count(const upbp_bre* bre) { return bre ? bre->n : 0; }
+
+upbp_status upbp_beams_build(upbp_ctx* ctx, const upbp_beam_input* beams, const upbp_build_params* params,
+                             upbp_beams** out_beams) {
+    if (!ctx) return fail(nullptr, UPBP_EINVAL, "ctx is NULL");
+    if (!beams || !params || !out_beams) return fail(ctx, UPBP_EINVAL, "beams, params and out_beams are required");
+    *out_beams = nullptr;
+    upbp_beams* b = new (std::nothrow) upbp_beams();
+    if (!b) return fail(ctx, UPBP_ENOMEM, "host allocation failed");
+    upbp_status st = guarded(ctx, [&] { beams_build(*ctx, *beams, *params, *b); });
+    if (st != UPBP_OK) {
+        delete b;
+        return st;
+    }
+    *out_beams = b;
+    return UPBP_OK;
+}
+
+upbp_status upbp_beams_destroy(upbp_beams* beams) {
+    if (!beams) return UPBP_OK;
+    if (beams->ctx) (void)hipSetDevice(beams->ctx->device);
+    delete beams;
+    return UPBP_OK;
+}
+
+int64_t upbp_beams_count(const upbp_beams* beams) { return beams ? beams->n : 0; }
+int64_t upbp_beams_piece_count(const upbp_beams* beams) { return beams ? beams->pieces : 0; }
+
+float upbp_beams_selection_pdf(const upbp_beams* beams, const float position[3]) {
+    // No per-cell beam reduction is performed (the reference default
+    // sMaxBeamsInCell = 0, PhBeams.cxx:51), so every beam is tested: pdf 1.
+    (void)beams;
+    (void)position;
+    return 1.0f;
+}
+
+upbp_status upbp_eval(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams* beams, const upbp_query_batch* queries,
+                      float* out_photons, float* out_beams, upbp_memory out_memory, upbp_eval_stats* stats) {
+    if (!ctx) return fail(nullptr, UPBP_EINVAL, "ctx is NULL");
+    return guarded(ctx, [&] {
+        UPBP_REQUIRE(!bre || out_photons, UPBP_EINVAL, "out_photons is NULL but a photon structure was given");
+        UPBP_REQUIRE(!beams || out_beams, UPBP_EINVAL, "out_beams is NULL but a beam structure was given");
+        RunOut o;
+        o.out_ph = bre ? out_photons : nullptr;
+        o.out_bm = beams ? out_beams : nullptr;
+        run_eval(*ctx, bre, beams, queries, Mode::Forward, o, out_memory, stats, nullptr, nullptr, false);
+    });
+}
+
+upbp_status upbp_hit_signatures(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams* beams,
+                                const upbp_query_batch* queries, uint32_t* photon_sig, uint32_t* beam_sig,
+                                upbp_memory out_memory) {
+    if (!ctx) return fail(nullptr, UPBP_EINVAL, "ctx is NULL");
+    return guarded(ctx, [&] {
+        RunOut o;
+        o.sig_ph = bre ? photon_sig : nullptr;
+        o.sig_bm = beams ? beam_sig : nullptr;
+        run_eval(*ctx, photon_sig ? bre : nullptr, beam_sig ? beams : nullptr, queries, Mode::Signature, o,
+                 out_memory, nullptr, nullptr, nullptr, true);
+    });
+}
+
+upbp_status upbp_adjoint(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams* beams,
+                         const upbp_query_batch* queries, const float* grad_out_photons, const float* grad_out_beams,
+                         float* grad_scattering, float* grad_absorption, float* grad_photon_power,
+                         float* grad_beam_power, upbp_memory memory, upbp_eval_stats* stats) {
+    if (!ctx) return fail(nullptr, UPBP_EINVAL, "ctx is NULL");
+    return guarded(ctx, [&] {
+        check_batch(*ctx, queries);
+        UPBP_REQUIRE(queries->beam_type == UPBP_LONG_BEAM, UPBP_EUNSUPPORTED,
+                     "the adjoint supports LONG query beams only");
+        const int64_t Q = queries->count;
+        const int M = ctx->n_media;
+        const bool do_ph = bre && grad_out_photons && bre->n > 0;
+        const bool do_bm = beams && grad_out_beams && beams->pieces > 0;
+        // upstream gradients on the device
+        const float* gph = do_ph ? stage_q(*ctx, 4, grad_out_photons, 3 * Q, memory) : nullptr;
+        const float* gbm = do_bm ? stage_q(*ctx, 5, grad_out_beams, 3 * Q, memory) : nullptr;
+        DevBuf gmedia, items_ph, items_bm, gs_d, ga_d;
+        gmedia.ensure((size_t)M * 6 * 4);
+        launch_clear_f32(*ctx, gmedia.as<float>(), (int64_t)M * 6);
+        float* dpp = grad_photon_power;
+        float* dbp = grad_beam_power;
+        if (memory == UPBP_MEM_HOST) {
+            if (do_ph && grad_photon_power) {
+                items_ph.ensure(bre->n_in * 12);
+                launch_clear_f32(*ctx, items_ph.as<float>(), 3 * bre->n_in);
+                dpp = items_ph.as<float>();
+            }
+            if (do_bm && grad_beam_power) {
+                items_bm.ensure(beams->n * 12);
+                launch_clear_f32(*ctx, items_bm.as<float>(), 3 * beams->n);
+                dbp = items_bm.as<float>();
+            }
+        }
+        // item gradients must have somewhere to go
+        DevBuf dummy_ph, dummy_bm;
+        if (do_ph && !dpp) {
+            dummy_ph.ensure(bre->n_in * 12);
+            dpp = dummy_ph.as<float>();
+        }
+        if (do_bm && !dbp) {
+            dummy_bm.ensure(beams->n * 12);
+            dbp = dummy_bm.as<float>();
+        }
+        AdjointOut aph{gph, gmedia.as<float>(), dpp};
+        AdjointOut abm{gbm, gmedia.as<float>(), dbp};
+        RunOut o;
+        run_eval(*ctx, do_ph ? bre : nullptr, do_bm ? beams : nullptr, queries, Mode::Adjoint, o, UPBP_MEM_DEVICE,
+                 stats, do_ph ? &aph : nullptr, do_bm ? &abm : nullptr, true);
+        // combine d/d sigma_t into d/d sigma_s and d/d sigma_a, accumulate into the outputs
+        if (memory == UPBP_MEM_DEVICE) {
+            launch_combine_media_grads(*ctx, gmedia.as<float>(), M, grad_scattering, grad_absorption);
+            UPBP_HIP(hipStreamSynchronize(ctx->stream));
+        } else {
+            std::vector<float> gm((size_t)M * 6);
+            UPBP_HIP(hipMemcpy(gm.data(), gmedia.p, gm.size() * 4, hipMemcpyDeviceToHost));
+            for (int m = 0; m < M; ++m)
+                for (int c = 0; c < 3; ++c) {
+                    if (grad_scattering) grad_scattering[3 * m + c] += gm[6 * m + c] + gm[6 * m + 3 + c];
+                    if (grad_absorption) grad_absorption[3 * m + c] += gm[6 * m + 3 + c];
+                }
+            if (do_ph && grad_photon_power) {
+                std::vector<float> h((size_t)bre->n_in * 3);
+                UPBP_HIP(hipMemcpy(h.data(), dpp, h.size() * 4, hipMemcpyDeviceToHost));
+                for (size_t i = 0; i < h.size(); ++i) grad_photon_power[i] += h[i];
+            }
+            if (do_bm && grad_beam_power) {
+                std::vector<float> h((size_t)beams->n * 3);
+                UPBP_HIP(hipMemcpy(h.data(), dbp, h.size() * 4, hipMemcpyDeviceToHost));
+                for (size_t i = 0; i < h.size(); ++i) grad_beam_power[i] += h[i];
+            }
+        }
+    });
+}
+
+// ---------------------------------------------------------------------------
+upbp_status upbp_batch_plan_create(upbp_ctx* ctx, const upbp_instance* instances, int32_t count,
+                                   upbp_batch_plan** out_plan) {
+    if (!ctx) return fail(nullptr, UPBP_EINVAL, "ctx is NULL");
+    if (!out_plan) return fail(ctx, UPBP_EINVAL, "out_plan is NULL");
+    *out_plan = nullptr;
+    upbp_batch_plan* plan = new (std::nothrow) upbp_batch_plan();
+    if (!plan) return fail(ctx, UPBP_ENOMEM, "host allocation failed");
+    upbp_status st = guarded(ctx, [&] {
+        UPBP_REQUIRE(count > 0 && instances, UPBP_EINVAL, "instances must be a non-empty array");
+        plan->ctx = ctx;
+        plan->n = count;
+        plan->host.resize(count);
+        int64_t tiles = 0;
+        for (int i = 0; i < count; ++i) {
+            const upbp_instance& in = instances[i];
+            check_batch(*ctx, &in.queries);
+            UPBP_REQUIRE(in.queries.memory == UPBP_MEM_DEVICE, UPBP_EINVAL, "batched queries must be device memory");
+            UPBP_REQUIRE(!in.bre || in.out_photons, UPBP_EINVAL, "instance out_photons is NULL");
+            UPBP_REQUIRE(!in.beams || in.out_beams, UPBP_EINVAL, "instance out_beams is NULL");
+            UPBP_REQUIRE(in.media_base >= 0 && in.media_base < ctx->n_media, UPBP_EINVAL, "media_base out of range");
+            const Bre* b = in.bre;
+            const Beams* m = in.beams;
+            InstanceDev& d = plan->host[i];
+            std::memset(&d, 0, sizeof(d));
+            d.has_photons = b && b->n > 0;
+            d.has_beams = m && m->pieces > 0;
+            if (d.has_photons) {
+                d.ph_nodes = b->bvh.nodes.as<NodeDev>();
+                d.ph_pos_r2 = b->pos_r2.as<float4>();
+                d.ph_power = b->power.as<float4>();
+                d.ph_inc = b->inc.as<float4>();
+            }
+            if (d.has_beams) {
+                d.bm_nodes = m->bvh.nodes.as<NodeDev>();
+                d.bm_pieces = m->piece.as<PieceDev>();
+            }
+            d.origin = in.queries.origin;
+            d.direction = in.queries.direction;
+            d.offsets = in.queries.segment_offsets;
+            d.segments = in.queries.segments;
+            d.count = in.queries.count;
+            d.beam_type = in.queries.beam_type;
+            d.lite = in.queries.lite_segments;
+            d.media_base = in.media_base;
+            d.out_photons = in.out_photons;
+            d.out_beams = in.out_beams;
+            d.tile_begin = tiles;
+            if (in.queries.order == UPBP_ORDER_SORTED && in.queries.count > 64) {
+                float lo[3], hi[3];
+                scene_bounds(b, m, lo, hi);
+                QueryView v;
+                v.count = in.queries.count;
+                v.origin = d.origin;
+                v.direction = d.direction;
+                v.offsets = d.offsets;
+                v.segments = d.segments;
+                DevBuf* perm = new DevBuf();
+                plan->perms.push_back(perm);
+                perm->ensure(v.count * 4);
+                ctx->qkeys.ensure(v.count * 4);
+                ctx->qkeys2.ensure(v.count * 4);
+                ctx->qperm.ensure(v.count * 4);
+                launch_query_keys(*ctx, v, lo, hi, ctx->qkeys.as<uint32_t>(), ctx->qperm.as<uint32_t>());
+                sort_pairs_u32(*ctx, ctx->qkeys.as<uint32_t>(), ctx->qkeys2.as<uint32_t>(), ctx->qperm.as<uint32_t>(),
+                               perm->as<uint32_t>(), v.count, 30);
+                d.perm = perm->as<uint32_t>();
+            }
+            tiles += (in.queries.count + 63) / 64;
+        }
+        plan->total_tiles = tiles;
+        UPBP_REQUIRE(tiles < 0xFFFFFFF0ll, UPBP_EINVAL, "too many query tiles");
+        plan->table.ensure(count * sizeof(InstanceDev));
+        UPBP_HIP(hipMemcpyAsync(plan->table.p, plan->host.data(), count * sizeof(InstanceDev), hipMemcpyHostToDevice,
+                                ctx->stream));
+        plan->counters.ensure(16);
+        UPBP_HIP(hipStreamSynchronize(ctx->stream));
+    });
+    if (st != UPBP_OK) {
+        delete plan;
+        return st;
+    }
+    *out_plan = plan;
+    return UPBP_OK;
+}
+
+upbp_status upbp_batch_plan_destroy(upbp_batch_plan* plan) {
+    if (!plan) return UPBP_OK;
+    if (plan->ctx) (void)hipSetDevice(plan->ctx->device);
+    delete plan;
+    return UPBP_OK;
+}
+
+static void enqueue_batch(Ctx& ctx, upbp_batch_plan* plan, int blocks) {
+    launch_clear_u64(ctx, ctx.stats.as<uint64_t>(), kStatSlots);
+    launch_fill_u32(ctx, plan->counters.as<uint32_t>(), 0u, 2);
+    launch_batched(ctx, plan->table.as<InstanceDev>(), plan->n, plan->total_tiles, plan->counters.as<uint32_t>(), 0,
+                   blocks, ctx.tuning.waves_per_block);
+    launch_batched(ctx, plan->table.as<InstanceDev>(), plan->n, plan->total_tiles, plan->counters.as<uint32_t>(), 1,
+                   blocks, ctx.tuning.waves_per_block);
+}
+
+upbp_status upbp_batch_run(upbp_ctx* ctx, upbp_batch_plan* plan, int use_graph, upbp_eval_stats* stats) {
+    if (!ctx) return fail(nullptr, UPBP_EINVAL, "ctx is NULL");
+    if (!plan || plan->ctx != ctx) return fail(ctx, UPBP_EINVAL, "plan is NULL or belongs to another context");
+    return guarded(ctx, [&] {
+        const int wpb = std::min(std::max(ctx->tuning.waves_per_block, 1), 4);
+        int per_cu = ctx->tuning.blocks_per_cu;
+        if (per_cu <= 0) per_cu = std::max(1, 16 / wpb);  // about 16 resident waves per CU
+        const int blocks = ctx->num_cus * per_cu;
+        if (stats) UPBP_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+        if (use_graph) {
+            UPBP_REQUIRE(ctx->stream != nullptr, UPBP_EINVAL, "graph capture needs a non-default stream");
+            if (!plan->exec || plan->graph_blocks != blocks) {
+                if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
+                if (plan->graph) (void)hipGraphDestroy(plan->graph);
+                plan->exec = nullptr;
+                plan->graph = nullptr;
+                UPBP_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+                try {
+                    enqueue_batch(*ctx, plan, blocks);
+                } catch (...) {
+                    hipGraph_t g = nullptr;
+                    (void)hipStreamEndCapture(ctx->stream, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    throw;
+                }
+                UPBP_HIP(hipStreamEndCapture(ctx->stream, &plan->graph));
+                UPBP_HIP(hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0));
+                plan->graph_blocks = blocks;
+            }
+            UPBP_HIP(hipGraphLaunch(plan->exec, ctx->stream));
+        } else {
+            enqueue_batch(*ctx, plan, blocks);
+        }
+        if (stats) UPBP_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+        if (!ctx->sync && !stats) return;
+        uint64_t st[kStatSlots];
+        UPBP_HIP(hipMemcpyAsync(st, ctx->stats.p, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+        UPBP_HIP(hipStreamSynchronize(ctx->stream));
+        float ms = 0.f;
+        if (stats) UPBP_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+        fill_stats(stats, st, ms, 0.f, 0.f, 2);
+        check_stats_errors(*ctx, st);
+    });
+}
+
+upbp_status upbp_set_tuning(upbp_ctx* ctx, const upbp_tuning* tuning) {
+    if (!ctx || !tuning) return fail(ctx, UPBP_EINVAL, "ctx and tuning are required");
+    if (tuning->waves_per_block < 1 || tuning->waves_per_block > 4 || tuning->blocks_per_cu < 0 ||
+        tuning->blocks_per_cu > 64)
+        return fail(ctx, UPBP_EINVAL, "waves_per_block must be 1..4 and blocks_per_cu 0..64");
+    ctx->tuning.waves_per_block = tuning->waves_per_block;
+    ctx->tuning.blocks_per_cu = tuning->blocks_per_cu;
+    return UPBP_OK;
+}
+
+upbp_status upbp_get_tuning(const upbp_ctx* ctx, upbp_tuning* tuning) {
+    if (!ctx || !tuning) return UPBP_EINVAL;
+    tuning->waves_per_block = ctx->tuning.waves_per_block;
+    tuning->blocks_per_cu = ctx->tuning.blocks_per_cu;
+    return UPBP_OK;
+}
+
+upbp_status upbp_autotune(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams* beams, const upbp_query_batch* queries,
+                          upbp_tuning* best, float* best_ms) {
+    if (!ctx) return fail(nullptr, UPBP_EINVAL, "ctx is NULL");
+    return guarded(ctx, [&] {
+        check_batch(*ctx, queries);
+        UPBP_REQUIRE(queries->memory == UPBP_MEM_DEVICE, UPBP_EINVAL, "autotune needs device-resident queries");
+        const int64_t Q = queries->count;
+        DevBuf o1, o2;
+        o1.ensure(std::max<int64_t>(Q, 1) * 12);
+        o2.ensure(std::max<int64_t>(Q, 1) * 12);
+        Tuning saved = ctx->tuning;
+        float best_t = INFINITY;
+        Tuning best_cfg = saved;
+        const int cand[3] = {1, 2, 4};
+        for (int wpb : cand) {
+            ctx->tuning.waves_per_block = wpb;
+            RunOut o;
+            o.out_ph = bre ? o1.as<float>() : nullptr;
+            o.out_bm = beams ? o2.as<float>() : nullptr;
+            upbp_eval_stats s;
+            run_eval(*ctx, bre, beams, queries, Mode::Forward, o, UPBP_MEM_DEVICE, &s, nullptr, nullptr, true);  // warm
+            float tsum = 0.f;
+            for (int rep = 0; rep < 3; ++rep) {
+                run_eval(*ctx, bre, beams, queries, Mode::Forward, o, UPBP_MEM_DEVICE, &s, nullptr, nullptr, true);
+                tsum += s.kernel_ms_photons + s.kernel_ms_beams;
+            }
+            if (tsum < best_t) {
+                best_t = tsum;
+                best_cfg = ctx->tuning;
+            }
+        }
+        ctx->tuning = best_cfg;
+        if (best) {
+            best->waves_per_block = best_cfg.waves_per_block;
+            best->blocks_per_cu = best_cfg.blocks_per_cu;
+        }
+        if (best_ms) *best_ms = best_t / 3.f;
+    });
+}
+
+}  // extern "C"

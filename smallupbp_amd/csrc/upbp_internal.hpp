@@ -1,0 +1,213 @@
+// upbp_internal.hpp -- host-side objects behind the opaque handles of include/upbp.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/upbp.h"
+#include "upbp_device.hpp"
+
+namespace upbp {
+
+// Error carried from a failing HIP call up to the C ABI.
+struct Error {
+    upbp_status status;
+    std::string message;
+};
+
+#define UPBP_HIP(call)                                                                       \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess) {                                                              \
+            throw ::upbp::Error{e_ == hipErrorOutOfMemory ? UPBP_ENOMEM : UPBP_EDEVICE,      \
+                                std::string(#call) + ": " + hipGetErrorString(e_)};         \
+        }                                                                                    \
+    } while (0)
+
+#define UPBP_REQUIRE(cond, status, msg)                  \
+    do {                                                 \
+        if (!(cond)) throw ::upbp::Error{(status), (msg)}; \
+    } while (0)
+
+// Owning device buffer (hipMalloc / hipFree), grows on demand.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void ensure(size_t n) {
+        if (n <= bytes) return;
+        release();
+        size_t want = n < 256 ? 256 : n;
+        UPBP_HIP(hipMalloc(&p, want));
+        bytes = want;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+// Counters written by the traversal kernels (device, uint64 each).
+enum StatSlot : int {
+    kStatPhotonHits = 0,
+    kStatBeamHits = 1,
+    kStatPhotonCands = 2,
+    kStatBeamCands = 3,
+    kStatNodes = 4,
+    kStatError = 5,     // bit mask of input errors found by kernels
+    kStatOverflow = 6,  // traversal stack overflow count
+    kStatSlots = 8
+};
+constexpr uint32_t kErrBadOffsets = 1u, kErrBadMedium = 2u, kErrBadSegment = 4u, kErrNonFinite = 8u;
+
+struct Tuning {
+    int waves_per_block = 4;
+    int blocks_per_cu = 0;
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    bool sync = true;
+    std::string last_error;
+    int n_media = 0;
+    std::vector<MediumDev> media_host;
+    DevBuf media;
+    DevBuf stats;       // kStatSlots x uint64
+    DevBuf sort_tmp;    // hipcub temporary storage
+    DevBuf qkeys, qkeys2, qperm, qperm2;
+    DevBuf staging[8];  // host-input staging copies
+    hipEvent_t ev[6] = {};
+    int num_cus = 256;
+    Tuning tuning;
+};
+
+// A built LBVH: node array plus the item order it implies.
+struct Lbvh {
+    DevBuf nodes;
+    int64_t n_nodes = 0;
+    int64_t n_items = 0;
+    float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};  // bounds of all item boxes
+};
+
+struct Bre {
+    Ctx* ctx = nullptr;
+    int64_t n_in = 0;  // photons given to build
+    int64_t n = 0;     // kept photons
+    Lbvh bvh;
+    DevBuf pos_r2;  // float4 [n]
+    DevBuf power;   // float4 [n]
+    DevBuf inc;     // float4 [n]
+    DevBuf orig;    // uint32 [n] original input index
+};
+
+struct Beams {
+    Ctx* ctx = nullptr;
+    int64_t n = 0;        // beams
+    int64_t pieces = 0;   // beam pieces in the LBVH
+    Lbvh bvh;
+    DevBuf piece;         // PieceDev [pieces]
+    DevBuf piece_beam;    // uint32 [pieces] original beam index
+};
+
+// Device-resident view of a query batch.
+struct QueryView {
+    int64_t count = 0;
+    const float* origin = nullptr;
+    const float* direction = nullptr;
+    const int64_t* offsets = nullptr;
+    const upbp_volume_segment* segments = nullptr;
+    uint32_t beam_type = UPBP_LONG_BEAM;
+    uint32_t lite = 0;
+    const uint32_t* perm = nullptr;  // lane slot -> query index, or null (identity)
+};
+
+// ---- lbvh.hip -------------------------------------------------------------
+// Builds an LBVH over n item boxes (lo/hi as float4 .xyz). On return
+// out_order[k] is the input index of the k-th item in leaf order.
+void build_lbvh(Ctx& ctx, int64_t n, const float4* box_lo, const float4* box_hi, int leaf_size,
+                Lbvh& out, DevBuf& out_order);
+void sort_pairs_u64(Ctx& ctx, uint64_t* keys_in, uint64_t* keys_out, uint32_t* vals_in,
+                    uint32_t* vals_out, int64_t n, int end_bit);
+void sort_pairs_u32(Ctx& ctx, uint32_t* keys_in, uint32_t* keys_out, uint32_t* vals_in,
+                    uint32_t* vals_out, int64_t n, int end_bit);
+void exclusive_scan_u32(Ctx& ctx, const uint32_t* in, uint32_t* out, int64_t n);
+
+// ---- build.hip ------------------------------------------------------------
+void bre_build(Ctx& ctx, const upbp_photon_input& in, const upbp_build_params& params, Bre& out);
+void bre_set_power(Bre& bre, const float* power_dev);
+void beams_build(Ctx& ctx, const upbp_beam_input& in, const upbp_build_params& params, Beams& out);
+
+// ---- traverse.hip ---------------------------------------------------------
+enum class Mode : int { Forward = 0, Signature = 1, Adjoint = 2 };
+
+struct PhotonArgs {
+    const NodeDev* nodes;
+    const float4* pos_r2;
+    const float4* power;
+    const float4* inc;
+    const uint32_t* orig;
+    int64_t n;
+};
+struct BeamArgs {
+    const NodeDev* nodes;
+    const PieceDev* pieces;
+    const uint32_t* piece_beam;
+    int64_t pieces_n;
+};
+struct AdjointOut {
+    const float* grad_out;  // [Q][3]
+    float* grad_media;      // [M][6]: dsigma_s direct (0..2), dsigma_t (3..5)
+    float* grad_items;      // photon [n_in][3] or beam [n][3]
+};
+
+void launch_photons(Ctx& ctx, Mode mode, const PhotonArgs& ph, const QueryView& q, float* out,
+                    uint32_t* sig, const AdjointOut* adj);
+void launch_beams(Ctx& ctx, Mode mode, const BeamArgs& bm, const QueryView& q, float* out,
+                  uint32_t* sig, const AdjointOut* adj);
+void launch_query_keys(Ctx& ctx, const QueryView& q, const float* lo, const float* hi, uint32_t* keys,
+                       uint32_t* idx);
+void launch_clear_u64(Ctx& ctx, uint64_t* p, int64_t n);
+void launch_clear_f32(Ctx& ctx, float* p, int64_t n);
+void launch_combine_media_grads(Ctx& ctx, const float* gm, int n_media, float* gs, float* ga);
+
+// ---- batched persistent kernels --------------------------------------------
+struct InstanceDev {
+    const NodeDev* ph_nodes;
+    const float4* ph_pos_r2;
+    const float4* ph_power;
+    const float4* ph_inc;
+    const NodeDev* bm_nodes;
+    const PieceDev* bm_pieces;
+    const float* origin;
+    const float* direction;
+    const int64_t* offsets;
+    const upbp_volume_segment* segments;
+    const uint32_t* perm;
+    float* out_photons;
+    float* out_beams;
+    int64_t count;
+    int64_t tile_begin;  // first global tile of this instance
+    int32_t media_base;
+    uint32_t beam_type;
+    uint32_t lite;
+    int32_t has_photons;
+    int32_t has_beams;
+    int32_t pad[3];
+};
+
+void launch_batched(Ctx& ctx, const InstanceDev* inst, int n_inst, int64_t total_tiles,
+                    uint32_t* counters, int which, int grid_blocks, int waves_per_block);
+void launch_fill_u32(Ctx& ctx, uint32_t* p, uint32_t v, int64_t n);
+
+}  // namespace upbp

@@ -1,0 +1,406 @@
+"""Host-side mirror of the reference estimator interface over the C ABI (ctypes).
+
+`EmbreeBre` and `PhotonBeamsEvaluator` keep the method shapes of the reference
+classes (SmallUPBP/src/Bre/Bre.hxx:40-184, SmallUPBP/src/Beams/PhBeams.hxx:46-160):
+build(...), evalBre / evalBeamBeamEstimate(...), destroy(). Evaluation is batched:
+one call evaluates every query ray of a `Queries` batch on the GPU.
+
+The library is libupbp_hip.so, built in-tree for gfx950. There is no CPU
+fallback: importing works anywhere, but creating a Context raises when the
+library is missing or no gfx950 device is present.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import _abi
+from .scenes import Beams, Photons, Queries
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libupbp_hip.so")
+_lib = None
+
+EXPORTED = [
+    "upbp_ctx_create", "upbp_ctx_destroy", "upbp_ctx_last_error", "upbp_ctx_set_stream",
+    "upbp_ctx_get_stream", "upbp_ctx_set_sync", "upbp_set_media", "upbp_bre_build", "upbp_bre_destroy",
+    "upbp_bre_set_power", "upbp_bre_count", "upbp_beams_build", "upbp_beams_destroy", "upbp_beams_count",
+    "upbp_beams_piece_count", "upbp_beams_selection_pdf", "upbp_eval", "upbp_hit_signatures",
+    "upbp_adjoint", "upbp_batch_plan_create", "upbp_batch_plan_destroy", "upbp_batch_run",
+    "upbp_set_tuning", "upbp_get_tuning", "upbp_autotune",
+]
+
+
+class UpbpError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__("%s: %s" % (_abi.STATUS_NAMES.get(status, status), message))
+        self.status = status
+        self.message = message
+
+
+def load():
+    """Load libupbp_hip.so (raises ImportError when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libupbp_hip.so is not built (run __graft_entry__.build() or "
+                          "python smallupbp_amd/build_ext.py)")
+    try:
+        # torch ships its own HIP runtime (SONAME libamdhip64.so.7). Loading it first makes the
+        # dynamic loader resolve this library's libamdhip64.so.7 to that same copy, so one HIP
+        # runtime serves the process; two runtimes in one process cannot share the GPU.
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    L = C.CDLL(LIB_PATH)
+    vp, i64, i32 = C.c_void_p, C.c_int64, C.c_int32
+    st = C.c_int
+    L.upbp_ctx_create.restype = st
+    L.upbp_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.upbp_ctx_destroy.restype = st
+    L.upbp_ctx_destroy.argtypes = [vp]
+    L.upbp_ctx_last_error.restype = C.c_char_p
+    L.upbp_ctx_last_error.argtypes = [vp]
+    L.upbp_ctx_set_stream.restype = st
+    L.upbp_ctx_set_stream.argtypes = [vp, vp]
+    L.upbp_ctx_get_stream.restype = vp
+    L.upbp_ctx_get_stream.argtypes = [vp]
+    L.upbp_ctx_set_sync.restype = st
+    L.upbp_ctx_set_sync.argtypes = [vp, C.c_int]
+    L.upbp_set_media.restype = st
+    L.upbp_set_media.argtypes = [vp, vp, i32]
+    L.upbp_bre_build.restype = st
+    L.upbp_bre_build.argtypes = [vp, C.POINTER(_abi.PhotonInput), C.POINTER(_abi.BuildParams), C.POINTER(vp),
+                                 C.POINTER(i64)]
+    L.upbp_bre_destroy.restype = st
+    L.upbp_bre_destroy.argtypes = [vp]
+    L.upbp_bre_set_power.restype = st
+    L.upbp_bre_set_power.argtypes = [vp, vp, C.c_int]
+    L.upbp_bre_count.restype = i64
+    L.upbp_bre_count.argtypes = [vp]
+    L.upbp_beams_build.restype = st
+    L.upbp_beams_build.argtypes = [vp, C.POINTER(_abi.BeamInput), C.POINTER(_abi.BuildParams), C.POINTER(vp)]
+    L.upbp_beams_destroy.restype = st
+    L.upbp_beams_destroy.argtypes = [vp]
+    L.upbp_beams_count.restype = i64
+    L.upbp_beams_count.argtypes = [vp]
+    L.upbp_beams_piece_count.restype = i64
+    L.upbp_beams_piece_count.argtypes = [vp]
+    L.upbp_beams_selection_pdf.restype = C.c_float
+    L.upbp_beams_selection_pdf.argtypes = [vp, vp]
+    L.upbp_eval.restype = st
+    L.upbp_eval.argtypes = [vp, vp, vp, C.POINTER(_abi.QueryBatch), vp, vp, C.c_int, C.POINTER(_abi.EvalStats)]
+    L.upbp_hit_signatures.restype = st
+    L.upbp_hit_signatures.argtypes = [vp, vp, vp, C.POINTER(_abi.QueryBatch), vp, vp, C.c_int]
+    L.upbp_adjoint.restype = st
+    L.upbp_adjoint.argtypes = [vp, vp, vp, C.POINTER(_abi.QueryBatch), vp, vp, vp, vp, vp, vp, C.c_int,
+                               C.POINTER(_abi.EvalStats)]
+    L.upbp_batch_plan_create.restype = st
+    L.upbp_batch_plan_create.argtypes = [vp, C.POINTER(_abi.Instance), i32, C.POINTER(vp)]
+    L.upbp_batch_plan_destroy.restype = st
+    L.upbp_batch_plan_destroy.argtypes = [vp]
+    L.upbp_batch_run.restype = st
+    L.upbp_batch_run.argtypes = [vp, vp, C.c_int, C.POINTER(_abi.EvalStats)]
+    L.upbp_set_tuning.restype = st
+    L.upbp_set_tuning.argtypes = [vp, C.POINTER(_abi.Tuning)]
+    L.upbp_get_tuning.restype = st
+    L.upbp_get_tuning.argtypes = [vp, C.POINTER(_abi.Tuning)]
+    L.upbp_autotune.restype = st
+    L.upbp_autotune.argtypes = [vp, vp, vp, C.POINTER(_abi.QueryBatch), C.POINTER(_abi.Tuning),
+                                C.POINTER(C.c_float)]
+    _lib = L
+    return L
+
+
+def _is_torch(x) -> bool:
+    return hasattr(x, "data_ptr") and hasattr(x, "is_cuda")
+
+
+class Context:
+    """One upbp_ctx on one GPU (the library's own stream unless set_stream is used)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = C.c_void_p()
+        r = self.lib.upbp_ctx_create(device, C.byref(h))
+        if r:
+            raise UpbpError(r, self.lib.upbp_ctx_last_error(None).decode())
+        self.h = h
+        self.device = device
+        self.n_media = 0
+
+    def check(self, r: int):
+        if r:
+            raise UpbpError(r, self.lib.upbp_ctx_last_error(self.h).decode())
+
+    def set_media(self, media: np.ndarray):
+        m = np.ascontiguousarray(media)
+        self.check(self.lib.upbp_set_media(self.h, m.ctypes.data, len(m)))
+        self.n_media = len(m)
+
+    def set_stream(self, stream_handle: Optional[int]):
+        self.check(self.lib.upbp_ctx_set_stream(self.h, stream_handle))
+
+    def stream(self) -> int:
+        return self.lib.upbp_ctx_get_stream(self.h) or 0
+
+    def set_sync(self, sync: bool):
+        self.check(self.lib.upbp_ctx_set_sync(self.h, 1 if sync else 0))
+
+    def set_tuning(self, waves_per_block: int, blocks_per_cu: int = 0):
+        t = _abi.Tuning(waves_per_block, blocks_per_cu)
+        self.check(self.lib.upbp_set_tuning(self.h, C.byref(t)))
+
+    def get_tuning(self):
+        t = _abi.Tuning()
+        self.check(self.lib.upbp_get_tuning(self.h, C.byref(t)))
+        return t.waves_per_block, t.blocks_per_cu
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.upbp_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+class QueryBatchView:
+    """A upbp_query_batch over host numpy arrays or device torch tensors."""
+
+    def __init__(self, origin, direction, seg_offsets, segments, beam_type=_abi.LONG_BEAM, lite=False,
+                 order=_abi.ORDER_SORTED):
+        self.device = _is_torch(origin) and origin.is_cuda
+        mem = _abi.MEM_DEVICE if self.device else _abi.MEM_HOST
+        self.count = int(origin.shape[0])
+        self._keep = (origin, direction, seg_offsets, segments)
+        self.qb = _abi.QueryBatch(self.count, _abi.ptr(origin), _abi.ptr(direction), _abi.ptr(seg_offsets),
+                                  _abi.ptr(segments), beam_type, 1 if lite else 0, order, mem)
+
+    @staticmethod
+    def host(q: Queries, order=_abi.ORDER_SORTED):
+        return QueryBatchView(np.ascontiguousarray(q.origin, np.float32), np.ascontiguousarray(q.direction, np.float32),
+                              np.ascontiguousarray(q.seg_offsets, np.int64), np.ascontiguousarray(q.segments),
+                              q.beam_type, q.lite, order)
+
+    @staticmethod
+    def to_device(q: Queries, device="cuda", order=_abi.ORDER_SORTED):
+        import torch
+
+        seg = np.ascontiguousarray(q.segments).view(np.float32).reshape(-1, 8)
+        return QueryBatchView(
+            torch.from_numpy(np.ascontiguousarray(q.origin, np.float32)).to(device),
+            torch.from_numpy(np.ascontiguousarray(q.direction, np.float32)).to(device),
+            torch.from_numpy(np.ascontiguousarray(q.seg_offsets, np.int64)).to(device),
+            torch.from_numpy(seg.copy()).to(device),
+            q.beam_type, q.lite, order)
+
+
+def _arr(x, dtype):
+    if x is None:
+        return None
+    if _is_torch(x):
+        return x.contiguous()
+    return np.ascontiguousarray(x, dtype=dtype)
+
+
+class EmbreeBre:
+    """PB2D photon-point estimator (reference: EmbreeBre, SmallUPBP/src/Bre/Bre.hxx:40-184)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.h = None
+        self.num_photons = 0
+
+    def build(self, photons: Photons, radius: Optional[float] = None, in_medium=None, leaf_size: int = 0,
+              radius_calculation: int = _abi.CONSTANT_RADIUS, knn: int = 0) -> int:
+        """EmbreeBre::build (Bre.cxx:526-632). Returns the number of photons kept."""
+        assert self.h is None, "destroy() before building again (Bre.cxx:529-532)"
+        arrs = [_arr(photons.position, np.float32), _arr(photons.inc_dir, np.float32),
+                _arr(photons.power, np.float32), _arr(photons.radius, np.float32), _arr(in_medium, np.uint8)]
+        dev = any(_is_torch(a) and a.is_cuda for a in arrs if a is not None)
+        pin = _abi.PhotonInput(photons.count, *[_abi.ptr(a) for a in arrs],
+                               _abi.MEM_DEVICE if dev else _abi.MEM_HOST)
+        r = photons.const_radius if radius is None else radius
+        params = _abi.BuildParams(radius_calculation, float(r or 0.0), knn, leaf_size, 0.0, 0)
+        h = C.c_void_p()
+        n = C.c_int64()
+        self.ctx.check(self.ctx.lib.upbp_bre_build(self.ctx.h, C.byref(pin), C.byref(params), C.byref(h), C.byref(n)))
+        self.h = h
+        self.num_photons = n.value
+        self.num_input = photons.count
+        return n.value
+
+    def set_power(self, power):
+        p = _arr(power, np.float32)
+        mem = _abi.MEM_DEVICE if (_is_torch(p) and p.is_cuda) else _abi.MEM_HOST
+        self.ctx.check(self.ctx.lib.upbp_bre_set_power(self.h, _abi.ptr(p), mem))
+
+    def evalBre(self, queries: QueryBatchView, out=None, stats=None):
+        """EmbreeBre::evalBre (Bre.cxx:812-964) for every query of the batch -> [Q,3]."""
+        return eval_batch(self.ctx, self, None, queries, out_photons=out, stats=stats)[0]
+
+    def destroy(self):
+        if self.h is not None:
+            self.ctx.lib.upbp_bre_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class PhotonBeamsEvaluator:
+    """BB1D photon-beam estimator (reference: PhotonBeamsEvaluator, SmallUPBP/src/Beams/PhBeams.hxx:46-160)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.h = None
+
+    def build(self, beams: Beams, radius: Optional[float] = None, start_radius=None, end_radius=None,
+              leaf_size: int = 0, piece_factor: float = 0.0, max_pieces: int = 0,
+              radius_calculation: int = _abi.CONSTANT_RADIUS, knn: int = 0):
+        """PhotonBeamsEvaluator::build (PhBeams.cxx:66-148)."""
+        assert self.h is None, "destroy() before building again"
+        arrs = [_arr(beams.origin, np.float32), _arr(beams.direction, np.float32), _arr(beams.length, np.float32),
+                _arr(beams.throughput, np.float32), _arr(beams.flags, np.uint32), _arr(beams.medium_id, np.int32),
+                _arr(start_radius, np.float32), _arr(end_radius, np.float32)]
+        dev = any(_is_torch(a) and a.is_cuda for a in arrs if a is not None)
+        bin_ = _abi.BeamInput(beams.count, *[_abi.ptr(a) for a in arrs], _abi.MEM_DEVICE if dev else _abi.MEM_HOST)
+        r = beams.radius if radius is None else radius
+        params = _abi.BuildParams(radius_calculation, float(r), knn, leaf_size, float(piece_factor), int(max_pieces))
+        h = C.c_void_p()
+        self.ctx.check(self.ctx.lib.upbp_beams_build(self.ctx.h, C.byref(bin_), C.byref(params), C.byref(h)))
+        self.h = h
+        self.num_beams = beams.count
+        return self
+
+    @property
+    def pieces(self) -> int:
+        return self.ctx.lib.upbp_beams_piece_count(self.h)
+
+    def getBeamSelectionPdf(self, pos) -> float:
+        p = np.ascontiguousarray(pos, np.float32)
+        return self.ctx.lib.upbp_beams_selection_pdf(self.h, p.ctypes.data)
+
+    def evalBeamBeamEstimate(self, queries: QueryBatchView, out=None, stats=None):
+        """PhotonBeamsEvaluator::evalBeamBeamEstimate (PhBeams.cxx:177-324) per query -> [Q,3]."""
+        return eval_batch(self.ctx, None, self, queries, out_beams=out, stats=stats)[1]
+
+    def destroy(self):
+        if self.h is not None:
+            self.ctx.lib.upbp_beams_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def _out_like(queries: QueryBatchView, given):
+    if given is not None:
+        return given
+    if queries.device:
+        import torch
+
+        return torch.empty((queries.count, 3), dtype=torch.float32, device=queries._keep[0].device)
+    return np.zeros((queries.count, 3), np.float32)
+
+
+def eval_batch(ctx: Context, bre: Optional[EmbreeBre], beams: Optional[PhotonBeamsEvaluator],
+               queries: QueryBatchView, out_photons=None, out_beams=None, stats: Optional[dict] = None):
+    """upbp_eval: returns (photon radiance [Q,3] or None, beam radiance [Q,3] or None)."""
+    op = _out_like(queries, out_photons) if bre is not None else None
+    ob = _out_like(queries, out_beams) if beams is not None else None
+    dev = (op is not None and _is_torch(op)) or (ob is not None and _is_torch(ob))
+    st = _abi.EvalStats()
+    ctx.check(ctx.lib.upbp_eval(ctx.h, bre.h if bre else None, beams.h if beams else None, C.byref(queries.qb),
+                                _abi.ptr(op), _abi.ptr(ob), _abi.MEM_DEVICE if dev else _abi.MEM_HOST,
+                                C.byref(st) if stats is not None else None))
+    if stats is not None:
+        stats.update(st.as_dict())
+    return op, ob
+
+
+def hit_signatures(ctx: Context, bre: Optional[EmbreeBre], beams: Optional[PhotonBeamsEvaluator],
+                   queries: QueryBatchView):
+    """Per-query (hit count, hash sum) for photons and beams, as host arrays."""
+    sp = np.zeros((queries.count, 2), np.uint32) if bre is not None else None
+    sb = np.zeros((queries.count, 2), np.uint32) if beams is not None else None
+    ctx.check(ctx.lib.upbp_hit_signatures(ctx.h, bre.h if bre else None, beams.h if beams else None,
+                                          C.byref(queries.qb), _abi.ptr(sp), _abi.ptr(sb), _abi.MEM_HOST))
+    return sp, sb
+
+
+def adjoint(ctx: Context, bre, beams, queries: QueryBatchView, grad_out_photons=None, grad_out_beams=None,
+            grad_scattering=None, grad_absorption=None, grad_photon_power=None, grad_beam_power=None,
+            stats: Optional[dict] = None):
+    """upbp_adjoint. Gradients accumulate into the given arrays (same memory kind as the upstream ones)."""
+    arrs = [grad_out_photons, grad_out_beams, grad_scattering, grad_absorption, grad_photon_power, grad_beam_power]
+    dev = any(a is not None and _is_torch(a) and a.is_cuda for a in arrs)
+    st = _abi.EvalStats()
+    ctx.check(ctx.lib.upbp_adjoint(ctx.h, bre.h if bre else None, beams.h if beams else None, C.byref(queries.qb),
+                                   *[_abi.ptr(a) for a in arrs], _abi.MEM_DEVICE if dev else _abi.MEM_HOST,
+                                   C.byref(st) if stats is not None else None))
+    if stats is not None:
+        stats.update(st.as_dict())
+
+
+class BatchPlan:
+    """upbp_batch_plan over several independent instances (device-resident queries)."""
+
+    def __init__(self, ctx: Context, instances):
+        """instances: list of (EmbreeBre|None, PhotonBeamsEvaluator|None, QueryBatchView, media_base,
+        out_photons tensor|None, out_beams tensor|None)."""
+        self.ctx = ctx
+        arr = (_abi.Instance * len(instances))()
+        self._keep = []
+        for i, (b, m, q, base, op, ob) in enumerate(instances):
+            arr[i].bre = b.h if b else None
+            arr[i].beams = m.h if m else None
+            arr[i].queries = q.qb
+            arr[i].media_base = base
+            arr[i].out_photons = _abi.ptr(op)
+            arr[i].out_beams = _abi.ptr(ob)
+            self._keep.append((b, m, q, op, ob))
+        h = C.c_void_p()
+        ctx.check(ctx.lib.upbp_batch_plan_create(ctx.h, arr, len(instances), C.byref(h)))
+        self.h = h
+
+    def run(self, use_graph: bool = True, stats: Optional[dict] = None):
+        st = _abi.EvalStats()
+        self.ctx.check(self.ctx.lib.upbp_batch_run(self.ctx.h, self.h, 1 if use_graph else 0,
+                                                   C.byref(st) if stats is not None else None))
+        if stats is not None:
+            stats.update(st.as_dict())
+
+    def destroy(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.upbp_batch_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def autotune(ctx: Context, bre, beams, queries: QueryBatchView):
+    t = _abi.Tuning()
+    ms = C.c_float()
+    ctx.check(ctx.lib.upbp_autotune(ctx.h, bre.h if bre else None, beams.h if beams else None,
+                                    C.byref(queries.qb), C.byref(t), C.byref(ms)))
+    return (t.waves_per_block, t.blocks_per_cu), ms.value

@@ -1,0 +1,439 @@
+"""Seeded synthetic workloads for the volumetric merging estimators.
+
+The reference produces photons, photon beams and query segments with its own
+light and camera tracers (SmallUPBP/src/Renderers/VolLightTracer.hxx:168-271,586-640,
+SmallUPBP/src/Renderers/UPBP.hxx:501-674,1783-1870, SmallUPBP/src/Scene/Scene.hxx:393-562).
+Those producers are not part of the estimator path; here the same kinds of
+inputs are generated analytically from a counter-based RNG (numpy Philox) so
+the GPU library and the CPU oracle consume byte-identical arrays.
+
+Geometry: an axis-aligned box medium, optionally with a sphere medium nested
+inside it, surrounded by a clear (vacuum) medium. Query rays are LONG camera
+beams; their volume segments carry what Scene::Intersect stores
+(Scene.hxx:537-545): dist_min/dist_max, RaySamplePdf/RevPdf with no sampling
+flags (Media.hxx:294-323) and attenuation exp(-sigma_t * length).
+
+Workloads (sizes as named):
+  plain_bre_256      unit cube, sigma_s=0.5, sigma_a=0.1, 100k photons r=0.01, 256x256 queries
+  bre_beams_1080p    box + sphere, 1M photons (lognormal radii), 200k SHORT beams r=0.003, 1920x1080
+  beams_stress       box sigma_s=1, sigma_a=0.1, 2M beams (50% SHORT / 50% LONG), 1M query beams
+  batched_64         64 unit-box scenes of lognormal size (10k-2M photons, 16k-2M queries)
+  adjoint_512        box + sphere, 500k photons + 100k beams, 512x512 queries
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import List, Optional
+
+import numpy as np
+
+SHORT_BEAM = 1
+LONG_BEAM = 2
+
+MEDIUM_DTYPE = np.dtype(
+    [
+        ("absorption", "<f4", (3,)),
+        ("scattering", "<f4", (3,)),
+        ("mean_cosine", "<f4"),
+        ("continuation_prob", "<f4"),
+    ]
+)
+assert MEDIUM_DTYPE.itemsize == 32
+
+SEGMENT_DTYPE = np.dtype(
+    [
+        ("dist_min", "<f4"),
+        ("dist_max", "<f4"),
+        ("ray_sample_pdf", "<f4"),
+        ("ray_sample_rev_pdf", "<f4"),
+        ("attenuation", "<f4", (3,)),
+        ("medium_id", "<i4"),
+    ]
+)
+assert SEGMENT_DTYPE.itemsize == 32
+
+
+@dataclasses.dataclass
+class Photons:
+    position: np.ndarray  # [N,3] f32
+    inc_dir: np.ndarray  # [N,3] f32
+    power: np.ndarray  # [N,3] f32
+    radius: Optional[np.ndarray]  # [N] f32 or None
+    const_radius: float = 0.0
+
+    @property
+    def count(self) -> int:
+        return int(self.position.shape[0])
+
+
+@dataclasses.dataclass
+class Beams:
+    origin: np.ndarray  # [B,3] f32
+    direction: np.ndarray  # [B,3] f32
+    length: np.ndarray  # [B] f32
+    throughput: np.ndarray  # [B,3] f32
+    flags: np.ndarray  # [B] u32
+    medium_id: np.ndarray  # [B] i32
+    radius: float = 0.003
+
+    @property
+    def count(self) -> int:
+        return int(self.origin.shape[0])
+
+
+@dataclasses.dataclass
+class Queries:
+    origin: np.ndarray  # [Q,3] f32
+    direction: np.ndarray  # [Q,3] f32
+    seg_offsets: np.ndarray  # [Q+1] i64
+    segments: np.ndarray  # [S] SEGMENT_DTYPE
+    beam_type: int = LONG_BEAM
+    lite: bool = False
+
+    @property
+    def count(self) -> int:
+        return int(self.origin.shape[0])
+
+    def subset(self, idx: np.ndarray) -> "Queries":
+        """Queries `idx` (sorted or not) as a new batch with its own CSR."""
+        idx = np.asarray(idx, dtype=np.int64)
+        starts = self.seg_offsets[idx]
+        counts = self.seg_offsets[idx + 1] - starts
+        off = np.zeros(len(idx) + 1, np.int64)
+        np.cumsum(counts, out=off[1:])
+        gather = np.repeat(starts - off[:-1], counts) + np.arange(off[-1], dtype=np.int64)
+        return Queries(
+            np.ascontiguousarray(self.origin[idx]),
+            np.ascontiguousarray(self.direction[idx]),
+            off,
+            np.ascontiguousarray(self.segments[gather]),
+            self.beam_type,
+            self.lite,
+        )
+
+
+@dataclasses.dataclass
+class Workload:
+    name: str
+    media: np.ndarray  # MEDIUM_DTYPE
+    photons: Optional[Photons]
+    beams: Optional[Beams]
+    queries: Queries
+    sphere: Optional[tuple] = None  # (center[3], radius, medium_id)
+    box_medium: int = 0
+    vacuum_medium: int = 1
+    seed: int = 0
+
+
+def rng_for(seed: int) -> np.random.Generator:
+    return np.random.Generator(np.random.Philox(seed))
+
+
+def make_media(rows) -> np.ndarray:
+    """rows: (sigma_a[3], sigma_s[3], g, continuation_prob)."""
+    m = np.zeros(len(rows), MEDIUM_DTYPE)
+    for i, (sa, ss, g, cp) in enumerate(rows):
+        m[i]["absorption"] = sa
+        m[i]["scattering"] = ss
+        m[i]["mean_cosine"] = g
+        m[i]["continuation_prob"] = cp
+    return m
+
+
+def _sigma_t(media: np.ndarray) -> np.ndarray:
+    return (np.abs(media["absorption"]) + np.abs(media["scattering"])).astype(np.float32)
+
+
+def _min_positive(st: np.ndarray) -> np.ndarray:
+    """Minimum positive component per medium, first index on ties (Media.hxx:142-145)."""
+    out = np.zeros(st.shape[0], np.float32)
+    for i in range(st.shape[0]):
+        v, k = st[i, 0], 0
+        for c in (1, 2):
+            if st[i, c] > 0 and (v == 0 or st[i, c] < v):
+                v, k = st[i, c], c
+        out[i] = v
+    return out
+
+
+def _unit(v: np.ndarray) -> np.ndarray:
+    n = np.sqrt((v.astype(np.float64) ** 2).sum(-1, keepdims=True))
+    return (v / n).astype(np.float32)
+
+
+def _isotropic_dirs(rng, n) -> np.ndarray:
+    z = rng.uniform(-1.0, 1.0, n)
+    phi = rng.uniform(0.0, 2.0 * math.pi, n)
+    s = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    return _unit(np.stack([s * np.cos(phi), s * np.sin(phi), z], -1))
+
+
+def _box_interval(o, d, lo, hi):
+    with np.errstate(divide="ignore", invalid="ignore"):
+        inv = 1.0 / d
+        t0 = (lo - o) * inv
+        t1 = (hi - o) * inv
+    tmin = np.nanmax(np.minimum(t0, t1), axis=-1)
+    tmax = np.nanmin(np.maximum(t0, t1), axis=-1)
+    return tmin, tmax
+
+
+def _sphere_interval(o, d, c, r):
+    oc = o - c
+    b = (d * oc).sum(-1)
+    cc = (oc * oc).sum(-1) - r * r
+    disc = b * b - cc
+    hit = disc > 0
+    sq = np.sqrt(np.where(hit, disc, 0.0))
+    return hit, -b - sq, -b + sq
+
+
+def segment_values(media: np.ndarray, medium_id: np.ndarray, tmin: np.ndarray, tmax: np.ndarray,
+                   end_in_medium: Optional[np.ndarray] = None) -> np.ndarray:
+    """Segment records as Scene::Intersect stores them for a ray that does not
+    sample the medium (Scene.hxx:528-545): attenuation = exp(-sigma_t*(tmax-tmin)),
+    pdf/revPdf = RaySamplePdf (Media.hxx:294-323)."""
+    st = _sigma_t(media)
+    mn = _min_positive(st)
+    has_sc = (np.abs(media["scattering"]) > 0).any(-1)
+    seg = np.zeros(len(tmin), SEGMENT_DTYPE)
+    t0 = tmin.astype(np.float32)
+    t1 = tmax.astype(np.float32)
+    seg["dist_min"] = t0
+    seg["dist_max"] = t1
+    seg["medium_id"] = medium_id
+    length = (t1 - t0).astype(np.float32)
+    seg["attenuation"] = np.exp(-st[medium_id] * length[:, None]).astype(np.float32)
+    att = np.maximum(np.exp(-mn[medium_id] * length), np.float32(1e-35)).astype(np.float32)
+    samplable = (mn[medium_id] > 0) & has_sc[medium_id]
+    pdf = np.where(samplable, att, np.float32(1.0)).astype(np.float32)
+    if end_in_medium is not None:
+        pdf = np.where(samplable & end_in_medium, (mn[medium_id] * att).astype(np.float32), pdf)
+    seg["ray_sample_pdf"] = pdf
+    seg["ray_sample_rev_pdf"] = np.where(samplable, att, np.float32(1.0)).astype(np.float32)
+    return seg
+
+
+def ray_segments(media, o, d, box_lo, box_hi, box_medium, vacuum_medium, sphere=None,
+                 leading_vacuum=True, far=10.0):
+    """Volume segments of rays through a box medium with an optional nested sphere."""
+    o64 = o.astype(np.float64)
+    d64 = d.astype(np.float64)
+    n = o.shape[0]
+    tin, tout = _box_interval(o64, d64, np.asarray(box_lo, np.float64), np.asarray(box_hi, np.float64))
+    tin = np.maximum(tin, 0.0)
+    hit_box = tout > tin
+    if sphere is not None:
+        hs, ts0, ts1 = _sphere_interval(o64, d64, np.asarray(sphere[0], np.float64), sphere[1])
+        hs &= hit_box & (ts0 > tin) & (ts1 < tout)
+    else:
+        hs = np.zeros(n, bool)
+    lead = leading_vacuum & hit_box & (tin > 0)
+    counts = lead.astype(np.int64) + np.where(hit_box, np.where(hs, 3, 1), 0)
+    counts = np.where(hit_box, counts, 1)
+    off = np.zeros(n + 1, np.int64)
+    np.cumsum(counts, out=off[1:])
+    total = int(off[-1])
+    tmn = np.zeros(total)
+    tmx = np.zeros(total)
+    mid = np.zeros(total, np.int32)
+    pos = off[:-1].copy()
+
+    def put(mask, a, b, m):
+        idx = pos[mask]
+        tmn[idx] = a[mask]
+        tmx[idx] = b[mask]
+        mid[idx] = m
+        pos[mask] += 1
+
+    miss = ~hit_box
+    put(miss, np.zeros(n), np.full(n, far), vacuum_medium)
+    put(lead, np.zeros(n), tin, vacuum_medium)
+    plain = hit_box & ~hs
+    put(plain, tin, tout, box_medium)
+    if sphere is not None:
+        put(hs, tin, ts0, box_medium)
+        put(hs, ts0, ts1, sphere[2])
+        put(hs, ts1, tout, box_medium)
+    assert np.array_equal(pos, off[1:])
+    seg = segment_values(media, mid, tmn, tmx)
+    return off, seg
+
+
+def pinhole_rays(width, height, eye=(0.5, 0.5, -2.0)):
+    """Pixel-centre rays from `eye` through the z=0 plane; the frame spans x in [0,1]
+    and is centred at y = 0.5 with square pixels. Scanline order."""
+    xs = (np.arange(width, dtype=np.float64) + 0.5) / width
+    ys = 0.5 + ((np.arange(height, dtype=np.float64) + 0.5) - height * 0.5) / width
+    X, Y = np.meshgrid(xs, ys)
+    tgt = np.stack([X.ravel(), Y.ravel(), np.zeros(X.size)], -1)
+    e = np.asarray(eye, np.float64)
+    d = tgt - e
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    o = np.broadcast_to(e, d.shape)
+    return np.ascontiguousarray(o, dtype=np.float32), np.ascontiguousarray(d, dtype=np.float32)
+
+
+def random_camera_rays(rng, n, eye=(0.5, 0.5, -2.0)):
+    tgt = np.stack([rng.uniform(0, 1, n), rng.uniform(0, 1, n), np.zeros(n)], -1)
+    e = np.asarray(eye, np.float64)
+    d = tgt - e
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    o = np.broadcast_to(e, d.shape)
+    return np.ascontiguousarray(o, dtype=np.float32), np.ascontiguousarray(d, dtype=np.float32)
+
+
+def _distance_to_boundary(o, d, sphere, box_lo=(0, 0, 0), box_hi=(1, 1, 1)):
+    """Distance from points inside the box along d to the first medium boundary."""
+    o64 = o.astype(np.float64)
+    d64 = d.astype(np.float64)
+    _, tout = _box_interval(o64, d64, np.asarray(box_lo, np.float64), np.asarray(box_hi, np.float64))
+    dist = tout
+    inside = np.zeros(o.shape[0], bool)
+    if sphere is not None:
+        c = np.asarray(sphere[0], np.float64)
+        inside = ((o64 - c) ** 2).sum(-1) < sphere[1] ** 2
+        hs, t0, t1 = _sphere_interval(o64, d64, c, sphere[1])
+        # inside: exit at t1; outside: entry at t0 when ahead of the origin
+        t_exit = np.where(inside, t1, np.where(hs & (t0 > 0), t0, np.inf))
+        dist = np.minimum(dist, t_exit)
+    return np.maximum(dist, 0.0), inside
+
+
+def make_photons(rng, n, lo=(0, 0, 0), hi=(1, 1, 1), radius=None, radius_lognormal=None,
+                 power_max=1e-4) -> Photons:
+    lo = np.asarray(lo, np.float64)
+    hi = np.asarray(hi, np.float64)
+    pos = (lo + rng.uniform(0, 1, (n, 3)) * (hi - lo)).astype(np.float32)
+    inc = _isotropic_dirs(rng, n)
+    power = rng.uniform(0, power_max, (n, 3)).astype(np.float32)
+    if radius_lognormal is not None:
+        mu, sigma, rmin, rmax = radius_lognormal
+        r = np.clip(rng.lognormal(mu, sigma, n), rmin, rmax).astype(np.float32)
+        return Photons(pos, inc, power, r, 0.0)
+    return Photons(pos, inc, power, None, float(radius))
+
+
+def make_beams(rng, n, media, sphere, box_medium, radius, short_fraction=1.0, short_mean=None,
+               power_max=1e-4) -> Beams:
+    """Beams with origin uniform in the unit box and isotropic direction.
+    SHORT beams: length ~ Exp(mean) with mean = 1/min positive sigma_t of the origin medium
+    (or `short_mean`), clipped at the first medium boundary. LONG beams: run to the boundary
+    (UPBP.hxx:1798-1868)."""
+    o = rng.uniform(0, 1, (n, 3)).astype(np.float32)
+    d = _isotropic_dirs(rng, n)
+    bound, inside = _distance_to_boundary(o, d, sphere)
+    med = np.full(n, box_medium, np.int32)
+    if sphere is not None:
+        med[inside] = sphere[2]
+    st = _sigma_t(media)
+    mn = _min_positive(st)
+    is_short = rng.uniform(0, 1, n) < short_fraction
+    mean = np.where(np.isfinite(1.0 / mn[med]), 1.0 / mn[med], 1.0) if short_mean is None else np.full(n, short_mean)
+    exp_len = rng.exponential(1.0, n) * mean
+    length = np.where(is_short, np.minimum(exp_len, bound), bound)
+    length = np.maximum(length, 1e-4).astype(np.float32)
+    flags = np.where(is_short, SHORT_BEAM, LONG_BEAM).astype(np.uint32)
+    thr = rng.uniform(0, power_max, (n, 3)).astype(np.float32)
+    return Beams(o, d, length, thr, flags, med, float(radius))
+
+
+def _medium_rows_box_sphere():
+    return [
+        ((0.02, 0.02, 0.02), (0.2, 0.2, 0.2), 0.0, 0.8),  # 0: box
+        ((0.0, 0.0, 0.0), (0.0, 0.0, 0.0), 0.0, 1.0),  # 1: vacuum (clear medium)
+        ((0.05, 0.05, 0.05), (2.0, 2.0, 2.0), 0.0, 0.8),  # 2: sphere
+    ]
+
+
+SPHERE = ((0.5, 0.5, 0.5), 0.3, 2)
+
+
+def plain_bre(width=256, height=256, n_photons=100_000, radius=0.01, seed=1, mean_cosine=0.0,
+              beam_type=LONG_BEAM) -> Workload:
+    """Plain BRE: unit cube, sigma_s=0.5, sigma_a=0.1, photons uniform, power U[0,1e-4], fixed radius."""
+    rng = rng_for(seed)
+    media = make_media([((0.1, 0.1, 0.1), (0.5, 0.5, 0.5), mean_cosine, 0.8),
+                        ((0, 0, 0), (0, 0, 0), 0.0, 1.0)])
+    ph = make_photons(rng, n_photons, radius=radius)
+    o, d = pinhole_rays(width, height)
+    off, seg = ray_segments(media, o, d, (0, 0, 0), (1, 1, 1), 0, 1)
+    q = Queries(o, d, off, seg, beam_type)
+    return Workload("plain_bre_%dx%d_%dph" % (width, height, n_photons), media, ph, None, q, None, 0, 1, seed)
+
+
+def bre_beams(width=1920, height=1080, n_photons=1_000_000, n_beams=200_000, beam_radius=0.003,
+              seed=2, mean_cosine=0.0) -> Workload:
+    """Box + nested sphere, lognormal photon radii (mu=ln 0.005, sigma=0.4, clamped to
+    [5e-4, 0.05]), SHORT beams with exponential lengths clipped at media boundaries."""
+    rng = rng_for(seed)
+    rows = _medium_rows_box_sphere()
+    rows = [(a, s, mean_cosine if i != 1 else 0.0, c) for i, (a, s, g, c) in enumerate(rows)]
+    media = make_media(rows)
+    ph = make_photons(rng, n_photons, radius_lognormal=(math.log(0.005), 0.4, 5e-4, 0.05))
+    bm = make_beams(rng, n_beams, media, SPHERE, 0, beam_radius, short_fraction=1.0)
+    o, d = pinhole_rays(width, height)
+    off, seg = ray_segments(media, o, d, (0, 0, 0), (1, 1, 1), 0, 1, SPHERE)
+    q = Queries(o, d, off, seg, LONG_BEAM)
+    return Workload("bre_beams_%dx%d_%dph_%dbm" % (width, height, n_photons, n_beams), media, ph, bm, q,
+                    SPHERE, 0, 1, seed)
+
+
+def beams_stress(n_queries=1_000_000, n_beams=2_000_000, beam_radius=0.003, seed=3) -> Workload:
+    """One box medium sigma_s=1, sigma_a=0.1; 50% SHORT beams (Exp(mean 0.05) clipped to the box),
+    50% LONG beams to the box boundary; query beams from random camera rays."""
+    rng = rng_for(seed)
+    media = make_media([((0.1, 0.1, 0.1), (1.0, 1.0, 1.0), 0.0, 0.8), ((0, 0, 0), (0, 0, 0), 0.0, 1.0)])
+    bm = make_beams(rng, n_beams, media, None, 0, beam_radius, short_fraction=0.5, short_mean=0.05)
+    o, d = random_camera_rays(rng, n_queries)
+    off, seg = ray_segments(media, o, d, (0, 0, 0), (1, 1, 1), 0, 1)
+    q = Queries(o, d, off, seg, LONG_BEAM)
+    return Workload("beams_stress_%dq_%dbm" % (n_queries, n_beams), media, None, bm, q, None, 0, 1, seed)
+
+
+def adjoint_scene(width=512, height=512, n_photons=500_000, n_beams=100_000, beam_radius=0.003,
+                  seed=5, scattering_scale=1.0) -> Workload:
+    """Box + sphere geometry with 500k photons + 100k beams at 512x512. `scattering_scale`
+    multiplies sigma_s of both media (the target image uses 1.1)."""
+    w = bre_beams(width, height, n_photons, n_beams, beam_radius, seed)
+    if scattering_scale != 1.0:
+        w = with_media(w, scale_scattering(w.media, scattering_scale))
+    return w
+
+
+def scale_scattering(media: np.ndarray, s: float) -> np.ndarray:
+    m = media.copy()
+    m["scattering"] = (m["scattering"] * np.float32(s)).astype(np.float32)
+    return m
+
+
+def with_media(w: Workload, media: np.ndarray) -> Workload:
+    """Same photons, beams and rays; segment attenuations and pdfs recomputed for `media`."""
+    q = w.queries
+    seg = segment_values(media, q.segments["medium_id"], q.segments["dist_min"], q.segments["dist_max"])
+    q2 = Queries(q.origin, q.direction, q.seg_offsets, seg, q.beam_type, q.lite)
+    return dataclasses.replace(w, media=media, queries=q2)
+
+
+def batched_instances(n_instances=64, seed=4, scale=1.0, max_queries=2_000_000) -> List[Workload]:
+    """Independent unit-box scenes: sigma_s, sigma_a ~ U[0.05, 3] per channel; photons
+    N ~ lognormal(mu=ln 1.4e5, sigma=1) clamped to [1e4, 2e6]; beams N/5; queries ~ U[16k, 2M]
+    random camera rays. `scale` shrinks every count (tests use small scales)."""
+    out = []
+    for i in range(n_instances):
+        rng = rng_for(seed * 1000 + i)
+        sa = rng.uniform(0.05, 3.0, 3)
+        ss = rng.uniform(0.05, 3.0, 3)
+        media = make_media([(tuple(sa), tuple(ss), 0.0, 0.8), ((0, 0, 0), (0, 0, 0), 0.0, 1.0)])
+        n_ph = int(np.clip(rng.lognormal(math.log(1.4e5), 1.0), 1e4, 2e6) * scale)
+        n_ph = max(n_ph, 1)
+        n_q = int(rng.uniform(16_000, max_queries) * scale)
+        n_q = max(n_q, 1)
+        ph = make_photons(rng, n_ph, radius_lognormal=(math.log(0.005), 0.4, 5e-4, 0.05))
+        bm = make_beams(rng, max(n_ph // 5, 1), media, None, 0, 0.003, short_fraction=1.0)
+        o, d = random_camera_rays(rng, n_q)
+        off, seg = ray_segments(media, o, d, (0, 0, 0), (1, 1, 1), 0, 1)
+        q = Queries(o, d, off, seg, LONG_BEAM)
+        out.append(Workload("instance_%02d" % i, media, ph, bm, q, None, 0, 1, seed * 1000 + i))
+    return out

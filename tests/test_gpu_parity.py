@@ -1,0 +1,183 @@
+"""GPU parity: the HIP estimators against the CPU oracle on the same seeded inputs.
+
+Bar: hit sets identical (per-query signature = count + hash of item indices);
+per-query radiance within REL_TOL = 1e-4 relative of the FP32 oracle (the reference
+arithmetic in reference order); FP64 re-evaluation errors are reported.
+"""
+import numpy as np
+import pytest
+
+from smallupbp_amd import _abi, native, scenes
+
+from gpu_helpers import REL_TOL, gpu_context, oracle_beams, oracle_photons, print_case, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _run_photons(w, leaf_size=0, order=_abi.ORDER_SORTED, device=True):
+    ctx = gpu_context(w.media)
+    bre = native.EmbreeBre(ctx)
+    bre.build(w.photons, leaf_size=leaf_size)
+    qv = native.QueryBatchView.to_device(w.queries, order=order) if device else native.QueryBatchView.host(
+        w.queries, order=order)
+    st = {}
+    out = bre.evalBre(qv, stats=st)
+    out = out.cpu().numpy() if hasattr(out, "cpu") else out
+    sig, _ = native.hit_signatures(ctx, bre, None, qv)
+    return out, sig, st
+
+
+def _run_beams(w, leaf_size=0, order=_abi.ORDER_SORTED, piece_factor=0.0, max_pieces=0):
+    ctx = gpu_context(w.media)
+    bm = native.PhotonBeamsEvaluator(ctx)
+    bm.build(w.beams, leaf_size=leaf_size, piece_factor=piece_factor, max_pieces=max_pieces)
+    qv = native.QueryBatchView.to_device(w.queries, order=order)
+    st = {}
+    out = bm.evalBeamBeamEstimate(qv, stats=st).cpu().numpy()
+    _, sig = native.hit_signatures(ctx, None, bm, qv)
+    return out, sig, st
+
+
+@pytest.mark.parametrize("leaf_size", [1, 4, 8, 32])
+def test_bre_small_matches_oracle(leaf_size):
+    print_case("bre_small", leaf_size=leaf_size)
+    w = scenes.plain_bre(64, 64, 20_000, 0.01, seed=11)
+    out, sig, st = _run_photons(w, leaf_size=leaf_size)
+    ref, rsig, cnt = oracle_photons(w, accel=0)
+    assert np.array_equal(sig, rsig), "photon hit sets differ"
+    assert st["photon_hits"] == int(cnt[0])
+    assert rel_err(out, ref).max() <= REL_TOL
+
+
+def test_bre_plain_256_full_config():
+    """Plain BRE, 256x256 queries x 100k photons, r = 0.01 (the CPU-runnable configuration)."""
+    print_case("bre_plain_256")
+    w = scenes.plain_bre(256, 256, 100_000, 0.01, seed=1)
+    out, sig, st = _run_photons(w)
+    ref, rsig, cnt = oracle_photons(w, accel=1)
+    assert np.array_equal(sig, rsig)
+    e = rel_err(out, ref)
+    print("max rel err vs FP32 oracle %.3g, hits %d" % (e.max(), st["photon_hits"]))
+    assert e.max() <= REL_TOL
+    ref64, _, _ = oracle_photons(w, accel=1, precision=64, signatures=False)
+    e64 = rel_err(out, ref64)
+    print("vs FP64: median %.3g p99.9 %.3g max %.3g" % (np.median(e64), np.percentile(e64, 99.9), e64.max()))
+    assert np.median(e64) < 1e-5
+
+
+def test_bre_host_memory_and_order_agree():
+    print_case("bre_host_order")
+    w = scenes.plain_bre(40, 30, 5_000, 0.02, seed=12)
+    a, sa, _ = _run_photons(w, order=_abi.ORDER_SORTED, device=False)
+    b, sb, _ = _run_photons(w, order=_abi.ORDER_AS_GIVEN, device=True)
+    assert np.array_equal(sa, sb)
+    assert rel_err(a, b).max() <= 1e-6
+
+
+def test_bre_short_query_and_anisotropic():
+    print_case("bre_short_aniso")
+    w = scenes.plain_bre(48, 48, 10_000, 0.02, seed=13, mean_cosine=0.6, beam_type=scenes.SHORT_BEAM)
+    out, sig, _ = _run_photons(w)
+    ref, rsig, _ = oracle_photons(w, accel=0)
+    assert np.array_equal(sig, rsig)
+    assert rel_err(out, ref).max() <= REL_TOL
+
+
+def test_bre_nested_media_lite_segments():
+    print_case("bre_nested_lite")
+    w = scenes.bre_beams(64, 36, 30_000, 10, seed=14)
+    w.queries.lite = True
+    out, sig, _ = _run_photons(w)
+    ref, rsig, _ = oracle_photons(w, accel=0)
+    assert np.array_equal(sig, rsig)
+    assert rel_err(out, ref).max() <= REL_TOL
+
+
+@pytest.mark.parametrize("leaf_size,piece_factor,max_pieces", [(4, 0.0, 0), (1, 2.0, 128), (16, 16.0, 8)])
+def test_beams_small_matches_oracle(leaf_size, piece_factor, max_pieces):
+    print_case("beams_small", leaf_size=leaf_size, piece_factor=piece_factor, max_pieces=max_pieces)
+    w = scenes.bre_beams(48, 27, 10, 3_000, seed=15)
+    out, sig, st = _run_beams(w, leaf_size, piece_factor=piece_factor, max_pieces=max_pieces)
+    ref, rsig, cnt = oracle_beams(w, accel=0)
+    assert np.array_equal(sig, rsig), "beam hit sets differ"
+    assert st["beam_hits"] == int(cnt[0])
+    assert rel_err(out, ref).max() <= REL_TOL
+
+
+def test_beams_stress_mix_short_long():
+    print_case("beams_mix")
+    w = scenes.beams_stress(n_queries=4_000, n_beams=20_000, seed=16)
+    out, sig, _ = _run_beams(w)
+    ref, rsig, _ = oracle_beams(w, accel=0)
+    assert np.array_equal(sig, rsig)
+    assert rel_err(out, ref).max() <= REL_TOL
+
+
+def test_beams_short_query():
+    print_case("beams_short_query")
+    w = scenes.bre_beams(32, 32, 10, 2_000, seed=17)
+    w.queries.beam_type = scenes.SHORT_BEAM
+    out, sig, _ = _run_beams(w)
+    ref, rsig, _ = oracle_beams(w, accel=0)
+    assert np.array_equal(sig, rsig)
+    assert rel_err(out, ref).max() <= REL_TOL
+
+
+def test_empty_structures_give_zero():
+    print_case("empty")
+    w = scenes.plain_bre(16, 16, 100, 0.01, seed=18)
+    ctx = gpu_context(w.media)
+    bre = native.EmbreeBre(ctx)
+    mask = np.zeros(w.photons.count, np.uint8)
+    assert bre.build(w.photons, in_medium=mask) == 0
+    out = bre.evalBre(native.QueryBatchView.host(w.queries))
+    assert np.all(out == 0)
+
+
+def test_in_medium_filter_matches_subset():
+    print_case("in_medium")
+    w = scenes.plain_bre(32, 32, 8_000, 0.03, seed=19)
+    rng = np.random.default_rng(0)
+    mask = (rng.uniform(size=w.photons.count) < 0.5).astype(np.uint8)
+    ctx = gpu_context(w.media)
+    bre = native.EmbreeBre(ctx)
+    n = bre.build(w.photons, in_medium=mask)
+    assert n == int(mask.sum())
+    out = bre.evalBre(native.QueryBatchView.host(w.queries))
+    from oracle import pyoracle as po
+
+    oph = po.OraclePhotons(w.photons, in_medium=mask)
+    ref, _, _ = po.bre_eval(oph, w.media, w.queries, accel=0)
+    assert rel_err(out, ref).max() <= REL_TOL
+
+
+def test_input_errors_are_reported():
+    print_case("errors")
+    w = scenes.plain_bre(8, 8, 500, 0.05, seed=20)
+    ctx = gpu_context(w.media)
+    bre = native.EmbreeBre(ctx)
+    bre.build(w.photons)
+    q = w.queries
+    bad = q.segments.copy()
+    bad["medium_id"][0] = 7
+    with pytest.raises(native.UpbpError) as e:
+        bre.evalBre(native.QueryBatchView(q.origin, q.direction, q.seg_offsets, bad))
+    assert e.value.status == _abi.UPBP_EINVAL
+    o = q.origin.copy()
+    o[3, 0] = np.nan
+    with pytest.raises(native.UpbpError) as e:
+        bre.evalBre(native.QueryBatchView(o, q.direction, q.seg_offsets, q.segments))
+    assert e.value.status == _abi.UPBP_ENONFINITE
+    off = q.seg_offsets.copy()
+    off[5] = off[6] + 1
+    with pytest.raises(native.UpbpError) as e:
+        bre.evalBre(native.QueryBatchView(q.origin, q.direction, off, q.segments))
+    assert e.value.status == _abi.UPBP_EINVAL
+    # the context still works after errors
+    out = bre.evalBre(native.QueryBatchView.host(q))
+    assert np.isfinite(out).all()
+    ph = scenes.Photons(w.photons.position.copy(), w.photons.inc_dir, w.photons.power, None, 0.05)
+    ph.position[2, 1] = np.inf
+    with pytest.raises(native.UpbpError) as e:
+        native.EmbreeBre(ctx).build(ph)
+    assert e.value.status == _abi.UPBP_ENONFINITE
