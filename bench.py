@@ -1,0 +1,292 @@
+#!/usr/bin/env python3
+"""Benchmark of the volumetric merging estimators on one MI355X.
+
+Workload (default): box + nested-sphere media, 1920x1080 LONG query beams,
+1,000,000 photon points (lognormal radii) + 200,000 SHORT photon beams
+(radius 0.003), FP32, synthetic seeded data. One step = one evaluation of both
+estimators (PB2D photon points and BB1D photon beams) over all queries, with
+inputs already resident in HBM (query regrouping by Morton key included in
+the step). Structures are built once before timing (the reference also builds
+once per iteration and evaluates every camera ray).
+
+Prints ONE JSON line: metric, value (Mqueries/s), roofline of the dominant
+kernel (algorithmic bytes / average HIP-event kernel time vs 8 TB/s), the CPU
+oracle timed on a bounded sample on the host cores, and the forward+adjoint vs
+forward cost ratio on the 512x512 differentiable workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def alg_bytes_queries(q) -> int:
+    """Per-query algorithmic bytes: 24 B ray + 12 B per segment (tmin, tmax, medium) + 12 B RGB out."""
+    nseg = int(q.seg_offsets[-1])
+    return 24 * q.count + 12 * nseg + 12 * q.count
+
+
+def host_cores() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_name() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def run_cpu_baseline(w, budget_s: float, threads: int):
+    """Oracle (clean-room CPU restatement of the reference estimator) on a random query sample:
+    photons through a binned-SAH BVH, beams through the reference uniform grid (256 cells on the
+    longest axis, 3D-DDA). Returns Mqueries/s and what the sample was."""
+    from oracle import pyoracle as po
+
+    t = time.time()
+    oph = po.OraclePhotons(w.photons)
+    obm = po.OracleBeams(w.beams, grid_res=256)
+    build_s = time.time() - t
+    rng = np.random.default_rng(12345)
+    probe = np.sort(rng.choice(w.queries.count, 2000, replace=False))
+    qp = w.queries.subset(probe)
+    t = time.time()
+    po.bre_eval(oph, w.media, qp, accel=1, threads=threads)
+    po.beams_eval(obm, w.media, qp, accel=1, threads=threads)
+    per_q = (time.time() - t) / 2000
+    n = int(min(w.queries.count, max(2000, budget_s / max(per_q, 1e-9))))
+    idx = np.sort(rng.choice(w.queries.count, n, replace=False))
+    qs = w.queries.subset(idx)
+    t = time.time()
+    po.bre_eval(oph, w.media, qs, accel=1, threads=threads)
+    po.beams_eval(obm, w.media, qs, accel=1, threads=threads)
+    el = time.time() - t
+    return n / el / 1e6, n, el, build_s
+
+
+def time_eval(ctx, bre, beams, qv, out_p, out_b, steps, warmup):
+    import torch
+
+    from smallupbp_amd import native
+
+    for _ in range(warmup):
+        native.eval_batch(ctx, bre, beams, qv, out_p, out_b)
+    torch.cuda.synchronize()
+    per = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = {}
+        native.eval_batch(ctx, bre, beams, qv, out_p, out_b, stats=st)
+        per.append(st)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    return el, per
+
+
+def adjoint_ratio(steps: int):
+    """Forward vs forward+adjoint on the 512x512 differentiable workload (500k photons + 100k beams)."""
+    import torch
+
+    from smallupbp_amd import native, scenes
+
+    w = scenes.adjoint_scene()
+    ctx = native.Context(0)
+    ctx.set_media(w.media)
+    bre = native.EmbreeBre(ctx)
+    bre.build(w.photons)
+    beams = native.PhotonBeamsEvaluator(ctx)
+    beams.build(w.beams)
+    qv = native.QueryBatchView.to_device(w.queries)
+    dev = "cuda"
+    op = torch.empty((w.queries.count, 3), device=dev)
+    ob = torch.empty((w.queries.count, 3), device=dev)
+    gp = torch.ones((w.queries.count, 3), device=dev)
+    gb = torch.ones((w.queries.count, 3), device=dev)
+    gs = torch.zeros((len(w.media), 3), device=dev)
+    ga = torch.zeros((len(w.media), 3), device=dev)
+    gpp = torch.zeros((w.photons.count, 3), device=dev)
+    gbp = torch.zeros((w.beams.count, 3), device=dev)
+    for _ in range(2):
+        native.eval_batch(ctx, bre, beams, qv, op, ob)
+        native.adjoint(ctx, bre, beams, qv, gp, gb, gs, ga, gpp, gbp)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        native.eval_batch(ctx, bre, beams, qv, op, ob)
+    torch.cuda.synchronize()
+    t_f = (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        native.eval_batch(ctx, bre, beams, qv, op, ob)
+        native.adjoint(ctx, bre, beams, qv, gp, gb, gs, ga, gpp, gbp)
+    torch.cuda.synchronize()
+    t_fa = (time.perf_counter() - t0) / steps
+    return t_fa / t_f, t_f * 1e3, t_fa * 1e3
+
+
+def calibrate_copy(nbytes: int = 1 << 30):
+    """A plain device copy of known size, so PMC runs can scale FETCH_SIZE / WRITE_SIZE."""
+    import torch
+
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    log("calibration copy: %d bytes read, %d bytes written" % (nbytes, nbytes))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds of CPU oracle work (0 = skip)")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--order", choices=["sorted", "as_given"], default="sorted")
+    ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--leaf-size", type=int, default=0)
+    ap.add_argument("--piece-factor", type=float, default=0.0)
+    ap.add_argument("--waves-per-block", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the estimators run on one MI355X and nothing here uses a "
+              "second GPU (no collectives, no sharding). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from smallupbp_amd import _abi, native, scenes
+
+    if args.calibrate_copy:
+        calibrate_copy()
+
+    t = time.time()
+    w = scenes.bre_beams()
+    log("workload %s generated in %.1fs" % (w.name, time.time() - t))
+    ctx = native.Context(0)
+    ctx.set_media(w.media)
+    if args.waves_per_block:
+        ctx.set_tuning(args.waves_per_block)
+    bre = native.EmbreeBre(ctx)
+    beams = native.PhotonBeamsEvaluator(ctx)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    bre.build(w.photons, leaf_size=args.leaf_size)
+    t_bp = time.perf_counter() - t
+    t = time.perf_counter()
+    beams.build(w.beams, leaf_size=args.leaf_size, piece_factor=args.piece_factor)
+    t_bb = time.perf_counter() - t
+    log("build: photons %.1f ms, beams %.1f ms (%d pieces)" % (t_bp * 1e3, t_bb * 1e3, beams.pieces))
+    order = _abi.ORDER_SORTED if args.order == "sorted" else _abi.ORDER_AS_GIVEN
+    qv = native.QueryBatchView.to_device(w.queries, order=order)
+    Q = w.queries.count
+    out_p = torch.empty((Q, 3), device="cuda")
+    out_b = torch.empty((Q, 3), device="cuda")
+
+    el, per = time_eval(ctx, bre, beams, qv, out_p, out_b, args.steps, args.warmup)
+    ms_step = el / args.steps * 1e3
+    value = Q * args.steps / el / 1e6
+    ms_ph = float(np.mean([p["kernel_ms_photons"] for p in per]))
+    ms_bm = float(np.mean([p["kernel_ms_beams"] for p in per]))
+    ms_ot = float(np.mean([p["kernel_ms_other"] for p in per]))
+    H_ph = int(per[-1]["photon_hits"])
+    H_bm = int(per[-1]["beam_hits"])
+    qbytes = alg_bytes_queries(w.queries)
+    bytes_ph = qbytes + 28 * H_ph
+    bytes_bm = qbytes + 48 * H_bm
+    if ms_bm >= ms_ph:
+        dom, dom_ms, dom_bytes = "k_eval<beams>", ms_bm, bytes_bm
+    else:
+        dom, dom_ms, dom_bytes = "k_eval<photons>", ms_ph, bytes_ph
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    log("step %.2f ms: photons %.2f ms, beams %.2f ms, sort+clear %.2f ms; hits %d + %d"
+        % (ms_step, ms_ph, ms_bm, ms_ot, H_ph, H_bm))
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        try:
+            tj = json.load(open(tpath))
+            traffic = tj.get(dom)
+        except (OSError, ValueError):
+            traffic = None
+
+    rec = {
+        "metric": "Mqueries/s (BRE + photon beams, 1920x1080 queries x 1M photons + 200k beams)",
+        "value": round(value, 3),
+        "unit": "Mqueries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Philox photons, beams and camera rays)",
+        "config": {
+            "workload": "bre_beams_1080p",
+            "queries": Q,
+            "photons": w.photons.count,
+            "beams": w.beams.count,
+            "photon_radius": "lognormal(mu=ln 0.005, sigma=0.4) clamped [5e-4, 0.05]",
+            "beam_radius": 0.003,
+            "media": "box sigma_s=0.2 sigma_a=0.02 + sphere r=0.3 sigma_s=2.0 sigma_a=0.05",
+            "query_order": args.order,
+            "beam_pieces": beams.pieces,
+            "parallelism": "single GPU",
+        },
+        "kernel_ms": {"photons": round(ms_ph, 4), "beams": round(ms_bm, 4), "sort_and_clear": round(ms_ot, 4)},
+        "hits": {"photon": H_ph, "beam": H_bm,
+                 "photon_evals_per_s": round(H_ph / (ms_step * 1e-3), 1),
+                 "beam_evals_per_s": round(H_bm / (ms_step * 1e-3), 1)},
+        "build_ms": {"photons": round(t_bp * 1e3, 2), "beams": round(t_bb * 1e3, 2)},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 5), "traffic": traffic,
+                     "alg_bytes_per_launch": dom_bytes},
+    }
+
+    if not args.no_adjoint:
+        ratio, tf, tfa = adjoint_ratio(max(3, args.steps // 2))
+        rec["adjoint"] = {"workload": "adjoint_512 (512x512 x 500k photons + 100k beams)",
+                          "fwd_ms": round(tf, 3), "fwd_plus_adjoint_ms": round(tfa, 3),
+                          "ratio": round(ratio, 3)}
+
+    if args.cpu_budget > 0:
+        threads = host_cores()
+        mq, n, el_cpu, bs = run_cpu_baseline(w, args.cpu_budget, threads)
+        rec["cpu_baseline"] = {
+            "value": round(mq, 5), "unit": "Mqueries/s", "cores": threads, "kind": "port",
+            "sample": "%d random queries of the same 1920x1080 workload, photons via binned-SAH BVH, beams via "
+                      "the reference uniform grid (256 cells, 3D-DDA); %.1f s of eval on %d threads (%s)"
+                      % (n, el_cpu, threads, cpu_name()),
+        }
+        rec["gpu_vs_cpu"] = round(value / mq, 1)
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -392,11 +392,22 @@ def beams_stress(n_queries=1_000_000, n_beams=2_000_000, beam_radius=0.003, seed
     return Workload("beams_stress_%dq_%dbm" % (n_queries, n_beams), media, None, bm, q, None, 0, 1, seed)
 
 
+CHROMATIC_ROWS = [
+    ((0.02, 0.03, 0.01), (0.2, 0.25, 0.3), 0.0, 0.8),  # 0: box
+    ((0.0, 0.0, 0.0), (0.0, 0.0, 0.0), 0.0, 1.0),  # 1: vacuum
+    ((0.05, 0.04, 0.06), (2.0, 1.8, 2.2), 0.0, 0.8),  # 2: sphere
+]
+
+
 def adjoint_scene(width=512, height=512, n_photons=500_000, n_beams=100_000, beam_radius=0.003,
-                  seed=5, scattering_scale=1.0) -> Workload:
+                  seed=5, scattering_scale=1.0, chromatic=False) -> Workload:
     """Box + sphere geometry with 500k photons + 100k beams at 512x512. `scattering_scale`
-    multiplies sigma_s of both media (the target image uses 1.1)."""
+    multiplies sigma_s of both media (the target image uses 1.1). `chromatic` swaps in media
+    whose minimum sigma_t channel is unique, so finite differences do not straddle the switch of
+    the SHORT-beam normalisation channel (Media.hxx:142-145) that grey media sit on."""
     w = bre_beams(width, height, n_photons, n_beams, beam_radius, seed)
+    if chromatic:
+        w = with_media(w, make_media(CHROMATIC_ROWS))
     if scattering_scale != 1.0:
         w = with_media(w, scale_scattering(w.media, scattering_scale))
     return w
