@@ -1,0 +1,124 @@
+"""Autotuner of the estimator kernels.
+
+Searches the parameters that the GPU path exposes and keeps the fastest per
+workload-size bucket:
+  build-time: LBVH leaf size (items per leaf tile, 1..32) for photons and for
+              beam pieces, and the beam piece length (in units of the beam radius);
+  launch-time: waves per workgroup (1, 2, 4) and the query order (Morton-sorted
+              bundles or the caller's order).
+Times come from the library's HIP events around each traversal kernel (median of
+`reps` runs). Results are cached in a JSON file keyed by
+(log2 photons, log2 beams, log2 queries) so later runs of similar size reuse them.
+"""
+from __future__ import annotations
+
+import itertools
+import json
+import math
+import os
+from typing import Dict, Optional
+
+import numpy as np
+
+from . import _abi, native, scenes
+
+DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_cache.json")
+
+PHOTON_LEAF = (1, 2, 4, 8, 16)
+BEAM_LEAF = (1, 2, 4, 8)
+PIECE_FACTOR = (2.0, 4.0, 8.0, 16.0)
+WAVES = (1, 2, 4)
+ORDERS = (_abi.ORDER_SORTED, _abi.ORDER_AS_GIVEN)
+
+
+def bucket(n_photons: int, n_beams: int, n_queries: int) -> str:
+    def b(n):
+        return 0 if n <= 0 else int(round(math.log2(n)))
+
+    return "%d/%d/%d" % (b(n_photons), b(n_beams), b(n_queries))
+
+
+def load_cache(path: str = DEFAULT_CACHE) -> Dict[str, dict]:
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def lookup(n_photons: int, n_beams: int, n_queries: int, path: str = DEFAULT_CACHE) -> Optional[dict]:
+    return load_cache(path).get(bucket(n_photons, n_beams, n_queries))
+
+
+def _median_kernel_ms(ctx, bre, beams, qv, reps, which):
+    ts = []
+    for _ in range(reps):
+        st = {}
+        native.eval_batch(ctx, bre if which == "photons" else None, beams if which == "beams" else None, qv,
+                          stats=st)
+        ts.append(st["kernel_ms_photons"] + st["kernel_ms_beams"] + st["kernel_ms_other"])
+    return float(np.median(ts))
+
+
+def tune(w: scenes.Workload, reps: int = 3, path: Optional[str] = DEFAULT_CACHE, log=print) -> dict:
+    """Sweep the parameter space for workload `w`; returns (and caches) the best configuration."""
+    ctx = native.Context(0)
+    ctx.set_media(w.media)
+    best = {"photons": None, "beams": None}
+    results = {"photons": [], "beams": []}
+    qviews = {o: native.QueryBatchView.to_device(w.queries, order=o) for o in ORDERS}
+    if w.photons is not None:
+        for leaf in PHOTON_LEAF:
+            bre = native.EmbreeBre(ctx)
+            bre.build(w.photons, leaf_size=leaf)
+            for wpb, order in itertools.product(WAVES, ORDERS):
+                ctx.set_tuning(wpb)
+                native.eval_batch(ctx, bre, None, qviews[order])  # warm
+                ms = _median_kernel_ms(ctx, bre, None, qviews[order], reps, "photons")
+                cfg = {"leaf_size": leaf, "waves_per_block": wpb, "order": order, "ms": ms}
+                results["photons"].append(cfg)
+                log("photons %s" % cfg)
+                if best["photons"] is None or ms < best["photons"]["ms"]:
+                    best["photons"] = cfg
+            bre.destroy()
+    if w.beams is not None:
+        for leaf, pf in itertools.product(BEAM_LEAF, PIECE_FACTOR):
+            bm = native.PhotonBeamsEvaluator(ctx)
+            bm.build(w.beams, leaf_size=leaf, piece_factor=pf)
+            for wpb, order in itertools.product(WAVES, ORDERS):
+                ctx.set_tuning(wpb)
+                native.eval_batch(ctx, None, bm, qviews[order])
+                ms = _median_kernel_ms(ctx, None, bm, qviews[order], reps, "beams")
+                cfg = {"leaf_size": leaf, "piece_factor": pf, "pieces": bm.pieces, "waves_per_block": wpb,
+                       "order": order, "ms": ms}
+                results["beams"].append(cfg)
+                log("beams %s" % cfg)
+                if best["beams"] is None or ms < best["beams"]["ms"]:
+                    best["beams"] = cfg
+            bm.destroy()
+    key = bucket(w.photons.count if w.photons is not None else 0, w.beams.count if w.beams is not None else 0,
+                 w.queries.count)
+    entry = {"workload": w.name, "best": best, "all": results}
+    if path:
+        cache = load_cache(path)
+        cache[key] = entry
+        tmp = path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(cache, f, indent=1)
+        os.replace(tmp, path)
+    return entry
+
+
+if __name__ == "__main__":
+    import argparse
+
+    ap = argparse.ArgumentParser(description="autotune the estimator kernels on a named workload")
+    ap.add_argument("--workload", default="bre_beams_1080p",
+                    choices=["bre_beams_1080p", "plain_bre_256", "beams_stress", "adjoint_512"])
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--cache", default=DEFAULT_CACHE)
+    a = ap.parse_args()
+    w = {"bre_beams_1080p": scenes.bre_beams, "plain_bre_256": scenes.plain_bre,
+         "beams_stress": scenes.beams_stress, "adjoint_512": scenes.adjoint_scene}[a.workload]()
+    e = tune(w, a.reps, a.cache)
+    print(json.dumps(e["best"]))

@@ -119,6 +119,26 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
+// v + DPP-moved v; lanes outside row_mask add 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_add(float v) {
+    const int m = __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xF, false);
+    return v + __int_as_float(m);
+}
+
+// Sum over the 64 lanes of a wave with DPP row operations (all lanes must be active).
+// quad_perm(1,0,3,2) and (2,3,0,1) sum quads, row_ror 4 and 8 sum rows of 16,
+// row_bcast15 / row_bcast31 carry row sums upward; lane 63 ends with the total.
+__device__ __forceinline__ float wave_sum_dpp(float v) {
+    v = dpp_add<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
+    v = dpp_add<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
+    v = dpp_add<0x124, 0xF>(v);  // row_ror:4
+    v = dpp_add<0x128, 0xF>(v);  // row_ror:8
+    v = dpp_add<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+    v = dpp_add<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
 // ---- leaf visitors ---------------------------------------------------------
 template <int MODE>
 __device__ __forceinline__ void photon_leaf(const EvalArgs& A, const Ray& r, SegState& s, LaneOut<MODE>& lo,
@@ -178,7 +198,7 @@ __device__ __forceinline__ void photon_leaf(const EvalArgs& A, const Ray& r, Seg
         }
         if (MODE == (int)Mode::Adjoint) {
             if (__ballot(hit)) {
-                const float r0 = wave_sum(gp0), r1 = wave_sum(gp1), r2 = wave_sum(gp2);
+                const float r0 = wave_sum_dpp(gp0), r1 = wave_sum_dpp(gp1), r2 = wave_sum_dpp(gp2);
                 if ((threadIdx.x & (kWave - 1)) == 0) {
                     const uint32_t o = A.orig[i];
                     unsafeAtomicAdd(&A.grad_items[3 * o + 0], r0);
@@ -271,7 +291,7 @@ __device__ __forceinline__ void beam_leaf(const EvalArgs& A, const Ray& r, SegSt
         }
         if (MODE == (int)Mode::Adjoint) {
             if (__ballot(hit)) {
-                const float r0 = wave_sum(gb0), r1 = wave_sum(gb1), r2 = wave_sum(gb2);
+                const float r0 = wave_sum_dpp(gb0), r1 = wave_sum_dpp(gb1), r2 = wave_sum_dpp(gb2);
                 if ((threadIdx.x & (kWave - 1)) == 0) {
                     const uint32_t o = A.orig[i];
                     unsafeAtomicAdd(&A.grad_items[3 * o + 0], r0);
@@ -593,7 +613,7 @@ __global__ __launch_bounds__(256) void k_eval(EvalArgs A) {
         const int lane = threadIdx.x & (kWave - 1);
         const int nm = min(A.n_media, kMaxLdsMedia);
         for (int e = 0; e < nm * 6; ++e) {
-            const float v = wave_sum(my_lds[(size_t)e * kWave + lane]);
+            const float v = wave_sum_dpp(my_lds[(size_t)e * kWave + lane]);
             if (lane == 0 && v != 0.f) unsafeAtomicAdd(&A.grad_media[(size_t)(A.media_base) * 6 + e], v);
         }
     }

@@ -181,3 +181,52 @@ def test_input_errors_are_reported():
     with pytest.raises(native.UpbpError) as e:
         native.EmbreeBre(ctx).build(ph)
     assert e.value.status == _abi.UPBP_ENONFINITE
+
+
+def _golden_names():
+    from golden import fixtures
+
+    return fixtures.names()
+
+
+@pytest.mark.parametrize("name", _golden_names())
+def test_gpu_reproduces_golden_fixture(name):
+    from golden import fixtures
+
+    print_case("golden", name=name)
+    w, z = fixtures.load(name)
+    ctx = gpu_context(w.media)
+    qv = native.QueryBatchView.to_device(w.queries)
+    if w.photons is not None:
+        bre = native.EmbreeBre(ctx)
+        bre.build(w.photons)
+        out = bre.evalBre(qv).cpu().numpy()
+        sig, _ = native.hit_signatures(ctx, bre, None, qv)
+        assert np.array_equal(sig, z["sig_photons"])
+        assert rel_err(out, z["out_photons_f32"]).max() <= REL_TOL
+    if w.beams is not None:
+        bm = native.PhotonBeamsEvaluator(ctx)
+        bm.build(w.beams)
+        out = bm.evalBeamBeamEstimate(qv).cpu().numpy()
+        _, sig = native.hit_signatures(ctx, None, bm, qv)
+        assert np.array_equal(sig, z["sig_beams"])
+        assert rel_err(out, z["out_beams_f32"]).max() <= REL_TOL
+
+
+def test_perpendicular_beam_is_rejected_like_the_reference():
+    """d1.d2 = 0 gives t2 = 0/0 in PhotonBeam.hxx:284 and the reference drops the hit."""
+    print_case("perpendicular")
+    media = scenes.make_media([((0.1, 0.1, 0.1), (0.5, 0.5, 0.5), 0.0, 1.0)])
+    o = np.array([[-1.0, 0.001, 2.0], [-0.5, 0.001, 1.0]], np.float32)
+    d = np.array([[1.0, 0.0, 0.0], [0.5, 0.0, np.sqrt(3) / 2]], np.float32)
+    bm = scenes.Beams(o, d, np.full(2, 3.0, np.float32), np.ones((2, 3), np.float32),
+                      np.full(2, scenes.LONG_BEAM, np.uint32), np.zeros(2, np.int32), 0.01)
+    q = scenes.Queries(np.zeros((1, 3), np.float32), np.array([[0, 0, 1]], np.float32), np.array([0, 1], np.int64),
+                       scenes.segment_values(media, np.array([0], np.int32), np.array([0.1]), np.array([3.0])))
+    ctx = gpu_context(media)
+    b = native.PhotonBeamsEvaluator(ctx)
+    b.build(bm)
+    _, sig = native.hit_signatures(ctx, None, b, native.QueryBatchView.host(q))
+    from oracle import pyoracle as po
+
+    assert sig[0, 0] == 1 and sig[0, 1] == po.hash32(np.array([1]))[0]
