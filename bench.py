@@ -147,6 +147,61 @@ def adjoint_ratio(steps: int):
     return t_fa / t_f, t_f * 1e3, t_fa * 1e3
 
 
+def batched_ratio(scale: float, steps: int, n_instances: int = 64):
+    """64 independent uneven instances (photons lognormal 10k-2M, beams N/5, queries U[16k, 2M]):
+    per-instance upbp_eval launches back to back vs one persistent launch (eager and hipGraph).
+    Both evaluate the same structures and queries; structures are built beforehand."""
+    import torch
+
+    from smallupbp_amd import native, scenes
+
+    t = time.time()
+    ws = scenes.batched_instances(n_instances=n_instances, seed=4, scale=scale, max_queries=2_000_000)
+    log("batched workload (%d instances, scale %.3g) generated in %.1fs" % (len(ws), scale, time.time() - t))
+    media = np.concatenate([w.media for w in ws])
+    ctx = native.Context(0)
+    ctx.set_media(media)
+    # per-instance contexts hold the instance's own media for the plain launches (media base 0)
+    inst, plain = [], []
+    for i, w in enumerate(ws):
+        bre = native.EmbreeBre(ctx)
+        bre.build(w.photons)
+        bm = native.PhotonBeamsEvaluator(ctx)
+        bm.build(w.beams)
+        qv = native.QueryBatchView.to_device(w.queries)
+        op = torch.empty((w.queries.count, 3), device="cuda")
+        ob = torch.empty((w.queries.count, 3), device="cuda")
+        inst.append((bre, bm, qv, 2 * i, op, ob))
+        pctx = native.Context(0)
+        pctx.set_media(w.media)
+        pbre = native.EmbreeBre(pctx)
+        pbre.build(w.photons)
+        pbm = native.PhotonBeamsEvaluator(pctx)
+        pbm.build(w.beams)
+        plain.append((pctx, pbre, pbm, qv, op, ob))
+    plan = native.BatchPlan(ctx, inst)
+    Q = sum(w.queries.count for w in ws)
+
+    def run_plain():
+        for pctx, pbre, pbm, qv, op, ob in plain:
+            native.eval_batch(pctx, pbre, pbm, qv, op, ob)
+
+    res = {}
+    for name, fn in (("plain", run_plain), ("batched", lambda: plan.run(use_graph=False)),
+                     ("batched_graph", lambda: plan.run(use_graph=True))):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        res[name] = (time.perf_counter() - t0) / steps * 1e3
+    return {"workload": "batched_64 (%d instances, scale %.3g, %d queries total)" % (len(ws), scale, Q),
+            "plain_ms": round(res["plain"], 3), "batched_ms": round(res["batched"], 3),
+            "batched_graph_ms": round(res["batched_graph"], 3),
+            "speedup": round(res["plain"] / min(res["batched"], res["batched_graph"]), 3)}
+
+
 def calibrate_copy(nbytes: int = 1 << 30):
     """A plain device copy of known size, so PMC runs can scale FETCH_SIZE / WRITE_SIZE."""
     import torch
@@ -171,6 +226,8 @@ def main():
     ap.add_argument("--leaf-size", type=int, default=0)
     ap.add_argument("--piece-factor", type=float, default=0.0)
     ap.add_argument("--waves-per-block", type=int, default=0)
+    ap.add_argument("--batched", type=float, default=0.0,
+                    help="also time the 64-instance batched form at this size scale (1 = full size)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -278,6 +335,9 @@ def main():
         rec["adjoint"] = {"workload": "adjoint_512 (512x512 x 500k photons + 100k beams)",
                           "fwd_ms": round(tf, 3), "fwd_plus_adjoint_ms": round(tfa, 3),
                           "ratio": round(ratio, 3)}
+
+    if args.batched > 0:
+        rec["batched"] = batched_ratio(args.batched, max(2, args.steps // 2))
 
     if args.cpu_budget > 0:
         threads = host_cores()

@@ -272,6 +272,116 @@ upbp_status upbp_adjoint(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams* b
                          float* grad_beam_power, upbp_memory memory, upbp_eval_stats* stats);
 
 /* ---------------------------------------------------------------------------
+ * UPBP mode: the same two estimators weighted by the UPBP multiple-importance-
+ * sampling weights, i.e. the callbacks breIntersectFuncHomogeneous2
+ * (SmallUPBP/src/Bre/Bre.cxx:293-464) and PhotonBeam::accumulate2
+ * (SmallUPBP/src/Beams/PhotonBeam.hxx:386-607) with the path-weight walks
+ * AccumulateCameraPathWeight / AccumulateLightPathWeight
+ * (SmallUPBP/src/Path/PathWeight.hxx:33-270). The inputs mirror
+ * AdditionalRayDataForMis (embree-2.0/embree/common/ray.h:31-99), MisData and
+ * UPBPLightVertex (SmallUPBP/src/Path/UPBPLightVertex.hxx:32-78).
+ * Nothing the caller passes is modified (the reference writes per-hit MIS data
+ * into the caller's camera and light-vertex arrays, Bre.cxx:395-422,
+ * PhotonBeam.hxx:499-595; here those values live in registers). */
+
+/* MisData flags (the four bools of MisData) */
+#define UPBP_MIS_DELTA 1u
+#define UPBP_MIS_ON_LIGHT_SOURCE 2u
+#define UPBP_MIS_SPECULAR 4u
+#define UPBP_MIS_IN_MEDIUM_WITH_BEAMS 8u
+
+/* MisData, UPBPLightVertex.hxx:32-52. 64 bytes. */
+typedef struct upbp_mis_data {
+    float pdf_a_inv;                 /* mPdfAInv */
+    float rev_pdf_a;                 /* mRevPdfA */
+    float rev_pdf_a_without_bsdf;    /* mRevPdfAWithoutBsdf */
+    float ray_sample_pdf_inv;        /* mRaySamplePdfInv */
+    float ray_sample_rev_pdf_inv;    /* mRaySampleRevPdfInv */
+    float ray_sample_pdfs_ratio;     /* mRaySamplePdfsRatio */
+    float ray_sample_rev_pdfs_ratio; /* mRaySampleRevPdfsRatio */
+    float sin_theta;                 /* mSinTheta */
+    float cos_theta_out;             /* mCosThetaOut */
+    float surf_factor;               /* mSurfMisWeightFactor */
+    float pp3d_factor;               /* mPP3DMisWeightFactor */
+    float pb2d_factor;               /* mPB2DMisWeightFactor */
+    float bb1d_factor;               /* mBB1DMisWeightFactor */
+    float bb1d_beam_selection_pdf;   /* mBB1DBeamSelectionPdf */
+    uint32_t flags;                  /* UPBP_MIS_* */
+    uint32_t pad;
+} upbp_mis_data;
+
+/* The UPBPLightVertex fields the estimators read (UPBPLightVertex.hxx:55-67);
+ * the position is the photon position given to upbp_bre_build. 112 bytes. */
+typedef struct upbp_light_vertex {
+    float throughput[3];        /* mThroughput */
+    float dir_fix[3];           /* mBSDF.WorldDirFix() */
+    float continuation_prob;    /* mBSDF.ContinuationProb() */
+    int32_t path_idx;           /* mPathIdx */
+    uint32_t path_length;       /* mPathLength */
+    uint32_t is_finite;         /* mIsFinite */
+    uint32_t pad[2];
+    upbp_mis_data mis;          /* mMisData */
+} upbp_light_vertex;
+
+/* PhotonBeam MIS fields (PhotonBeam.hxx:48-54), one per beam. 32 bytes. */
+typedef struct upbp_beam_mis {
+    int32_t light_vertex;         /* mLightVertex as an index into the light vertices */
+    uint32_t ray_sampling_flags;  /* mRaySamplingFlags */
+    float ray_sample_pdf;         /* mRaySamplePdf */
+    float ray_sample_rev_pdf;     /* mRaySampleRevPdf */
+    float last_pdf_w_inv;         /* mLastPdfWInv */
+    uint32_t pad[3];
+} upbp_beam_mis;
+
+/* Per-query part of AdditionalRayDataForMis plus the camera sub-path's MisData
+ * (mCameraVerticesMisData): entries camera_offset .. camera_offset + camera_path_length
+ * of the camera array; the last entry is recomputed per hit (Bre.cxx:396-419). 24 bytes. */
+typedef struct upbp_query_mis {
+    int64_t camera_offset;
+    uint32_t camera_path_length;  /* mCameraPathLength, >= 1 */
+    uint32_t ray_sampling_flags;  /* raySamplingFlags of evalBre: 0 or UPBP_ORIGIN_IN_MEDIUM */
+    float last_pdf_w_inv;         /* mLastPdfWInv */
+    uint32_t pad;
+} upbp_query_mis;
+
+/* Per-call constants of AdditionalRayDataForMis and the renderer (UPBP.hxx:423-460,780-907). */
+typedef struct upbp_mis_params {
+    uint32_t techniques;          /* estimatorTechniques | OtherSettings flags (PREVIOUS, BB1D_PREVIOUS, NO_SINE...) */
+    uint32_t query_beam_type;     /* mQueryBeamType */
+    uint32_t photon_beam_type;    /* mPhotonBeamType */
+    uint32_t min_path_length;     /* mMinPathLength */
+    uint32_t max_path_length;     /* mMaxPathLength */
+    float surf_factor;            /* mSurfMisWeightFactor */
+    float pp3d_factor;            /* mPP3DMisWeightFactor */
+    float pb2d_factor;            /* mPB2DMisWeightFactor */
+    float bb1d_factor;            /* mBB1DMisWeightFactor */
+    float bb1d_min_mfp;           /* mBB1DMinMFP */
+    float min_dist_to_med;        /* mMinDistToMed */
+    int32_t has_bb1d_beams;       /* mBB1DPhotonBeams != NULL (PB2D sets the beam selection pdf from it) */
+} upbp_mis_params;
+
+typedef struct upbp_mis_inputs {
+    const upbp_query_mis* queries;            /* [queries.count] */
+    const upbp_mis_data* camera;              /* [camera_count] */
+    int64_t camera_count;
+    const upbp_light_vertex* light_vertices;  /* mLightVertices: every vertex of every light sub-path */
+    int64_t light_vertex_count;
+    const int32_t* path_ends;                 /* mPathEnds */
+    int64_t path_count;
+    const upbp_beam_mis* beams;               /* [beams count], required with a beam structure */
+    upbp_mis_params params;
+    upbp_memory memory;                       /* of every pointer above */
+} upbp_mis_inputs;
+
+/* UPBP evaluation. The photon structure must have been built from the light
+ * vertices themselves (photon i of the build input = light vertex i, filtered by
+ * in_medium), as EmbreeBre::build does for UPBPLightVertex (Bre.cxx:649-755). */
+upbp_status upbp_eval_upbp(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams* beams,
+                           const upbp_query_batch* queries, const upbp_mis_inputs* mis,
+                           float* out_photons, float* out_beams, upbp_memory out_memory,
+                           upbp_eval_stats* stats);
+
+/* ---------------------------------------------------------------------------
  * Batched evaluation of many independent instances in one persistent launch.
  * Each instance names its structures, a device-resident query batch, its
  * media (a range of the context's media table: medium_id + media_base) and

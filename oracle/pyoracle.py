@@ -53,6 +53,9 @@ def lib():
         L.or_adjoint.restype = C.c_int
         L.or_adjoint.argtypes = [vp, vp, vp, i32, C.POINTER(_abi.QueryBatch), vp, vp, C.c_int, vp, vp,
                                  vp, vp]
+        L.or_upbp_eval.restype = C.c_int
+        L.or_upbp_eval.argtypes = [vp, vp, vp, i32, C.POINTER(_abi.QueryBatch), C.POINTER(_abi.MisInputs), C.c_int,
+                                   C.c_int, C.c_int, vp, vp, vp]
         L.or_exp_cephes.restype = f32
         L.or_exp_cephes.argtypes = [f32]
         L.or_hash32.restype = C.c_uint32
@@ -179,6 +182,22 @@ def adjoint(oph, obm, media, q: Queries, w_photons, w_beams, threads=None):
     if r:
         raise RuntimeError("or_adjoint failed: %s" % _abi.STATUS_NAMES.get(r, r))
     return {"grad_sigma_s": gs, "grad_sigma_a": ga, "grad_photon_power": gp, "grad_beam_power": gb}
+
+
+def upbp_eval(oph, obm, media, q: Queries, mi, which: int, precision=32, threads=None):
+    """UPBP-mode estimate (MIS weights) per query; which = 0 photons, 1 beams. Returns (out, hits)."""
+    threads = threads or os.cpu_count()
+    qb = query_batch(q)
+    m, mp, nm = _media_ptr(media)
+    ms = _abi.mis_inputs(mi)
+    out = np.zeros((q.count, 3), np.float64 if precision == 64 else np.float32)
+    cnt = np.zeros(2, np.uint64)
+    r = lib().or_upbp_eval(oph.h if oph else None, obm.h if obm else None, mp, nm, C.byref(qb), C.byref(ms), which,
+                           precision, threads, out.ctypes.data if precision != 64 else None,
+                           out.ctypes.data if precision == 64 else None, cnt.ctypes.data)
+    if r:
+        raise RuntimeError("or_upbp_eval failed: %s" % _abi.STATUS_NAMES.get(r, r))
+    return out, int(cnt[0])
 
 
 def hash32(i: np.ndarray) -> np.ndarray:

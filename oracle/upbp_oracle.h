@@ -101,6 +101,16 @@ int or_beams_eval(const or_beams* bm, const upbp_medium* media, int32_t n_media,
                   const upbp_query_batch* q, int accel, int precision, int threads,
                   float* out32, double* out64, uint32_t* sig, uint64_t* counters);
 
+/* ---- UPBP mode ----------------------------------------------------------- */
+/* evalBre / evalBeamBeamEstimate with MIS data: breIntersectFuncHomogeneous2
+ * (Bre.cxx:293-464), PhotonBeam::accumulate2 (PhotonBeam.hxx:386-607),
+ * AccumulateCameraPathWeight / AccumulateLightPathWeight (PathWeight.hxx:33-270),
+ * exact path walks, brute force over items. which: 0 photons (photon i = light
+ * vertex orig[i]), 1 beams. counters[0] = hits. */
+int or_upbp_eval(const or_photons* ph, const or_beams* bm, const upbp_medium* media, int32_t n_media,
+                 const upbp_query_batch* q, const upbp_mis_inputs* mis, int which, int precision, int threads,
+                 float* out32, double* out64, uint64_t* counters);
+
 /* ---- adjoint (FP64 analytic gradient of the plain estimators) ----------- */
 /* Gradient of L = sum_q <w_photons[q], out_photons[q]> + <w_beams[q], out_beams[q]>
  * (w are the upstream gradients). Brute-force traversal, double arithmetic,

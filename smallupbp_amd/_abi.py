@@ -104,3 +104,30 @@ def media_array(media: np.ndarray):
     arr = (Medium * len(media))()
     C.memmove(arr, np.ascontiguousarray(media).ctypes.data, 32 * len(media))
     return arr
+
+
+class MisParams(C.Structure):
+    _fields_ = [("techniques", C.c_uint32), ("query_beam_type", C.c_uint32), ("photon_beam_type", C.c_uint32),
+                ("min_path_length", C.c_uint32), ("max_path_length", C.c_uint32), ("surf_factor", C.c_float),
+                ("pp3d_factor", C.c_float), ("pb2d_factor", C.c_float), ("bb1d_factor", C.c_float),
+                ("bb1d_min_mfp", C.c_float), ("min_dist_to_med", C.c_float), ("has_bb1d_beams", C.c_int32)]
+
+
+class MisInputs(C.Structure):
+    _fields_ = [("queries", C.c_void_p), ("camera", C.c_void_p), ("camera_count", C.c_int64),
+                ("light_vertices", C.c_void_p), ("light_vertex_count", C.c_int64), ("path_ends", C.c_void_p),
+                ("path_count", C.c_int64), ("beams", C.c_void_p), ("params", MisParams), ("memory", C.c_int)]
+
+
+def mis_inputs(mi, memory=MEM_HOST, arrays=None):
+    """upbp_mis_inputs over a scenes.MisInputs (host numpy) or over `arrays` (dict of torch tensors)."""
+    src = arrays if arrays is not None else {
+        "queries": np.ascontiguousarray(mi.queries), "camera": np.ascontiguousarray(mi.camera),
+        "light_vertices": np.ascontiguousarray(mi.light_vertices),
+        "path_ends": np.ascontiguousarray(mi.path_ends, np.int32),
+        "beams": None if mi.beams is None else np.ascontiguousarray(mi.beams)}
+    p = MisParams(**mi.params)
+    s = MisInputs(ptr(src["queries"]), ptr(src["camera"]), len(mi.camera), ptr(src["light_vertices"]),
+                  len(mi.light_vertices), ptr(src["path_ends"]), len(mi.path_ends), ptr(src["beams"]), p, memory)
+    s._keep = src
+    return s

@@ -66,8 +66,10 @@ __global__ void k_photon_gather(int64_t n, const uint32_t* order, const uint32_t
     uint32_t k = order[s];
     int64_t i = keep ? keep[k] : k;
     float r = radius ? radius[i] : const_radius;
-    pos_r2[s] = make_float4(position[3 * i], position[3 * i + 1], position[3 * i + 2], r * r);
-    pw[s] = make_float4(power[3 * i], power[3 * i + 1], power[3 * i + 2], 0.f);
+    const float r2 = r * r;
+    pos_r2[s] = make_float4(position[3 * i], position[3 * i + 1], position[3 * i + 2], r2);
+    // kernel normalisation 1 / (r^2 pi / 2) of Bre.cxx:273, once per photon
+    pw[s] = make_float4(power[3 * i], power[3 * i + 1], power[3 * i + 2], 1.0f / ((r2 * kPi) * 0.5f));
     if (inc)
         inc_out[s] = make_float4(inc[3 * i], inc[3 * i + 1], inc[3 * i + 2], 0.f);
     else
@@ -80,7 +82,7 @@ __global__ void k_photon_set_power(int64_t n, const uint32_t* orig, const float*
     int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (s >= n) return;
     uint32_t i = orig[s];
-    pw[s] = make_float4(power[3 * i], power[3 * i + 1], power[3 * i + 2], 0.f);
+    pw[s] = make_float4(power[3 * i], power[3 * i + 1], power[3 * i + 2], pw[s].w);
 }
 
 struct BeamIn {
@@ -144,7 +146,7 @@ __device__ __forceinline__ float cut_at(float len, uint32_t k, uint32_t c) {
 }
 
 __global__ void k_beam_pieces(int64_t n, BeamIn b, const uint32_t* count, const uint32_t* offset,
-                              PieceDev* pieces, uint32_t* piece_beam, float4* lo, float4* hi) {
+                              PieceDev* pieces, float4* flux, uint32_t* piece_beam, float4* lo, float4* hi) {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     float r0, r1, rmax, dr;
@@ -157,6 +159,7 @@ __global__ void k_beam_pieces(int64_t n, BeamIn b, const uint32_t* count, const 
     const uint32_t word = ((uint32_t)b.medium[i] & kMediumMask) | (b.flags[i] == UPBP_SHORT_BEAM ? kShortBeamBit : 0u);
     const uint32_t c = count[i];
     const uint32_t base = offset[i];
+    const float d2o2 = dot3(dx, dy, dz, ox, oy, oz);  // dot(d2, O2) of PhotonBeam.hxx:278
     for (uint32_t k = 0; k < c; ++k) {
         const float t0 = cut_at(len, k, c), t1 = cut_at(len, k + 1, c);
         const float rs = r0 + t0 * dr, re = r0 + t1 * dr;
@@ -170,19 +173,21 @@ __global__ void k_beam_pieces(int64_t n, BeamIn b, const uint32_t* count, const 
         PieceDev pc;
         pc.p0 = make_float4(ox, oy, oz, len);
         pc.p1 = make_float4(dx, dy, dz, r0);
-        pc.p2 = make_float4(b.throughput[3 * i], b.throughput[3 * i + 1], b.throughput[3 * i + 2], dr);
-        pc.p3 = make_float4(t0, t1, __uint_as_float(word), rmax * rmax);
+        pc.p2 = make_float4(t0, t1, __uint_as_float(word), rmax * rmax);
+        pc.p3 = make_float4(d2o2, dr, 0.f, 0.f);
         pieces[p] = pc;
+        flux[p] = make_float4(b.throughput[3 * i], b.throughput[3 * i + 1], b.throughput[3 * i + 2], 0.f);
         piece_beam[p] = (uint32_t)i;
     }
 }
 
-__global__ void k_piece_gather(int64_t n, const uint32_t* order, const PieceDev* in, const uint32_t* in_beam,
-                               PieceDev* out, uint32_t* out_beam) {
+__global__ void k_piece_gather(int64_t n, const uint32_t* order, const PieceDev* in, const float4* in_flux,
+                               const uint32_t* in_beam, PieceDev* out, float4* out_flux, uint32_t* out_beam) {
     int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (s >= n) return;
     uint32_t k = order[s];
     out[s] = in[k];
+    out_flux[s] = in_flux[k];
     out_beam[s] = in_beam[k];
 }
 
@@ -329,22 +334,24 @@ void beams_build(Ctx& ctx, const upbp_beam_input& in, const upbp_build_params& p
     const int64_t total = (int64_t)read_u32(ctx, offset.as<uint32_t>() + (n - 1)) +
                           read_u32(ctx, count.as<uint32_t>() + (n - 1));
     UPBP_REQUIRE(total < (int64_t)0x7FFFFFFF, UPBP_EINVAL, "too many beam pieces");
-    DevBuf pieces, piece_beam, lo, hi, order;
+    DevBuf pieces, flux, piece_beam, lo, hi, order;
     pieces.ensure(total * sizeof(PieceDev));
+    flux.ensure(total * 16);
     piece_beam.ensure(total * 4);
     lo.ensure(total * 16);
     hi.ensure(total * 16);
     hipLaunchKernelGGL(k_beam_pieces, dim3(grid_for(n)), dim3(kBlock), 0, st, n, b, count.as<uint32_t>(),
-                       offset.as<uint32_t>(), pieces.as<PieceDev>(), piece_beam.as<uint32_t>(), lo.as<float4>(),
-                       hi.as<float4>());
+                       offset.as<uint32_t>(), pieces.as<PieceDev>(), flux.as<float4>(), piece_beam.as<uint32_t>(),
+                       lo.as<float4>(), hi.as<float4>());
     UPBP_HIP(hipGetLastError());
     build_lbvh(ctx, total, lo.as<float4>(), hi.as<float4>(), leaf, out.bvh, order);
     out.pieces = total;
     out.piece.ensure(total * sizeof(PieceDev));
+    out.piece_flux.ensure(total * 16);
     out.piece_beam.ensure(total * 4);
     hipLaunchKernelGGL(k_piece_gather, dim3(grid_for(total)), dim3(kBlock), 0, st, total, order.as<uint32_t>(),
-                       pieces.as<PieceDev>(), piece_beam.as<uint32_t>(), out.piece.as<PieceDev>(),
-                       out.piece_beam.as<uint32_t>());
+                       pieces.as<PieceDev>(), flux.as<float4>(), piece_beam.as<uint32_t>(), out.piece.as<PieceDev>(),
+                       out.piece_flux.as<float4>(), out.piece_beam.as<uint32_t>());
     UPBP_HIP(hipGetLastError());
     UPBP_HIP(hipStreamSynchronize(st));
 }

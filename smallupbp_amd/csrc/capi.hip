@@ -84,6 +84,8 @@ MediumDev derive_medium(const upbp_medium& in) {
     m.cont = in.continuation_prob;
     m.has_scattering = (m.ss[0] > 0 || m.ss[1] > 0 || m.ss[2] > 0) ? 1 : 0;
     m.isotropic = std::fabs(in.mean_cosine) < 1e-3f ? 1 : 0;
+    const float mx = std::max(m.st[0], std::max(m.st[1], m.st[2]));
+    m.mfp = mx > 0.f ? 1.0f / mx : INFINITY;
     return m;
 }
 
@@ -167,6 +169,8 @@ void check_stats_errors(Ctx& ctx, const uint64_t* st) {
     UPBP_REQUIRE(!(e & kErrBadMedium), UPBP_EINVAL, "segment medium_id outside the media table");
     UPBP_REQUIRE(!(e & kErrNonFinite), UPBP_ENONFINITE, "non-finite query origin, direction or segment value");
     UPBP_REQUIRE(!(e & kErrBadSegment), UPBP_EINVAL, "segment with dist_min > dist_max");
+    UPBP_REQUIRE(!(e & kBadLightIndex), UPBP_EINVAL,
+                 "UPBP inputs: camera range, light vertex, beam light vertex or path index out of range");
     UPBP_REQUIRE(st[kStatOverflow] == 0, UPBP_EDEVICE, "LBVH traversal stack overflow (tree deeper than 128)");
     (void)ctx;
 }
@@ -199,6 +203,7 @@ BeamArgs beam_args(const Beams* b) {
     BeamArgs a;
     a.nodes = b->bvh.nodes.as<NodeDev>();
     a.pieces = b->piece.as<PieceDev>();
+    a.piece_flux = b->piece_flux.as<float4>();
     a.piece_beam = b->piece_beam.as<uint32_t>();
     a.pieces_n = b->pieces;
     return a;
@@ -214,7 +219,7 @@ struct RunOut {
 
 void run_eval(Ctx& ctx, const Bre* bre, const Beams* bm, const upbp_query_batch* q, Mode mode, RunOut o,
               upbp_memory out_mem, upbp_eval_stats* stats, const AdjointOut* adj_ph, const AdjointOut* adj_bm,
-              bool force_sync) {
+              bool force_sync, const MisDev* mis = nullptr) {
     check_batch(ctx, q);
     if (bre) UPBP_REQUIRE(bre->ctx == &ctx, UPBP_EINVAL, "photon structure belongs to another context");
     if (bm) UPBP_REQUIRE(bm->ctx == &ctx, UPBP_EINVAL, "beam structure belongs to another context");
@@ -242,7 +247,7 @@ void run_eval(Ctx& ctx, const Bre* bre, const Beams* bm, const upbp_query_batch*
     uint32_t launches = 0;
     if (want_ph) {
         if (bre->n > 0) {
-            launch_photons(ctx, mode, photon_args(bre), v, dph, sph, adj_ph);
+            launch_photons(ctx, mode, photon_args(bre), v, dph, sph, adj_ph, mis);
             launches++;
         } else if (mode != Mode::Adjoint) {
             if (dph) launch_clear_f32(ctx, dph, 3 * Q);
@@ -252,7 +257,7 @@ void run_eval(Ctx& ctx, const Bre* bre, const Beams* bm, const upbp_query_batch*
     if (timing) UPBP_HIP(hipEventRecord(ctx.ev[2], ctx.stream));
     if (want_bm) {
         if (bm->pieces > 0) {
-            launch_beams(ctx, mode, beam_args(bm), v, dbm, sbm, adj_bm);
+            launch_beams(ctx, mode, beam_args(bm), v, dbm, sbm, adj_bm, mis);
             launches++;
         } else if (mode != Mode::Adjoint) {
             if (dbm) launch_clear_f32(ctx, dbm, 3 * Q);
@@ -453,6 +458,57 @@ upbp_status upbp_eval(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams* beam
     });
 }
 
+upbp_status upbp_eval_upbp(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams* beams,
+                           const upbp_query_batch* queries, const upbp_mis_inputs* mis, float* out_photons,
+                           float* out_beams, upbp_memory out_memory, upbp_eval_stats* stats) {
+    if (!ctx) return fail(nullptr, UPBP_EINVAL, "ctx is NULL");
+    return guarded(ctx, [&] {
+        check_batch(*ctx, queries);
+        UPBP_REQUIRE(mis != nullptr, UPBP_EINVAL, "mis inputs are required");
+        UPBP_REQUIRE(!bre || out_photons, UPBP_EINVAL, "out_photons is NULL but a photon structure was given");
+        UPBP_REQUIRE(!beams || out_beams, UPBP_EINVAL, "out_beams is NULL but a beam structure was given");
+        const int64_t Q = queries->count;
+        UPBP_REQUIRE(Q == 0 || mis->queries, UPBP_EINVAL, "mis->queries is required");
+        UPBP_REQUIRE(mis->camera_count > 0 && mis->camera, UPBP_EINVAL, "mis->camera is required");
+        UPBP_REQUIRE(mis->light_vertex_count >= 0 && (mis->light_vertex_count == 0 || mis->light_vertices),
+                     UPBP_EINVAL, "mis->light_vertices is required");
+        UPBP_REQUIRE(mis->path_count >= 0 && (mis->path_count == 0 || mis->path_ends), UPBP_EINVAL,
+                     "mis->path_ends is required");
+        UPBP_REQUIRE(!beams || beams->n == 0 || mis->beams, UPBP_EINVAL, "mis->beams is required with beams");
+        UPBP_REQUIRE(!bre || bre->n_in <= mis->light_vertex_count, UPBP_EINVAL,
+                     "the photon structure must be built from the light vertices (photon i = light vertex i)");
+        const upbp_mis_params& P = mis->params;
+        UPBP_REQUIRE(P.query_beam_type == queries->beam_type, UPBP_EINVAL,
+                     "params.query_beam_type must equal the batch beam_type");
+        UPBP_REQUIRE(P.pb2d_factor != 0.f || !bre, UPBP_EINVAL, "pb2d_factor must be nonzero with photons");
+        UPBP_REQUIRE(P.bb1d_factor != 0.f || !beams, UPBP_EINVAL, "bb1d_factor must be nonzero with beams");
+        MisDev d;
+        d.params = P;
+        // staging slots 0..3 hold the query batch; 4..7 the outputs; MIS arrays get their own buffers
+        DevBuf sq, sc, sl, sp, sb;
+        auto stage_mis = [&](DevBuf& buf, const void* src, size_t bytes) -> const void* {
+            if (!src || mis->memory == UPBP_MEM_DEVICE) return src;
+            buf.ensure(std::max<size_t>(bytes, 16));
+            UPBP_HIP(hipMemcpyAsync(buf.p, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+            return buf.p;
+        };
+        d.queries = (const upbp_query_mis*)stage_mis(sq, mis->queries, (size_t)Q * sizeof(upbp_query_mis));
+        d.camera = (const upbp_mis_data*)stage_mis(sc, mis->camera, (size_t)mis->camera_count * sizeof(upbp_mis_data));
+        d.camera_count = mis->camera_count;
+        d.lv = (const upbp_light_vertex*)stage_mis(sl, mis->light_vertices,
+                                                   (size_t)mis->light_vertex_count * sizeof(upbp_light_vertex));
+        d.lv_count = mis->light_vertex_count;
+        d.path_ends = (const int32_t*)stage_mis(sp, mis->path_ends, (size_t)mis->path_count * 4);
+        d.path_count = mis->path_count;
+        d.beams = (const upbp_beam_mis*)stage_mis(sb, mis->beams,
+                                                  beams ? (size_t)beams->n * sizeof(upbp_beam_mis) : 0);
+        RunOut o;
+        o.out_ph = bre ? out_photons : nullptr;
+        o.out_bm = beams ? out_beams : nullptr;
+        run_eval(*ctx, bre, beams, queries, Mode::Upbp, o, out_memory, stats, nullptr, nullptr, true, &d);
+    });
+}
+
 upbp_status upbp_hit_signatures(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams* beams,
                                 const upbp_query_batch* queries, uint32_t* photon_sig, uint32_t* beam_sig,
                                 upbp_memory out_memory) {
@@ -576,6 +632,7 @@ upbp_status upbp_batch_plan_create(upbp_ctx* ctx, const upbp_instance* instances
             if (d.has_beams) {
                 d.bm_nodes = m->bvh.nodes.as<NodeDev>();
                 d.bm_pieces = m->piece.as<PieceDev>();
+                d.bm_flux = m->piece_flux.as<float4>();
             }
             d.origin = in.queries.origin;
             d.direction = in.queries.direction;

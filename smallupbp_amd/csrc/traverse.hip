@@ -20,6 +20,7 @@
 #include <algorithm>
 
 #include "upbp_internal.hpp"
+#include "upbp_mis.hpp"
 
 namespace upbp {
 
@@ -38,6 +39,7 @@ struct EvalArgs {
     const float4* __restrict__ inc;
     const uint32_t* __restrict__ orig;  // photon original index / piece -> beam index
     const PieceDev* __restrict__ pieces;
+    const float4* __restrict__ piece_flux;
     // queries
     const float* __restrict__ origin;
     const float* __restrict__ direction;
@@ -59,6 +61,9 @@ struct EvalArgs {
     const float* __restrict__ grad_out;
     float* __restrict__ grad_media;  // [M][6]
     float* __restrict__ grad_items;  // [items][3]
+    // UPBP mode
+    const upbp_query_mis* __restrict__ qmis;
+    MisGlobals mis;
     // persistent batching
     const InstanceDev* __restrict__ inst;
     int32_t n_inst;
@@ -76,6 +81,7 @@ struct Ray {
     float ox, oy, oz, dx, dy, dz;
     float ix, iy, iz;     // guarded reciprocal direction
     float oix, oiy, oiz;  // o * inv
+    float d1o1;           // dot3(d, o), loop invariant of the beam test
 };
 
 __device__ __forceinline__ bool slab(const Ray& r, float tmin, float tmax, float lx, float ly, float lz,
@@ -99,6 +105,7 @@ struct SegState {
     int32_t iso;
     int32_t kidx;
     int32_t medium;     // medium id (instance-local, for the beam medium test)
+    float min_comp, mfp, cont;  // UPBP mode: min sigma_t component, mean free path, continuation
     float acc0, acc1, acc2;
     // adjoint per segment: w (upstream) * prefix, and accumulators
     float w0, w1, w2;
@@ -110,6 +117,8 @@ template <int MODE>
 struct LaneOut {
     uint32_t hits = 0;
     uint32_t sig_n = 0, sig_h = 0;
+    uint32_t err = 0;
+    MisLane ml;  // UPBP mode only
 };
 
 __device__ __forceinline__ float pick3(int k, float a, float b, float c) { return k == 0 ? a : (k == 1 ? b : c); }
@@ -140,166 +149,363 @@ __device__ __forceinline__ float wave_sum_dpp(float v) {
 }
 
 // ---- leaf visitors ---------------------------------------------------------
+// Exponent of the transmittance of channel c along `dist` with the SHORT
+// normalisation T_c / T_k folded in: exp(-st_c d + st_k d) = T_c(d) / T_k(d).
+__device__ __forceinline__ float neg_tau(float st_c, float st_k, float d, bool short_norm) {
+    return short_norm ? (st_k * d - st_c * d) : -(st_c * d);
+}
+
+// Sums the three channels' item gradients over the wave (DPP) and adds them once.
+__device__ __forceinline__ void add_item_gradient(float* grad, uint32_t o, float g0, float g1, float g2) {
+    const float r0 = wave_sum_dpp(g0), r1 = wave_sum_dpp(g1), r2 = wave_sum_dpp(g2);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        unsafeAtomicAdd(&grad[3 * o + 0], r0);
+        unsafeAtomicAdd(&grad[3 * o + 1], r1);
+        unsafeAtomicAdd(&grad[3 * o + 2], r2);
+    }
+}
+
+__device__ __forceinline__ MisF merge_vertex_mis(const upbp_mis_params& P, float pdf_a_inv, float ray_pdf_inv,
+                                                 float ratio, float bb1d_sel, uint32_t flags) {
+    MisF m = {};
+    m.pdf_a_inv = pdf_a_inv;
+    m.ray_sample_pdf_inv = ray_pdf_inv;
+    m.ray_sample_pdfs_ratio = ratio;
+    m.surf = 0.f;
+    m.pp3d = P.pp3d_factor;
+    m.pb2d = P.pb2d_factor;
+    m.bb1d = P.bb1d_factor;
+    m.bb1d_sel = bb1d_sel;
+    m.flags = flags;
+    return m;
+}
+
+// breIntersectFuncHomogeneous2 after the disc test (Bre.cxx:305-463), per lane; the
+// photon's light vertex is read by index (lanes hit different photons only through
+// the same wave-uniform photon, so this is one gather per hit).
+__device__ void photon_hit_mis(const EvalArgs& A, const Ray& r, SegState& s, const MisLane& ml, float t, float sq,
+                               float r2, uint32_t lv_idx, bool short_q, uint32_t& err) {
+    const MisGlobals& G = A.mis;
+    const upbp_mis_params& P = G.p;
+    if ((int64_t)lv_idx >= G.lv_count) {
+        err |= kBadLightIndex;
+        return;
+    }
+    const upbp_light_vertex* __restrict__ lv = G.lv + lv_idx;
+    const uint32_t path_len = lv->path_length;
+    const uint32_t cam_len = (uint32_t)ml.cam_len;
+    if ((path_len + cam_len > P.max_path_length) || (path_len + cam_len < P.min_path_length)) return;
+    if (cam_len == 1 && t < P.min_dist_to_med) return;
+    const float dt = t - s.tmin;
+    float a0 = fast_exp(-s.st0 * dt), a1 = fast_exp(-s.st1 * dt), a2 = fast_exp(-s.st2 * dt);
+    const float pdf = pick3(s.kidx, a0, a1, a2);
+    if (short_q) {
+        a0 = a0 / pdf;
+        a1 = a1 / pdf;
+        a2 = a2 / pdf;
+    }
+    float ray_pdf = s.min_comp * pdf;
+    float ray_rev = (ml.seg_flags & UPBP_ORIGIN_IN_MEDIUM) ? ray_pdf : pdf;
+    const float ratio = 1.0f / s.min_comp;
+    if (!(a0 > 1e-20f && a1 > 1e-20f && a2 > 1e-20f)) return;
+    ray_pdf *= ml.pref_pdf;
+    ray_rev *= ml.pref_rev;
+    // PhaseFunction::Evaluate(-ray.dir, lightDirection): cos = -dot(gen, fix) = dot(dir_fix, d)
+    const float lx = lv->dir_fix[0], ly = lv->dir_fix[1], lz = lv->dir_fix[2];
+    const float cos_t = -dot3(lx, ly, lz, -r.dx, -r.dy, -r.dz);
+    const float bsdf = phase_pdf(s.g, s.iso != 0, cos_t);
+    const float sin_t = __builtin_sqrtf(fmaxf(0.f, 1.f - cos_t * cos_t));
+    if (bsdf <= 0.f) return;
+    const float dir_pdf = bsdf * s.cont;
+    const float rev_pdf = bsdf * lv->continuation_prob;
+    const float kernel = (1.0f - sq / r2) / ((r2 * kPi) * 0.5f);
+    if (!(kernel > 1e-20f)) return;
+    const float u0 = (((lv->throughput[0] * a0) * s.ss0) * bsdf) * kernel;
+    const float u1 = (((lv->throughput[1] * a1) * s.ss1) * bsdf) * kernel;
+    const float u2 = (((lv->throughput[2] * a2) * s.ss2) * bsdf) * kernel;
+    if (u0 <= 0.f && u1 <= 0.f && u2 <= 0.f) return;
+    const float dist_sq = t * t;
+    const float ray_pdf_inv = 1.0f / ray_pdf;
+    CamView cv;
+    cv.base = ml.cam_base;
+    cv.len = ml.cam_len;
+    cv.last = merge_vertex_mis(P, ml.last_pdf_w_inv * dist_sq * ray_pdf_inv, ray_pdf_inv, ratio,
+                               P.has_bb1d_beams ? 1.f : 0.f, s.mfp > P.bb1d_min_mfp ? UPBP_MIS_IN_MEDIUM_WITH_BEAMS : 0u);
+    cv.prev_ray_rev_inv = 1.0f / ray_rev;
+    const float last = short_q ? (float)(1.0 / (double)(ratio * cv.last.pb2d)) : ray_pdf / cv.last.pb2d;
+    const uint32_t tech = A.short_query ? (UPBP_SHORT_BEAM | P.techniques) : (UPBP_LONG_BEAM | P.techniques);
+    const MisF lvm = mis_load(&lv->mis);
+    const float w_cam = camera_path_weight(cv, last, sin_t, lvm.ray_sample_pdf_inv, lvm.ray_sample_pdfs_ratio,
+                                           rev_pdf * ray_rev / dist_sq, P.query_beam_type, P.photon_beam_type, tech);
+    LightView lview{&G, nullptr, -1, 0.f};
+    const float w_light = light_path_weight(lview, lv->path_idx, (int)path_len, last, dir_pdf, UPBP_PB2D,
+                                            P.query_beam_type, P.photon_beam_type, tech, err);
+    const float w = 1.f / (w_light + w_cam);
+    s.acc0 += w * u0;
+    s.acc1 += w * u1;
+    s.acc2 += w * u2;
+}
+
+// PhotonBeam::accumulate2 after the beam-beam test (PhotonBeam.hxx:393-605). The beam,
+// its MIS record and its origin light vertex are wave-uniform (scalar loads).
+__device__ void beam_hit_mis(const EvalArgs& A, const Ray& r, SegState& s, const MisLane& ml, float dist,
+                             float sin_theta, float t1, float t2, float radius, bool short_b, float4 flux,
+                             float d2x, float d2y, float d2z, uint32_t beam_idx, bool short_q, uint32_t& err) {
+    const MisGlobals& G = A.mis;
+    const upbp_mis_params& P = G.p;
+    const upbp_beam_mis bm = G.beams[beam_idx];
+    if (bm.light_vertex < 0 || (int64_t)bm.light_vertex >= G.lv_count) {
+        err |= kBadLightIndex;
+        return;
+    }
+    const upbp_light_vertex* __restrict__ lv = G.lv + bm.light_vertex;
+    const uint32_t path_len = lv->path_length;
+    const uint32_t cam_len = (uint32_t)ml.cam_len;
+    if ((path_len + 1 + cam_len > P.max_path_length) || (path_len + 1 + cam_len < P.min_path_length)) return;
+    if (cam_len == 1 && t1 < P.min_dist_to_med) return;
+    const float sel_pdf = 1.0f;  // no per-cell beam reduction
+    const float dq = t1 - s.tmin;
+    float a0 = fast_exp(-s.st0 * dq), a1 = fast_exp(-s.st1 * dq), a2 = fast_exp(-s.st2 * dq);
+    const float pq = pick3(s.kidx, a0, a1, a2);
+    if (short_q) {
+        a0 = a0 / pq;
+        a1 = a1 / pq;
+        a2 = a2 / pq;
+    }
+    float rsp_q = s.min_comp * pq;
+    float rsr_q = (ml.seg_flags & UPBP_ORIGIN_IN_MEDIUM) ? rsp_q : pq;
+    const float ratio_q = 1.0f / s.min_comp;
+    const float e0 = fast_exp(-s.st0 * t2), e1 = fast_exp(-s.st1 * t2), e2 = fast_exp(-s.st2 * t2);
+    a0 *= e0;
+    a1 *= e1;
+    a2 *= e2;
+    const float pb = pick3(s.kidx, e0, e1, e2);
+    if (short_b) {
+        a0 = a0 / pb;
+        a1 = a1 / pb;
+        a2 = a2 / pb;
+    }
+    float rsp_b = s.min_comp * pb;
+    float rsr_b = (bm.ray_sampling_flags & UPBP_ORIGIN_IN_MEDIUM) ? rsp_b : pb;
+    const float ratio_b = 1.0f / s.min_comp;
+    if (!(a0 > 1e-20f && a1 > 1e-20f && a2 > 1e-20f)) return;
+    rsp_q *= ml.pref_pdf;
+    rsr_q *= ml.pref_rev;
+    rsp_b *= bm.ray_sample_pdf;
+    rsr_b *= bm.ray_sample_rev_pdf;
+    // PhaseFunction::Evaluate(-ray.direction, -beam.direction): cos = -dot(-d2, -d1)
+    const float cos_t = -dot3(-d2x, -d2y, -d2z, -r.dx, -r.dy, -r.dz);
+    const float bsdf = phase_pdf(s.g, s.iso != 0, cos_t);
+    if (bsdf <= 0.f) return;
+    const float dir_pdf = bsdf * s.cont;
+    const float rev_pdf = bsdf * s.cont;
+    const float k1 = (1.0f - dist * dist / (radius * radius)) * 3.0f;
+    const float den = (4.0f * radius) * sin_theta;
+    const float inv_sel = 1.0f / sel_pdf;
+    const float u0 = ((((((inv_sel * flux.x) * s.ss0) * a0) * bsdf) * k1)) / den;
+    const float u1 = ((((((inv_sel * flux.y) * s.ss1) * a1) * bsdf) * k1)) / den;
+    const float u2 = ((((((inv_sel * flux.z) * s.ss2) * a2) * bsdf) * k1)) / den;
+    if (u0 <= 0.f && u1 <= 0.f && u2 <= 0.f) return;
+    const float dist_sq_q = t1 * t1;
+    const float rsp_inv_q = 1.0f / rsp_q;
+    CamView cv;
+    cv.base = ml.cam_base;
+    cv.len = ml.cam_len;
+    cv.last = merge_vertex_mis(P, ml.last_pdf_w_inv * dist_sq_q * rsp_inv_q, rsp_inv_q, ratio_q, sel_pdf,
+                               UPBP_MIS_IN_MEDIUM_WITH_BEAMS);
+    cv.prev_ray_rev_inv = 1.0f / rsr_q;
+    const float dist_sq_b = lv->is_finite ? t2 * t2 : 1.0f;
+    const float rsp_inv_b = 1.0f / rsp_b;
+    const MisF bv = merge_vertex_mis(P, bm.last_pdf_w_inv * dist_sq_b * rsp_inv_b, rsp_inv_b, ratio_b, sel_pdf,
+                                     UPBP_MIS_IN_MEDIUM_WITH_BEAMS);
+    const uint32_t tech = A.short_query ? (UPBP_SHORT_BEAM | P.techniques) : (UPBP_LONG_BEAM | P.techniques);
+    const float sin_t = (tech & UPBP_NO_SINE_IN_WEIGHTS) ? 1.0f : sin_theta;
+    float last;
+    if (short_q)
+        last = short_b ? (float)(1.0 / (double)(ratio_q * ratio_b * cv.last.bb1d * sel_pdf * sin_t))
+                       : rsp_b / (ratio_q * cv.last.bb1d * sel_pdf * sin_t);
+    else
+        last = short_b ? rsp_q / (ratio_b * cv.last.bb1d * sel_pdf * sin_t)
+                       : rsp_q * rsp_b / (cv.last.bb1d * sel_pdf * sin_t);
+    const float w_cam = camera_path_weight(cv, last, sin_t, rsp_inv_b, ratio_b, rev_pdf * rsr_q / dist_sq_q,
+                                           P.query_beam_type, P.photon_beam_type, tech);
+    LightView lview{&G, &bv, (int64_t)bm.light_vertex, 1.0f / rsr_b};
+    const uint32_t lvf = lv->mis.flags;
+    const bool on_light_delta = (lvf & UPBP_MIS_ON_LIGHT_SOURCE) && (lvf & UPBP_MIS_DELTA);
+    const float ntl = on_light_delta ? 0.f : dir_pdf * rsr_b * __builtin_fabsf(lv->mis.cos_theta_out) / dist_sq_b;
+    const float w_light = light_path_weight(lview, lv->path_idx, (int)path_len + 1, last, ntl, UPBP_BB1D,
+                                            P.query_beam_type, P.photon_beam_type, tech, err);
+    const float w = 1.f / (w_light + w_cam);
+    s.acc0 += w * u0;
+    s.acc1 += w * u1;
+    s.acc2 += w * u2;
+}
+
 template <int MODE>
 __device__ __forceinline__ void photon_leaf(const EvalArgs& A, const Ray& r, SegState& s, LaneOut<MODE>& lo,
                                             uint32_t start, uint32_t cnt, bool short_q) {
-    for (uint32_t i = start; i < start + cnt; ++i) {
-        const float4 p = A.pos_r2[i];
+    const uint32_t end = start + cnt;
+    float4 p = A.pos_r2[start];
+    for (uint32_t i = start; i < end; ++i) {
+        // fetch the next photon while this one is tested (scalar loads, wave-uniform index)
+        float4 pn = p;
+        if (i + 1 < end) pn = A.pos_r2[i + 1];
         float t, sq;
         const bool hit = test_bre(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, s.tmin, s.tmax, p.x, p.y, p.z, p.w, t, sq);
-        float gp0 = 0.f, gp1 = 0.f, gp2 = 0.f;
-        if (hit) {
-            const float4 w = A.power[i];
-            const float dt = t - s.tmin;
-            float a0 = fast_exp(-s.st0 * dt), a1 = fast_exp(-s.st1 * dt), a2 = fast_exp(-s.st2 * dt);
-            if (short_q) {
-                const float ak = pick3(s.kidx, a0, a1, a2);
-                a0 = a0 / ak;
-                a1 = a1 / ak;
-                a2 = a2 / ak;
-            }
-            float ph;
-            if (s.iso) {
-                ph = kInvPi * 0.25f;
-            } else {
-                const float4 in = A.inc[i];
-                ph = phase_pdf(s.g, false, dot3(in.x, in.y, in.z, r.dx, r.dy, r.dz));
-            }
-            const float kern = 1.0f - sq / p.w;
-            const float den = (p.w * kPi) * 0.5f;
-            const float scale = (ph * kern) / den;
-            // base_c = a_c * scale; contribution = power_c * base_c * sigma_s_c
-            const float b0 = a0 * scale, b1 = a1 * scale, b2 = a2 * scale;
-            const float c0 = w.x * b0 * s.ss0, c1 = w.y * b1 * s.ss1, c2 = w.z * b2 * s.ss2;
-            if (MODE == (int)Mode::Adjoint) {
-                // w holds dL/dout * P (prefix transmittance)
-                gp0 = s.w0 * b0 * s.ss0;
-                gp1 = s.w1 * b1 * s.ss1;
-                gp2 = s.w2 * b2 * s.ss2;
-                s.gs0 += s.w0 * b0 * w.x;
-                s.gs1 += s.w1 * b1 * w.y;
-                s.gs2 += s.w2 * b2 * w.z;
-                s.gt0 -= s.w0 * c0 * dt;
-                s.gt1 -= s.w1 * c1 * dt;
-                s.gt2 -= s.w2 * c2 * dt;
-                s.acc0 += s.w0 * c0;
-                s.acc1 += s.w1 * c1;
-                s.acc2 += s.w2 * c2;
-            } else {
-                s.acc0 += c0;
-                s.acc1 += c1;
-                s.acc2 += c2;
-            }
-            lo.hits++;
-            if (MODE == (int)Mode::Signature) {
-                lo.sig_n++;
-                lo.sig_h += hash32(A.orig[i]);
-            }
-        }
-        if (MODE == (int)Mode::Adjoint) {
+        if (MODE == (int)Mode::Upbp) {
             if (__ballot(hit)) {
-                const float r0 = wave_sum_dpp(gp0), r1 = wave_sum_dpp(gp1), r2 = wave_sum_dpp(gp2);
-                if ((threadIdx.x & (kWave - 1)) == 0) {
-                    const uint32_t o = A.orig[i];
-                    unsafeAtomicAdd(&A.grad_items[3 * o + 0], r0);
-                    unsafeAtomicAdd(&A.grad_items[3 * o + 1], r1);
-                    unsafeAtomicAdd(&A.grad_items[3 * o + 2], r2);
+                const uint32_t lv_idx = A.orig[i];
+                if (hit) {
+                    photon_hit_mis(A, r, s, lo.ml, t, sq, p.w, lv_idx, short_q, lo.err);
+                    lo.hits++;
                 }
             }
+            p = pn;
+            continue;
         }
+        if (__ballot(hit)) {
+            const float4 w = A.power[i];  // (rgb, 1 / (r^2 pi / 2))
+            float gp0 = 0.f, gp1 = 0.f, gp2 = 0.f;
+            if (hit) {
+                const float dt = t - s.tmin;
+                const float stk = pick3(s.kidx, s.st0, s.st1, s.st2);
+                const float a0 = fast_exp(neg_tau(s.st0, stk, dt, short_q));
+                const float a1 = fast_exp(neg_tau(s.st1, stk, dt, short_q));
+                const float a2 = fast_exp(neg_tau(s.st2, stk, dt, short_q));
+                float ph;
+                if (s.iso) {
+                    ph = kInvPi * 0.25f;
+                } else {
+                    const float4 in = A.inc[i];
+                    ph = phase_pdf(s.g, false, dot3(in.x, in.y, in.z, r.dx, r.dy, r.dz));
+                }
+                // Epanechnikov kernel (1 - s/r^2) / (r^2 pi / 2), Bre.cxx:272-273
+                const float kern = 1.0f - sq / p.w;
+                const float scale = (ph * kern) * w.w;
+                // base_c = a_c * scale; contribution_c = power_c * base_c * sigma_s_c
+                const float b0 = a0 * scale, b1 = a1 * scale, b2 = a2 * scale;
+                const float c0 = w.x * b0 * s.ss0, c1 = w.y * b1 * s.ss1, c2 = w.z * b2 * s.ss2;
+                if (MODE == (int)Mode::Adjoint) {
+                    // s.w = dL/dout * P (prefix transmittance of earlier segments)
+                    gp0 = s.w0 * b0 * s.ss0;
+                    gp1 = s.w1 * b1 * s.ss1;
+                    gp2 = s.w2 * b2 * s.ss2;
+                    s.gs0 += s.w0 * b0 * w.x;
+                    s.gs1 += s.w1 * b1 * w.y;
+                    s.gs2 += s.w2 * b2 * w.z;
+                    s.gt0 -= s.w0 * c0 * dt;
+                    s.gt1 -= s.w1 * c1 * dt;
+                    s.gt2 -= s.w2 * c2 * dt;
+                    s.acc0 += s.w0 * c0;
+                    s.acc1 += s.w1 * c1;
+                    s.acc2 += s.w2 * c2;
+                } else {
+                    s.acc0 += c0;
+                    s.acc1 += c1;
+                    s.acc2 += c2;
+                }
+                lo.hits++;
+                if (MODE == (int)Mode::Signature) {
+                    lo.sig_n++;
+                    lo.sig_h += hash32(A.orig[i]);
+                }
+            }
+            if (MODE == (int)Mode::Adjoint) add_item_gradient(A.grad_items, A.orig[i], gp0, gp1, gp2);
+        }
+        p = pn;
     }
 }
 
 template <int MODE>
 __device__ __forceinline__ void beam_leaf(const EvalArgs& A, const Ray& r, SegState& s, LaneOut<MODE>& lo,
                                           uint32_t start, uint32_t cnt, bool short_q) {
-    for (uint32_t i = start; i < start + cnt; ++i) {
-        const PieceDev pc = A.pieces[i];
-        const uint32_t word = __float_as_uint(pc.p3.z);
+    const uint32_t end = start + cnt;
+    PieceDev pc = A.pieces[start];
+    for (uint32_t i = start; i < end; ++i) {
+        PieceDev pn = pc;
+        if (i + 1 < end) pn = A.pieces[i + 1];
+        const uint32_t word = __float_as_uint(pc.p2.z);
         const int32_t bmed = (int32_t)(word & kMediumMask);
         float dist, st, t1, t2;
         bool hit = (bmed == s.medium) &&
-                   test_beam_beam(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, s.tmin, s.tmax, pc.p0.x, pc.p0.y, pc.p0.z,
-                                  pc.p1.x, pc.p1.y, pc.p1.z, pc.p0.w, pc.p3.w, pc.p3.x, pc.p3.y, dist, st, t1, t2);
+                   test_beam_beam(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, r.d1o1, s.tmin, s.tmax, pc.p0.x, pc.p0.y,
+                                  pc.p0.z, pc.p1.x, pc.p1.y, pc.p1.z, pc.p3.x, pc.p0.w, pc.p2.w, pc.p2.x, pc.p2.y,
+                                  dist, st, t1, t2);
         float radius = 0.f;
         if (hit) {
-            radius = pc.p1.w + t2 * pc.p2.w;  // mStartRadius + t2 * mRadiusChange
+            radius = pc.p1.w + t2 * pc.p3.y;  // mStartRadius + t2 * mRadiusChange
             hit = !(radius < dist);
         }
-        float gb0 = 0.f, gb1 = 0.f, gb2 = 0.f;
-        if (hit) {
-            const float dq = t1 - s.tmin;
-            float a0 = fast_exp(-s.st0 * dq), a1 = fast_exp(-s.st1 * dq), a2 = fast_exp(-s.st2 * dq);
-            const float e0 = fast_exp(-s.st0 * t2), e1 = fast_exp(-s.st1 * t2), e2 = fast_exp(-s.st2 * t2);
-            if (short_q) {
-                const float ak = pick3(s.kidx, a0, a1, a2);
-                a0 = a0 / ak;
-                a1 = a1 / ak;
-                a2 = a2 / ak;
-            }
-            a0 *= e0;
-            a1 *= e1;
-            a2 *= e2;
-            const bool short_b = (word & kShortBeamBit) != 0;
-            if (short_b) {
-                const float bk = pick3(s.kidx, e0, e1, e2);
-                a0 = a0 / bk;
-                a1 = a1 / bk;
-                a2 = a2 / bk;
-            }
-            // phase: Evaluate(ray.direction, beam direction): cos = -dot(d2, d1)
-            const float ph = s.iso ? kInvPi * 0.25f
-                                   : phase_pdf(s.g, false, -dot3(pc.p1.x, pc.p1.y, pc.p1.z, r.dx, r.dy, r.dz));
-            const float kern = 1.0f - (dist * dist) / (radius * radius);
-            const float scale = (ph * kern * 3.0f) / ((4.0f * radius) * st);
-            const float b0 = a0 * scale, b1 = a1 * scale, b2 = a2 * scale;
-            const float c0 = pc.p2.x * s.ss0 * b0, c1 = pc.p2.y * s.ss1 * b1, c2 = pc.p2.z * s.ss2 * b2;
-            if (MODE == (int)Mode::Adjoint) {
-                gb0 = s.w0 * b0 * s.ss0;
-                gb1 = s.w1 * b1 * s.ss1;
-                gb2 = s.w2 * b2 * s.ss2;
-                s.gs0 += s.w0 * b0 * pc.p2.x;
-                s.gs1 += s.w1 * b1 * pc.p2.y;
-                s.gs2 += s.w2 * b2 * pc.p2.z;
-                const float g0 = s.w0 * c0, g1 = s.w1 * c1, g2 = s.w2 * c2;
-                // d log C_c / d sigma_t_c = -(t1 - tmin) - t2, except the SHORT-beam
-                // normalisation channel k (its t2 terms cancel) ...
-                const int k = s.kidx;
-                s.gt0 -= g0 * (dq + ((short_b && k == 0) ? 0.f : t2));
-                s.gt1 -= g1 * (dq + ((short_b && k == 1) ? 0.f : t2));
-                s.gt2 -= g2 * (dq + ((short_b && k == 2) ? 0.f : t2));
-                // ... which instead gains +t2 from every other channel of a SHORT beam
-                if (short_b) {
-                    const float other = (k == 0 ? 0.f : g0) + (k == 1 ? 0.f : g1) + (k == 2 ? 0.f : g2);
-                    if (k == 0) s.gt0 += other * t2;
-                    else if (k == 1) s.gt1 += other * t2;
-                    else s.gt2 += other * t2;
-                }
-                s.acc0 += g0;
-                s.acc1 += g1;
-                s.acc2 += g2;
-            } else {
-                s.acc0 += c0;
-                s.acc1 += c1;
-                s.acc2 += c2;
-            }
-            lo.hits++;
-            if (MODE == (int)Mode::Signature) {
-                lo.sig_n++;
-                lo.sig_h += hash32(A.orig[i]);
-            }
-        }
-        if (MODE == (int)Mode::Adjoint) {
+        if (MODE == (int)Mode::Upbp) {
             if (__ballot(hit)) {
-                const float r0 = wave_sum_dpp(gb0), r1 = wave_sum_dpp(gb1), r2 = wave_sum_dpp(gb2);
-                if ((threadIdx.x & (kWave - 1)) == 0) {
-                    const uint32_t o = A.orig[i];
-                    unsafeAtomicAdd(&A.grad_items[3 * o + 0], r0);
-                    unsafeAtomicAdd(&A.grad_items[3 * o + 1], r1);
-                    unsafeAtomicAdd(&A.grad_items[3 * o + 2], r2);
+                const float4 flux = A.piece_flux[i];
+                const uint32_t beam_idx = A.orig[i];
+                if (hit) {
+                    beam_hit_mis(A, r, s, lo.ml, dist, st, t1, t2, radius, (word & kShortBeamBit) != 0, flux, pc.p1.x,
+                                 pc.p1.y, pc.p1.z, beam_idx, short_q, lo.err);
+                    lo.hits++;
                 }
             }
+            pc = pn;
+            continue;
         }
+        if (__ballot(hit)) {
+            const float4 flux = A.piece_flux[i];
+            float gb0 = 0.f, gb1 = 0.f, gb2 = 0.f;
+            if (hit) {
+                const bool short_b = (word & kShortBeamBit) != 0;
+                const float dq = t1 - s.tmin;
+                const float stk = pick3(s.kidx, s.st0, s.st1, s.st2);
+                // T(t1 - tmin) [/ T_k] * T(t2) [/ T_k(t2)], one exponential per channel
+                const float a0 = fast_exp(neg_tau(s.st0, stk, dq, short_q) + neg_tau(s.st0, stk, t2, short_b));
+                const float a1 = fast_exp(neg_tau(s.st1, stk, dq, short_q) + neg_tau(s.st1, stk, t2, short_b));
+                const float a2 = fast_exp(neg_tau(s.st2, stk, dq, short_q) + neg_tau(s.st2, stk, t2, short_b));
+                // phase: Evaluate(ray.direction, beam direction), cos = -dot(d2, d1)
+                const float ph = s.iso ? kInvPi * 0.25f
+                                       : phase_pdf(s.g, false, -dot3(pc.p1.x, pc.p1.y, pc.p1.z, r.dx, r.dy, r.dz));
+                // 1D kernel (1 - dist^2/r^2) * 3 / (4 r sin(theta)), PhotonBeam.hxx:364-365
+                const float kern = 1.0f - (dist * dist) / (radius * radius);
+                const float scale = (ph * kern * 3.0f) * __builtin_amdgcn_rcpf((4.0f * radius) * st);
+                const float b0 = a0 * scale, b1 = a1 * scale, b2 = a2 * scale;
+                const float c0 = flux.x * s.ss0 * b0, c1 = flux.y * s.ss1 * b1, c2 = flux.z * s.ss2 * b2;
+                if (MODE == (int)Mode::Adjoint) {
+                    gb0 = s.w0 * b0 * s.ss0;
+                    gb1 = s.w1 * b1 * s.ss1;
+                    gb2 = s.w2 * b2 * s.ss2;
+                    s.gs0 += s.w0 * b0 * flux.x;
+                    s.gs1 += s.w1 * b1 * flux.y;
+                    s.gs2 += s.w2 * b2 * flux.z;
+                    const float g0 = s.w0 * c0, g1 = s.w1 * c1, g2 = s.w2 * c2;
+                    // d log C_c / d sigma_t_c = -(t1 - tmin) - t2, except the SHORT-beam
+                    // normalisation channel k (its t2 terms cancel) ...
+                    const int k = s.kidx;
+                    s.gt0 -= g0 * (dq + ((short_b && k == 0) ? 0.f : t2));
+                    s.gt1 -= g1 * (dq + ((short_b && k == 1) ? 0.f : t2));
+                    s.gt2 -= g2 * (dq + ((short_b && k == 2) ? 0.f : t2));
+                    // ... which instead gains +t2 from every other channel of a SHORT beam
+                    if (short_b) {
+                        const float other = (k == 0 ? 0.f : g0) + (k == 1 ? 0.f : g1) + (k == 2 ? 0.f : g2);
+                        if (k == 0) s.gt0 += other * t2;
+                        else if (k == 1) s.gt1 += other * t2;
+                        else s.gt2 += other * t2;
+                    }
+                    s.acc0 += g0;
+                    s.acc1 += g1;
+                    s.acc2 += g2;
+                } else {
+                    s.acc0 += c0;
+                    s.acc1 += c1;
+                    s.acc2 += c2;
+                }
+                lo.hits++;
+                if (MODE == (int)Mode::Signature) {
+                    lo.sig_n++;
+                    lo.sig_h += hash32(A.orig[i]);
+                }
+            }
+            if (MODE == (int)Mode::Adjoint) add_item_gradient(A.grad_items, A.orig[i], gb0, gb1, gb2);
+        }
+        pc = pn;
     }
 }
 
@@ -367,6 +573,9 @@ __device__ __forceinline__ bool finite3(float x, float y, float z) {
     return __builtin_isfinite(x) && __builtin_isfinite(y) && __builtin_isfinite(z);
 }
 
+__device__ __forceinline__ float s_tmin_of(const EvalArgs& A, int64_t i) { return A.segments[i].dist_min; }
+__device__ __forceinline__ float s_tmax_of(const EvalArgs& A, int64_t i) { return A.segments[i].dist_max; }
+
 // Processes the 64 queries of one tile (wave). `slot0` is the first lane slot.
 template <int EST, int MODE>
 __device__ void run_tile(const EvalArgs& A, int64_t slot0, unsigned long long* local_stats,
@@ -406,6 +615,7 @@ __device__ void run_tile(const EvalArgs& A, int64_t slot0, unsigned long long* l
     r.oix = r.ox * r.ix;
     r.oiy = r.oy * r.iy;
     r.oiz = r.oz * r.iz;
+    r.d1o1 = dot3(r.dx, r.dy, r.dz, r.ox, r.oy, r.oz);
     int nseg = alive ? (int)(s1 - s0) : 0;
     int maxseg = nseg;
     for (int off = 32; off > 0; off >>= 1) maxseg = max(maxseg, __shfl_xor(maxseg, off, kWave));
@@ -421,6 +631,30 @@ __device__ void run_tile(const EvalArgs& A, int64_t slot0, unsigned long long* l
         W2 = A.grad_out[3 * qi + 2];
     }
     LaneOut<MODE> lo;
+    if (MODE == (int)Mode::Upbp) {
+        // per-query AdditionalRayDataForMis (Bre.cxx:842-849 runs it per segment below)
+        lo.ml.cam_base = A.mis.camera;
+        lo.ml.cam_len = 1;
+        lo.ml.ray_flags = 0;
+        lo.ml.last_pdf_w_inv = 1.f;
+        lo.ml.pref_pdf = 1.f;
+        lo.ml.pref_rev = 1.f;
+        lo.ml.seg_flags = UPBP_END_IN_MEDIUM;
+        if (valid) {
+            const upbp_query_mis qm = A.qmis[qi];
+            if (qm.camera_path_length < 1 || qm.camera_offset < 0 ||
+                qm.camera_offset + (int64_t)qm.camera_path_length >= A.mis.camera_count) {
+                err |= kBadLightIndex;
+                alive = false;
+                nseg = 0;
+            } else {
+                lo.ml.cam_base = A.mis.camera + qm.camera_offset;
+                lo.ml.cam_len = (int)qm.camera_path_length;
+                lo.ml.ray_flags = qm.ray_sampling_flags;
+                lo.ml.last_pdf_w_inv = qm.last_pdf_w_inv;
+            }
+        }
+    }
     uint64_t nodes_visited = 0, tests = 0;
     uint32_t overflow = 0;
     for (int k = 0; k < maxseg; ++k) {
@@ -478,7 +712,12 @@ __device__ void run_tile(const EvalArgs& A, int64_t slot0, unsigned long long* l
                     s.g = m->g;
                     s.iso = m->isotropic;
                     s.kidx = m->min_idx;
+                    s.min_comp = m->min_comp;
+                    s.mfp = m->mfp;
+                    s.cont = m->cont;
                 }
+                if (MODE == (int)Mode::Upbp)
+                    lo.ml.seg_flags = UPBP_END_IN_MEDIUM | (k == 0 ? lo.ml.ray_flags : 0u);
                 if (A.lite) {
                     const float len = tmax - tmin;
                     a0 = fast_exp(-m->st[0] * len);
@@ -547,8 +786,27 @@ __device__ void run_tile(const EvalArgs& A, int64_t slot0, unsigned long long* l
                 P2 *= a2 / pdf;
             }
             if (!(P0 > 1e-20f && P1 > 1e-20f && P2 > 1e-20f)) alive = false;
+            if (MODE == (int)Mode::Upbp) {
+                // ray-sampling pdf prefixes of the segment loop (Bre.cxx:871-873, lite :955-958)
+                if (A.lite) {
+                    float rev = 1.f, spdf = 1.f;
+                    if (m->min_comp != 0.f && m->has_scattering) {
+                        float att = fmaxf(__expf(-m->min_comp * (s_tmax_of(A, s0 + k) - s_tmin_of(A, s0 + k))), 1e-35f);
+                        const float minatt = m->min_comp * att;
+                        const uint32_t f = k == 0 ? lo.ml.ray_flags : 0u;
+                        spdf = (f & UPBP_END_IN_MEDIUM) ? minatt : att;
+                        rev = (f & UPBP_ORIGIN_IN_MEDIUM) ? minatt : att;
+                    }
+                    lo.ml.pref_pdf *= spdf;
+                    lo.ml.pref_rev *= rev;
+                } else {
+                    lo.ml.pref_pdf *= pdf;
+                    lo.ml.pref_rev *= A.segments[s0 + k].ray_sample_rev_pdf;
+                }
+            }
         }
     }
+    err |= lo.err;
     if (valid && MODE != (int)Mode::Adjoint) {
         if (MODE == (int)Mode::Signature) {
             A.sig[2 * qi] = lo.sig_n;
@@ -647,6 +905,7 @@ __global__ __launch_bounds__(256) void k_eval_batched(EvalArgs A) {
         B.power = I.ph_power;
         B.inc = I.ph_inc;
         B.pieces = I.bm_pieces;
+        B.piece_flux = I.bm_flux;
         B.origin = I.origin;
         B.direction = I.direction;
         B.offsets = I.offsets;
@@ -732,6 +991,9 @@ void launch_mode(Ctx& ctx, Mode mode, const EvalArgs& A, unsigned blocks, unsign
         case Mode::Adjoint:
             hipLaunchKernelGGL((k_eval<EST, (int)Mode::Adjoint>), dim3(blocks), dim3(threads), 0, ctx.stream, A);
             break;
+        case Mode::Upbp:
+            hipLaunchKernelGGL((k_eval<EST, (int)Mode::Upbp>), dim3(blocks), dim3(threads), 0, ctx.stream, A);
+            break;
     }
     UPBP_HIP(hipGetLastError());
 }
@@ -755,10 +1017,24 @@ EvalArgs base_args(Ctx& ctx, const QueryView& q) {
 
 }  // namespace
 
+static void set_mis(EvalArgs& A, const MisDev* mis) {
+    if (!mis) return;
+    A.qmis = mis->queries;
+    A.mis.camera = mis->camera;
+    A.mis.camera_count = mis->camera_count;
+    A.mis.lv = mis->lv;
+    A.mis.lv_count = mis->lv_count;
+    A.mis.path_ends = mis->path_ends;
+    A.mis.path_count = mis->path_count;
+    A.mis.beams = mis->beams;
+    A.mis.p = mis->params;
+}
+
 void launch_photons(Ctx& ctx, Mode mode, const PhotonArgs& ph, const QueryView& q, float* out, uint32_t* sig,
-                    const AdjointOut* adj) {
+                    const AdjointOut* adj, const MisDev* mis) {
     if (q.count == 0) return;
     EvalArgs A = base_args(ctx, q);
+    set_mis(A, mis);
     A.nodes = ph.nodes;
     A.pos_r2 = ph.pos_r2;
     A.power = ph.power;
@@ -777,11 +1053,13 @@ void launch_photons(Ctx& ctx, Mode mode, const PhotonArgs& ph, const QueryView& 
 }
 
 void launch_beams(Ctx& ctx, Mode mode, const BeamArgs& bm, const QueryView& q, float* out, uint32_t* sig,
-                  const AdjointOut* adj) {
+                  const AdjointOut* adj, const MisDev* mis) {
     if (q.count == 0) return;
     EvalArgs A = base_args(ctx, q);
+    set_mis(A, mis);
     A.nodes = bm.nodes;
     A.pieces = bm.pieces;
+    A.piece_flux = bm.piece_flux;
     A.orig = bm.piece_beam;
     A.out = out;
     A.sig = sig;

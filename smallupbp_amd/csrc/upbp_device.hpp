@@ -18,6 +18,7 @@ constexpr float kInvPi = 1.f / kPi;                         // INV_PI_F, Defs.hx
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;
 constexpr uint32_t kShortBeamBit = 1u << 24;  // piece word: medium id in bits 0-23, SHORT flag bit 24
 constexpr uint32_t kMediumMask = 0x00FFFFFFu;
+constexpr uint32_t kBadLightIndex = 16u;  // error bit: MIS light-vertex / path index out of range
 
 // Device medium, 64 bytes. Derived like HomogeneousMedium's constructor
 // (SmallUPBP/src/Scene/Media.hxx:129-159).
@@ -31,7 +32,7 @@ struct alignas(16) MediumDev {
     int32_t min_idx;   // index of min_comp
     int32_t has_scattering;
     int32_t isotropic; // |g| < 1e-3 (Media.hxx:111)
-    int32_t pad;
+    float mfp;         // mean free path 1 / max sigma_t, or +inf (Media.hxx:158)
 };
 static_assert(sizeof(MediumDev) == 64, "MediumDev layout");
 
@@ -45,11 +46,12 @@ struct alignas(16) NodeDev {
 };
 static_assert(sizeof(NodeDev) == 64, "NodeDev layout");
 
-// Photon record in leaf order: pos_r2 = (pos.xyz, r^2), power = (rgb, unused),
+// Photon record in leaf order: pos_r2 = (pos.xyz, r^2), power = (rgb, 1 / (r^2 pi / 2)),
 // inc = (WorldDirFix.xyz, unused) read only for anisotropic media.
-// Beam piece record (64 bytes) in leaf order:
+// Beam piece record (64 bytes) in leaf order; the throughput lives in a separate
+// array (piece_flux) because it is read only when some lane of the wave hits:
 //   p0 = (origin.xyz, length)  p1 = (dir.xyz, start radius)
-//   p2 = (throughput.rgb, radius change)  p3 = (t_lo, t_hi, word, r^2 max)
+//   p2 = (t_lo, t_hi, word, r^2 max)  p3 = (dot(dir, origin), radius change, 0, 0)
 struct alignas(16) PieceDev {
     float4 p0, p1, p2, p3;
 };
@@ -81,12 +83,13 @@ __device__ __forceinline__ bool test_bre(float ox, float oy, float oz, float dx,
 
 // PhotonBeam::testIntersectionBeamBeam (PhotonBeam.hxx:238-296) with
 // minT2 = 0 and maxT2 = length, plus the piece's half-open range [t_lo, t_hi)
-// that makes a beam cut into pieces count once.
+// that makes a beam cut into pieces count once. d1o1 = dot3(d1, o1) and
+// d2o2 = dot3(d2, o2) are passed in precomputed (same operations, same values).
 __device__ __forceinline__ bool test_beam_beam(float o1x, float o1y, float o1z, float d1x, float d1y,
-                                               float d1z, float min_t1, float max_t1, float o2x,
+                                               float d1z, float d1o1, float min_t1, float max_t1, float o2x,
                                                float o2y, float o2z, float d2x, float d2y, float d2z,
-                                               float len, float max_dist_sqr, float t_lo, float t_hi,
-                                               float& dist, float& sin_theta, float& t1_out,
+                                               float d2o2, float len, float max_dist_sqr, float t_lo,
+                                               float t_hi, float& dist, float& sin_theta, float& t1_out,
                                                float& t2_out) {
     const float cx = d1y * d2z - d1z * d2y;
     const float cy = d1z * d2x - d1x * d2z;
@@ -97,11 +100,8 @@ __device__ __forceinline__ bool test_beam_beam(float o1x, float o1y, float o1z, 
     const float q = dot3(d1x, d1y, d1z, d2x, d2y, d2z);
     const float q2m1 = q * q - 1.0f;
     if (q2m1 < 1e-5f && q2m1 > -1e-5f) return false;
-    const float d1o1 = dot3(d1x, d1y, d1z, o1x, o1y, o1z);
     const float d1o2 = dot3(d1x, d1y, d1z, o2x, o2y, o2z);
-    const float t1 =
-        ((d1o1 - d1o2) - q * (dot3(d2x, d2y, d2z, o1x, o1y, o1z) - dot3(d2x, d2y, d2z, o2x, o2y, o2z))) /
-        q2m1;
+    const float t1 = ((d1o1 - d1o2) - q * (dot3(d2x, d2y, d2z, o1x, o1y, o1z) - d2o2)) / q2m1;
     if (t1 <= min_t1 || t1 >= max_t1) return false;
     const float t2 = ((t1 + d1o1) - d1o2) / q;
     if (t2 <= 0.0f || t2 >= len || __builtin_isnan(t2)) return false;

@@ -117,6 +117,7 @@ struct Beams {
     int64_t pieces = 0;   // beam pieces in the LBVH
     Lbvh bvh;
     DevBuf piece;         // PieceDev [pieces]
+    DevBuf piece_flux;    // float4 [pieces] throughput (rgb, 0)
     DevBuf piece_beam;    // uint32 [pieces] original beam index
 };
 
@@ -149,7 +150,7 @@ void bre_set_power(Bre& bre, const float* power_dev);
 void beams_build(Ctx& ctx, const upbp_beam_input& in, const upbp_build_params& params, Beams& out);
 
 // ---- traverse.hip ---------------------------------------------------------
-enum class Mode : int { Forward = 0, Signature = 1, Adjoint = 2 };
+enum class Mode : int { Forward = 0, Signature = 1, Adjoint = 2, Upbp = 3 };
 
 struct PhotonArgs {
     const NodeDev* nodes;
@@ -162,6 +163,7 @@ struct PhotonArgs {
 struct BeamArgs {
     const NodeDev* nodes;
     const PieceDev* pieces;
+    const float4* piece_flux;
     const uint32_t* piece_beam;
     int64_t pieces_n;
 };
@@ -171,10 +173,23 @@ struct AdjointOut {
     float* grad_items;      // photon [n_in][3] or beam [n][3]
 };
 
+// UPBP mode inputs on the device (upbp_mis_inputs after staging).
+struct MisDev {
+    const upbp_query_mis* queries;
+    const upbp_mis_data* camera;
+    int64_t camera_count;
+    const upbp_light_vertex* lv;
+    int64_t lv_count;
+    const int32_t* path_ends;
+    int64_t path_count;
+    const upbp_beam_mis* beams;
+    upbp_mis_params params;
+};
+
 void launch_photons(Ctx& ctx, Mode mode, const PhotonArgs& ph, const QueryView& q, float* out,
-                    uint32_t* sig, const AdjointOut* adj);
+                    uint32_t* sig, const AdjointOut* adj, const MisDev* mis = nullptr);
 void launch_beams(Ctx& ctx, Mode mode, const BeamArgs& bm, const QueryView& q, float* out,
-                  uint32_t* sig, const AdjointOut* adj);
+                  uint32_t* sig, const AdjointOut* adj, const MisDev* mis = nullptr);
 void launch_query_keys(Ctx& ctx, const QueryView& q, const float* lo, const float* hi, uint32_t* keys,
                        uint32_t* idx);
 void launch_clear_u64(Ctx& ctx, uint64_t* p, int64_t n);
@@ -189,6 +204,7 @@ struct InstanceDev {
     const float4* ph_inc;
     const NodeDev* bm_nodes;
     const PieceDev* bm_pieces;
+    const float4* bm_flux;
     const float* origin;
     const float* direction;
     const int64_t* offsets;

@@ -1,0 +1,192 @@
+// upbp_mis.hpp -- device side of the UPBP multiple-importance-sampling weights.
+//
+// Restates, per hit and in registers, the MIS part of breIntersectFuncHomogeneous2
+// (SmallUPBP/src/Bre/Bre.cxx:293-464) and PhotonBeam::accumulate2
+// (SmallUPBP/src/Beams/PhotonBeam.hxx:386-607) with the two path-weight walks
+// AccumulateCameraPathWeight / AccumulateLightPathWeight
+// (SmallUPBP/src/Path/PathWeight.hxx:33-270). The reference writes the hit's MIS
+// data into the caller's camera array and temporarily into the beam's origin light
+// vertex; here those values are register overrides of the walks, so no input is
+// ever written and concurrent lanes never race.
+#pragma once
+
+#include "upbp_device.hpp"
+#include "../../include/upbp.h"
+
+namespace upbp {
+
+struct MisF {
+    float pdf_a_inv, rev_pdf_a, rev_pdf_a_without_bsdf, ray_sample_pdf_inv, ray_sample_rev_pdf_inv,
+        ray_sample_pdfs_ratio, ray_sample_rev_pdfs_ratio, sin_theta, cos_theta_out, surf, pp3d, pb2d, bb1d, bb1d_sel;
+    uint32_t flags;
+};
+
+__device__ __forceinline__ MisF mis_load(const upbp_mis_data* __restrict__ p) {
+    const float4* v = reinterpret_cast<const float4*>(p);
+    const float4 a = v[0], b = v[1], c = v[2], d = v[3];
+    MisF m;
+    m.pdf_a_inv = a.x;
+    m.rev_pdf_a = a.y;
+    m.rev_pdf_a_without_bsdf = a.z;
+    m.ray_sample_pdf_inv = a.w;
+    m.ray_sample_rev_pdf_inv = b.x;
+    m.ray_sample_pdfs_ratio = b.y;
+    m.ray_sample_rev_pdfs_ratio = b.z;
+    m.sin_theta = b.w;
+    m.cos_theta_out = c.x;
+    m.surf = c.y;
+    m.pp3d = c.z;
+    m.pb2d = c.w;
+    m.bb1d = d.x;
+    m.bb1d_sel = d.y;
+    m.flags = __float_as_uint(d.z);
+    return m;
+}
+
+struct MisGlobals {
+    const upbp_mis_data* __restrict__ camera;
+    int64_t camera_count;
+    const upbp_light_vertex* __restrict__ lv;
+    int64_t lv_count;
+    const int32_t* __restrict__ path_ends;
+    int64_t path_count;
+    const upbp_beam_mis* __restrict__ beams;
+    upbp_mis_params p;
+};
+
+// Camera sub-path of one query with the per-hit entry [len] and the override of
+// entry [len-1].mRaySampleRevPdfInv (Bre.cxx:396-422, PhotonBeam.hxx:502-521).
+struct CamView {
+    const upbp_mis_data* base;
+    int len;
+    MisF last;
+    float prev_ray_rev_inv;
+};
+
+__device__ __forceinline__ MisF cam_get(const CamView& c, int i) {
+    if (i == c.len) return c.last;
+    MisF m = mis_load(c.base + i);
+    if (i == c.len - 1) m.ray_sample_rev_pdf_inv = c.prev_ray_rev_inv;
+    return m;
+}
+
+// AccumulateCameraPathWeight, PathWeight.hxx:33-142.
+__device__ inline float camera_path_weight(const CamView& cam, float last_rev, float last_sin, float last_ray_rev_inv,
+                                           float last_ray_rev_ratio, float next_to_last_partial, uint32_t qbt,
+                                           uint32_t pbt, uint32_t tech) {
+    float weight = 0.f, product = 1.f, weight_bb1d = 0.f;
+    const int len = cam.len;
+    MisF cur = cam_get(cam, len);
+    for (int index = 0; index < len; ++index) {
+        const MisF next = index < len - 1 ? cam_get(cam, len - index - 1) : cur;
+        const bool cur_spec = (cur.flags & UPBP_MIS_SPECULAR) != 0;
+        if ((tech & UPBP_BB1D_PREVIOUS) && !cur_spec) weight_bb1d = 0.f;
+        float rev = cur.rev_pdf_a, sin_t = cur.sin_theta, ray_rev = cur.ray_sample_rev_pdf_inv,
+              ray_rev_ratio = cur.ray_sample_rev_pdfs_ratio;
+        if (index == 0) {
+            rev = last_rev;
+            sin_t = last_sin;
+            ray_rev = last_ray_rev_inv;
+            ray_rev_ratio = last_ray_rev_ratio;
+        } else if (index == 1) {
+            rev *= next_to_last_partial;
+        }
+        product *= rev;
+        const bool usable = index == 0 || !cur_spec;
+        if ((tech & UPBP_SURF) && usable) weight += product * cur.surf;
+        if (tech & UPBP_PP3D) weight += product * cur.pp3d;
+        if (tech & UPBP_PB2D)
+            weight += product * cur.pb2d * ((qbt & UPBP_LONG_BEAM) ? cur.ray_sample_pdf_inv : cur.ray_sample_pdfs_ratio);
+        if ((tech & UPBP_BB1D) && (cur.flags & UPBP_MIS_IN_MEDIUM_WITH_BEAMS)) {
+            if (tech & UPBP_NO_SINE_IN_WEIGHTS) sin_t = 1.f;
+            const float a = (qbt & UPBP_LONG_BEAM) ? cur.ray_sample_pdf_inv : cur.ray_sample_pdfs_ratio;
+            const float b = (pbt & UPBP_LONG_BEAM) ? ray_rev : ray_rev_ratio;
+            weight_bb1d += product * cur.bb1d * cur.bb1d_sel * a * sin_t * b;
+        }
+        product *= cur.pdf_a_inv;
+        if ((tech & UPBP_BPT) && usable && !(next.flags & UPBP_MIS_SPECULAR)) weight += product;
+        cur = next;
+    }
+    return weight + weight_bb1d;
+}
+
+struct LightView {
+    const MisGlobals* g;
+    const MisF* beam_mis;       // BB1D virtual vertex (walk index 0) or null
+    int64_t override_idx;       // vertex whose mRaySampleRevPdfInv is replaced, or -1
+    float override_ray_rev_inv;
+};
+
+__device__ __forceinline__ MisF light_get(const LightView& l, int64_t i, uint32_t& err) {
+    if (i < 0 || i >= l.g->lv_count) {
+        err |= kBadLightIndex;
+        MisF m = {};
+        m.pdf_a_inv = 1.f;
+        return m;
+    }
+    MisF m = mis_load(&l.g->lv[i].mis);
+    if (i == l.override_idx) m.ray_sample_rev_pdf_inv = l.override_ray_rev_inv;
+    return m;
+}
+
+// AccumulateLightPathWeight, PathWeight.hxx:145-270 (camera connection = false).
+__device__ inline float light_path_weight(const LightView& l, int path_idx, int path_length, float last_rev,
+                                          float next_to_last_partial, uint32_t evaluated, uint32_t qbt, uint32_t pbt,
+                                          uint32_t tech, uint32_t& err) {
+    float weight = 0.f, product = 1.f;
+    int64_t last_index;
+    if (path_idx == 0) {
+        last_index = path_length;
+    } else {
+        if (path_idx - 1 < 0 || path_idx - 1 >= l.g->path_count) {
+            err |= kBadLightIndex;
+            return 0.f;
+        }
+        last_index = (int64_t)l.g->path_ends[path_idx - 1] + path_length;
+    }
+    MisF cur = (evaluated == UPBP_BB1D) ? *l.beam_mis : light_get(l, last_index, err);
+    for (int index = 0; index <= path_length; ++index) {
+        const MisF next = index < path_length ? light_get(l, last_index - index - 1, err) : cur;
+        float rev = cur.rev_pdf_a, sin_t = cur.sin_theta, ray_rev = cur.ray_sample_rev_pdf_inv,
+              ray_rev_ratio = cur.ray_sample_rev_pdfs_ratio;
+        if (index == 0) {
+            rev = last_rev;
+            sin_t = 0.f;
+            ray_rev = 0.f;
+            ray_rev_ratio = 0.f;
+        } else if (index == 1) {
+            rev = (evaluated == UPBP_BB1D) ? next_to_last_partial : cur.rev_pdf_a_without_bsdf * next_to_last_partial;
+        }
+        if (rev == 0.f) break;
+        product *= rev;
+        const bool usable = index == 0 || !(cur.flags & UPBP_MIS_SPECULAR);
+        if (index != 0 || evaluated == UPBP_BPT) {
+            if ((tech & UPBP_SURF) && usable) weight += product * cur.surf;
+            if (tech & UPBP_PP3D) weight += product * cur.pp3d;
+            if (tech & UPBP_PB2D) weight += product * cur.pb2d * ((qbt & UPBP_LONG_BEAM) ? ray_rev : ray_rev_ratio);
+            if ((tech & UPBP_BB1D) && !(tech & UPBP_BB1D_PREVIOUS) && (cur.flags & UPBP_MIS_IN_MEDIUM_WITH_BEAMS)) {
+                if (tech & UPBP_NO_SINE_IN_WEIGHTS) sin_t = 1.f;
+                const float a = (qbt & UPBP_LONG_BEAM) ? ray_rev : ray_rev_ratio;
+                const float b = (pbt & UPBP_LONG_BEAM) ? cur.ray_sample_pdf_inv : cur.ray_sample_pdfs_ratio;
+                weight += product * cur.bb1d * cur.bb1d_sel * a * sin_t * b;
+            }
+        }
+        product *= cur.pdf_a_inv;
+        if ((tech & UPBP_BPT) && usable && !(next.flags & UPBP_MIS_SPECULAR)) weight += product;
+        if (tech & UPBP_PREVIOUS) break;
+        cur = next;
+    }
+    return weight;
+}
+
+// Per-lane query state of the UPBP callbacks: AdditionalRayDataForMis of the current segment.
+struct MisLane {
+    const upbp_mis_data* cam_base;
+    int cam_len;
+    uint32_t ray_flags;    // raySamplingFlags of evalBre (first segment only)
+    float last_pdf_w_inv;
+    float pref_pdf, pref_rev;  // mRaySamplePdf, mRaySampleRevPdf of the segment loop
+    uint32_t seg_flags;        // mRaySamplingFlags of the current segment
+};
+
+}  // namespace upbp

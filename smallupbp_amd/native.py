@@ -30,7 +30,7 @@ EXPORTED = [
     "upbp_bre_set_power", "upbp_bre_count", "upbp_beams_build", "upbp_beams_destroy", "upbp_beams_count",
     "upbp_beams_piece_count", "upbp_beams_selection_pdf", "upbp_eval", "upbp_hit_signatures",
     "upbp_adjoint", "upbp_batch_plan_create", "upbp_batch_plan_destroy", "upbp_batch_run",
-    "upbp_set_tuning", "upbp_get_tuning", "upbp_autotune",
+    "upbp_set_tuning", "upbp_get_tuning", "upbp_autotune", "upbp_eval_upbp",
 ]
 
 
@@ -94,6 +94,9 @@ def load():
     L.upbp_beams_selection_pdf.argtypes = [vp, vp]
     L.upbp_eval.restype = st
     L.upbp_eval.argtypes = [vp, vp, vp, C.POINTER(_abi.QueryBatch), vp, vp, C.c_int, C.POINTER(_abi.EvalStats)]
+    L.upbp_eval_upbp.restype = st
+    L.upbp_eval_upbp.argtypes = [vp, vp, vp, C.POINTER(_abi.QueryBatch), C.POINTER(_abi.MisInputs), vp, vp, C.c_int,
+                                 C.POINTER(_abi.EvalStats)]
     L.upbp_hit_signatures.restype = st
     L.upbp_hit_signatures.argtypes = [vp, vp, vp, C.POINTER(_abi.QueryBatch), vp, vp, C.c_int]
     L.upbp_adjoint.restype = st
@@ -329,6 +332,22 @@ def eval_batch(ctx: Context, bre: Optional[EmbreeBre], beams: Optional[PhotonBea
     ctx.check(ctx.lib.upbp_eval(ctx.h, bre.h if bre else None, beams.h if beams else None, C.byref(queries.qb),
                                 _abi.ptr(op), _abi.ptr(ob), _abi.MEM_DEVICE if dev else _abi.MEM_HOST,
                                 C.byref(st) if stats is not None else None))
+    if stats is not None:
+        stats.update(st.as_dict())
+    return op, ob
+
+
+def eval_upbp(ctx: Context, bre: Optional[EmbreeBre], beams: Optional[PhotonBeamsEvaluator],
+              queries: QueryBatchView, mis, stats: Optional[dict] = None):
+    """upbp_eval_upbp: UPBP-weighted estimates. `mis` is a scenes.MisInputs (host arrays)."""
+    ms = _abi.mis_inputs(mis)
+    op = _out_like(queries, None) if bre is not None else None
+    ob = _out_like(queries, None) if beams is not None else None
+    dev = (op is not None and _is_torch(op)) or (ob is not None and _is_torch(ob))
+    st = _abi.EvalStats()
+    ctx.check(ctx.lib.upbp_eval_upbp(ctx.h, bre.h if bre else None, beams.h if beams else None, C.byref(queries.qb),
+                                     C.byref(ms), _abi.ptr(op), _abi.ptr(ob), _abi.MEM_DEVICE if dev else _abi.MEM_HOST,
+                                     C.byref(st) if stats is not None else None))
     if stats is not None:
         stats.update(st.as_dict())
     return op, ob

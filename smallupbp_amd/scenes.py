@@ -448,3 +448,114 @@ def batched_instances(n_instances=64, seed=4, scale=1.0, max_queries=2_000_000) 
         q = Queries(o, d, off, seg, LONG_BEAM)
         out.append(Workload("instance_%02d" % i, media, ph, bm, q, None, 0, 1, seed * 1000 + i))
     return out
+
+
+# ---------------------------------------------------------------------------
+# UPBP mode inputs (MIS data). Layouts match upbp_mis_data / upbp_light_vertex /
+# upbp_beam_mis / upbp_query_mis of include/upbp.h.
+MIS_DTYPE = np.dtype(
+    [("pdf_a_inv", "<f4"), ("rev_pdf_a", "<f4"), ("rev_pdf_a_without_bsdf", "<f4"), ("ray_sample_pdf_inv", "<f4"),
+     ("ray_sample_rev_pdf_inv", "<f4"), ("ray_sample_pdfs_ratio", "<f4"), ("ray_sample_rev_pdfs_ratio", "<f4"),
+     ("sin_theta", "<f4"), ("cos_theta_out", "<f4"), ("surf_factor", "<f4"), ("pp3d_factor", "<f4"),
+     ("pb2d_factor", "<f4"), ("bb1d_factor", "<f4"), ("bb1d_beam_selection_pdf", "<f4"), ("flags", "<u4"),
+     ("pad", "<u4")])
+assert MIS_DTYPE.itemsize == 64
+LIGHT_VERTEX_DTYPE = np.dtype(
+    [("throughput", "<f4", (3,)), ("dir_fix", "<f4", (3,)), ("continuation_prob", "<f4"), ("path_idx", "<i4"),
+     ("path_length", "<u4"), ("is_finite", "<u4"), ("pad", "<u4", (2,)), ("mis", MIS_DTYPE)])
+assert LIGHT_VERTEX_DTYPE.itemsize == 112
+BEAM_MIS_DTYPE = np.dtype([("light_vertex", "<i4"), ("ray_sampling_flags", "<u4"), ("ray_sample_pdf", "<f4"),
+                           ("ray_sample_rev_pdf", "<f4"), ("last_pdf_w_inv", "<f4"), ("pad", "<u4", (3,))])
+assert BEAM_MIS_DTYPE.itemsize == 32
+QUERY_MIS_DTYPE = np.dtype([("camera_offset", "<i8"), ("camera_path_length", "<u4"), ("ray_sampling_flags", "<u4"),
+                            ("last_pdf_w_inv", "<f4"), ("pad", "<u4")])
+assert QUERY_MIS_DTYPE.itemsize == 24
+
+MIS_DELTA, MIS_ON_LIGHT_SOURCE, MIS_SPECULAR, MIS_IN_MEDIUM_WITH_BEAMS = 1, 2, 4, 8
+BPT, SURF, PP3D, PB2D, BB1D = 4, 8, 16, 32, 64
+NO_SINE_IN_WEIGHTS, PREVIOUS, COMPATIBLE, BB1D_PREVIOUS = 128, 256, 512, 1024
+
+
+@dataclasses.dataclass
+class MisInputs:
+    queries: np.ndarray  # QUERY_MIS_DTYPE [Q]
+    camera: np.ndarray  # MIS_DTYPE
+    light_vertices: np.ndarray  # LIGHT_VERTEX_DTYPE
+    path_ends: np.ndarray  # int32
+    beams: Optional[np.ndarray]  # BEAM_MIS_DTYPE [B]
+    params: dict
+
+
+def random_mis(rng, n, specular_p=0.1, beams_p=0.7) -> np.ndarray:
+    m = np.zeros(n, MIS_DTYPE)
+    for f in ("pdf_a_inv", "rev_pdf_a", "rev_pdf_a_without_bsdf", "ray_sample_pdfs_ratio", "ray_sample_rev_pdfs_ratio"):
+        m[f] = rng.uniform(0.5, 2.0, n)
+    m["ray_sample_pdf_inv"] = rng.uniform(1.0, 3.0, n)
+    m["ray_sample_rev_pdf_inv"] = rng.uniform(1.0, 3.0, n)
+    m["sin_theta"] = rng.uniform(0.2, 1.0, n)
+    m["cos_theta_out"] = rng.uniform(-1.0, 1.0, n)
+    for f in ("surf_factor", "pp3d_factor", "pb2d_factor", "bb1d_factor"):
+        m[f] = rng.uniform(0.1, 2.0, n)
+    m["bb1d_beam_selection_pdf"] = 1.0
+    fl = np.where(rng.uniform(size=n) < specular_p, MIS_SPECULAR, 0)
+    fl |= np.where(rng.uniform(size=n) < beams_p, MIS_IN_MEDIUM_WITH_BEAMS, 0)
+    m["flags"] = fl.astype(np.uint32)
+    return m
+
+
+def upbp_scene(width=32, height=18, n_paths=2_000, n_beams=500, photon_radius=0.03, beam_radius=0.01, seed=41,
+               techniques=PB2D | BB1D | BPT | PP3D | SURF, query_beam_type=LONG_BEAM, photon_beam_type=SHORT_BEAM,
+               max_light_length=4, max_camera_length=3):
+    """Box + sphere media; light sub-paths of 1..max_light_length segments whose non-source
+    vertices are photons in the media; beams leave random light vertices; each query has a
+    camera sub-path of 1..max_camera_length vertices with random MIS data."""
+    rng = rng_for(seed)
+    media = make_media(_medium_rows_box_sphere())
+    lens = rng.integers(1, max_light_length + 1, n_paths)
+    counts = lens + 1
+    ends = np.cumsum(counts).astype(np.int32)
+    nv = int(ends[-1])
+    starts = np.concatenate([[0], ends[:-1]])
+    lv = np.zeros(nv, LIGHT_VERTEX_DTYPE)
+    path_idx = np.repeat(np.arange(n_paths), counts)
+    path_len = np.arange(nv) - np.repeat(starts, counts)
+    lv["path_idx"] = path_idx
+    lv["path_length"] = path_len
+    lv["throughput"] = rng.uniform(0, 1e-4, (nv, 3))
+    lv["dir_fix"] = _isotropic_dirs(rng, nv)
+    lv["continuation_prob"] = 0.8
+    lv["is_finite"] = 1
+    lv["mis"] = random_mis(rng, nv)
+    src = path_len == 0
+    # light-source vertices: on the light, no merging factors (they are not in a medium)
+    lv["mis"]["flags"][src] = MIS_ON_LIGHT_SOURCE
+    for f in ("surf_factor", "pp3d_factor", "pb2d_factor", "bb1d_factor"):
+        lv["mis"][f][src] = 0.0
+    pos = rng.uniform(0, 1, (nv, 3)).astype(np.float32)
+    in_medium = (~src).astype(np.uint8)
+    photons = Photons(pos, np.ascontiguousarray(lv["dir_fix"]), np.ascontiguousarray(lv["throughput"]), None,
+                      photon_radius)
+    bm = make_beams(rng, n_beams, media, SPHERE, 0, beam_radius, short_fraction=1.0 if photon_beam_type == SHORT_BEAM else 0.0)
+    bmis = np.zeros(n_beams, BEAM_MIS_DTYPE)
+    bmis["light_vertex"] = rng.integers(0, nv, n_beams)
+    bmis["ray_sampling_flags"] = 2 | np.where(rng.uniform(size=n_beams) < 0.3, 1, 0)
+    bmis["ray_sample_pdf"] = rng.uniform(0.5, 1.0, n_beams)
+    bmis["ray_sample_rev_pdf"] = rng.uniform(0.5, 1.0, n_beams)
+    bmis["last_pdf_w_inv"] = rng.uniform(0.5, 2.0, n_beams)
+    o, d = pinhole_rays(width, height)
+    off, seg = ray_segments(media, o, d, (0, 0, 0), (1, 1, 1), 0, 1, SPHERE)
+    q = Queries(o, d, off, seg, query_beam_type)
+    nq = q.count
+    clen = rng.integers(1, max_camera_length + 1, nq)
+    qm = np.zeros(nq, QUERY_MIS_DTYPE)
+    coff = np.concatenate([[0], np.cumsum(clen + 1)[:-1]])
+    qm["camera_offset"] = coff
+    qm["camera_path_length"] = clen
+    qm["ray_sampling_flags"] = np.where((clen > 1) & (rng.uniform(size=nq) < 0.5), 1, 0)
+    qm["last_pdf_w_inv"] = rng.uniform(0.5, 2.0, nq)
+    cam = random_mis(rng, int((clen + 1).sum()))
+    params = dict(techniques=techniques, query_beam_type=query_beam_type, photon_beam_type=photon_beam_type,
+                  min_path_length=0, max_path_length=20, surf_factor=0.7, pp3d_factor=0.9, pb2d_factor=0.6,
+                  bb1d_factor=1.3, bb1d_min_mfp=0.5, min_dist_to_med=0.0, has_bb1d_beams=1)
+    w = Workload("upbp_%dx%d" % (width, height), media, photons, bm, q, SPHERE, 0, 1, seed)
+    return w, in_medium, MisInputs(qm, cam, lv, ends, bmis, params)

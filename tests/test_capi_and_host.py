@@ -99,3 +99,31 @@ def test_generators_are_deterministic():
     assert np.array_equal(a.queries.segments, b.queries.segments)
     c = scenes.plain_bre(16, 16, 1000, 0.01, seed=10)
     assert not np.array_equal(a.photons.position, c.photons.position)
+
+
+def test_cpp_facade_compiles(tmp_path):
+    """include/upbp.hpp (the C++ facade a renderer includes) compiles against upbp.h."""
+    import shutil
+    import subprocess
+
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    src = tmp_path / "use.cpp"
+    src.write_text('#include "upbp.hpp"\n'
+                   "int use(upbp::Context& c, const float* p, const float* w) {\n"
+                   "  upbp::EmbreeBre bre(c);\n"
+                   "  int kept = bre.build(p, nullptr, w, nullptr, 4, UPBP_CONSTANT_RADIUS, 0.01f, 0);\n"
+                   "  upbp_volume_segment s{0.f, 1.f, 1.f, 1.f, {1.f, 1.f, 1.f}, 0};\n"
+                   "  float o[3] = {0, 0, 0}, d[3] = {0, 0, 1};\n"
+                   "  auto rgb = bre.evalBre(UPBP_LONG_BEAM, o, d, &s, 1);\n"
+                   "  upbp::QueryBatch qb; qb.add(o, d, &s, 1);\n"
+                   "  upbp::PhotonBeamsEvaluator pb(c);\n"
+                   "  return kept + (int)rgb[0] + (int)qb.size() + (int)pb.getBeamSelectionPdf(o);\n"
+                   "}\n")
+    r = subprocess.run([gxx, "-std=c++17", "-fsyntax-only", "-Wall", "-Werror", "-I",
+                        os.path.join(ROOT, "include"), str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    c = subprocess.run(["gcc", "-std=c99", "-fsyntax-only", "-Wall", "-Werror", "-x", "c",
+                        os.path.join(ROOT, "include", "upbp.h")], capture_output=True, text=True)
+    assert c.returncode == 0, c.stderr

@@ -1,0 +1,70 @@
+"""GPU parity of the UPBP (MIS-weighted) estimators against the CPU oracle."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from smallupbp_amd import native, scenes
+
+from gpu_helpers import REL_TOL, gpu_context, print_case, rel_err
+
+pytestmark = pytest.mark.gpu
+
+ALL = scenes.PB2D | scenes.BB1D | scenes.BPT | scenes.PP3D | scenes.SURF
+
+
+def _gpu(w, inm, mi, device=True):
+    ctx = gpu_context(w.media)
+    bre = native.EmbreeBre(ctx)
+    bre.build(w.photons, in_medium=inm)
+    bm = native.PhotonBeamsEvaluator(ctx)
+    bm.build(w.beams)
+    qv = native.QueryBatchView.to_device(w.queries) if device else native.QueryBatchView.host(w.queries)
+    op, ob = native.eval_upbp(ctx, bre, bm, qv, mi)
+    if hasattr(op, "cpu"):
+        op, ob = op.cpu().numpy(), ob.cpu().numpy()
+    return op, ob
+
+
+def _oracle(w, inm, mi):
+    oph = po.OraclePhotons(w.photons, in_medium=inm, build_bvh=False)
+    obm = po.OracleBeams(w.beams, grid_res=0)
+    a, _ = po.upbp_eval(oph, None, w.media, w.queries, mi, 0)
+    b, _ = po.upbp_eval(None, obm, w.media, w.queries, mi, 1)
+    return a, b
+
+
+CASES = [
+    ("all_long_short", ALL, scenes.LONG_BEAM, scenes.SHORT_BEAM, False),
+    ("all_long_long", ALL, scenes.LONG_BEAM, scenes.LONG_BEAM, False),
+    ("all_short_short", ALL, scenes.SHORT_BEAM, scenes.SHORT_BEAM, False),
+    ("no_sine", ALL | scenes.NO_SINE_IN_WEIGHTS, scenes.LONG_BEAM, scenes.SHORT_BEAM, False),
+    ("previous", ALL | scenes.PREVIOUS, scenes.LONG_BEAM, scenes.SHORT_BEAM, False),
+    ("bb1d_previous", ALL | scenes.BB1D_PREVIOUS, scenes.LONG_BEAM, scenes.SHORT_BEAM, False),
+    ("lite_segments", ALL, scenes.LONG_BEAM, scenes.SHORT_BEAM, True),
+]
+
+
+@pytest.mark.parametrize("name,tech,qbt,pbt,lite", CASES)
+def test_upbp_matches_oracle(name, tech, qbt, pbt, lite):
+    print_case("upbp", name=name)
+    w, inm, mi = scenes.upbp_scene(40, 24, 3000, 800, seed=61, techniques=tech, query_beam_type=qbt,
+                                   photon_beam_type=pbt)
+    w.queries.lite = lite
+    gp, gb = _gpu(w, inm, mi)
+    rp, rb = _oracle(w, inm, mi)
+    assert (rp.max(1) > 0).sum() > 100 and (rb.max(1) > 0).sum() > 50
+    ep, eb = rel_err(gp, rp), rel_err(gb, rb)
+    print("max rel err photons %.2e beams %.2e" % (ep.max(), eb.max()))
+    assert ep.max() <= REL_TOL and eb.max() <= REL_TOL
+
+
+def test_upbp_host_inputs_and_bad_indices():
+    print_case("upbp_host_errors")
+    w, inm, mi = scenes.upbp_scene(24, 14, 1500, 400, seed=62)
+    gp, gb = _gpu(w, inm, mi, device=False)
+    rp, rb = _oracle(w, inm, mi)
+    assert rel_err(gp, rp).max() <= REL_TOL and rel_err(gb, rb).max() <= REL_TOL
+    bad = scenes.MisInputs(mi.queries.copy(), mi.camera, mi.light_vertices, mi.path_ends, mi.beams, mi.params)
+    bad.queries["camera_offset"][3] = len(mi.camera)  # past the camera array
+    with pytest.raises(native.UpbpError):
+        _gpu(w, inm, bad)
