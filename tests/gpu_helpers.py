@@ -27,8 +27,8 @@ def gpu_context(media) -> native.Context:
     return ctx
 
 
-def print_case(name: str, **kw):
-    print("[case] %s %s" % (name, " ".join("%s=%s" % (k, v) for k, v in kw.items())), flush=True)
+def print_case(case: str, **kw):
+    print("[case] %s %s" % (case, " ".join("%s=%s" % (k, v) for k, v in kw.items())), flush=True)
 
 
 def oracle_photons(w, precision=32, accel=1, signatures=True, queries=None):
