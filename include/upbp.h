@@ -46,7 +46,7 @@ typedef enum upbp_status {
     UPBP_ENOMEM = 2,        /* device or host allocation failed */
     UPBP_EDEVICE = 3,       /* HIP runtime error, no device, or a kernel reported an internal overflow */
     UPBP_ENONFINITE = 4,    /* NaN/Inf in an input position, direction, radius, power or segment */
-    UPBP_EUNSUPPORTED = 5,  /* valid request this build does not serve (e.g. KNN radius, SHORT query in the adjoint) */
+    UPBP_EUNSUPPORTED = 5,  /* valid request this build does not serve (e.g. SHORT query in the adjoint) */
     UPBP_ENOTBUILT = 6      /* handle is empty (build found no photons/beams) where data is required */
 } upbp_status;
 
@@ -116,13 +116,27 @@ typedef struct upbp_volume_segment {
 /* ---------------------------------------------------------------------------
  * Build inputs. */
 typedef struct upbp_build_params {
-    int32_t radius_calculation; /* UPBP_CONSTANT_RADIUS; UPBP_KNN_RADIUS returns UPBP_EUNSUPPORTED */
-    float radius;               /* photon radius / beam radius, > 0, used when no per-item radius array */
-    int32_t knn;                /* reserved for UPBP_KNN_RADIUS */
+    int32_t radius_calculation; /* UPBP_CONSTANT_RADIUS or UPBP_KNN_RADIUS (see below) */
+    float radius;               /* photon radius / beam radius, > 0, used when no per-item radius array;
+                                   with UPBP_KNN_RADIUS the multiplier of 2 sqrt(d2_knn) */
+    int32_t knn;                /* UPBP_KNN_RADIUS: which nearest neighbour (>= 1, the item itself
+                                   counts as the first) sets the radius */
     int32_t leaf_size;          /* max items per LBVH leaf, 1..32; 0 = library default */
     float beam_piece_factor;    /* beams only: target piece length in units of the max beam radius; 0 = default */
     int32_t max_pieces_per_beam;/* beams only: cap on pieces per beam, 1..256; 0 = default */
 } upbp_build_params;
+
+/* UPBP_KNN_RADIUS (photons: Bre.cxx:562-602; beams: PhBeams.cxx:77-117):
+ * d2_knn is the squared distance to the knn-th nearest item of the kd-tree the
+ * reference builds (in-medium photon positions; beam origins), the query's own
+ * item included, computed in float as ((dx*dx) + dy*dy) + dz*dz.
+ *   photon radius      = radius * (2 sqrt(d2_knn at the photon))
+ *   beam start radius  = max(2 sqrt(d2_knn at the origin) * radius, 0.001)
+ *   beam end radius    = max(2 sqrt(d2_knn at origin + direction * length) * radius, 0.001)
+ * With n <= knn items the largest of the n distances is used (the reference's
+ * kd-tree then returns whichever item it visited first). Per-item radius arrays
+ * must be NULL. A photon whose radius comes out 0 (knn coincident photons) makes
+ * the build fail with UPBP_EINVAL. */
 
 /* Photons: light vertices in media. Replaces the inputs of EmbreeBre::build
  * (SmallUPBP/src/Bre/Bre.cxx:526-632): position = mHitpoint,
@@ -214,12 +228,16 @@ upbp_status upbp_bre_destroy(upbp_bre* bre);
  * photons) without rebuilding the tree. Used by differentiable callers. */
 upbp_status upbp_bre_set_power(upbp_bre* bre, const float* power, upbp_memory memory);
 int64_t upbp_bre_count(const upbp_bre* bre);
+/* Radius the build gave each input photon, in input order (0 for photons the
+ * in-medium filter dropped): the `radius` of EmbreePhoton::set (Bre.cxx:596),
+ * e.g. to read back UPBP_KNN_RADIUS radii. `radius` holds photons->count floats. */
+upbp_status upbp_bre_radii(const upbp_bre* bre, float* radius, upbp_memory memory);
 
 /* ---------------------------------------------------------------------------
  * Photon beams (BB1D). */
 /* Replaces PhotonBeamsEvaluator::build (SmallUPBP/src/Beams/PhBeams.cxx:66-148):
  * assigns radii like the reference (constant radius: start = end = r,
- * mRadiusChange = 0), then cuts every beam into pieces and builds a GPU LBVH
+ * mRadiusChange = 0; UPBP_KNN_RADIUS as above), then cuts every beam into pieces and builds a GPU LBVH
  * over the pieces' cone bounding boxes (PhotonBeam.hxx:186-198). Each
  * (query, beam) pair is counted once: a piece accepts only closest-approach
  * parameters t2 in its own half-open range. */
@@ -229,6 +247,10 @@ upbp_status upbp_beams_build(upbp_ctx* ctx, const upbp_beam_input* beams,
 upbp_status upbp_beams_destroy(upbp_beams* beams);
 int64_t upbp_beams_count(const upbp_beams* beams);
 int64_t upbp_beams_piece_count(const upbp_beams* beams);
+/* mStartRadius / mEndRadius the build gave each beam (PhBeams.cxx:92-117), in
+ * input order; each array holds beams->count floats. */
+upbp_status upbp_beams_radii(const upbp_beams* beams, float* start_radius, float* end_radius,
+                             upbp_memory memory);
 /* Replaces PhotonBeamsEvaluator::getBeamSelectionPdf (PhBeams.hxx, Grid.hxx:409-415).
  * Without beam reduction (sMaxBeamsInCell = 0, the reference default) it is 1. */
 float upbp_beams_selection_pdf(const upbp_beams* beams, const float position[3]);

@@ -19,6 +19,15 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 _lib = None
 
 
+def _threads() -> int:
+    """Worker threads for the oracle: the CPUs this process may run on, at most 16."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
 def build(force: bool = False) -> str:
     if force or not os.path.exists(LIB_PATH):
         subprocess.check_call(["make", "-s", "-C", HERE] + (["-B"] if force else []))
@@ -57,6 +66,12 @@ def lib():
         L.or_upbp_eval.argtypes = [vp, vp, vp, i32, C.POINTER(_abi.QueryBatch), C.POINTER(_abi.MisInputs), C.c_int,
                                    C.c_int, C.c_int, vp, vp, vp]
         L.or_exp_cephes.restype = f32
+        L.or_knn_d2.argtypes = [vp, i64, vp, i64, i32, C.c_int, vp]
+        L.or_knn_d2.restype = None
+        L.or_photon_knn_radius.argtypes = [i64, vp, vp, f32, i32, C.c_int, vp]
+        L.or_photon_knn_radius.restype = None
+        L.or_beam_knn_radius.argtypes = [i64, vp, vp, vp, f32, i32, C.c_int, vp, vp]
+        L.or_beam_knn_radius.restype = None
         L.or_exp_cephes.argtypes = [f32]
         L.or_hash32.restype = C.c_uint32
         L.or_hash32.argtypes = [C.c_uint32]
@@ -134,7 +149,7 @@ def _media_ptr(media):
 def bre_eval(oph: OraclePhotons, media, q: Queries, accel=1, precision=32, threads=None,
              signatures=False):
     """Returns (out [Q,3] f32 or f64, sig [Q,2] u32 or None, counters [hits, candidates])."""
-    threads = threads or os.cpu_count()
+    threads = threads or _threads()
     qb = query_batch(q)
     m, mp, nm = _media_ptr(media)
     out = np.zeros((q.count, 3), np.float64 if precision == 64 else np.float32)
@@ -150,7 +165,7 @@ def bre_eval(oph: OraclePhotons, media, q: Queries, accel=1, precision=32, threa
 
 def beams_eval(obm: OracleBeams, media, q: Queries, accel=0, precision=32, threads=None, signatures=False):
     """accel 0 = brute force (every beam), 1 = the reference uniform grid."""
-    threads = threads or os.cpu_count()
+    threads = threads or _threads()
     qb = query_batch(q)
     m, mp, nm = _media_ptr(media)
     out = np.zeros((q.count, 3), np.float64 if precision == 64 else np.float32)
@@ -167,7 +182,7 @@ def beams_eval(obm: OracleBeams, media, q: Queries, accel=0, precision=32, threa
 def adjoint(oph, obm, media, q: Queries, w_photons, w_beams, threads=None):
     """FP64 analytic gradient. Returns dict of grad_sigma_s, grad_sigma_a [M,3],
     grad_photon_power [N_in,3], grad_beam_power [B,3]."""
-    threads = threads or os.cpu_count()
+    threads = threads or _threads()
     qb = query_batch(q)
     m, mp, nm = _media_ptr(media)
     gs = np.zeros((nm, 3))
@@ -186,7 +201,7 @@ def adjoint(oph, obm, media, q: Queries, w_photons, w_beams, threads=None):
 
 def upbp_eval(oph, obm, media, q: Queries, mi, which: int, precision=32, threads=None):
     """UPBP-mode estimate (MIS weights) per query; which = 0 photons, 1 beams. Returns (out, hits)."""
-    threads = threads or os.cpu_count()
+    threads = threads or _threads()
     qb = query_batch(q)
     m, mp, nm = _media_ptr(media)
     ms = _abi.mis_inputs(mi)
@@ -209,3 +224,34 @@ def hash32(i: np.ndarray) -> np.ndarray:
     x = (x * 0x846CA68B) & 0xFFFFFFFF
     x ^= x >> 16
     return x.astype(np.uint32)
+
+
+def knn_d2(points: np.ndarray, queries: np.ndarray, k: int, threads=None) -> np.ndarray:
+    """k-th smallest squared distance from each query to the points (KdTmpl.hxx:600-792 semantics)."""
+    p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    q = np.ascontiguousarray(queries, np.float32).reshape(-1, 3)
+    out = np.zeros(len(q), np.float32)
+    lib().or_knn_d2(p.ctypes.data, len(p), q.ctypes.data, len(q), int(k), threads or _threads(), out.ctypes.data)
+    return out
+
+
+def photon_knn_radius(position, radius: float, knn: int, in_medium=None, threads=None) -> np.ndarray:
+    """Per-photon radius of EmbreeBre::build with KNN_RADIUS (Bre.cxx:562-602); 0 for dropped photons."""
+    p = np.ascontiguousarray(position, np.float32).reshape(-1, 3)
+    m = None if in_medium is None else np.ascontiguousarray(in_medium, np.uint8)
+    out = np.zeros(len(p), np.float32)
+    lib().or_photon_knn_radius(len(p), p.ctypes.data, None if m is None else m.ctypes.data, float(radius), int(knn),
+                               threads or _threads(), out.ctypes.data)
+    return out
+
+
+def beam_knn_radius(bm: Beams, radius: float, knn: int, threads=None):
+    """(start, end) radius per beam of PhotonBeamsEvaluator::build with KNN_RADIUS (PhBeams.cxx:77-117)."""
+    o = np.ascontiguousarray(bm.origin, np.float32)
+    d = np.ascontiguousarray(bm.direction, np.float32)
+    ln = np.ascontiguousarray(bm.length, np.float32)
+    r0 = np.zeros(bm.count, np.float32)
+    r1 = np.zeros(bm.count, np.float32)
+    lib().or_beam_knn_radius(bm.count, o.ctypes.data, d.ctypes.data, ln.ctypes.data, float(radius), int(knn),
+                             threads or _threads(), r0.ctypes.data, r1.ctypes.data)
+    return r0, r1

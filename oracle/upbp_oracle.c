@@ -870,3 +870,114 @@ int or_upbp_eval(const or_photons* ph, const or_beams* bm, const upbp_medium* me
     free(m);
     return r;
 }
+
+/* ------------------------------------------------------------------------- */
+/* k-nearest-neighbour radii.
+ *
+ * SmallUPBP answers "k-th nearest neighbour" with Jensen's kd-tree query
+ * (SmallUPBP/src/Misc/KdTmpl.hxx:600-792): a bounded max-heap of k squared
+ * distances, each computed as ((dx*dx) + dy*dy) + dz*dz with d = item - query
+ * (:604-610), an item entering only if its distance is strictly below the heap
+ * maximum (:612), and the result read from dist2[1], the heap maximum. Plane
+ * pruning (:751) never drops an item that could enter, so for more than k items
+ * the result is exactly the k-th smallest squared distance, the query's own item
+ * (distance 0) included. This restatement keeps the same bounded max-heap but
+ * scans every item (brute force). With n <= k items the kd-tree never builds its
+ * heap and dist2[1] is the first item it happened to visit; this restatement
+ * returns the largest of the n distances instead, and so does the GPU library. */
+static void knn_heap_push(float* heap, int* found, int k, float d2) {
+    /* heap[1..k], max at heap[1] once full (1-based like KdTmpl.hxx) */
+    if (*found < k) {
+        int c = ++(*found);
+        heap[c] = d2;
+        while (c > 1 && heap[c / 2] < heap[c]) {
+            float t = heap[c / 2];
+            heap[c / 2] = heap[c];
+            heap[c] = t;
+            c /= 2;
+        }
+        return;
+    }
+    if (!(d2 < heap[1])) return;
+    int p = 1;
+    for (;;) {
+        int j = 2 * p;
+        if (j > k) break;
+        if (j < k && heap[j] < heap[j + 1]) ++j;
+        if (!(d2 < heap[j])) break;
+        heap[p] = heap[j];
+        p = j;
+    }
+    heap[p] = d2;
+}
+
+void or_knn_d2(const float* pts, int64_t n, const float* q, int64_t nq, int32_t k, int threads, float* out) {
+    /* initial search radius MAX_FLOAT_SQUARE_ROOT (Misc/Defs.hxx:48), squared as KdTmpl.hxx:139 does */
+    const float init_r = sqrtf(FLT_MAX);
+    const float init_d2 = init_r * init_r;
+    const int keff = (int64_t)k < n ? k : (int)n;
+    if (threads <= 0) threads = 1;
+#pragma omp parallel num_threads(threads)
+    {
+        float* heap = (float*)malloc(sizeof(float) * (size_t)(keff + 2));
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t i = 0; i < nq; ++i) {
+            const float qx = q[3 * i], qy = q[3 * i + 1], qz = q[3 * i + 2];
+            int found = 0;
+            for (int64_t j = 0; j < n; ++j) {
+                const float dx = pts[3 * j] - qx, dy = pts[3 * j + 1] - qy, dz = pts[3 * j + 2] - qz;
+                const float d2 = (dx * dx + dy * dy) + dz * dz;
+                if (d2 < init_d2) knn_heap_push(heap, &found, keff, d2);
+            }
+            out[i] = found > 0 ? heap[1] : init_d2;
+        }
+        free(heap);
+    }
+}
+
+/* EmbreeBre::build with KNN_RADIUS (SmallUPBP/src/Bre/Bre.cxx:562-602): the
+ * kd-tree holds the in-medium vertices; radius_i = radius * (2 * sqrt(d2_k)).
+ * out[i] is written for in-medium photons only. */
+void or_photon_knn_radius(int64_t n, const float* pos, const uint8_t* in_medium, float radius, int32_t knn,
+                          int threads, float* out) {
+    int64_t m = 0;
+    for (int64_t i = 0; i < n; ++i) m += (!in_medium || in_medium[i]) ? 1 : 0;
+    float* pts = (float*)calloc(3 * (size_t)(m + 1), sizeof(float));
+    int64_t* idx = (int64_t*)malloc(sizeof(int64_t) * (size_t)(m + 1));
+    float* d2 = (float*)malloc(sizeof(float) * (size_t)(m + 1));
+    int64_t c = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        if (in_medium && !in_medium[i]) continue;
+        memcpy(&pts[3 * c], &pos[3 * i], 12);
+        idx[c++] = i;
+    }
+    or_knn_d2(pts, m, pts, m, knn, threads, d2);
+    for (int64_t j = 0; j < m; ++j) out[idx[j]] = radius * (2.0f * sqrtf(d2[j]));
+    free(pts);
+    free(idx);
+    free(d2);
+}
+
+/* PhotonBeamsEvaluator::build with KNN_RADIUS (SmallUPBP/src/Beams/PhBeams.cxx:77-117):
+ * the kd-tree holds the beam origins; each beam queries at its origin and at
+ * origin + direction * length (Ray::target, Path/Ray.hxx:57-59). */
+void or_beam_knn_radius(int64_t n, const float* origin, const float* dir, const float* length, float radius,
+                        int32_t knn, int threads, float* r0, float* r1) {
+    const float smallest = 0.001f; /* SMALLEST_RADIUS, PhBeams.cxx:72 */
+    float* ends = (float*)malloc(sizeof(float) * 3 * (size_t)(n + 1));
+    float* d2 = (float*)malloc(sizeof(float) * (size_t)(n + 1));
+    for (int64_t i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) ends[3 * i + a] = origin[3 * i + a] + dir[3 * i + a] * length[i];
+    or_knn_d2(origin, n, origin, n, knn, threads, d2);
+    for (int64_t i = 0; i < n; ++i) {
+        const float r = (2.0f * sqrtf(d2[i])) * radius;
+        r0[i] = r < smallest ? smallest : r;
+    }
+    or_knn_d2(origin, n, ends, n, knn, threads, d2);
+    for (int64_t i = 0; i < n; ++i) {
+        const float r = (2.0f * sqrtf(d2[i])) * radius;
+        r1[i] = r < smallest ? smallest : r;
+    }
+    free(ends);
+    free(d2);
+}

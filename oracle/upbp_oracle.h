@@ -120,6 +120,17 @@ int or_adjoint(const or_photons* ph, const or_beams* bm, const upbp_medium* medi
                int threads, double* grad_sigma_s, double* grad_sigma_a,
                double* grad_photon_power, double* grad_beam_power);
 
+/* ---- k-nearest-neighbour radii (KdTmpl.hxx:600-792 result semantics) ----- */
+/* out[i] = k-th smallest squared distance from q[i] to pts (brute force, bounded
+ * max-heap); with n <= k the largest of the n distances. */
+void or_knn_d2(const float* pts, int64_t n, const float* q, int64_t nq, int32_t k, int threads, float* out);
+/* Bre.cxx:562-602: out[i] = radius * 2 sqrt(d2_k) over in-medium photons (in_medium NULL = all). */
+void or_photon_knn_radius(int64_t n, const float* pos, const uint8_t* in_medium, float radius, int32_t knn,
+                          int threads, float* out);
+/* PhBeams.cxx:77-117: start / end radius of each beam from the k-th nearest beam origin. */
+void or_beam_knn_radius(int64_t n, const float* origin, const float* dir, const float* length, float radius,
+                        int32_t knn, int threads, float* r0, float* r1);
+
 #ifdef __cplusplus
 }
 #endif

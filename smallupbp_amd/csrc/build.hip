@@ -1,7 +1,7 @@
 // build.hip -- photon and beam structure builds on the GPU.
 //
 // Photons (EmbreeBre::build, SmallUPBP/src/Bre/Bre.cxx:526-632): in-medium
-// filter, radius (constant or per photon), r^2 = r*r, cube box pos +- r
+// filter, radius (constant, per photon, or from the k-th nearest photon), r^2 = r*r, cube box pos +- r
 // (Bre.cxx:174-177), LBVH, then photon records gathered into leaf order.
 // Beams (PhotonBeamsEvaluator::build, SmallUPBP/src/Beams/PhBeams.cxx:66-148):
 // radii as the reference (constant: start = end = r, change 0), then each beam
@@ -60,7 +60,7 @@ __global__ void k_photon_boxes(int64_t n, const uint32_t* keep, const float* pos
 __global__ void k_photon_gather(int64_t n, const uint32_t* order, const uint32_t* keep,
                                 const float* position, const float* radius, float const_radius,
                                 const float* power, const float* inc, float4* pos_r2, float4* pw,
-                                float4* inc_out, uint32_t* orig) {
+                                float4* inc_out, uint32_t* orig, float* radius_in) {
     int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (s >= n) return;
     uint32_t k = order[s];
@@ -75,6 +75,7 @@ __global__ void k_photon_gather(int64_t n, const uint32_t* order, const uint32_t
     else
         inc_out[s] = make_float4(0.f, 0.f, 0.f, 0.f);
     orig[s] = (uint32_t)i;
+    radius_in[i] = r;
 }
 
 // new power in original input order -> leaf order
@@ -146,11 +147,14 @@ __device__ __forceinline__ float cut_at(float len, uint32_t k, uint32_t c) {
 }
 
 __global__ void k_beam_pieces(int64_t n, BeamIn b, const uint32_t* count, const uint32_t* offset,
-                              PieceDev* pieces, float4* flux, uint32_t* piece_beam, float4* lo, float4* hi) {
+                              PieceDev* pieces, float4* flux, uint32_t* piece_beam, float4* lo, float4* hi,
+                              float* r_start, float* r_end) {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     float r0, r1, rmax, dr;
     beam_radii(b, i, r0, r1, rmax, dr);
+    r_start[i] = r0;
+    r_end[i] = r1;
     const float ox = b.origin[3 * i], oy = b.origin[3 * i + 1], oz = b.origin[3 * i + 2];
     const float dx = b.direction[3 * i], dy = b.direction[3 * i + 1], dz = b.direction[3 * i + 2];
     const float len = b.length[i];
@@ -219,8 +223,11 @@ void check_err_word(uint32_t e) {
 
 void bre_build(Ctx& ctx, const upbp_photon_input& in, const upbp_build_params& params, Bre& out) {
     UPBP_REQUIRE(in.count >= 0 && in.count < (int64_t)0x7FFFFFFF, UPBP_EINVAL, "photon count out of range");
-    UPBP_REQUIRE(params.radius_calculation == UPBP_CONSTANT_RADIUS, UPBP_EUNSUPPORTED,
-                 "only UPBP_CONSTANT_RADIUS (or a per-photon radius array) is supported");
+    const bool knn = params.radius_calculation == UPBP_KNN_RADIUS;
+    UPBP_REQUIRE(knn || params.radius_calculation == UPBP_CONSTANT_RADIUS, UPBP_EINVAL,
+                 "radius_calculation must be UPBP_CONSTANT_RADIUS or UPBP_KNN_RADIUS");
+    UPBP_REQUIRE(!knn || (params.knn >= 1 && !in.radius), UPBP_EINVAL,
+                 "UPBP_KNN_RADIUS needs knn >= 1 and no per-photon radius array");
     UPBP_REQUIRE(in.radius || (params.radius > 0.f && std::isfinite(params.radius)), UPBP_EINVAL,
                  "photon radius must be > 0");
     const int leaf = params.leaf_size > 0 ? params.leaf_size : 4;
@@ -258,6 +265,12 @@ void bre_build(Ctx& ctx, const upbp_photon_input& in, const upbp_build_params& p
     }
     out.n = n;
     if (n == 0) return;
+    DevBuf knn_radius;
+    if (knn) {
+        // radius of each kept photon from its knn-th nearest kept photon (Bre.cxx:562-602)
+        photon_knn_radius(ctx, n, pos, keep, params.radius, params.knn, knn_radius, n_in);
+        rad = knn_radius.as<float>();
+    }
     DevBuf lo, hi, err, order;
     lo.ensure(n * 16);
     hi.ensure(n * 16);
@@ -272,9 +285,11 @@ void bre_build(Ctx& ctx, const upbp_photon_input& in, const upbp_build_params& p
     out.power.ensure(n * 16);
     out.inc.ensure(n * 16);
     out.orig.ensure(n * 4);
+    out.radius_in.ensure(n_in * 4);
+    launch_clear_f32(ctx, out.radius_in.as<float>(), n_in);
     hipLaunchKernelGGL(k_photon_gather, dim3(grid_for(n)), dim3(kBlock), 0, st, n, order.as<uint32_t>(), keep,
                        pos, rad, params.radius, pw, inc, out.pos_r2.as<float4>(), out.power.as<float4>(),
-                       out.inc.as<float4>(), out.orig.as<uint32_t>());
+                       out.inc.as<float4>(), out.orig.as<uint32_t>(), out.radius_in.as<float>());
     UPBP_HIP(hipGetLastError());
     UPBP_HIP(hipStreamSynchronize(st));
 }
@@ -289,8 +304,11 @@ void bre_set_power(Bre& bre, const float* power_dev) {
 
 void beams_build(Ctx& ctx, const upbp_beam_input& in, const upbp_build_params& params, Beams& out) {
     UPBP_REQUIRE(in.count >= 0 && in.count < (int64_t)0x7FFFFFFF, UPBP_EINVAL, "beam count out of range");
-    UPBP_REQUIRE(params.radius_calculation == UPBP_CONSTANT_RADIUS, UPBP_EUNSUPPORTED,
-                 "only UPBP_CONSTANT_RADIUS (or per-beam radius arrays) is supported");
+    const bool knn = params.radius_calculation == UPBP_KNN_RADIUS;
+    UPBP_REQUIRE(knn || params.radius_calculation == UPBP_CONSTANT_RADIUS, UPBP_EINVAL,
+                 "radius_calculation must be UPBP_CONSTANT_RADIUS or UPBP_KNN_RADIUS");
+    UPBP_REQUIRE(!knn || (params.knn >= 1 && !in.start_radius && !in.end_radius), UPBP_EINVAL,
+                 "UPBP_KNN_RADIUS needs knn >= 1 and no per-beam radius arrays");
     UPBP_REQUIRE(in.start_radius || (params.radius > 0.f && std::isfinite(params.radius)), UPBP_EINVAL,
                  "beam radius must be > 0");
     UPBP_REQUIRE(ctx.n_media > 0, UPBP_EINVAL, "upbp_set_media must be called before building beams");
@@ -316,6 +334,13 @@ void beams_build(Ctx& ctx, const upbp_beam_input& in, const upbp_build_params& p
     b.medium = stage(ctx, s[5], in.medium_id, n, in.memory);
     b.r0 = stage(ctx, s[6], in.start_radius, n, in.memory);
     b.r1 = stage(ctx, s[7], in.end_radius, n, in.memory);
+    DevBuf knn_r0, knn_r1;
+    if (knn) {
+        // start / end radius from the knn-th nearest beam origin (PhBeams.cxx:77-117)
+        beam_knn_radius(ctx, n, b.origin, b.direction, b.length, params.radius, params.knn, knn_r0, knn_r1);
+        b.r0 = knn_r0.as<float>();
+        b.r1 = knn_r1.as<float>();
+    }
     b.const_radius = params.radius;
     b.piece_factor = factor;
     b.max_pieces = maxp;
@@ -340,9 +365,11 @@ void beams_build(Ctx& ctx, const upbp_beam_input& in, const upbp_build_params& p
     piece_beam.ensure(total * 4);
     lo.ensure(total * 16);
     hi.ensure(total * 16);
+    out.r_start.ensure(n * 4);
+    out.r_end.ensure(n * 4);
     hipLaunchKernelGGL(k_beam_pieces, dim3(grid_for(n)), dim3(kBlock), 0, st, n, b, count.as<uint32_t>(),
                        offset.as<uint32_t>(), pieces.as<PieceDev>(), flux.as<float4>(), piece_beam.as<uint32_t>(),
-                       lo.as<float4>(), hi.as<float4>());
+                       lo.as<float4>(), hi.as<float4>(), out.r_start.as<float>(), out.r_end.as<float>());
     UPBP_HIP(hipGetLastError());
     build_lbvh(ctx, total, lo.as<float4>(), hi.as<float4>(), leaf, out.bvh, order);
     out.pieces = total;

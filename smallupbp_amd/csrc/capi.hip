@@ -411,6 +411,33 @@ upbp_status upbp_bre_set_power(upbp_bre* bre, const float* power, upbp_memory me
 
 int64_t upbp_bre_count(const upbp_bre* bre) { return bre ? bre->n : 0; }
 
+namespace {
+// Copies n floats of a device buffer (or zeros when it is empty) to a caller buffer.
+void copy_out_f32(Ctx& ctx, const DevBuf& src, int64_t n, float* dst, upbp_memory memory) {
+    if (n <= 0) return;
+    const hipMemcpyKind kind = memory == UPBP_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    if (src.p) {
+        UPBP_HIP(hipMemcpyAsync(dst, src.p, (size_t)n * 4, kind, ctx.stream));
+    } else if (memory == UPBP_MEM_DEVICE) {
+        launch_clear_f32(ctx, dst, n);
+    } else {
+        std::fill(dst, dst + n, 0.f);
+    }
+    UPBP_HIP(hipStreamSynchronize(ctx.stream));
+}
+}  // namespace
+
+upbp_status upbp_bre_radii(const upbp_bre* bre, float* radius, upbp_memory memory) {
+    if (!bre) return fail(nullptr, UPBP_EINVAL, "bre is NULL");
+    Ctx* ctx = bre->ctx;
+    return guarded(ctx, [&] {
+        UPBP_REQUIRE(radius || bre->n_in == 0, UPBP_EINVAL, "radius is NULL");
+        UPBP_REQUIRE(memory == UPBP_MEM_HOST || memory == UPBP_MEM_DEVICE, UPBP_EINVAL, "bad memory kind");
+        // radius_in is empty when the build kept no photon: zeros then
+        copy_out_f32(*ctx, bre->radius_in, bre->n_in, radius, memory);
+    });
+}
+
 upbp_status upbp_beams_build(upbp_ctx* ctx, const upbp_beam_input* beams, const upbp_build_params* params,
                              upbp_beams** out_beams) {
     if (!ctx) return fail(nullptr, UPBP_EINVAL, "ctx is NULL");
@@ -436,6 +463,19 @@ upbp_status upbp_beams_destroy(upbp_beams* beams) {
 
 int64_t upbp_beams_count(const upbp_beams* beams) { return beams ? beams->n : 0; }
 int64_t upbp_beams_piece_count(const upbp_beams* beams) { return beams ? beams->pieces : 0; }
+
+upbp_status upbp_beams_radii(const upbp_beams* beams, float* start_radius, float* end_radius,
+                             upbp_memory memory) {
+    if (!beams) return fail(nullptr, UPBP_EINVAL, "beams is NULL");
+    Ctx* ctx = beams->ctx;
+    return guarded(ctx, [&] {
+        UPBP_REQUIRE((start_radius && end_radius) || beams->n == 0, UPBP_EINVAL,
+                     "start_radius and end_radius are required");
+        UPBP_REQUIRE(memory == UPBP_MEM_HOST || memory == UPBP_MEM_DEVICE, UPBP_EINVAL, "bad memory kind");
+        copy_out_f32(*ctx, beams->r_start, beams->n, start_radius, memory);
+        copy_out_f32(*ctx, beams->r_end, beams->n, end_radius, memory);
+    });
+}
 
 float upbp_beams_selection_pdf(const upbp_beams* beams, const float position[3]) {
     // No per-cell beam reduction is performed (the reference default

@@ -109,6 +109,7 @@ struct Bre {
     DevBuf power;   // float4 [n]
     DevBuf inc;     // float4 [n]
     DevBuf orig;    // uint32 [n] original input index
+    DevBuf radius_in;  // float [n_in] radius the build gave each input photon (0 if filtered out)
 };
 
 struct Beams {
@@ -119,6 +120,7 @@ struct Beams {
     DevBuf piece;         // PieceDev [pieces]
     DevBuf piece_flux;    // float4 [pieces] throughput (rgb, 0)
     DevBuf piece_beam;    // uint32 [pieces] original beam index
+    DevBuf r_start, r_end;  // float [n] start / end radius the build gave each beam
 };
 
 // Device-resident view of a query batch.
@@ -143,6 +145,26 @@ void sort_pairs_u64(Ctx& ctx, uint64_t* keys_in, uint64_t* keys_out, uint32_t* v
 void sort_pairs_u32(Ctx& ctx, uint32_t* keys_in, uint32_t* keys_out, uint32_t* vals_in,
                     uint32_t* vals_out, int64_t n, int end_bit);
 void exclusive_scan_u32(Ctx& ctx, const uint32_t* in, uint32_t* out, int64_t n);
+
+// ---- knn.hip --------------------------------------------------------------
+// Point LBVH answering "squared distance to the k-th nearest point" queries.
+struct KnnIndex {
+    int64_t n = 0;
+    Lbvh bvh;
+    DevBuf points;  // float4 [n] in leaf order
+    // points: device xyz [*][3]; keep: optional device index list of the points to use
+    void build(Ctx& ctx, int64_t n_points, const float* pts, const uint32_t* keep);
+    // out[i] (device) = k-th smallest squared distance from q[i] (device xyz) to the points;
+    // with n <= k the largest of the n.
+    void kth_d2(Ctx& ctx, int64_t nq, const float* q, int k, float* out);
+};
+// KNN_RADIUS of EmbreeBre::build: out_radius[i] (device, [n_in]) = radius * 2 sqrt(d2_k)
+// for the kept photons (keep: kept -> input index, or null for all).
+void photon_knn_radius(Ctx& ctx, int64_t n_kept, const float* pos, const uint32_t* keep, float radius, int knn,
+                       DevBuf& out_radius, int64_t n_in);
+// KNN_RADIUS of PhotonBeamsEvaluator::build: start / end radius per beam (device).
+void beam_knn_radius(Ctx& ctx, int64_t n, const float* origin, const float* dir, const float* length, float radius,
+                     int knn, DevBuf& r0, DevBuf& r1);
 
 // ---- build.hip ------------------------------------------------------------
 void bre_build(Ctx& ctx, const upbp_photon_input& in, const upbp_build_params& params, Bre& out);

@@ -30,7 +30,8 @@ EXPORTED = [
     "upbp_bre_set_power", "upbp_bre_count", "upbp_beams_build", "upbp_beams_destroy", "upbp_beams_count",
     "upbp_beams_piece_count", "upbp_beams_selection_pdf", "upbp_eval", "upbp_hit_signatures",
     "upbp_adjoint", "upbp_batch_plan_create", "upbp_batch_plan_destroy", "upbp_batch_run",
-    "upbp_set_tuning", "upbp_get_tuning", "upbp_autotune", "upbp_eval_upbp",
+    "upbp_set_tuning", "upbp_get_tuning", "upbp_autotune", "upbp_eval_upbp", "upbp_bre_radii",
+    "upbp_beams_radii",
 ]
 
 
@@ -82,6 +83,10 @@ def load():
     L.upbp_bre_set_power.argtypes = [vp, vp, C.c_int]
     L.upbp_bre_count.restype = i64
     L.upbp_bre_count.argtypes = [vp]
+    L.upbp_bre_radii.restype = st
+    L.upbp_bre_radii.argtypes = [vp, vp, C.c_int]
+    L.upbp_beams_radii.restype = st
+    L.upbp_beams_radii.argtypes = [vp, vp, vp, C.c_int]
     L.upbp_beams_build.restype = st
     L.upbp_beams_build.argtypes = [vp, C.POINTER(_abi.BeamInput), C.POINTER(_abi.BuildParams), C.POINTER(vp)]
     L.upbp_beams_destroy.restype = st
@@ -242,6 +247,12 @@ class EmbreeBre:
         self.num_input = photons.count
         return n.value
 
+    def radii(self) -> np.ndarray:
+        """Radius the build gave each input photon (0 where the in-medium filter dropped it)."""
+        r = np.zeros(self.num_input, np.float32)
+        self.ctx.check(self.ctx.lib.upbp_bre_radii(self.h, r.ctypes.data, _abi.MEM_HOST))
+        return r
+
     def set_power(self, power):
         p = _arr(power, np.float32)
         mem = _abi.MEM_DEVICE if (_is_torch(p) and p.is_cuda) else _abi.MEM_HOST
@@ -291,6 +302,13 @@ class PhotonBeamsEvaluator:
     @property
     def pieces(self) -> int:
         return self.ctx.lib.upbp_beams_piece_count(self.h)
+
+    def radii(self):
+        """(mStartRadius, mEndRadius) the build gave each beam."""
+        r0 = np.zeros(self.num_beams, np.float32)
+        r1 = np.zeros(self.num_beams, np.float32)
+        self.ctx.check(self.ctx.lib.upbp_beams_radii(self.h, r0.ctypes.data, r1.ctypes.data, _abi.MEM_HOST))
+        return r0, r1
 
     def getBeamSelectionPdf(self, pos) -> float:
         p = np.ascontiguousarray(pos, np.float32)
