@@ -117,3 +117,123 @@ def test_mis_fp32_against_fp64(which):
     b = _run(w, inm, mi, which, precision=64)
     e = np.abs(a - b).max(1) / np.maximum(np.abs(b).max(1), 1e-30)
     assert np.median(e[b.max(1) > 0]) < 1e-5
+
+
+# ---------------------------------------------------------------------------
+# Structure of 1 / w = w_light + w_camera. For a query with a single hit the
+# UPBP output is w times the plain output, so 1 / w is observable. Every input
+# field of every path vertex enters each term of the balance-heuristic sums at
+# most once, so 1 / w is affine in any one of them; a field that is squared,
+# read from the wrong vertex twice, or dropped shows up here. Which vertices the
+# walks visit is fixed too: never the camera vertex itself (index 0), and on the
+# light side the merged vertex's own merging factors are left to the camera side.
+CAM_FIELDS = ("pdf_a_inv", "rev_pdf_a", "ray_sample_pdf_inv", "ray_sample_pdfs_ratio", "sin_theta",
+              "ray_sample_rev_pdf_inv", "ray_sample_rev_pdfs_ratio", "surf_factor", "pp3d_factor", "pb2d_factor",
+              "bb1d_factor", "bb1d_beam_selection_pdf")
+LIGHT_FIELDS = CAM_FIELDS + ("rev_pdf_a_without_bsdf",)
+
+
+def _single_hit(w, inm, which):
+    if which == 0:
+        oph = po.OraclePhotons(w.photons, in_medium=inm, build_bvh=False)
+        p, sig, _ = po.bre_eval(oph, w.media, w.queries, accel=0, precision=64, signatures=True)
+    else:
+        obm = po.OracleBeams(w.beams, grid_res=0)
+        p, sig, _ = po.beams_eval(obm, w.media, w.queries, accel=0, precision=64, signatures=True)
+    return p, (sig[:, 0] == 1) & (p.max(1) > 0)
+
+
+def _inv_w(w, inm, mi, which, p, qs):
+    u = _run(w, inm, mi, which)
+    ch = p[qs].argmax(1)
+    return p[qs, ch] / u[qs, ch]
+
+
+def _with_camera(mi, cam):
+    return scenes.MisInputs(mi.queries, cam, mi.light_vertices, mi.path_ends, mi.beams, dict(mi.params))
+
+
+def _with_lights(mi, lv):
+    return scenes.MisInputs(mi.queries, mi.camera, lv, mi.path_ends, mi.beams, dict(mi.params))
+
+
+def _affine(iw1, iw2, iw4):
+    # values at scale 1, 2 and 4 (exact in float32): f(4) - f(1) = 3 (f(2) - f(1))
+    return np.all(np.abs(iw4 - 3 * iw2 + 2 * iw1) <= 1e-12 * np.maximum(np.abs(iw4), 1.0))
+
+
+@pytest.mark.parametrize("which", [0, 1])
+def test_inverse_weight_is_affine_in_each_camera_field(which):
+    w, inm, mi = scenes.upbp_scene(24, 14, 1500, 400, seed=58, max_camera_length=3)
+    p, one = _single_hit(w, inm, which)
+    clen = mi.queries["camera_path_length"].astype(np.int64)
+    qs = np.nonzero(one & (clen >= 2))[0]
+    assert len(qs) >= 10
+    off = mi.queries["camera_offset"][qs].astype(np.int64)
+    base = _inv_w(w, inm, mi, which, p, qs)
+    for field in CAM_FIELDS:
+        for pos in (0, 1, "before_last"):
+            idx = off + (clen[qs] - 1 if pos == "before_last" else pos)
+            iw = [base]
+            for s in (2.0, 4.0):
+                cam = mi.camera.copy()
+                cam[field][idx] *= s
+                iw.append(_inv_w(w, inm, _with_camera(mi, cam), which, p, qs))
+            assert _affine(*iw), (field, pos)
+            if pos == 0:
+                # the camera vertex itself is never visited
+                assert np.array_equal(iw[1], base), field
+    # the merging factor of a visited camera vertex counts iff its technique is enabled
+    idx = off + 1
+    cam = mi.camera.copy()
+    cam["pb2d_factor"][idx] *= 2.0
+    assert np.all(_inv_w(w, inm, _with_camera(mi, cam), which, p, qs) > base)
+    mi_off = _with_camera(mi, mi.camera)
+    mi_off.params["techniques"] = mi.params["techniques"] & ~scenes.PB2D
+    p_off = _inv_w(w, inm, mi_off, which, p, qs)
+    assert np.array_equal(_inv_w(w, inm, _with_camera(mi_off, cam), which, p, qs), p_off)
+
+
+def _hit_photon(w, inm):
+    """Single-hit queries and the photon (= light vertex) each one merged with, read back
+    from the hit signature (count 1, hash of the photon index)."""
+    oph = po.OraclePhotons(w.photons, in_medium=inm, build_bvh=False)
+    p, sig, _ = po.bre_eval(oph, w.media, w.queries, accel=0, precision=64, signatures=True)
+    one = (sig[:, 0] == 1) & (p.max(1) > 0)
+    h = po.hash32(np.arange(w.photons.count, dtype=np.uint32))
+    assert len(np.unique(h)) == len(h)
+    lookup = dict(zip(h.tolist(), range(len(h))))
+    qs = np.nonzero(one)[0]
+    return p, qs, np.array([lookup[int(x)] for x in sig[qs, 1]], np.int64)
+
+
+def test_inverse_weight_is_affine_in_each_light_field():
+    which = 0
+    w, inm, mi = scenes.upbp_scene(64, 36, 800, 400, seed=59, max_light_length=4)
+    p, qs, v = _hit_photon(w, inm)
+    lv = mi.light_vertices
+    depth = lv["path_length"].astype(np.int64)
+    # one query per light path, photons at depth >= 2 so that indices 0, 1 and 2 of the walk exist
+    keep = depth[v] >= 2
+    _, first = np.unique(lv["path_idx"][v[keep]], return_index=True)
+    qs, v = qs[keep][first], v[keep][first]
+    assert len(qs) >= 10
+    base = _inv_w(w, inm, mi, which, p, qs)
+    for field in LIGHT_FIELDS:
+        for index in (0, 1, 2):
+            iw = [base]
+            for s in (2.0, 4.0):
+                lv2 = lv.copy()
+                lv2["mis"][field][v - index] *= s
+                iw.append(_inv_w(w, inm, _with_lights(mi, lv2), which, p, qs))
+            assert _affine(*iw), (field, index)
+            unused = (index == 0 and field in ("rev_pdf_a", "surf_factor", "pp3d_factor", "pb2d_factor",
+                                               "bb1d_factor", "rev_pdf_a_without_bsdf", "sin_theta",
+                                               "ray_sample_rev_pdf_inv", "ray_sample_rev_pdfs_ratio")) or \
+                     (index == 1 and field == "rev_pdf_a")
+            if unused:
+                # index 0 takes its reverse data from the hit and leaves merging to the camera side;
+                # index 1 uses the reverse pdf without the BSDF term
+                assert np.array_equal(iw[1], base), (field, index)
+            elif field in ("rev_pdf_a_without_bsdf", "pdf_a_inv") and index == 1:
+                assert np.all(iw[1] != base), (field, index)
