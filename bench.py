@@ -221,11 +221,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds of CPU oracle work (0 = skip)")
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--order", choices=["sorted", "as_given"], default="sorted")
+    ap.add_argument("--order", choices=["sorted", "as_given", "tuned"], default="tuned")
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--leaf-size", type=int, default=0)
     ap.add_argument("--piece-factor", type=float, default=0.0)
     ap.add_argument("--waves-per-block", type=int, default=0)
+    ap.add_argument("--no-tuning", action="store_true",
+                    help="ignore smallupbp_amd/tuning_cache.json and use the library defaults")
     ap.add_argument("--batched", type=float, default=0.0,
                     help="also time the 64-instance batched form at this size scale (1 = full size)")
     args = ap.parse_args()
@@ -245,18 +247,30 @@ def main():
     t = time.time()
     w = scenes.bre_beams()
     log("workload %s generated in %.1fs" % (w.name, time.time() - t))
+    # launch / build parameters: command line, else the autotuner's cached best for this size, else defaults
+    from smallupbp_amd import autotune
+
+    tuned = None if args.no_tuning else autotune.lookup(w.photons.count, w.beams.count, w.queries.count)
+    best_ph = (tuned or {}).get("best", {}).get("photons") or {}
+    best_bm = (tuned or {}).get("best", {}).get("beams") or {}
+    leaf_ph = args.leaf_size or int(best_ph.get("leaf_size", 0))
+    leaf_bm = args.leaf_size or int(best_bm.get("leaf_size", 0))
+    piece_factor = args.piece_factor or float(best_bm.get("piece_factor", 0.0))
+    waves = args.waves_per_block or int(best_bm.get("waves_per_block", 0))
+    if args.order == "tuned":
+        args.order = "as_given" if best_bm.get("order") == _abi.ORDER_AS_GIVEN else "sorted"
     ctx = native.Context(0)
     ctx.set_media(w.media)
-    if args.waves_per_block:
-        ctx.set_tuning(args.waves_per_block)
+    if waves:
+        ctx.set_tuning(waves)
     bre = native.EmbreeBre(ctx)
     beams = native.PhotonBeamsEvaluator(ctx)
     torch.cuda.synchronize()
     t = time.perf_counter()
-    bre.build(w.photons, leaf_size=args.leaf_size)
+    bre.build(w.photons, leaf_size=leaf_ph)
     t_bp = time.perf_counter() - t
     t = time.perf_counter()
-    beams.build(w.beams, leaf_size=args.leaf_size, piece_factor=args.piece_factor)
+    beams.build(w.beams, leaf_size=leaf_bm, piece_factor=piece_factor)
     t_bb = time.perf_counter() - t
     log("build: photons %.1f ms, beams %.1f ms (%d pieces)" % (t_bp * 1e3, t_bb * 1e3, beams.pieces))
     order = _abi.ORDER_SORTED if args.order == "sorted" else _abi.ORDER_AS_GIVEN
@@ -316,6 +330,9 @@ def main():
             "media": "box sigma_s=0.2 sigma_a=0.02 + sphere r=0.3 sigma_s=2.0 sigma_a=0.05",
             "query_order": args.order,
             "beam_pieces": beams.pieces,
+            "tuning": {"photon_leaf_size": leaf_ph or "default", "beam_leaf_size": leaf_bm or "default",
+                       "beam_piece_factor": piece_factor or "default", "waves_per_block": waves or "default",
+                       "source": "command line / tuning_cache.json" if tuned else "command line / defaults"},
             "parallelism": "single GPU",
         },
         "kernel_ms": {"photons": round(ms_ph, 4), "beams": round(ms_bm, 4), "sort_and_clear": round(ms_ot, 4)},

@@ -199,6 +199,9 @@ typedef struct upbp_eval_stats {
 /* ---------------------------------------------------------------------------
  * Context. */
 upbp_status upbp_ctx_create(int device, upbp_ctx** out_ctx);
+/* Drops the caller's reference. Photon and beam structures and batch plans
+ * built on the context hold references of their own, so they may be destroyed
+ * before or after it, in any order; the context is freed with the last one. */
 upbp_status upbp_ctx_destroy(upbp_ctx* ctx);
 const char* upbp_ctx_last_error(const upbp_ctx* ctx);
 /* Use an existing HIP stream (hipStream_t passed as void*); NULL restores the

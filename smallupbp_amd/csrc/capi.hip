@@ -49,6 +49,8 @@ upbp_status fail(Ctx* ctx, upbp_status st, const std::string& msg) {
 template <class F>
 upbp_status guarded(Ctx* ctx, F&& f) {
     try {
+        // start from a clear HIP error state, so that launch checks report this call's errors only
+        (void)hipGetLastError();
         if (ctx) UPBP_HIP(hipSetDevice(ctx->device));
         f();
         if (ctx) ctx->last_error.clear();
@@ -323,14 +325,26 @@ upbp_status upbp_ctx_create(int device, upbp_ctx** out_ctx) {
     }
 }
 
-upbp_status upbp_ctx_destroy(upbp_ctx* ctx) {
-    if (!ctx) return UPBP_OK;
+namespace {
+void ctx_retain(Ctx* ctx) { ctx->refs.fetch_add(1); }
+
+// Frees the context once the caller and every structure built on it have let go.
+void ctx_release(Ctx* ctx) {
+    if (ctx->refs.fetch_sub(1) != 1) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto& ev : ctx->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
-    delete ctx;
+    delete static_cast<upbp_ctx*>(ctx);
+    // a failed teardown call must not surface as the error of an unrelated later launch
+    (void)hipGetLastError();
+}
+}  // namespace
+
+upbp_status upbp_ctx_destroy(upbp_ctx* ctx) {
+    if (!ctx) return UPBP_OK;
+    ctx_release(ctx);
     return UPBP_OK;
 }
 
@@ -385,6 +399,7 @@ upbp_status upbp_bre_build(upbp_ctx* ctx, const upbp_photon_input* photons, cons
         delete b;
         return st;
     }
+    ctx_retain(ctx);
     *out_bre = b;
     if (out_num_photons) *out_num_photons = b->n;
     return UPBP_OK;
@@ -392,8 +407,10 @@ upbp_status upbp_bre_build(upbp_ctx* ctx, const upbp_photon_input* photons, cons
 
 upbp_status upbp_bre_destroy(upbp_bre* bre) {
     if (!bre) return UPBP_OK;
-    if (bre->ctx) (void)hipSetDevice(bre->ctx->device);
+    Ctx* ctx = bre->ctx;
+    (void)hipSetDevice(ctx->device);
     delete bre;
+    ctx_release(ctx);
     return UPBP_OK;
 }
 
@@ -450,14 +467,17 @@ upbp_status upbp_beams_build(upbp_ctx* ctx, const upbp_beam_input* beams, const 
         delete b;
         return st;
     }
+    ctx_retain(ctx);
     *out_beams = b;
     return UPBP_OK;
 }
 
 upbp_status upbp_beams_destroy(upbp_beams* beams) {
     if (!beams) return UPBP_OK;
-    if (beams->ctx) (void)hipSetDevice(beams->ctx->device);
+    Ctx* ctx = beams->ctx;
+    (void)hipSetDevice(ctx->device);
     delete beams;
+    ctx_release(ctx);
     return UPBP_OK;
 }
 
@@ -719,14 +739,18 @@ upbp_status upbp_batch_plan_create(upbp_ctx* ctx, const upbp_instance* instances
         delete plan;
         return st;
     }
+    plan->ctx = ctx;
+    ctx_retain(ctx);
     *out_plan = plan;
     return UPBP_OK;
 }
 
 upbp_status upbp_batch_plan_destroy(upbp_batch_plan* plan) {
     if (!plan) return UPBP_OK;
-    if (plan->ctx) (void)hipSetDevice(plan->ctx->device);
+    Ctx* ctx = plan->ctx;
+    (void)hipSetDevice(ctx->device);
     delete plan;
+    ctx_release(ctx);
     return UPBP_OK;
 }
 

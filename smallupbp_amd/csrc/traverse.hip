@@ -53,6 +53,7 @@ struct EvalArgs {
     const MediumDev* __restrict__ media;
     int32_t n_media;
     int32_t media_base;
+    int32_t lds_media_n;  // adjoint: media with an LDS gradient slab (min(n_media, kMaxLdsMedia))
     // outputs
     float* __restrict__ out;
     uint32_t* __restrict__ sig;
@@ -579,7 +580,7 @@ __device__ __forceinline__ float s_tmax_of(const EvalArgs& A, int64_t i) { retur
 // Processes the 64 queries of one tile (wave). `slot0` is the first lane slot.
 template <int EST, int MODE>
 __device__ void run_tile(const EvalArgs& A, int64_t slot0, unsigned long long* local_stats,
-                         float* lds_media /* adjoint: [kMaxLdsMedia][6][64] for this wave or null */) {
+                         float* lds_media /* adjoint: [lds_media_n][6][64] for this wave or null */) {
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t slot = slot0 + lane;
     const bool valid = slot < A.count;
@@ -739,7 +740,7 @@ __device__ void run_tile(const EvalArgs& A, int64_t slot0, unsigned long long* l
             if (MODE == (int)Mode::Adjoint && scatter) {
                 // medium gradients of this segment plus the prefix terms of earlier segments
                 const int mloc = A.media_base + s.medium;
-                float* slab_m = (lds_media && s.medium < kMaxLdsMedia) ? lds_media : nullptr;
+                float* slab_m = (lds_media && s.medium < A.lds_media_n) ? lds_media : nullptr;
                 if (slab_m) {
                     float* base = slab_m + (size_t)s.medium * 6 * kWave + lane;
                     base[0 * kWave] += s.gs0;
@@ -763,7 +764,7 @@ __device__ void run_tile(const EvalArgs& A, int64_t slot0, unsigned long long* l
                     const float len = sp->dist_max - sp->dist_min;
                     const int32_t mj = sp->medium_id;
                     const float d0 = -s.acc0 * len, d1 = -s.acc1 * len, d2 = -s.acc2 * len;
-                    if (lds_media && mj < kMaxLdsMedia) {
+                    if (lds_media && mj < A.lds_media_n) {
                         float* base = lds_media + (size_t)mj * 6 * kWave + lane;
                         base[3 * kWave] += d0;
                         base[4 * kWave] += d1;
@@ -859,17 +860,19 @@ __global__ __launch_bounds__(256) void k_eval(EvalArgs A) {
     const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
     const int64_t wave = blk * waves_per_block + wave_in_block;
     unsigned long long ls[kStatSlots] = {0, 0, 0, 0, 0, 0, 0, 0};
-    __shared__ float lds[MODE == (int)Mode::Adjoint ? 4 * kMaxLdsMedia * 6 * kWave : 1];
+    // adjoint: per-lane medium gradient slabs, [lds_media_n][6][64] floats per wave, sized at launch
+    extern __shared__ float lds_dyn[];
     float* my_lds = nullptr;
     if (MODE == (int)Mode::Adjoint) {
-        my_lds = lds + (size_t)wave_in_block * kMaxLdsMedia * 6 * kWave;
-        for (int k = threadIdx.x & (kWave - 1); k < kMaxLdsMedia * 6 * kWave; k += kWave) my_lds[k] = 0.f;
+        const int slab = A.lds_media_n * 6 * kWave;
+        my_lds = lds_dyn + (size_t)wave_in_block * slab;
+        for (int k = threadIdx.x & (kWave - 1); k < slab; k += kWave) my_lds[k] = 0.f;
     }
     if (wave * kWave < A.count) run_tile<EST, MODE>(A, wave * kWave, ls, my_lds);
     if (MODE == (int)Mode::Adjoint) {
         // reduce the per-lane medium slabs of this wave and add them once
         const int lane = threadIdx.x & (kWave - 1);
-        const int nm = min(A.n_media, kMaxLdsMedia);
+        const int nm = A.lds_media_n;
         for (int e = 0; e < nm * 6; ++e) {
             const float v = wave_sum_dpp(my_lds[(size_t)e * kWave + lane]);
             if (lane == 0 && v != 0.f) unsafeAtomicAdd(&A.grad_media[(size_t)(A.media_base) * 6 + e], v);
@@ -988,9 +991,12 @@ void launch_mode(Ctx& ctx, Mode mode, const EvalArgs& A, unsigned blocks, unsign
         case Mode::Signature:
             hipLaunchKernelGGL((k_eval<EST, (int)Mode::Signature>), dim3(blocks), dim3(threads), 0, ctx.stream, A);
             break;
-        case Mode::Adjoint:
-            hipLaunchKernelGGL((k_eval<EST, (int)Mode::Adjoint>), dim3(blocks), dim3(threads), 0, ctx.stream, A);
+        case Mode::Adjoint: {
+            // LDS gradient slabs for the media actually present: fewer bytes, more resident waves
+            const size_t lds = (size_t)(threads / kWave) * A.lds_media_n * 6 * kWave * sizeof(float);
+            hipLaunchKernelGGL((k_eval<EST, (int)Mode::Adjoint>), dim3(blocks), dim3(threads), lds, ctx.stream, A);
             break;
+        }
         case Mode::Upbp:
             hipLaunchKernelGGL((k_eval<EST, (int)Mode::Upbp>), dim3(blocks), dim3(threads), 0, ctx.stream, A);
             break;
@@ -1011,6 +1017,7 @@ EvalArgs base_args(Ctx& ctx, const QueryView& q) {
     A.media = ctx.media.as<MediumDev>();
     A.n_media = ctx.n_media;
     A.media_base = 0;
+    A.lds_media_n = std::max(1, std::min(ctx.n_media, kMaxLdsMedia));
     A.stats = ctx.stats.as<unsigned long long>();
     return A;
 }

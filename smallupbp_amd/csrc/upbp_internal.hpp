@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -75,6 +76,9 @@ struct Tuning {
 };
 
 struct Ctx {
+    // References: the caller's (dropped by upbp_ctx_destroy) plus one per photon
+    // structure, beam structure and batch plan built on this context.
+    std::atomic<int> refs{1};
     int device = 0;
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;

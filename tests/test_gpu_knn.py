@@ -75,6 +75,8 @@ def test_beam_knn_radii_bit_exact(knn):
 def test_knn_estimates_match_oracle_with_those_radii():
     print_case("knn_estimates")
     w = scenes.bre_beams(96, 54, 40_000, 8_000, seed=14)
+    # KNN radii replace the scene's per-photon radii
+    w.photons = scenes.Photons(w.photons.position, w.photons.inc_dir, w.photons.power, None, 0.0)
     ctx = gpu_context(w.media)
     bre = native.EmbreeBre(ctx)
     bre.build(w.photons, radius=0.3, radius_calculation=native._abi.KNN_RADIUS, knn=10)

@@ -183,6 +183,30 @@ def test_input_errors_are_reported():
     assert e.value.status == _abi.UPBP_ENONFINITE
 
 
+def test_structures_keep_their_context_alive_in_any_destroy_order():
+    # Python may finalise a context before the structures built on it (garbage
+    # collection of reference cycles); the library must tolerate either order and
+    # leave no error behind for the next call
+    print_case("lifetime")
+    w = scenes.bre_beams(16, 9, 2000, 300, seed=21)
+    for ctx_first in (True, False):
+        ctx = gpu_context(w.media)
+        bre = native.EmbreeBre(ctx)
+        bre.build(w.photons)
+        bm = native.PhotonBeamsEvaluator(ctx)
+        bm.build(w.beams)
+        if ctx_first:
+            ctx.close()
+        bre.destroy()
+        bm.destroy()
+        ctx.close()
+    ctx = gpu_context(w.media)
+    bm = native.PhotonBeamsEvaluator(ctx)
+    bm.build(w.beams)
+    out = bm.evalBeamBeamEstimate(native.QueryBatchView.to_device(w.queries)).cpu().numpy()
+    assert np.isfinite(out).all()
+
+
 def _golden_names():
     from golden import fixtures
 
