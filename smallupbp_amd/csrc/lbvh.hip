@@ -32,13 +32,10 @@ __host__ __device__ inline float ord2f(uint32_t u) {
 
 // bounds[0..2] = min item-box lo, [3..5] = max item-box hi,
 // [6..8] = min centroid, [9..11] = max centroid (order-preserving uint encoding).
-__global__ void k_bounds_init(uint32_t* b) {
-    int i = threadIdx.x;
-    if (i < 12) b[i] = ((i / 3) % 2 == 0) ? 0xFFFFFFFFu : 0u;
-}
-
+// Per-block partial bounds; k_bounds_final folds the partials (no global atomics:
+// thousands of atomics on the same twelve words serialise).
 __global__ __launch_bounds__(kBlock) void k_bounds(int64_t n, const float4* lo, const float4* hi,
-                                                   uint32_t* b) {
+                                                   uint32_t* partial) {
     uint32_t v[12];
     for (int k = 0; k < 12; ++k) v[k] = ((k / 3) % 2 == 0) ? 0xFFFFFFFFu : 0u;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -53,21 +50,32 @@ __global__ __launch_bounds__(kBlock) void k_bounds(int64_t n, const float4* lo, 
             v[9 + a] = max(v[9 + a], f2ord(c[a]));
         }
     }
-    // wave reduction, one atomic per wave and value
     for (int off = 32; off > 0; off >>= 1) {
         for (int k = 0; k < 12; ++k) {
             uint32_t o = __shfl_xor(v[k], off, 64);
             v[k] = ((k / 3) % 2 == 0) ? min(v[k], o) : max(v[k], o);
         }
     }
-    if ((threadIdx.x & 63) == 0) {
-        for (int k = 0; k < 12; ++k) {
-            if ((k / 3) % 2 == 0)
-                atomicMin(&b[k], v[k]);
-            else
-                atomicMax(&b[k], v[k]);
-        }
+    __shared__ uint32_t sh[kBlock / 64][12];
+    const int w = threadIdx.x / 64;
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 12; ++k) sh[w][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x < 12) {
+        const int k = threadIdx.x;
+        uint32_t r = sh[0][k];
+        for (int j = 1; j < (int)(blockDim.x / 64); ++j) r = ((k / 3) % 2 == 0) ? min(r, sh[j][k]) : max(r, sh[j][k]);
+        partial[(size_t)blockIdx.x * 12 + k] = r;
     }
+}
+
+__global__ void k_bounds_final(int nb, const uint32_t* partial, uint32_t* b) {
+    const int k = threadIdx.x;
+    if (k >= 12) return;
+    const bool is_min = (k / 3) % 2 == 0;
+    uint32_t r = is_min ? 0xFFFFFFFFu : 0u;
+    for (int i = 0; i < nb; ++i) r = is_min ? min(r, partial[(size_t)i * 12 + k]) : max(r, partial[(size_t)i * 12 + k]);
+    b[k] = r;
 }
 
 __device__ __forceinline__ uint64_t spread21(uint64_t x) {
@@ -151,44 +159,64 @@ __global__ __launch_bounds__(kBlock) void k_karras(int64_t n, const uint64_t* ke
     last[i] = (uint32_t)hi;
 }
 
-// Bottom-up refit: every leaf climbs; the second thread to reach a node merges
-// its two children (agent-scope acquire/release on the arrival counter).
-__global__ __launch_bounds__(kBlock) void k_refit(int64_t n, const uint32_t* order, const float4* lo,
-                                                  const float4* hi, const uint32_t* left,
-                                                  const uint32_t* right, const uint32_t* leaf_parent,
-                                                  const uint32_t* int_parent, uint32_t* arrive,
-                                                  float4* ilo, float4* ihi) {
-    int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    uint32_t node = leaf_parent[k];
-    while (true) {
-        uint32_t prev = __hip_atomic_fetch_add(&arrive[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (prev == 0) return;
-        float4 b0l, b0h, b1l, b1h;
-        uint32_t c0 = left[node], c1 = right[node];
-        if (c0 & 0x80000000u) {
-            uint32_t it = order[c0 & 0x7FFFFFFFu];
-            b0l = lo[it];
-            b0h = hi[it];
-        } else {
-            b0l = ilo[c0];
-            b0h = ihi[c0];
-        }
-        if (c1 & 0x80000000u) {
-            uint32_t it = order[c1 & 0x7FFFFFFFu];
-            b1l = lo[it];
-            b1h = hi[it];
-        } else {
-            b1l = ilo[c1];
-            b1h = ihi[c1];
-        }
-        float4 ml = make_float4(fminf(b0l.x, b1l.x), fminf(b0l.y, b1l.y), fminf(b0l.z, b1l.z), 0.f);
-        float4 mh = make_float4(fmaxf(b0h.x, b1h.x), fmaxf(b0h.y, b1h.y), fmaxf(b0h.z, b1h.z), 0.f);
-        ilo[node] = ml;
-        ihi[node] = mh;
-        if (node == 0) return;
+// Level-synchronous refit. Every internal node's depth is found by walking its
+// parent links; nodes are grouped by depth and merged one level per launch, deepest
+// first, so a node's children are final before it reads them. Kernel boundaries
+// order the levels: no atomics or device-scope fences, which on a multi-XCD GPU
+// would write back and invalidate L2 at every level of every climb.
+constexpr uint32_t kMaxDepth = 128;  // > Karras depth bound (63 key bits + 31 index bits)
+
+__global__ __launch_bounds__(kBlock) void k_depth(int64_t ni, const uint32_t* int_parent, uint32_t* depth,
+                                                  uint32_t* ids, uint32_t* err) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= ni) return;
+    uint32_t node = (uint32_t)i, d = 0;
+    while (node != 0 && d < kMaxDepth) {
         node = int_parent[node];
+        ++d;
     }
+    if (d >= kMaxDepth) {
+        atomicOr(err, 1u);
+        d = kMaxDepth - 1;
+    }
+    depth[i] = d;
+    ids[i] = (uint32_t)i;
+}
+
+// start[d] = first position of depth d in the depth-sorted node list
+__global__ __launch_bounds__(kBlock) void k_level_starts(int64_t ni, const uint32_t* sorted_depth, uint32_t* start) {
+    int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (j >= ni) return;
+    const uint32_t d = sorted_depth[j];
+    if (j == 0 || sorted_depth[j - 1] != d) start[d] = (uint32_t)j;
+}
+
+__global__ __launch_bounds__(kBlock) void k_refit_level(int64_t count, const uint32_t* ids, const uint32_t* order,
+                                                        const float4* lo, const float4* hi, const uint32_t* left,
+                                                        const uint32_t* right, float4* ilo, float4* ihi) {
+    int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const uint32_t node = ids[t];
+    float4 b0l, b0h, b1l, b1h;
+    const uint32_t c0 = left[node], c1 = right[node];
+    if (c0 & 0x80000000u) {
+        const uint32_t it = order[c0 & 0x7FFFFFFFu];
+        b0l = lo[it];
+        b0h = hi[it];
+    } else {
+        b0l = ilo[c0];
+        b0h = ihi[c0];
+    }
+    if (c1 & 0x80000000u) {
+        const uint32_t it = order[c1 & 0x7FFFFFFFu];
+        b1l = lo[it];
+        b1h = hi[it];
+    } else {
+        b1l = ilo[c1];
+        b1h = ihi[c1];
+    }
+    ilo[node] = make_float4(fminf(b0l.x, b1l.x), fminf(b0l.y, b1l.y), fminf(b0l.z, b1l.z), 0.f);
+    ihi[node] = make_float4(fmaxf(b0h.x, b1h.x), fmaxf(b0h.y, b1h.y), fmaxf(b0h.z, b1h.z), 0.f);
 }
 
 __global__ __launch_bounds__(kBlock) void k_emit(int64_t n, int leaf_size, float eps,
@@ -278,11 +306,13 @@ void build_lbvh(Ctx& ctx, int64_t n, const float4* box_lo, const float4* box_hi,
     UPBP_REQUIRE(n > 0 && n < (int64_t)0x7FFFFFFF, UPBP_EINVAL, "LBVH item count out of range");
     hipStream_t st = ctx.stream;
     out.n_items = n;
-    DevBuf bounds;
+    DevBuf bounds, partial;
     bounds.ensure(12 * 4);
-    hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, st, bounds.as<uint32_t>());
-    unsigned gb = std::min<unsigned>(grid_for(n), 2048u);
-    hipLaunchKernelGGL(k_bounds, dim3(gb), dim3(kBlock), 0, st, n, box_lo, box_hi, bounds.as<uint32_t>());
+    const unsigned gb = std::min<unsigned>(grid_for(n), 1024u);
+    partial.ensure((size_t)gb * 12 * 4);
+    hipLaunchKernelGGL(k_bounds, dim3(gb), dim3(kBlock), 0, st, n, box_lo, box_hi, partial.as<uint32_t>());
+    UPBP_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(64), 0, st, (int)gb, partial.as<uint32_t>(), bounds.as<uint32_t>());
     UPBP_HIP(hipGetLastError());
     uint32_t hb[12];
     UPBP_HIP(hipMemcpyAsync(hb, bounds.p, sizeof(hb), hipMemcpyDeviceToHost, st));
@@ -324,26 +354,54 @@ void build_lbvh(Ctx& ctx, int64_t n, const float4* box_lo, const float4* box_hi,
     sort_pairs_u64(ctx, keys.as<uint64_t>(), keys2.as<uint64_t>(), vals.as<uint32_t>(),
                    out_order.as<uint32_t>(), n, 63);
     const int64_t ni = n - 1;
-    DevBuf left, right, first, last, leaf_parent, int_parent, arrive, ilo, ihi;
+    DevBuf left, right, first, last, leaf_parent, int_parent, ilo, ihi;
     left.ensure(ni * 4);
     right.ensure(ni * 4);
     first.ensure(ni * 4);
     last.ensure(ni * 4);
     leaf_parent.ensure(n * 4);
     int_parent.ensure(ni * 4);
-    arrive.ensure(ni * 4);
     ilo.ensure(ni * 16);
     ihi.ensure(ni * 16);
-    UPBP_HIP(hipMemsetAsync(arrive.p, 0, ni * 4, st));
     hipLaunchKernelGGL(k_karras, dim3(grid_for(ni)), dim3(kBlock), 0, st, n, keys2.as<uint64_t>(),
                        left.as<uint32_t>(), right.as<uint32_t>(), first.as<uint32_t>(),
                        last.as<uint32_t>(), leaf_parent.as<uint32_t>(), int_parent.as<uint32_t>());
     UPBP_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_refit, dim3(grid_for(n)), dim3(kBlock), 0, st, n, out_order.as<uint32_t>(),
-                       box_lo, box_hi, left.as<uint32_t>(), right.as<uint32_t>(),
-                       leaf_parent.as<uint32_t>(), int_parent.as<uint32_t>(), arrive.as<uint32_t>(),
-                       ilo.as<float4>(), ihi.as<float4>());
-    UPBP_HIP(hipGetLastError());
+    {
+        // group internal nodes by depth, then merge bottom-up one level per launch
+        DevBuf depth, depth_sorted, ids, ids_sorted, starts, err;
+        depth.ensure(ni * 4);
+        depth_sorted.ensure(ni * 4);
+        ids.ensure(ni * 4);
+        ids_sorted.ensure(ni * 4);
+        starts.ensure(kMaxDepth * 4);
+        err.ensure(4);
+        launch_fill_u32(ctx, err.as<uint32_t>(), 0u, 1);
+        hipLaunchKernelGGL(k_depth, dim3(grid_for(ni)), dim3(kBlock), 0, st, ni, int_parent.as<uint32_t>(),
+                           depth.as<uint32_t>(), ids.as<uint32_t>(), err.as<uint32_t>());
+        UPBP_HIP(hipGetLastError());
+        sort_pairs_u32(ctx, depth.as<uint32_t>(), depth_sorted.as<uint32_t>(), ids.as<uint32_t>(),
+                       ids_sorted.as<uint32_t>(), ni, 8);
+        launch_fill_u32(ctx, starts.as<uint32_t>(), (uint32_t)ni, kMaxDepth);
+        hipLaunchKernelGGL(k_level_starts, dim3(grid_for(ni)), dim3(kBlock), 0, st, ni,
+                           depth_sorted.as<uint32_t>(), starts.as<uint32_t>());
+        UPBP_HIP(hipGetLastError());
+        uint32_t hs[kMaxDepth + 1], herr = 0, dmax = 0;
+        UPBP_HIP(hipMemcpyAsync(hs, starts.p, kMaxDepth * 4, hipMemcpyDeviceToHost, st));
+        UPBP_HIP(hipMemcpyAsync(&herr, err.p, 4, hipMemcpyDeviceToHost, st));
+        UPBP_HIP(hipMemcpyAsync(&dmax, depth_sorted.as<uint32_t>() + (ni - 1), 4, hipMemcpyDeviceToHost, st));
+        UPBP_HIP(hipStreamSynchronize(st));
+        UPBP_REQUIRE(herr == 0 && dmax < kMaxDepth, UPBP_EDEVICE, "LBVH deeper than expected");
+        hs[dmax + 1] = (uint32_t)ni;
+        for (int d = (int)dmax; d >= 0; --d) {
+            const int64_t cnt = (int64_t)hs[d + 1] - hs[d];
+            UPBP_REQUIRE(cnt > 0 && hs[d] < (uint32_t)ni, UPBP_EDEVICE, "LBVH level table inconsistent");
+            hipLaunchKernelGGL(k_refit_level, dim3(grid_for(cnt)), dim3(kBlock), 0, st, cnt,
+                               ids_sorted.as<uint32_t>() + hs[d], out_order.as<uint32_t>(), box_lo, box_hi,
+                               left.as<uint32_t>(), right.as<uint32_t>(), ilo.as<float4>(), ihi.as<float4>());
+            UPBP_HIP(hipGetLastError());
+        }
+    }
     out.nodes.ensure((size_t)ni * sizeof(NodeDev));
     out.n_nodes = ni;
     hipLaunchKernelGGL(k_emit, dim3(grid_for(ni)), dim3(kBlock), 0, st, n, leaf_size, eps,

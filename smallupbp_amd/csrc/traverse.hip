@@ -381,7 +381,8 @@ __device__ __forceinline__ void photon_leaf(const EvalArgs& A, const Ray& r, Seg
                     ph = phase_pdf(s.g, false, dot3(in.x, in.y, in.z, r.dx, r.dy, r.dz));
                 }
                 // Epanechnikov kernel (1 - s/r^2) / (r^2 pi / 2), Bre.cxx:272-273
-                const float kern = 1.0f - sq / p.w;
+                // the kernel value only scales the contribution: a 1-ulp reciprocal suffices
+                const float kern = 1.0f - sq * __builtin_amdgcn_rcpf(p.w);
                 const float scale = (ph * kern) * w.w;
                 // base_c = a_c * scale; contribution_c = power_c * base_c * sigma_s_c
                 const float b0 = a0 * scale, b1 = a1 * scale, b2 = a2 * scale;
@@ -465,7 +466,7 @@ __device__ __forceinline__ void beam_leaf(const EvalArgs& A, const Ray& r, SegSt
                 const float ph = s.iso ? kInvPi * 0.25f
                                        : phase_pdf(s.g, false, -dot3(pc.p1.x, pc.p1.y, pc.p1.z, r.dx, r.dy, r.dz));
                 // 1D kernel (1 - dist^2/r^2) * 3 / (4 r sin(theta)), PhotonBeam.hxx:364-365
-                const float kern = 1.0f - (dist * dist) / (radius * radius);
+                const float kern = 1.0f - (dist * dist) * __builtin_amdgcn_rcpf(radius * radius);
                 const float scale = (ph * kern * 3.0f) * __builtin_amdgcn_rcpf((4.0f * radius) * st);
                 const float b0 = a0 * scale, b1 = a1 * scale, b2 = a2 * scale;
                 const float c0 = flux.x * s.ss0 * b0, c1 = flux.y * s.ss1 * b1, c2 = flux.z * s.ss2 * b2;
