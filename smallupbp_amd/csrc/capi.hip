@@ -23,6 +23,7 @@ struct upbp_batch_plan {
     std::vector<InstanceDev> host;
     DevBuf table;
     DevBuf counters;  // [2] queue counters (photons, beams)
+    DevBuf args[2];   // per-instance kernel arguments (photons, beams), refreshed by every run
     std::vector<DevBuf*> perms;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -245,6 +246,20 @@ void run_eval(Ctx& ctx, const Bre* bre, const Beams* bm, const upbp_query_batch*
         if (sph) { ctx.staging[6].ensure(Q * 8); sph = ctx.staging[6].as<uint32_t>(); }
         if (sbm) { ctx.staging[7].ensure(Q * 8); sbm = ctx.staging[7].as<uint32_t>(); }
     }
+    // Photons go to the side stream when beams also run, so the two kernels overlap; each
+    // kernel's events are recorded on the stream it runs on.
+    const bool overlap = want_ph && want_bm && bre->n > 0 && bm->pieces > 0 && Q > 0;
+    hipStream_t main_stream = ctx.stream;
+    struct StreamRestore {
+        Ctx& c;
+        hipStream_t s;
+        ~StreamRestore() { c.stream = s; }
+    } restore{ctx, main_stream};
+    if (overlap) {
+        UPBP_HIP(hipEventRecord(ctx.fork, main_stream));
+        UPBP_HIP(hipStreamWaitEvent(ctx.side, ctx.fork, 0));
+        ctx.stream = ctx.side;
+    }
     if (timing) UPBP_HIP(hipEventRecord(ctx.ev[1], ctx.stream));
     uint32_t launches = 0;
     if (want_ph) {
@@ -257,6 +272,11 @@ void run_eval(Ctx& ctx, const Bre* bre, const Beams* bm, const upbp_query_batch*
         }
     }
     if (timing) UPBP_HIP(hipEventRecord(ctx.ev[2], ctx.stream));
+    if (overlap) {
+        UPBP_HIP(hipEventRecord(ctx.join, ctx.side));
+        ctx.stream = main_stream;
+    }
+    if (timing) UPBP_HIP(hipEventRecord(ctx.ev[4], ctx.stream));
     if (want_bm) {
         if (bm->pieces > 0) {
             launch_beams(ctx, mode, beam_args(bm), v, dbm, sbm, adj_bm, mis);
@@ -267,6 +287,7 @@ void run_eval(Ctx& ctx, const Bre* bre, const Beams* bm, const upbp_query_batch*
         }
     }
     if (timing) UPBP_HIP(hipEventRecord(ctx.ev[3], ctx.stream));
+    if (overlap) UPBP_HIP(hipStreamWaitEvent(ctx.stream, ctx.join, 0));
     if (out_mem == UPBP_MEM_HOST && Q > 0) {
         if (o.out_ph && want_ph) UPBP_HIP(hipMemcpyAsync(o.out_ph, dph, Q * 12, hipMemcpyDeviceToHost, ctx.stream));
         if (o.out_bm && want_bm) UPBP_HIP(hipMemcpyAsync(o.out_bm, dbm, Q * 12, hipMemcpyDeviceToHost, ctx.stream));
@@ -282,7 +303,7 @@ void run_eval(Ctx& ctx, const Bre* bre, const Beams* bm, const upbp_query_batch*
     if (timing) {
         UPBP_HIP(hipEventElapsedTime(&ms_other, ctx.ev[0], ctx.ev[1]));
         UPBP_HIP(hipEventElapsedTime(&ms_ph, ctx.ev[1], ctx.ev[2]));
-        UPBP_HIP(hipEventElapsedTime(&ms_bm, ctx.ev[2], ctx.ev[3]));
+        UPBP_HIP(hipEventElapsedTime(&ms_bm, ctx.ev[4], ctx.ev[3]));
     }
     fill_stats(stats, st, ms_ph, ms_bm, ms_other, launches);
     check_stats_errors(ctx, st);
@@ -313,6 +334,9 @@ upbp_status upbp_ctx_create(int device, upbp_ctx** out_ctx) {
         UPBP_HIP(hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking));
         ctx->stream = ctx->own_stream;
         for (auto& ev : ctx->ev) UPBP_HIP(hipEventCreate(&ev));
+        UPBP_HIP(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+        UPBP_HIP(hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
+        UPBP_HIP(hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming));
         ctx->stats.ensure(kStatSlots * 8);
         *out_ctx = ctx;
         return UPBP_OK;
@@ -335,6 +359,10 @@ void ctx_release(Ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto& ev : ctx->ev)
         if (ev) (void)hipEventDestroy(ev);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
+    if (ctx->fork) (void)hipEventDestroy(ctx->fork);
+    if (ctx->join) (void)hipEventDestroy(ctx->join);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete static_cast<upbp_ctx*>(ctx);
     // a failed teardown call must not surface as the error of an unrelated later launch
@@ -733,6 +761,8 @@ upbp_status upbp_batch_plan_create(upbp_ctx* ctx, const upbp_instance* instances
         UPBP_HIP(hipMemcpyAsync(plan->table.p, plan->host.data(), count * sizeof(InstanceDev), hipMemcpyHostToDevice,
                                 ctx->stream));
         plan->counters.ensure(16);
+        plan->args[0].ensure(batched_args_bytes(count));
+        plan->args[1].ensure(batched_args_bytes(count));
         UPBP_HIP(hipStreamSynchronize(ctx->stream));
     });
     if (st != UPBP_OK) {
@@ -754,13 +784,28 @@ upbp_status upbp_batch_plan_destroy(upbp_batch_plan* plan) {
     return UPBP_OK;
 }
 
+// The photon kernel runs on the side stream (fork/join by events, which a graph
+// capture records as edges), so the beam kernel's blocks fill the CUs the photon
+// kernel frees as its queue drains.
 static void enqueue_batch(Ctx& ctx, upbp_batch_plan* plan, int blocks) {
     launch_clear_u64(ctx, ctx.stats.as<uint64_t>(), kStatSlots);
     launch_fill_u32(ctx, plan->counters.as<uint32_t>(), 0u, 2);
-    launch_batched(ctx, plan->table.as<InstanceDev>(), plan->n, plan->total_tiles, plan->counters.as<uint32_t>(), 0,
-                   blocks, ctx.tuning.waves_per_block);
-    launch_batched(ctx, plan->table.as<InstanceDev>(), plan->n, plan->total_tiles, plan->counters.as<uint32_t>(), 1,
-                   blocks, ctx.tuning.waves_per_block);
+    hipStream_t main_stream = ctx.stream;
+    UPBP_HIP(hipEventRecord(ctx.fork, main_stream));
+    UPBP_HIP(hipStreamWaitEvent(ctx.side, ctx.fork, 0));
+    ctx.stream = ctx.side;
+    try {
+        launch_batched(ctx, plan->table.as<InstanceDev>(), plan->n, plan->total_tiles, plan->counters.as<uint32_t>(),
+                       plan->args[0].p, 0, blocks, ctx.tuning.waves_per_block);
+        UPBP_HIP(hipEventRecord(ctx.join, ctx.side));
+    } catch (...) {
+        ctx.stream = main_stream;
+        throw;
+    }
+    ctx.stream = main_stream;
+    launch_batched(ctx, plan->table.as<InstanceDev>(), plan->n, plan->total_tiles, plan->counters.as<uint32_t>(),
+                   plan->args[1].p, 1, blocks, ctx.tuning.waves_per_block);
+    UPBP_HIP(hipStreamWaitEvent(main_stream, ctx.join, 0));
 }
 
 upbp_status upbp_batch_run(upbp_ctx* ctx, upbp_batch_plan* plan, int use_graph, upbp_eval_stats* stats) {

@@ -65,12 +65,9 @@ struct EvalArgs {
     // UPBP mode
     const upbp_query_mis* __restrict__ qmis;
     MisGlobals mis;
-    // persistent batching
-    const InstanceDev* __restrict__ inst;
-    int32_t n_inst;
-    int64_t total_tiles;
-    uint32_t* __restrict__ queue;
-    int32_t which;
+    // persistent batching: this instance's first global tile, and whether it has items to gather
+    int64_t tile_begin;
+    int32_t has_items;
 };
 
 __device__ __forceinline__ float guard_inv(float d) {
@@ -346,11 +343,8 @@ template <int MODE>
 __device__ __forceinline__ void photon_leaf(const EvalArgs& A, const Ray& r, SegState& s, LaneOut<MODE>& lo,
                                             uint32_t start, uint32_t cnt, bool short_q) {
     const uint32_t end = start + cnt;
-    float4 p = A.pos_r2[start];
     for (uint32_t i = start; i < end; ++i) {
-        // fetch the next photon while this one is tested (scalar loads, wave-uniform index)
-        float4 pn = p;
-        if (i + 1 < end) pn = A.pos_r2[i + 1];
+        const float4 p = A.pos_r2[i];  // scalar load, wave-uniform index
         float t, sq;
         const bool hit = test_bre(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, s.tmin, s.tmax, p.x, p.y, p.z, p.w, t, sq);
         if (MODE == (int)Mode::Upbp) {
@@ -361,7 +355,6 @@ __device__ __forceinline__ void photon_leaf(const EvalArgs& A, const Ray& r, Seg
                     lo.hits++;
                 }
             }
-            p = pn;
             continue;
         }
         if (__ballot(hit)) {
@@ -414,7 +407,6 @@ __device__ __forceinline__ void photon_leaf(const EvalArgs& A, const Ray& r, Seg
             }
             if (MODE == (int)Mode::Adjoint) add_item_gradient(A.grad_items, A.orig[i], gp0, gp1, gp2);
         }
-        p = pn;
     }
 }
 
@@ -422,10 +414,10 @@ template <int MODE>
 __device__ __forceinline__ void beam_leaf(const EvalArgs& A, const Ray& r, SegState& s, LaneOut<MODE>& lo,
                                           uint32_t start, uint32_t cnt, bool short_q) {
     const uint32_t end = start + cnt;
-    PieceDev pc = A.pieces[start];
+    // No software prefetch of the next piece: the rotation of two 64-byte records through
+    // SGPRs cost more scalar moves and SGPR pressure than the latency it hid (measured 14 %).
     for (uint32_t i = start; i < end; ++i) {
-        PieceDev pn = pc;
-        if (i + 1 < end) pn = A.pieces[i + 1];
+        const PieceDev pc = A.pieces[i];
         const uint32_t word = __float_as_uint(pc.p2.z);
         const int32_t bmed = (int32_t)(word & kMediumMask);
         float dist, st, t1, t2;
@@ -448,7 +440,6 @@ __device__ __forceinline__ void beam_leaf(const EvalArgs& A, const Ray& r, SegSt
                     lo.hits++;
                 }
             }
-            pc = pn;
             continue;
         }
         if (__ballot(hit)) {
@@ -507,7 +498,6 @@ __device__ __forceinline__ void beam_leaf(const EvalArgs& A, const Ray& r, SegSt
             }
             if (MODE == (int)Mode::Adjoint) add_item_gradient(A.grad_items, A.orig[i], gb0, gb1, gb2);
         }
-        pc = pn;
     }
 }
 
@@ -523,12 +513,17 @@ __device__ __forceinline__ void traverse(const EvalArgs& A, const Ray& r, SegSta
     const uint32_t active = (uint32_t)__popcll(__ballot(s.tmin <= s.tmax));
     while (true) {
         node = uni(node);
-        const NodeDev* nd = A.nodes + node;
-        const float4 a = nd->a, b = nd->b, c = nd->c;
-        const uint4 rr = nd->r;
+        // One 64-byte scalar load for the whole node, then both slab tests without
+        // short-circuit: testing the child references first would make the box loads
+        // wait on them (three dependent scalar round trips per node instead of one).
+        const NodeDev nd = A.nodes[node];
+        const float4 a = nd.a, b = nd.b, c = nd.c;
+        const uint4 rr = nd.r;
         nodes_visited++;
-        const bool h0 = (rr.z != kInvalid) && slab(r, s.tmin, s.tmax, a.x, a.y, a.z, a.w, b.x, b.y);
-        const bool h1 = (rr.w != kInvalid) && slab(r, s.tmin, s.tmax, b.z, b.w, c.x, c.y, c.z, c.w);
+        const bool s0 = slab(r, s.tmin, s.tmax, a.x, a.y, a.z, a.w, b.x, b.y);
+        const bool s1 = slab(r, s.tmin, s.tmax, b.z, b.w, c.x, c.y, c.z, c.w);
+        const bool h0 = s0 & (rr.z != kInvalid);
+        const bool h1 = s1 & (rr.w != kInvalid);
         const bool any0 = __ballot(h0) != 0;
         const bool any1 = __ballot(h1) != 0;
         if (any0 && rr.z != 0) {
@@ -854,8 +849,10 @@ __device__ __forceinline__ int64_t xcd_block(int64_t bid, int64_t nblk) {
     return xcd < rmd ? xcd * (q + 1) + local : rmd * (q + 1) + (xcd - rmd) * q + local;
 }
 
+// Forward kernels ask for 8 resident waves per SIMD (64 VGPRs): a few bytes of
+// register spill cost less than the latency the extra wave hides (measured 3-4 %).
 template <int EST, int MODE>
-__global__ __launch_bounds__(256) void k_eval(EvalArgs A) {
+__global__ __launch_bounds__(256, (MODE == (int)Mode::Forward ? 8 : 1)) void k_eval(EvalArgs A) {
     const int wave_in_block = threadIdx.x / kWave;
     const int waves_per_block = blockDim.x / kWave;
     const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
@@ -884,45 +881,64 @@ __global__ __launch_bounds__(256) void k_eval(EvalArgs A) {
 
 // Persistent batched kernel: waves take (instance, tile) items from a device
 // counter until the queue drains; every wave exits when an item index passes
-// total_tiles.
+// total_tiles. The per-instance arguments are a table in device memory that no
+// kernel writes while this one runs; reading it through the constant address
+// space lets the compiler reload fields as needed instead of pinning them in
+// registers for the whole tile (fewer VGPRs, more resident waves).
+typedef const __attribute__((address_space(4))) EvalArgs* ConstArgs;
+
 template <int EST>
-__global__ __launch_bounds__(256) void k_eval_batched(EvalArgs A) {
+__global__ __launch_bounds__(256) void k_eval_batched(const EvalArgs* args_g, int n_inst, int64_t total_tiles,
+                                                      uint32_t* queue, unsigned long long* stats) {
     const int lane = threadIdx.x & (kWave - 1);
+    ConstArgs args = (ConstArgs)args_g;
     unsigned long long ls[kStatSlots] = {0, 0, 0, 0, 0, 0, 0, 0};
     while (true) {
         uint32_t item = 0;
-        if (lane == 0) item = atomicAdd(A.queue, 1u);
+        if (lane == 0) item = atomicAdd(queue, 1u);
         item = uni((uint32_t)__shfl(item, 0, kWave));
-        if ((int64_t)item >= A.total_tiles) break;
+        if ((int64_t)item >= total_tiles) break;
         // binary search of the instance owning this tile (uniform)
-        int lo_i = 0, hi_i = A.n_inst - 1;
+        int lo_i = 0, hi_i = n_inst - 1;
         while (lo_i < hi_i) {
             const int mid = (lo_i + hi_i + 1) >> 1;
-            if (A.inst[mid].tile_begin <= (int64_t)item) lo_i = mid; else hi_i = mid - 1;
+            if (args[mid].tile_begin <= (int64_t)item) lo_i = mid; else hi_i = mid - 1;
         }
-        const InstanceDev& I = A.inst[lo_i];
-        const bool has = EST == kEstPhotons ? I.has_photons : I.has_beams;
-        if (!has) continue;
-        EvalArgs B = A;
-        B.nodes = EST == kEstPhotons ? I.ph_nodes : I.bm_nodes;
-        B.pos_r2 = I.ph_pos_r2;
-        B.power = I.ph_power;
-        B.inc = I.ph_inc;
-        B.pieces = I.bm_pieces;
-        B.piece_flux = I.bm_flux;
-        B.origin = I.origin;
-        B.direction = I.direction;
-        B.offsets = I.offsets;
-        B.segments = I.segments;
-        B.perm = I.perm;
-        B.count = I.count;
-        B.short_query = I.beam_type == UPBP_SHORT_BEAM;
-        B.lite = (int32_t)I.lite;
-        B.media_base = I.media_base;
-        B.out = EST == kEstPhotons ? I.out_photons : I.out_beams;
-        run_tile<EST, (int)Mode::Forward>(B, ((int64_t)item - I.tile_begin) * kWave, ls, nullptr);
+        const EvalArgs& B = *(const EvalArgs*)(args + lo_i);
+        if (!B.has_items) continue;
+        run_tile<EST, (int)Mode::Forward>(B, ((int64_t)item - B.tile_begin) * kWave, ls, nullptr);
     }
-    flush_stats(A, ls);
+    EvalArgs S = {};
+    S.stats = stats;
+    flush_stats(S, ls);
+}
+
+// One EvalArgs per instance for estimator `which` (0 photons, 1 beams).
+__global__ void k_make_batched_args(int n, const InstanceDev* __restrict__ inst, EvalArgs base, int which,
+                                    EvalArgs* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const InstanceDev I = inst[i];
+    EvalArgs B = base;
+    B.nodes = which == kEstPhotons ? I.ph_nodes : I.bm_nodes;
+    B.pos_r2 = I.ph_pos_r2;
+    B.power = I.ph_power;
+    B.inc = I.ph_inc;
+    B.pieces = I.bm_pieces;
+    B.piece_flux = I.bm_flux;
+    B.origin = I.origin;
+    B.direction = I.direction;
+    B.offsets = I.offsets;
+    B.segments = I.segments;
+    B.perm = I.perm;
+    B.count = I.count;
+    B.short_query = I.beam_type == UPBP_SHORT_BEAM;
+    B.lite = (int32_t)I.lite;
+    B.media_base = I.media_base;
+    B.out = which == kEstPhotons ? I.out_photons : I.out_beams;
+    B.tile_begin = I.tile_begin;
+    B.has_items = which == kEstPhotons ? I.has_photons : I.has_beams;
+    out[i] = B;
 }
 
 __global__ void k_clear_u64(unsigned long long* p, int64_t n) {
@@ -1081,22 +1097,27 @@ void launch_beams(Ctx& ctx, Mode mode, const BeamArgs& bm, const QueryView& q, f
     launch_mode<kEstBeams>(ctx, mode, A, (unsigned)((waves + wpb - 1) / wpb), (unsigned)(wpb * kWave));
 }
 
+size_t batched_args_bytes(int n_inst) { return (size_t)std::max(n_inst, 1) * sizeof(EvalArgs); }
+
 void launch_batched(Ctx& ctx, const InstanceDev* inst, int n_inst, int64_t total_tiles, uint32_t* counters,
-                    int which, int grid_blocks, int waves_per_block) {
+                    void* args_table, int which, int grid_blocks, int waves_per_block) {
     EvalArgs A = {};
     A.media = ctx.media.as<MediumDev>();
     A.n_media = ctx.n_media;
     A.stats = ctx.stats.as<unsigned long long>();
-    A.inst = inst;
-    A.n_inst = n_inst;
-    A.total_tiles = total_tiles;
-    A.queue = counters + which;
-    A.which = which;
+    EvalArgs* args = static_cast<EvalArgs*>(args_table);
+    hipLaunchKernelGGL(k_make_batched_args, dim3((n_inst + 63) / 64), dim3(64), 0, ctx.stream, n_inst, inst, A, which,
+                       args);
+    UPBP_HIP(hipGetLastError());
     const unsigned threads = (unsigned)(std::min(std::max(waves_per_block, 1), 4) * kWave);
+    uint32_t* queue = counters + which;
+    unsigned long long* stats = ctx.stats.as<unsigned long long>();
     if (which == kEstPhotons)
-        hipLaunchKernelGGL(k_eval_batched<kEstPhotons>, dim3(grid_blocks), dim3(threads), 0, ctx.stream, A);
+        hipLaunchKernelGGL(k_eval_batched<kEstPhotons>, dim3(grid_blocks), dim3(threads), 0, ctx.stream, args,
+                           n_inst, total_tiles, queue, stats);
     else
-        hipLaunchKernelGGL(k_eval_batched<kEstBeams>, dim3(grid_blocks), dim3(threads), 0, ctx.stream, A);
+        hipLaunchKernelGGL(k_eval_batched<kEstBeams>, dim3(grid_blocks), dim3(threads), 0, ctx.stream, args, n_inst,
+                           total_tiles, queue, stats);
     UPBP_HIP(hipGetLastError());
 }
 

@@ -103,15 +103,7 @@ __device__ __forceinline__ bool test_beam_beam(float o1x, float o1y, float o1z, 
     const float d1o2 = dot3(d1x, d1y, d1z, o2x, o2y, o2z);
     const float t1 = ((d1o1 - d1o2) - q * (dot3(d2x, d2y, d2z, o1x, o1y, o1z) - d2o2)) / q2m1;
     if (t1 <= min_t1 || t1 >= max_t1) return false;
-    const float n2 = (t1 + d1o1) - d1o2;
-    // Most candidates that get this far have their closest point on another piece of
-    // the beam. n2 * rcp(q) is within 2 ulp of the correctly rounded n2 / q, so a
-    // margin of 2^-20 relative (8 ulp) rejects those without the IEEE division and
-    // without changing any decision.
-    const float t2a = n2 * __builtin_amdgcn_rcpf(q);
-    const float m = __builtin_fabsf(t2a) * 9.5367431640625e-07f + 1e-30f;
-    if (__builtin_isfinite(t2a) && (t2a < t_lo - m || t2a >= t_hi + m)) return false;
-    const float t2 = n2 / q;
+    const float t2 = ((t1 + d1o1) - d1o2) / q;
     if (t2 <= 0.0f || t2 >= len || __builtin_isnan(t2)) return false;
     if (!(t2 >= t_lo && t2 < t_hi)) return false;
     const float st = __builtin_sqrtf(s2);

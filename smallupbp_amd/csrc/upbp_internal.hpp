@@ -92,6 +92,10 @@ struct Ctx {
     DevBuf qkeys, qkeys2, qperm, qperm2;
     DevBuf staging[8];  // host-input staging copies
     hipEvent_t ev[6] = {};
+    // second stream: the photon and beam kernels of one eval call run concurrently
+    // (fork from `stream`, join back), so one kernel's tail overlaps the other's work
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     int num_cus = 256;
     Tuning tuning;
 };
@@ -248,8 +252,10 @@ struct InstanceDev {
     int32_t pad[3];
 };
 
-void launch_batched(Ctx& ctx, const InstanceDev* inst, int n_inst, int64_t total_tiles,
-                    uint32_t* counters, int which, int grid_blocks, int waves_per_block);
+// Bytes of the per-instance argument table launch_batched fills (device memory owned by the plan).
+size_t batched_args_bytes(int n_inst);
+void launch_batched(Ctx& ctx, const InstanceDev* inst, int n_inst, int64_t total_tiles, uint32_t* counters,
+                    void* args_table, int which, int grid_blocks, int waves_per_block);
 void launch_fill_u32(Ctx& ctx, uint32_t* p, uint32_t v, int64_t n);
 
 }  // namespace upbp
