@@ -423,8 +423,14 @@ typedef struct upbp_instance {
 
 typedef struct upbp_batch_plan upbp_batch_plan;
 /* Prepare a reusable plan: validates the instances, uploads the descriptor
- * table and sizes the work queue. The instances' structures and buffers must
- * outlive the plan. */
+ * table, sorts each instance's queries once, and COPIES every instance's photon
+ * and beam structures into merged arrays owned by the plan (one node array and
+ * one item array per estimator, child references rebased), so the persistent
+ * kernel reads them through kernel arguments like a single evaluation. The
+ * structures may be destroyed after this call; rebuilding or re-powering them
+ * (upbp_bre_set_power) does not reach an existing plan. Query arrays and
+ * output buffers are referenced, not copied: they must outlive the plan and
+ * may be refilled between runs. */
 upbp_status upbp_batch_plan_create(upbp_ctx* ctx, const upbp_instance* instances,
                                    int32_t count, upbp_batch_plan** out_plan);
 upbp_status upbp_batch_plan_destroy(upbp_batch_plan* plan);

@@ -23,7 +23,9 @@ struct upbp_batch_plan {
     std::vector<InstanceDev> host;
     DevBuf table;
     DevBuf counters;  // [2] queue counters (photons, beams)
-    DevBuf args[2];   // per-instance kernel arguments (photons, beams), refreshed by every run
+    // copies of every instance's structures, merged (see MergedDev)
+    DevBuf m_ph_nodes, m_ph_pos_r2, m_ph_power, m_ph_inc, m_bm_nodes, m_bm_pieces, m_bm_flux;
+    MergedDev merged = {};
     std::vector<DevBuf*> perms;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -685,6 +687,69 @@ upbp_status upbp_adjoint(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams* b
 }
 
 // ---------------------------------------------------------------------------
+// Copies every instance's photon and beam structures into the plan's merged arrays
+// (device-to-device) and rebases their child references; records each instance's roots.
+static void merge_structures(Ctx& ctx, upbp_batch_plan* plan, const upbp_instance* instances) {
+    int64_t pn = 0, pi = 0, bn = 0, bi = 0;  // photon nodes / items, beam nodes / items
+    for (int i = 0; i < plan->n; ++i) {
+        const Bre* b = instances[i].bre;
+        const Beams* m = instances[i].beams;
+        if (plan->host[i].has_photons) {
+            pn += b->bvh.n_nodes;
+            pi += b->n;
+        }
+        if (plan->host[i].has_beams) {
+            bn += m->bvh.n_nodes;
+            bi += m->pieces;
+        }
+    }
+    UPBP_REQUIRE(pn < 0x7FFFFFFF && pi < 0x7FFFFFFF && bn < 0x7FFFFFFF && bi < 0x7FFFFFFF, UPBP_EINVAL,
+                 "batched structures too large");
+    plan->m_ph_nodes.ensure(std::max<int64_t>(pn, 1) * sizeof(NodeDev));
+    plan->m_ph_pos_r2.ensure(std::max<int64_t>(pi, 1) * 16);
+    plan->m_ph_power.ensure(std::max<int64_t>(pi, 1) * 16);
+    plan->m_ph_inc.ensure(std::max<int64_t>(pi, 1) * 16);
+    plan->m_bm_nodes.ensure(std::max<int64_t>(bn, 1) * sizeof(NodeDev));
+    plan->m_bm_pieces.ensure(std::max<int64_t>(bi, 1) * sizeof(PieceDev));
+    plan->m_bm_flux.ensure(std::max<int64_t>(bi, 1) * 16);
+    const hipMemcpyKind dd = hipMemcpyDeviceToDevice;
+    pn = pi = bn = bi = 0;
+    for (int i = 0; i < plan->n; ++i) {
+        InstanceDev& d = plan->host[i];
+        const Bre* b = instances[i].bre;
+        const Beams* m = instances[i].beams;
+        if (d.has_photons) {
+            NodeDev* dst = plan->m_ph_nodes.as<NodeDev>() + pn;
+            UPBP_HIP(hipMemcpyAsync(dst, b->bvh.nodes.p, b->bvh.n_nodes * sizeof(NodeDev), dd, ctx.stream));
+            UPBP_HIP(hipMemcpyAsync(plan->m_ph_pos_r2.as<float4>() + pi, b->pos_r2.p, b->n * 16, dd, ctx.stream));
+            UPBP_HIP(hipMemcpyAsync(plan->m_ph_power.as<float4>() + pi, b->power.p, b->n * 16, dd, ctx.stream));
+            UPBP_HIP(hipMemcpyAsync(plan->m_ph_inc.as<float4>() + pi, b->inc.p, b->n * 16, dd, ctx.stream));
+            launch_rebase_nodes(ctx, dst, b->bvh.n_nodes, (uint32_t)pn, (uint32_t)pi);
+            d.ph_root = (uint32_t)pn;
+            pn += b->bvh.n_nodes;
+            pi += b->n;
+        }
+        if (d.has_beams) {
+            NodeDev* dst = plan->m_bm_nodes.as<NodeDev>() + bn;
+            UPBP_HIP(hipMemcpyAsync(dst, m->bvh.nodes.p, m->bvh.n_nodes * sizeof(NodeDev), dd, ctx.stream));
+            UPBP_HIP(hipMemcpyAsync(plan->m_bm_pieces.as<PieceDev>() + bi, m->piece.p, m->pieces * sizeof(PieceDev), dd,
+                                    ctx.stream));
+            UPBP_HIP(hipMemcpyAsync(plan->m_bm_flux.as<float4>() + bi, m->piece_flux.p, m->pieces * 16, dd, ctx.stream));
+            launch_rebase_nodes(ctx, dst, m->bvh.n_nodes, (uint32_t)bn, (uint32_t)bi);
+            d.bm_root = (uint32_t)bn;
+            bn += m->bvh.n_nodes;
+            bi += m->pieces;
+        }
+    }
+    plan->merged.ph_nodes = plan->m_ph_nodes.as<NodeDev>();
+    plan->merged.ph_pos_r2 = plan->m_ph_pos_r2.as<float4>();
+    plan->merged.ph_power = plan->m_ph_power.as<float4>();
+    plan->merged.ph_inc = plan->m_ph_inc.as<float4>();
+    plan->merged.bm_nodes = plan->m_bm_nodes.as<NodeDev>();
+    plan->merged.bm_pieces = plan->m_bm_pieces.as<PieceDev>();
+    plan->merged.bm_flux = plan->m_bm_flux.as<float4>();
+}
+
 upbp_status upbp_batch_plan_create(upbp_ctx* ctx, const upbp_instance* instances, int32_t count,
                                    upbp_batch_plan** out_plan) {
     if (!ctx) return fail(nullptr, UPBP_EINVAL, "ctx is NULL");
@@ -757,12 +822,11 @@ upbp_status upbp_batch_plan_create(upbp_ctx* ctx, const upbp_instance* instances
         }
         plan->total_tiles = tiles;
         UPBP_REQUIRE(tiles < 0xFFFFFFF0ll, UPBP_EINVAL, "too many query tiles");
+        merge_structures(*ctx, plan, instances);
         plan->table.ensure(count * sizeof(InstanceDev));
         UPBP_HIP(hipMemcpyAsync(plan->table.p, plan->host.data(), count * sizeof(InstanceDev), hipMemcpyHostToDevice,
                                 ctx->stream));
         plan->counters.ensure(16);
-        plan->args[0].ensure(batched_args_bytes(count));
-        plan->args[1].ensure(batched_args_bytes(count));
         UPBP_HIP(hipStreamSynchronize(ctx->stream));
     });
     if (st != UPBP_OK) {
@@ -796,7 +860,7 @@ static void enqueue_batch(Ctx& ctx, upbp_batch_plan* plan, int blocks) {
     ctx.stream = ctx.side;
     try {
         launch_batched(ctx, plan->table.as<InstanceDev>(), plan->n, plan->total_tiles, plan->counters.as<uint32_t>(),
-                       plan->args[0].p, 0, blocks, ctx.tuning.waves_per_block);
+                       plan->merged, 0, blocks, ctx.tuning.waves_per_block);
         UPBP_HIP(hipEventRecord(ctx.join, ctx.side));
     } catch (...) {
         ctx.stream = main_stream;
@@ -804,7 +868,7 @@ static void enqueue_batch(Ctx& ctx, upbp_batch_plan* plan, int blocks) {
     }
     ctx.stream = main_stream;
     launch_batched(ctx, plan->table.as<InstanceDev>(), plan->n, plan->total_tiles, plan->counters.as<uint32_t>(),
-                   plan->args[1].p, 1, blocks, ctx.tuning.waves_per_block);
+                   plan->merged, 1, blocks, ctx.tuning.waves_per_block);
     UPBP_HIP(hipStreamWaitEvent(main_stream, ctx.join, 0));
 }
 

@@ -65,9 +65,8 @@ struct EvalArgs {
     // UPBP mode
     const upbp_query_mis* __restrict__ qmis;
     MisGlobals mis;
-    // persistent batching: this instance's first global tile, and whether it has items to gather
-    int64_t tile_begin;
-    int32_t has_items;
+    // traversal starts here (0, or an instance's root in a batched plan's merged tree array)
+    uint32_t root;
 };
 
 __device__ __forceinline__ float guard_inv(float d) {
@@ -509,7 +508,7 @@ __device__ __forceinline__ void traverse(const EvalArgs& A, const Ray& r, SegSta
     const int lane = threadIdx.x & (kWave - 1);
     uint32_t stk0 = 0, stk1 = 0;
     int sp = 0;
-    uint32_t node = 0;
+    uint32_t node = A.root;
     const uint32_t active = (uint32_t)__popcll(__ballot(s.tmin <= s.tmax));
     while (true) {
         node = uni(node);
@@ -881,17 +880,32 @@ __global__ __launch_bounds__(256, (MODE == (int)Mode::Forward ? 8 : 1)) void k_e
 
 // Persistent batched kernel: waves take (instance, tile) items from a device
 // counter until the queue drains; every wave exits when an item index passes
-// total_tiles. The per-instance arguments are a table in device memory that no
-// kernel writes while this one runs; reading it through the constant address
-// space lets the compiler reload fields as needed instead of pinning them in
-// registers for the whole tile (fewer VGPRs, more resident waves).
-typedef const __attribute__((address_space(4))) EvalArgs* ConstArgs;
+// total_tiles. Structures come from the plan's merged arrays through the kernel
+// argument A (wave-uniform scalar loads, as in k_eval); only the instance's root
+// node, query pointers and flags come from the instance table.
+// A wave-uniform global pointer read from memory, made a scalar: the two halves pass
+// through readfirstlane and the result is tagged as global memory, so loads through it
+// use a scalar base (as kernel-argument pointers do) instead of a 64-bit VGPR pair.
+// 64-bit readfirstlane from two 32-bit halves. readfirstlane returns int: each half
+// goes through uint32_t before widening, or a low word with bit 31 set would
+// sign-extend over the high word.
+__device__ __forceinline__ uint64_t uni_u64(uint64_t v) {
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+    return ((uint64_t)hi << 32) | (uint64_t)lo;
+}
+
+template <class T>
+__device__ __forceinline__ T* uni_global(T* p) {
+    return (T*)(__attribute__((address_space(1))) T*)uni_u64((uint64_t)p);
+}
+
+__device__ __forceinline__ int64_t uni64(int64_t x) { return (int64_t)uni_u64((uint64_t)x); }
 
 template <int EST>
-__global__ __launch_bounds__(256) void k_eval_batched(const EvalArgs* args_g, int n_inst, int64_t total_tiles,
-                                                      uint32_t* queue, unsigned long long* stats) {
+__global__ __launch_bounds__(256) void k_eval_batched(EvalArgs A, const InstanceDev* __restrict__ inst,
+                                                         int n_inst, int64_t total_tiles, uint32_t* queue) {
     const int lane = threadIdx.x & (kWave - 1);
-    ConstArgs args = (ConstArgs)args_g;
     unsigned long long ls[kStatSlots] = {0, 0, 0, 0, 0, 0, 0, 0};
     while (true) {
         uint32_t item = 0;
@@ -902,43 +916,37 @@ __global__ __launch_bounds__(256) void k_eval_batched(const EvalArgs* args_g, in
         int lo_i = 0, hi_i = n_inst - 1;
         while (lo_i < hi_i) {
             const int mid = (lo_i + hi_i + 1) >> 1;
-            if (args[mid].tile_begin <= (int64_t)item) lo_i = mid; else hi_i = mid - 1;
+            if (inst[mid].tile_begin <= (int64_t)item) lo_i = mid; else hi_i = mid - 1;
         }
-        const EvalArgs& B = *(const EvalArgs*)(args + lo_i);
-        if (!B.has_items) continue;
-        run_tile<EST, (int)Mode::Forward>(B, ((int64_t)item - B.tile_begin) * kWave, ls, nullptr);
+        const InstanceDev& I = inst[lo_i];
+        const bool has = EST == kEstPhotons ? I.has_photons : I.has_beams;
+        if (!uni((uint32_t)has)) continue;
+        EvalArgs B = A;
+        B.root = uni(EST == kEstPhotons ? I.ph_root : I.bm_root);
+        B.origin = uni_global(I.origin);
+        B.direction = uni_global(I.direction);
+        B.offsets = uni_global(I.offsets);
+        B.segments = uni_global(I.segments);
+        B.perm = I.perm ? uni_global(I.perm) : nullptr;
+        B.count = uni64(I.count);
+        B.short_query = (int32_t)uni((uint32_t)(I.beam_type == UPBP_SHORT_BEAM));
+        B.lite = (int32_t)uni(I.lite);
+        B.media_base = (int32_t)uni((uint32_t)I.media_base);
+        B.out = uni_global(EST == kEstPhotons ? I.out_photons : I.out_beams);
+        const int64_t tile_begin = uni64(I.tile_begin);
+        run_tile<EST, (int)Mode::Forward>(B, ((int64_t)item - tile_begin) * kWave, ls, nullptr);
     }
-    EvalArgs S = {};
-    S.stats = stats;
-    flush_stats(S, ls);
+    flush_stats(A, ls);
 }
 
-// One EvalArgs per instance for estimator `which` (0 photons, 1 beams).
-__global__ void k_make_batched_args(int n, const InstanceDev* __restrict__ inst, EvalArgs base, int which,
-                                    EvalArgs* __restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// Rebases one instance's copied nodes into the merged array.
+__global__ void k_rebase_nodes(NodeDev* nodes, int64_t n, uint32_t node_off, uint32_t item_off) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const InstanceDev I = inst[i];
-    EvalArgs B = base;
-    B.nodes = which == kEstPhotons ? I.ph_nodes : I.bm_nodes;
-    B.pos_r2 = I.ph_pos_r2;
-    B.power = I.ph_power;
-    B.inc = I.ph_inc;
-    B.pieces = I.bm_pieces;
-    B.piece_flux = I.bm_flux;
-    B.origin = I.origin;
-    B.direction = I.direction;
-    B.offsets = I.offsets;
-    B.segments = I.segments;
-    B.perm = I.perm;
-    B.count = I.count;
-    B.short_query = I.beam_type == UPBP_SHORT_BEAM;
-    B.lite = (int32_t)I.lite;
-    B.media_base = I.media_base;
-    B.out = which == kEstPhotons ? I.out_photons : I.out_beams;
-    B.tile_begin = I.tile_begin;
-    B.has_items = which == kEstPhotons ? I.has_photons : I.has_beams;
-    out[i] = B;
+    uint4 r = nodes[i].r;
+    if (r.z != kInvalid) r.x += r.z == 0 ? node_off : item_off;
+    if (r.w != kInvalid) r.y += r.w == 0 ? node_off : item_off;
+    nodes[i].r = r;
 }
 
 __global__ void k_clear_u64(unsigned long long* p, int64_t n) {
@@ -1097,27 +1105,36 @@ void launch_beams(Ctx& ctx, Mode mode, const BeamArgs& bm, const QueryView& q, f
     launch_mode<kEstBeams>(ctx, mode, A, (unsigned)((waves + wpb - 1) / wpb), (unsigned)(wpb * kWave));
 }
 
-size_t batched_args_bytes(int n_inst) { return (size_t)std::max(n_inst, 1) * sizeof(EvalArgs); }
+void launch_rebase_nodes(Ctx& ctx, NodeDev* nodes, int64_t n, uint32_t node_off, uint32_t item_off) {
+    if (n <= 0 || (node_off == 0 && item_off == 0)) return;
+    hipLaunchKernelGGL(k_rebase_nodes, dim3(grid_for(n)), dim3(256), 0, ctx.stream, nodes, n, node_off, item_off);
+    UPBP_HIP(hipGetLastError());
+}
 
 void launch_batched(Ctx& ctx, const InstanceDev* inst, int n_inst, int64_t total_tiles, uint32_t* counters,
-                    void* args_table, int which, int grid_blocks, int waves_per_block) {
+                    const MergedDev& merged, int which, int grid_blocks, int waves_per_block) {
     EvalArgs A = {};
     A.media = ctx.media.as<MediumDev>();
     A.n_media = ctx.n_media;
     A.stats = ctx.stats.as<unsigned long long>();
-    EvalArgs* args = static_cast<EvalArgs*>(args_table);
-    hipLaunchKernelGGL(k_make_batched_args, dim3((n_inst + 63) / 64), dim3(64), 0, ctx.stream, n_inst, inst, A, which,
-                       args);
-    UPBP_HIP(hipGetLastError());
+    if (which == kEstPhotons) {
+        A.nodes = merged.ph_nodes;
+        A.pos_r2 = merged.ph_pos_r2;
+        A.power = merged.ph_power;
+        A.inc = merged.ph_inc;
+    } else {
+        A.nodes = merged.bm_nodes;
+        A.pieces = merged.bm_pieces;
+        A.piece_flux = merged.bm_flux;
+    }
     const unsigned threads = (unsigned)(std::min(std::max(waves_per_block, 1), 4) * kWave);
     uint32_t* queue = counters + which;
-    unsigned long long* stats = ctx.stats.as<unsigned long long>();
     if (which == kEstPhotons)
-        hipLaunchKernelGGL(k_eval_batched<kEstPhotons>, dim3(grid_blocks), dim3(threads), 0, ctx.stream, args,
-                           n_inst, total_tiles, queue, stats);
+        hipLaunchKernelGGL(k_eval_batched<kEstPhotons>, dim3(grid_blocks), dim3(threads), 0, ctx.stream, A, inst,
+                           n_inst, total_tiles, queue);
     else
-        hipLaunchKernelGGL(k_eval_batched<kEstBeams>, dim3(grid_blocks), dim3(threads), 0, ctx.stream, args, n_inst,
-                           total_tiles, queue, stats);
+        hipLaunchKernelGGL(k_eval_batched<kEstBeams>, dim3(grid_blocks), dim3(threads), 0, ctx.stream, A, inst, n_inst,
+                           total_tiles, queue);
     UPBP_HIP(hipGetLastError());
 }
 

@@ -249,13 +249,28 @@ struct InstanceDev {
     uint32_t lite;
     int32_t has_photons;
     int32_t has_beams;
-    int32_t pad[3];
+    uint32_t ph_root;  // root of this instance's photon tree in the plan's merged node array
+    uint32_t bm_root;  // root of its beam tree in the merged beam node array
+    int32_t pad;
 };
 
-// Bytes of the per-instance argument table launch_batched fills (device memory owned by the plan).
-size_t batched_args_bytes(int n_inst);
+// A batched plan's copy of every instance's structures, one array each, so that the
+// persistent kernel reads them through kernel-argument pointers (scalar loads).
+// Child references are rebased: inner refs by the instance's node offset, leaf refs
+// by its item offset.
+struct MergedDev {
+    const NodeDev* ph_nodes;
+    const float4* ph_pos_r2;
+    const float4* ph_power;
+    const float4* ph_inc;
+    const NodeDev* bm_nodes;
+    const PieceDev* bm_pieces;
+    const float4* bm_flux;
+};
+void launch_rebase_nodes(Ctx& ctx, NodeDev* nodes, int64_t n, uint32_t node_off, uint32_t item_off);
+
 void launch_batched(Ctx& ctx, const InstanceDev* inst, int n_inst, int64_t total_tiles, uint32_t* counters,
-                    void* args_table, int which, int grid_blocks, int waves_per_block);
+                    const MergedDev& merged, int which, int grid_blocks, int waves_per_block);
 void launch_fill_u32(Ctx& ctx, uint32_t* p, uint32_t v, int64_t n);
 
 }  // namespace upbp

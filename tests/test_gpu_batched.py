@@ -77,3 +77,42 @@ def test_batched_plan_rejects_host_queries():
     op = torch.zeros((w.queries.count, 3), device="cuda")
     with pytest.raises(native.UpbpError):
         native.BatchPlan(ctx, [(bre, None, qv, 0, op, None)])
+
+
+def test_batched_plan_owns_structure_copies_and_mixed_instances():
+    # the plan copies the structures: they may be destroyed after the plan is made;
+    # instances may carry photons only, beams only, or both
+    print_case("batched_copies", n=5, scale=0.01)
+    ws = _instances(5, 0.01, seed=9)
+    media = np.concatenate([w.media for w in ws])
+    ctx = native.Context(0)
+    ctx.set_media(media)
+    built, inst, outs = [], [], []
+    for i, w in enumerate(ws):
+        bre = None if i == 1 else native.EmbreeBre(ctx)
+        bm = None if i == 0 else native.PhotonBeamsEvaluator(ctx)
+        if bre is not None:
+            bre.build(w.photons)
+        if bm is not None:
+            bm.build(w.beams)
+        qv = native.QueryBatchView.to_device(w.queries)
+        op = torch.full((w.queries.count, 3), -1.0, device="cuda") if bre is not None else None
+        ob = torch.full((w.queries.count, 3), -1.0, device="cuda") if bm is not None else None
+        built.append((bre, bm))
+        inst.append((bre, bm, qv, 2 * i, op, ob))
+        outs.append((op, ob))
+    plan = native.BatchPlan(ctx, inst)
+    refs = [_plain_reference(w) for w in ws]
+    for bre, bm in built:
+        if bre is not None:
+            bre.destroy()
+        if bm is not None:
+            bm.destroy()
+    for use_graph in (False, True, True):
+        plan.run(use_graph=use_graph)
+        torch.cuda.synchronize()
+        for (rp, rb), (op, ob) in zip(refs, outs):
+            if op is not None:
+                assert np.array_equal(op.cpu().numpy(), rp)
+            if ob is not None:
+                assert np.array_equal(ob.cpu().numpy(), rb)
