@@ -30,6 +30,7 @@ struct upbp_batch_plan {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int graph_blocks = 0;
+    int graph_wpb = 0;
     ~upbp_batch_plan() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -821,7 +822,8 @@ upbp_status upbp_batch_plan_create(upbp_ctx* ctx, const upbp_instance* instances
             tiles += (in.queries.count + 63) / 64;
         }
         plan->total_tiles = tiles;
-        UPBP_REQUIRE(tiles < 0xFFFFFFF0ll, UPBP_EINVAL, "too many query tiles");
+        // the 32-bit work counter also absorbs one extra increment per exiting wave
+        UPBP_REQUIRE(tiles < (1ll << 31), UPBP_EINVAL, "too many query tiles");
         merge_structures(*ctx, plan, instances);
         plan->table.ensure(count * sizeof(InstanceDev));
         UPBP_HIP(hipMemcpyAsync(plan->table.p, plan->host.data(), count * sizeof(InstanceDev), hipMemcpyHostToDevice,
@@ -876,14 +878,14 @@ upbp_status upbp_batch_run(upbp_ctx* ctx, upbp_batch_plan* plan, int use_graph, 
     if (!ctx) return fail(nullptr, UPBP_EINVAL, "ctx is NULL");
     if (!plan || plan->ctx != ctx) return fail(ctx, UPBP_EINVAL, "plan is NULL or belongs to another context");
     return guarded(ctx, [&] {
-        const int wpb = std::min(std::max(ctx->tuning.waves_per_block, 1), 4);
-        int per_cu = ctx->tuning.blocks_per_cu;
-        if (per_cu <= 0) per_cu = std::max(1, 16 / wpb);  // about 16 resident waves per CU
+        // blocks_per_cu 0: each persistent kernel fills the GPU to its own occupancy limit
+        const int per_cu = std::max(ctx->tuning.blocks_per_cu, 0);
         const int blocks = ctx->num_cus * per_cu;
+        if (per_cu == 0) prepare_batched(*ctx, ctx->tuning.waves_per_block);
         if (stats) UPBP_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
         if (use_graph) {
             UPBP_REQUIRE(ctx->stream != nullptr, UPBP_EINVAL, "graph capture needs a non-default stream");
-            if (!plan->exec || plan->graph_blocks != blocks) {
+            if (!plan->exec || plan->graph_blocks != blocks || plan->graph_wpb != ctx->tuning.waves_per_block) {
                 if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
                 if (plan->graph) (void)hipGraphDestroy(plan->graph);
                 plan->exec = nullptr;
@@ -900,6 +902,7 @@ upbp_status upbp_batch_run(upbp_ctx* ctx, upbp_batch_plan* plan, int use_graph, 
                 UPBP_HIP(hipStreamEndCapture(ctx->stream, &plan->graph));
                 UPBP_HIP(hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0));
                 plan->graph_blocks = blocks;
+                plan->graph_wpb = ctx->tuning.waves_per_block;
             }
             UPBP_HIP(hipGraphLaunch(plan->exec, ctx->stream));
         } else {

@@ -93,6 +93,25 @@ __device__ __forceinline__ bool slab(const Ray& r, float tmin, float tmax, float
 
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// A wave-uniform global pointer read from memory, made a scalar: the two halves pass
+// through readfirstlane and the result is tagged as global memory, so loads through it
+// use a scalar base (as kernel-argument pointers do) instead of a 64-bit VGPR pair.
+// 64-bit readfirstlane from two 32-bit halves. readfirstlane returns int: each half
+// goes through uint32_t before widening, or a low word with bit 31 set would
+// sign-extend over the high word.
+__device__ __forceinline__ uint64_t uni_u64(uint64_t v) {
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+    return ((uint64_t)hi << 32) | (uint64_t)lo;
+}
+
+template <class T>
+__device__ __forceinline__ T* uni_global(T* p) {
+    return (T*)(__attribute__((address_space(1))) T*)uni_u64((uint64_t)p);
+}
+
+__device__ __forceinline__ int64_t uni64(int64_t x) { return (int64_t)uni_u64((uint64_t)x); }
+
 // Lane state of one segment being gathered.
 struct SegState {
     float tmin, tmax;
@@ -814,17 +833,17 @@ __device__ void run_tile(const EvalArgs& A, int64_t slot0, unsigned long long* l
         }
     }
     // per-wave counters
+    // Reduced over the wave and made scalar: the running counters then live in SGPRs,
+    // not in 64-bit VGPR pairs held across every tile of a persistent wave.
     uint32_t h = lo.hits;
     for (int off = 32; off > 0; off >>= 1) h += __shfl_xor(h, off, kWave);
     uint32_t e = err;
     for (int off = 32; off > 0; off >>= 1) e |= __shfl_xor(e, off, kWave);
-    if (lane == 0) {
-        local_stats[EST == kEstPhotons ? kStatPhotonHits : kStatBeamHits] += h;
-        local_stats[EST == kEstPhotons ? kStatPhotonCands : kStatBeamCands] += tests;
-        local_stats[kStatNodes] += nodes_visited;
-        local_stats[kStatError] |= e;
-        local_stats[kStatOverflow] += overflow;
-    }
+    local_stats[EST == kEstPhotons ? kStatPhotonHits : kStatBeamHits] += uni(h);
+    local_stats[EST == kEstPhotons ? kStatPhotonCands : kStatBeamCands] += uni_u64(tests);
+    local_stats[kStatNodes] += uni_u64(nodes_visited);
+    local_stats[kStatError] |= uni(e);
+    local_stats[kStatOverflow] += uni(overflow);
 }
 
 __device__ __forceinline__ void flush_stats(const EvalArgs& A, const unsigned long long* ls) {
@@ -882,29 +901,11 @@ __global__ __launch_bounds__(256, (MODE == (int)Mode::Forward ? 8 : 1)) void k_e
 // counter until the queue drains; every wave exits when an item index passes
 // total_tiles. Structures come from the plan's merged arrays through the kernel
 // argument A (wave-uniform scalar loads, as in k_eval); only the instance's root
-// node, query pointers and flags come from the instance table.
-// A wave-uniform global pointer read from memory, made a scalar: the two halves pass
-// through readfirstlane and the result is tagged as global memory, so loads through it
-// use a scalar base (as kernel-argument pointers do) instead of a 64-bit VGPR pair.
-// 64-bit readfirstlane from two 32-bit halves. readfirstlane returns int: each half
-// goes through uint32_t before widening, or a low word with bit 31 set would
-// sign-extend over the high word.
-__device__ __forceinline__ uint64_t uni_u64(uint64_t v) {
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
-    return ((uint64_t)hi << 32) | (uint64_t)lo;
-}
-
-template <class T>
-__device__ __forceinline__ T* uni_global(T* p) {
-    return (T*)(__attribute__((address_space(1))) T*)uni_u64((uint64_t)p);
-}
-
-__device__ __forceinline__ int64_t uni64(int64_t x) { return (int64_t)uni_u64((uint64_t)x); }
-
+// node, query pointers and flags come from the instance table. Six waves per SIMD
+// (at most 80 VGPRs) measured faster than the unconstrained allocation.
 template <int EST>
-__global__ __launch_bounds__(256) void k_eval_batched(EvalArgs A, const InstanceDev* __restrict__ inst,
-                                                         int n_inst, int64_t total_tiles, uint32_t* queue) {
+__global__ __launch_bounds__(256, 6) void k_eval_batched(EvalArgs A, const InstanceDev* __restrict__ inst,
+                                                            int n_inst, int64_t total_tiles, uint32_t* queue) {
     const int lane = threadIdx.x & (kWave - 1);
     unsigned long long ls[kStatSlots] = {0, 0, 0, 0, 0, 0, 0, 0};
     while (true) {
@@ -1111,6 +1112,19 @@ void launch_rebase_nodes(Ctx& ctx, NodeDev* nodes, int64_t n, uint32_t node_off,
     UPBP_HIP(hipGetLastError());
 }
 
+void prepare_batched(Ctx& ctx, int waves_per_block) {
+    const int wpb = std::min(std::max(waves_per_block, 1), 4);
+    if (ctx.batched_wpb == wpb) return;
+    int p = 0, b = 0;
+    UPBP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &p, reinterpret_cast<const void*>(&k_eval_batched<kEstPhotons>), wpb * kWave, 0));
+    UPBP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &b, reinterpret_cast<const void*>(&k_eval_batched<kEstBeams>), wpb * kWave, 0));
+    ctx.batched_per_cu[0] = std::max(p, 1);
+    ctx.batched_per_cu[1] = std::max(b, 1);
+    ctx.batched_wpb = wpb;
+}
+
 void launch_batched(Ctx& ctx, const InstanceDev* inst, int n_inst, int64_t total_tiles, uint32_t* counters,
                     const MergedDev& merged, int which, int grid_blocks, int waves_per_block) {
     EvalArgs A = {};
@@ -1129,6 +1143,12 @@ void launch_batched(Ctx& ctx, const InstanceDev* inst, int n_inst, int64_t total
     }
     const unsigned threads = (unsigned)(std::min(std::max(waves_per_block, 1), 4) * kWave);
     uint32_t* queue = counters + which;
+    if (grid_blocks <= 0) {
+        // persistent grid: as many blocks as can be resident at once on every CU
+        UPBP_REQUIRE(ctx.batched_wpb == (int)(threads / kWave) && ctx.batched_per_cu[which] > 0, UPBP_EDEVICE,
+                     "batched occupancy not prepared");
+        grid_blocks = ctx.num_cus * ctx.batched_per_cu[which];
+    }
     if (which == kEstPhotons)
         hipLaunchKernelGGL(k_eval_batched<kEstPhotons>, dim3(grid_blocks), dim3(threads), 0, ctx.stream, A, inst,
                            n_inst, total_tiles, queue);

@@ -96,6 +96,9 @@ struct Ctx {
     // (fork from `stream`, join back), so one kernel's tail overlaps the other's work
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
+    // resident blocks per CU of the two persistent batched kernels at batched_wpb waves per block
+    int batched_wpb = 0;
+    int batched_per_cu[2] = {0, 0};
     int num_cus = 256;
     Tuning tuning;
 };
@@ -269,6 +272,8 @@ struct MergedDev {
 };
 void launch_rebase_nodes(Ctx& ctx, NodeDev* nodes, int64_t n, uint32_t node_off, uint32_t item_off);
 
+// Queries (outside any graph capture) how many blocks of each persistent kernel fit on a CU.
+void prepare_batched(Ctx& ctx, int waves_per_block);
 void launch_batched(Ctx& ctx, const InstanceDev* inst, int n_inst, int64_t total_tiles, uint32_t* counters,
                     const MergedDev& merged, int which, int grid_blocks, int waves_per_block);
 void launch_fill_u32(Ctx& ctx, uint32_t* p, uint32_t v, int64_t n);
