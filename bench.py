@@ -11,8 +11,9 @@ once per iteration and evaluates every camera ray).
 
 Prints ONE JSON line: metric, value (Mqueries/s), roofline of the dominant
 kernel (algorithmic bytes / average HIP-event kernel time vs 8 TB/s), the CPU
-oracle timed on a bounded sample on the host cores, and the forward+adjoint vs
-forward cost ratio on the 512x512 differentiable workload.
+oracle timed on a bounded sample on the host cores, the forward+adjoint vs
+forward cost ratio on the 512x512 differentiable workload, the 64-instance
+batched speedup, and the PP3D vertex-merging rate.
 """
 from __future__ import annotations
 
@@ -202,6 +203,73 @@ def batched_ratio(scale: float, steps: int, n_instances: int = 64):
             "speedup": round(res["plain"] / min(res["batched"], res["batched_graph"]), 3)}
 
 
+def pp3d_measure(steps: int, cpu_budget: float, threads: int):
+    """PP3D vertex merging (the reference's hashed grid + RangeQuery::Process with UPBP MIS weights):
+    1,000,000 camera vertices in the box + sphere media against the in-medium vertices of 1,000,000
+    light paths (radius 0.01, one hash bucket per light path). Inputs resident in HBM; one step =
+    one upbp_pp3d_eval call (lane sort + grid walk + weights). The oracle's restatement of the same
+    hashed grid is timed on a query sample as the CPU comparison."""
+    import torch
+
+    from smallupbp_amd import native, scenes
+
+    t = time.time()
+    sc = scenes.pp3d_scene(n_paths=1_000_000, n_queries=1_000_000, radius=0.01, cell_count=1_000_000, seed=91)
+    log("pp3d workload: %d light vertices, %d camera vertices, generated in %.1fs"
+        % (len(sc.positions), len(sc.queries), time.time() - t))
+    ctx = native.Context(0)
+    ctx.set_media(sc.media)
+    pos = torch.from_numpy(np.ascontiguousarray(sc.positions)).cuda()
+    lvt = torch.from_numpy(np.ascontiguousarray(sc.mis.light_vertices).view(np.uint8)).cuda()
+    grid = native.HashGridPP3D(ctx)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    grid.build(pos, lvt, sc.radius, sc.cell_count)
+    t_build = time.perf_counter() - t
+    ms = native.device_mis(sc.mis)
+    q = torch.from_numpy(np.ascontiguousarray(sc.queries).view(np.uint8)).cuda()
+    Q = len(sc.queries)
+    out = torch.empty((Q, 3), device="cuda")
+    for _ in range(2):
+        grid.eval(q, ms, out=out)
+    torch.cuda.synchronize()
+    per = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = {}
+        grid.eval(q, ms, out=out, stats=st)
+        per.append(st)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    ms_step = el / steps * 1e3
+    rec = {"workload": "pp3d_1m (1M camera vertices x %d in-medium light vertices, r = 0.01, MIS on)" % grid.count,
+           "mqueries_per_s": round(Q / (el / steps) / 1e6, 3), "ms_per_step": round(ms_step, 4),
+           "kernel_ms": round(float(np.mean([p["kernel_ms_photons"] for p in per])), 4),
+           "process_calls": int(per[-1]["photon_hits"]), "distance_tests": int(per[-1]["photon_candidates"]),
+           "build_ms": round(t_build * 1e3, 2)}
+    if cpu_budget > 0:
+        from oracle import pyoracle as po
+
+        og = po.OraclePp3d(sc)
+        rng = np.random.default_rng(7)
+        probe = np.sort(rng.choice(Q, 20000, replace=False))
+        t = time.time()
+        po.pp3d_eval(og, sc, queries=sc.queries[probe], threads=threads)
+        per_q = (time.time() - t) / len(probe)
+        n = int(min(Q, max(20000, cpu_budget / max(per_q, 1e-9))))
+        idx = np.sort(rng.choice(Q, n, replace=False))
+        t = time.time()
+        po.pp3d_eval(og, sc, queries=sc.queries[idx], threads=threads)
+        el_cpu = time.time() - t
+        rec["cpu_baseline"] = {"value": round(n / el_cpu / 1e6, 5), "unit": "Mqueries/s", "cores": threads,
+                               "kind": "port", "sample": "%d random camera vertices of the same workload through "
+                               "the oracle's restatement of the reference hashed grid; %.1f s on %d threads"
+                               % (n, el_cpu, threads)}
+        rec["gpu_vs_cpu"] = round(rec["mqueries_per_s"] / rec["cpu_baseline"]["value"], 1)
+    grid.destroy()
+    return rec
+
+
 def calibrate_copy(nbytes: int = 1 << 30):
     """A plain device copy of known size, so PMC runs can scale FETCH_SIZE / WRITE_SIZE."""
     import torch
@@ -228,8 +296,9 @@ def main():
     ap.add_argument("--waves-per-block", type=int, default=0)
     ap.add_argument("--no-tuning", action="store_true",
                     help="ignore smallupbp_amd/tuning_cache.json and use the library defaults")
-    ap.add_argument("--batched", type=float, default=0.0,
-                    help="also time the 64-instance batched form at this size scale (1 = full size)")
+    ap.add_argument("--batched", type=float, default=0.25,
+                    help="also time the 64-instance batched form at this size scale (1 = full size, 0 = skip)")
+    ap.add_argument("--no-pp3d", action="store_true", help="skip the PP3D vertex-merging measurement")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -360,6 +429,9 @@ def main():
 
     if args.batched > 0:
         rec["batched"] = batched_ratio(args.batched, max(2, args.steps // 2))
+
+    if not args.no_pp3d:
+        rec["pp3d"] = pp3d_measure(max(3, args.steps // 2), min(args.cpu_budget, 5.0), host_cores())
 
     if args.cpu_budget > 0:
         threads = host_cores()

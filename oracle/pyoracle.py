@@ -66,6 +66,14 @@ def lib():
         L.or_upbp_eval.argtypes = [vp, vp, vp, i32, C.POINTER(_abi.QueryBatch), C.POINTER(_abi.MisInputs), C.c_int,
                                    C.c_int, C.c_int, vp, vp, vp]
         L.or_exp_cephes.restype = f32
+        L.or_pp3d_create.restype = vp
+        L.or_pp3d_create.argtypes = [i64, vp, vp, f32, i64]
+        L.or_pp3d_free.argtypes = [vp]
+        L.or_pp3d_count.restype = i64
+        L.or_pp3d_count.argtypes = [vp]
+        L.or_pp3d_grid_info.argtypes = [vp, vp, vp, vp]
+        L.or_pp3d_eval.argtypes = [vp, vp, i32, vp, i64, C.POINTER(_abi.MisInputs), C.c_uint32, C.c_int, C.c_int,
+                                   C.c_int, vp, vp, vp, vp]
         L.or_knn_d2.argtypes = [vp, i64, vp, i64, i32, C.c_int, vp]
         L.or_knn_d2.restype = None
         L.or_photon_knn_radius.argtypes = [i64, vp, vp, f32, i32, C.c_int, vp]
@@ -213,6 +221,52 @@ def upbp_eval(oph, obm, media, q: Queries, mi, which: int, precision=32, threads
     if r:
         raise RuntimeError("or_upbp_eval failed: %s" % _abi.STATUS_NAMES.get(r, r))
     return out, int(cnt[0])
+
+
+class OraclePp3d:
+    """HashGrid::Build over the in-medium light vertices of a scenes.Pp3dScene (HashGrid.hxx:46-128)."""
+
+    def __init__(self, sc):
+        self.pos = np.ascontiguousarray(sc.positions, np.float32)
+        self.lv = np.ascontiguousarray(sc.mis.light_vertices)
+        self.cell_count = int(sc.cell_count)
+        self.h = lib().or_pp3d_create(len(self.pos), self.pos.ctypes.data, self.lv.ctypes.data, float(sc.radius),
+                                      self.cell_count)
+        if not self.h:
+            raise RuntimeError("or_pp3d_create rejected its arguments")
+
+    def count(self):
+        return int(lib().or_pp3d_count(self.h))
+
+    def grid_info(self):
+        """(bounding box (6,), mIndices, mCellEnds) of the built grid."""
+        bb = np.zeros(6, np.float32)
+        idx = np.zeros(max(self.count(), 1), np.int32)
+        ends = np.zeros(self.cell_count, np.int32)
+        lib().or_pp3d_grid_info(self.h, bb.ctypes.data, idx.ctypes.data, ends.ctypes.data)
+        return bb, idx[:self.count()], ends
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_pp3d_free(self.h)
+            self.h = None
+
+
+def pp3d_eval(og: OraclePp3d, sc, flags=0, accel=1, precision=32, threads=None, queries=None):
+    """RangeQuery::GetContrib per camera vertex (UPBP.hxx:84-129). Returns (out, sig, counters)."""
+    threads = threads or _threads()
+    q = np.ascontiguousarray(sc.queries if queries is None else queries)
+    m, mp, nm = _media_ptr(sc.media)
+    ms = _abi.mis_inputs(sc.mis)
+    out = np.zeros((len(q), 3), np.float64 if precision == 64 else np.float32)
+    sig = np.zeros((len(q), 2), np.uint32)
+    cnt = np.zeros(2, np.uint64)
+    r = lib().or_pp3d_eval(og.h, mp, nm, q.ctypes.data, len(q), C.byref(ms), int(flags), int(accel), int(precision),
+                           threads, out.ctypes.data if precision != 64 else None,
+                           out.ctypes.data if precision == 64 else None, sig.ctypes.data, cnt.ctypes.data)
+    if r:
+        raise RuntimeError("or_pp3d_eval failed: %s" % _abi.STATUS_NAMES.get(r, r))
+    return out, sig, cnt
 
 
 def hash32(i: np.ndarray) -> np.ndarray:

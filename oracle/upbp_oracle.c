@@ -614,6 +614,169 @@ void or_beams_grid_info(const or_beams* bm, int64_t* info6) {
 }
 
 /* ------------------------------------------------------------------------- */
+/* PP3D hashed grid (SmallUPBP/src/Misc/HashGrid.hxx:38-230), FP32 throughout: the
+ * grid decides which vertices a camera vertex visits, so it is restated in the
+ * reference's operation order. */
+struct or_pp3d {
+    int64_t n_src;          /* light vertices given to the build */
+    int64_t n;              /* vertices in the grid (mInMedium) */
+    float radius, radius_sqr, cell_size, inv_cell_size;
+    float bb_min[3], bb_max[3];
+    int64_t cells;          /* mCellEnds.size() */
+    int32_t* indices;       /* mIndices: light-vertex indices in bucket order */
+    int32_t* cell_ends;     /* mCellEnds after the build: one past each bucket's last index */
+    uint32_t* bucket;       /* bucket of each grid slot's vertex (brute-force check) */
+    float* pos;             /* [n_src][3] */
+};
+
+/* GetCellIndex(x, y, z), HashGrid.hxx:195-206: unsigned 32-bit products, XOR, modulo. */
+static inline int32_t pp3d_hash(const or_pp3d* g, int32_t x, int32_t y, int32_t z) {
+    const uint32_t ux = (uint32_t)x, uy = (uint32_t)y, uz = (uint32_t)z;
+    return (int32_t)(((ux * 73856093u) ^ (uy * 19349663u) ^ (uz * 83492791u)) % (uint32_t)g->cells);
+}
+
+/* GetCellIndex(Pos), HashGrid.hxx:208-218. */
+static inline int32_t pp3d_cell_of(const or_pp3d* g, const float* p) {
+    const float dx = p[0] - g->bb_min[0], dy = p[1] - g->bb_min[1], dz = p[2] - g->bb_min[2];
+    return pp3d_hash(g, (int32_t)floorf(g->inv_cell_size * dx), (int32_t)floorf(g->inv_cell_size * dy),
+                     (int32_t)floorf(g->inv_cell_size * dz));
+}
+
+/* HashGrid::Build, HashGrid.hxx:46-128 (type PP3D: MatchesType = mInMedium). */
+or_pp3d* or_pp3d_create(int64_t n, const float* pos, const upbp_light_vertex* lv, float radius, int64_t cell_count) {
+    if (n < 0 || cell_count < 1 || !(radius > 0.f)) return NULL;
+    or_pp3d* g = (or_pp3d*)calloc(1, sizeof(or_pp3d));
+    g->n_src = n;
+    g->radius = radius;
+    g->radius_sqr = radius * radius;
+    g->cell_size = radius * 2.f;
+    g->inv_cell_size = 1.f / g->cell_size;
+    g->cells = cell_count;
+    for (int a = 0; a < 3; ++a) {
+        g->bb_min[a] = 1e36f;
+        g->bb_max[a] = -1e36f;
+    }
+    g->pos = (float*)malloc(sizeof(float) * 3 * (size_t)(n > 0 ? n : 1));
+    if (n > 0) memcpy(g->pos, pos, sizeof(float) * 3 * (size_t)n);
+    int64_t matched = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        if (!(lv[i].flags & UPBP_LV_IN_MEDIUM)) continue;
+        for (int a = 0; a < 3; ++a) {
+            /* std::max(a, b) = (a < b) ? b : a; std::min(a, b) = (b < a) ? b : a */
+            if (g->bb_max[a] < pos[3 * i + a]) g->bb_max[a] = pos[3 * i + a];
+            if (pos[3 * i + a] < g->bb_min[a]) g->bb_min[a] = pos[3 * i + a];
+        }
+        matched++;
+    }
+    g->n = matched;
+    g->indices = (int32_t*)malloc(sizeof(int32_t) * (size_t)(matched > 0 ? matched : 1));
+    g->bucket = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(matched > 0 ? matched : 1));
+    g->cell_ends = (int32_t*)calloc((size_t)cell_count, sizeof(int32_t));
+    for (int64_t i = 0; i < n; ++i)
+        if (lv[i].flags & UPBP_LV_IN_MEDIUM) g->cell_ends[pp3d_cell_of(g, pos + 3 * i)]++;
+    int32_t sum = 0;
+    for (int64_t c = 0; c < cell_count; ++c) {
+        const int32_t t = g->cell_ends[c];
+        g->cell_ends[c] = sum;
+        sum += t;
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        if (!(lv[i].flags & UPBP_LV_IN_MEDIUM)) continue;
+        const int32_t c = pp3d_cell_of(g, pos + 3 * i);
+        const int32_t slot = g->cell_ends[c]++;
+        g->indices[slot] = (int32_t)i;
+        g->bucket[slot] = (uint32_t)c;
+    }
+    return g;
+}
+
+void or_pp3d_free(or_pp3d* g) {
+    if (!g) return;
+    free(g->indices);
+    free(g->bucket);
+    free(g->cell_ends);
+    free(g->pos);
+    free(g);
+}
+
+int64_t or_pp3d_count(const or_pp3d* g) { return g->n; }
+
+void or_pp3d_grid_info(const or_pp3d* g, float* bb6, int32_t* indices, int32_t* cell_ends) {
+    if (bb6)
+        for (int a = 0; a < 3; ++a) {
+            bb6[a] = g->bb_min[a];
+            bb6[3 + a] = g->bb_max[a];
+        }
+    if (indices && g->n > 0) memcpy(indices, g->indices, sizeof(int32_t) * (size_t)g->n);
+    if (cell_ends) memcpy(cell_ends, g->cell_ends, sizeof(int32_t) * (size_t)g->cells);
+}
+
+/* The eight buckets a query visits, HashGrid::Process, HashGrid.hxx:130-186.
+ * Returns 0 when the query lies outside the grid's bounding box. */
+static int pp3d_buckets(const or_pp3d* g, const float* q, int32_t* h8) {
+    float dmin[3], dmax[3];
+    for (int a = 0; a < 3; ++a) {
+        dmin[a] = q[a] - g->bb_min[a];
+        dmax[a] = g->bb_max[a] - q[a];
+    }
+    for (int a = 0; a < 3; ++a) {
+        if (dmin[a] < 0.f) return 0;
+        if (dmax[a] < 0.f) return 0;
+    }
+    int32_t p[3], po[3];
+    for (int a = 0; a < 3; ++a) {
+        const float cp = g->inv_cell_size * dmin[a];
+        const float cf = floorf(cp);
+        p[a] = (int32_t)cf;
+        const float fr = cp - cf;
+        po[a] = p[a] + (fr < 0.5f ? -1 : +1);
+    }
+    for (int j = 0; j < 8; ++j) {
+        /* case order of HashGrid.hxx:163-172: x varies slowest, z fastest */
+        const int32_t x = (j & 4) ? po[0] : p[0], y = (j & 2) ? po[1] : p[1], z = (j & 1) ? po[2] : p[2];
+        h8[j] = pp3d_hash(g, x, y, z);
+    }
+    return 1;
+}
+
+/* distSqr of HashGrid.hxx:178-179: (query - particle).square() = dot(d, d). */
+static inline float pp3d_dist_sqr(const float* q, const float* p) {
+    const float dx = q[0] - p[0], dy = q[1] - p[1], dz = q[2] - p[2];
+    return (dx * dx + dy * dy) + dz * dz;
+}
+
+/* Calls visit(ctx, vertex) for every Process call of one query, in the grid's order
+ * (accel = 1), or, as an independent statement of the same multiset (accel = 0), for
+ * every grid vertex within the radius once per neighbour bucket equal to its own. */
+typedef void (*pp3d_visit_fn)(void* ctx, int32_t vertex);
+static int64_t pp3d_query(const or_pp3d* g, const float* q, int accel, pp3d_visit_fn visit, void* ctx) {
+    int32_t h8[8];
+    if (g->n == 0 || !pp3d_buckets(g, q, h8)) return 0;
+    int64_t tested = 0;
+    if (accel) {
+        for (int j = 0; j < 8; ++j) {
+            const int32_t c = h8[j];
+            int32_t k = c == 0 ? 0 : g->cell_ends[c - 1];
+            const int32_t e = g->cell_ends[c];
+            for (; k < e; ++k) {
+                const int32_t v = g->indices[k];
+                tested++;
+                if (pp3d_dist_sqr(q, g->pos + 3 * (int64_t)v) <= g->radius_sqr) visit(ctx, v);
+            }
+        }
+    } else {
+        for (int64_t k = 0; k < g->n; ++k) {
+            const int32_t v = g->indices[k];
+            tested++;
+            if (!(pp3d_dist_sqr(q, g->pos + 3 * (int64_t)v) <= g->radius_sqr)) continue;
+            for (int j = 0; j < 8; ++j)
+                if ((uint32_t)h8[j] == g->bucket[k]) visit(ctx, v);
+        }
+    }
+    return tested;
+}
+
+/* ------------------------------------------------------------------------- */
 /* The estimator mathematics, in FP32 and FP64. */
 #define REAL float
 #define SUF(name) name##_f32
@@ -867,6 +1030,23 @@ int or_upbp_eval(const or_photons* ph, const or_beams* bm, const upbp_medium* me
         r = upbp_eval_all_f64(ph, bm, m, n_media, q, mis, which, threads, out64, counters);
     else
         r = upbp_eval_all_f32(ph, bm, m, n_media, q, mis, which, threads, out32, counters);
+    free(m);
+    return r;
+}
+
+int or_pp3d_eval(const or_pp3d* g, const upbp_medium* media, int32_t n_media, const upbp_pp3d_query* q, int64_t nq,
+                 const upbp_mis_inputs* mis, uint32_t flags, int accel, int precision, int threads, float* out32,
+                 double* out64, uint32_t* sig, uint64_t* counters) {
+    if (!g || nq < 0 || (nq > 0 && !q) || !mis || n_media < 1 || !media) return UPBP_EINVAL;
+    if (g->n_src > mis->light_vertex_count) return UPBP_EINVAL;
+    or_medium* m = (or_medium*)malloc(sizeof(or_medium) * (size_t)n_media);
+    or_prepare_media(media, n_media, m);
+    if (threads <= 0) threads = 1;
+    int r;
+    if (precision == 64)
+        r = pp3d_eval_all_f64(g, m, n_media, q, nq, mis, flags, accel, threads, out64, sig, counters);
+    else
+        r = pp3d_eval_all_f32(g, m, n_media, q, nq, mis, flags, accel, threads, out32, sig, counters);
     free(m);
     return r;
 }

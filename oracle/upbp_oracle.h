@@ -131,6 +131,25 @@ void or_photon_knn_radius(int64_t n, const float* pos, const uint8_t* in_medium,
 void or_beam_knn_radius(int64_t n, const float* origin, const float* dir, const float* length, float radius,
                         int32_t knn, int threads, float* r0, float* r1);
 
+/* ---- PP3D vertex merging (HashGrid.hxx:38-230, UPBP.hxx:63-133) ----------- */
+typedef struct or_pp3d or_pp3d;
+/* HashGrid::Reserve(cell_count) + Build(vertices, radius, PP3D): vertices whose
+ * flags hold UPBP_LV_IN_MEDIUM, FP32 bounding box, cell size 2r, hashed buckets. */
+or_pp3d* or_pp3d_create(int64_t n, const float* pos, const upbp_light_vertex* lv, float radius, int64_t cell_count);
+void or_pp3d_free(or_pp3d* g);
+int64_t or_pp3d_count(const or_pp3d* g);
+/* bb6 = (min.xyz, max.xyz); indices [count] (mIndices); cell_ends [cell_count] (mCellEnds). */
+void or_pp3d_grid_info(const or_pp3d* g, float* bb6, int32_t* indices, int32_t* cell_ends);
+/* RangeQuery over HashGrid::Process per camera vertex. accel 1: the grid's own walk
+ * (HashGrid.hxx:130-186); accel 0: every grid vertex within the radius, once per
+ * neighbour bucket equal to its own bucket (the same multiset, stated without the
+ * grid's bucket ranges). out32/out64 [nq][3] by precision, sig [nq][2] optional
+ * (Process calls, sum of or_hash32(vertex)); counters[0] = Process calls,
+ * counters[1] = distance tests. Returns 0 or an upbp_status. */
+int or_pp3d_eval(const or_pp3d* g, const upbp_medium* media, int32_t n_media, const upbp_pp3d_query* q, int64_t nq,
+                 const upbp_mis_inputs* mis, uint32_t flags, int accel, int precision, int threads, float* out32,
+                 double* out64, uint32_t* sig, uint64_t* counters);
+
 #ifdef __cplusplus
 }
 #endif

@@ -122,7 +122,8 @@ class MisInputs(C.Structure):
 def mis_inputs(mi, memory=MEM_HOST, arrays=None):
     """upbp_mis_inputs over a scenes.MisInputs (host numpy) or over `arrays` (dict of torch tensors)."""
     src = arrays if arrays is not None else {
-        "queries": np.ascontiguousarray(mi.queries), "camera": np.ascontiguousarray(mi.camera),
+        "queries": None if mi.queries is None else np.ascontiguousarray(mi.queries),
+        "camera": np.ascontiguousarray(mi.camera),
         "light_vertices": np.ascontiguousarray(mi.light_vertices),
         "path_ends": np.ascontiguousarray(mi.path_ends, np.int32),
         "beams": None if mi.beams is None else np.ascontiguousarray(mi.beams)}

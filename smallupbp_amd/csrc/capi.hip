@@ -3,6 +3,7 @@
 // Every entry point catches all errors and returns an upbp_status; the message
 // is kept in the context (upbp_ctx_last_error). Nothing here exits the process.
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -15,6 +16,7 @@ using namespace upbp;
 struct upbp_ctx : Ctx {};
 struct upbp_bre : Bre {};
 struct upbp_beams : Beams {};
+struct upbp_pp3d : Pp3d {};
 
 struct upbp_batch_plan {
     Ctx* ctx = nullptr;
@@ -597,6 +599,146 @@ upbp_status upbp_eval_upbp(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams*
         o.out_ph = bre ? out_photons : nullptr;
         o.out_bm = beams ? out_beams : nullptr;
         run_eval(*ctx, bre, beams, queries, Mode::Upbp, o, out_memory, stats, nullptr, nullptr, true, &d);
+    });
+}
+
+upbp_status upbp_pp3d_build(upbp_ctx* ctx, const float* positions, const upbp_light_vertex* vertices,
+                            int64_t count, float radius, int64_t cell_count, upbp_memory memory,
+                            upbp_pp3d** out_grid) {
+    if (!ctx) return fail(nullptr, UPBP_EINVAL, "ctx is NULL");
+    if (!out_grid) return fail(ctx, UPBP_EINVAL, "out_grid is NULL");
+    *out_grid = nullptr;
+    upbp_pp3d* g = new (std::nothrow) upbp_pp3d();
+    if (!g) return fail(ctx, UPBP_ENOMEM, "host allocation failed");
+    upbp_status st = guarded(ctx, [&] {
+        UPBP_REQUIRE(count >= 0 && count < ((int64_t)1 << 31), UPBP_EINVAL, "light vertex count out of range");
+        UPBP_REQUIRE(count == 0 || (positions && vertices), UPBP_EINVAL, "positions and vertices are required");
+        UPBP_REQUIRE(std::isfinite(radius) && radius > 0.f, UPBP_EINVAL, "PP3D radius must be finite and > 0");
+        UPBP_REQUIRE(cell_count >= 1 && cell_count <= ((int64_t)1 << 30), UPBP_EINVAL,
+                     "cell_count must be in [1, 2^30]");
+        UPBP_REQUIRE(memory == UPBP_MEM_HOST || memory == UPBP_MEM_DEVICE, UPBP_EINVAL, "bad memory kind");
+        g->ctx = ctx;
+        const float* pos = positions;
+        const uint32_t* flags = nullptr;
+        int64_t stride = 1;
+        DevBuf pos_d, flags_d;
+        if (count > 0) {
+            if (memory == UPBP_MEM_HOST) {
+                pos_d.ensure((size_t)count * 12);
+                UPBP_HIP(hipMemcpyAsync(pos_d.p, positions, (size_t)count * 12, hipMemcpyHostToDevice, ctx->stream));
+                std::vector<uint32_t> fl((size_t)count);
+                for (int64_t i = 0; i < count; ++i) fl[(size_t)i] = vertices[i].flags;
+                flags_d.ensure((size_t)count * 4);
+                UPBP_HIP(hipMemcpyAsync(flags_d.p, fl.data(), (size_t)count * 4, hipMemcpyHostToDevice, ctx->stream));
+                UPBP_HIP(hipStreamSynchronize(ctx->stream));  // fl leaves scope
+                pos = pos_d.as<float>();
+                flags = flags_d.as<uint32_t>();
+            } else {
+                flags = reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(vertices) +
+                                                          offsetof(upbp_light_vertex, flags));
+                stride = (int64_t)(sizeof(upbp_light_vertex) / sizeof(uint32_t));
+            }
+        }
+        pp3d_build(*ctx, pos, flags, stride, count, radius, cell_count, *g);
+    });
+    if (st != UPBP_OK) {
+        delete g;
+        return st;
+    }
+    ctx_retain(ctx);
+    *out_grid = g;
+    return UPBP_OK;
+}
+
+upbp_status upbp_pp3d_destroy(upbp_pp3d* grid) {
+    if (!grid) return UPBP_OK;
+    Ctx* ctx = grid->ctx;
+    (void)hipSetDevice(ctx->device);
+    delete grid;
+    ctx_release(ctx);
+    return UPBP_OK;
+}
+
+int64_t upbp_pp3d_count(const upbp_pp3d* grid) { return grid ? grid->n : 0; }
+
+upbp_status upbp_pp3d_eval(upbp_ctx* ctx, const upbp_pp3d* grid, const upbp_pp3d_query* queries, int64_t count,
+                           const upbp_mis_inputs* mis, uint32_t flags, float* out, uint32_t* sig,
+                           upbp_memory memory, upbp_eval_stats* stats) {
+    if (!ctx) return fail(nullptr, UPBP_EINVAL, "ctx is NULL");
+    return guarded(ctx, [&] {
+        UPBP_REQUIRE(grid != nullptr, UPBP_EINVAL, "grid is NULL");
+        UPBP_REQUIRE(grid->ctx == ctx, UPBP_EINVAL, "grid belongs to another context");
+        UPBP_REQUIRE(count >= 0 && count < ((int64_t)1 << 32), UPBP_EINVAL, "query count out of range");
+        UPBP_REQUIRE(count == 0 || queries, UPBP_EINVAL, "queries is NULL");
+        UPBP_REQUIRE(out || sig, UPBP_EINVAL, "out and sig are both NULL");
+        UPBP_REQUIRE((flags & ~UPBP_PP3D_NO_MIS) == 0, UPBP_EINVAL, "unknown PP3D flags");
+        UPBP_REQUIRE(memory == UPBP_MEM_HOST || memory == UPBP_MEM_DEVICE, UPBP_EINVAL, "bad memory kind");
+        UPBP_REQUIRE(ctx->n_media > 0, UPBP_EINVAL, "upbp_set_media must be called before evaluation");
+        UPBP_REQUIRE(mis != nullptr, UPBP_EINVAL, "mis inputs are required (light vertices)");
+        UPBP_REQUIRE(mis->light_vertex_count >= grid->n_src && (mis->light_vertex_count == 0 || mis->light_vertices),
+                     UPBP_EINVAL, "mis->light_vertices must hold the light vertices the grid was built from");
+        const bool use_mis = !(flags & UPBP_PP3D_NO_MIS);
+        UPBP_REQUIRE(!use_mis || (mis->camera_count > 0 && mis->camera), UPBP_EINVAL, "mis->camera is required");
+        UPBP_REQUIRE(mis->path_count >= 0 && (mis->path_count == 0 || mis->path_ends), UPBP_EINVAL,
+                     "mis->path_ends is required");
+        UPBP_REQUIRE(!use_mis || mis->params.pp3d_factor != 0.f, UPBP_EINVAL, "pp3d_factor must be nonzero");
+        const bool timing = stats != nullptr;
+        if (timing) UPBP_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+        MisDev d = {};
+        d.params = mis->params;
+        DevBuf sc, sl, sp, sq;
+        auto stage = [&](DevBuf& buf, const void* src, size_t bytes, upbp_memory mem) -> const void* {
+            if (!src || mem == UPBP_MEM_DEVICE) return src;
+            buf.ensure(std::max<size_t>(bytes, 16));
+            UPBP_HIP(hipMemcpyAsync(buf.p, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+            return buf.p;
+        };
+        d.camera = use_mis ? (const upbp_mis_data*)stage(sc, mis->camera,
+                                                         (size_t)mis->camera_count * sizeof(upbp_mis_data), mis->memory)
+                           : nullptr;
+        d.camera_count = use_mis ? mis->camera_count : 0;
+        d.lv = (const upbp_light_vertex*)stage(sl, mis->light_vertices,
+                                               (size_t)mis->light_vertex_count * sizeof(upbp_light_vertex), mis->memory);
+        d.lv_count = mis->light_vertex_count;
+        d.path_ends = (const int32_t*)stage(sp, mis->path_ends, (size_t)mis->path_count * 4, mis->memory);
+        d.path_count = mis->path_count;
+        const upbp_pp3d_query* dq =
+            (const upbp_pp3d_query*)stage(sq, queries, (size_t)count * sizeof(upbp_pp3d_query), memory);
+        float* dout = out;
+        uint32_t* dsig = sig;
+        if (memory == UPBP_MEM_HOST && count > 0) {
+            if (out) {
+                ctx->staging[4].ensure(count * 12);
+                dout = ctx->staging[4].as<float>();
+            }
+            if (sig) {
+                ctx->staging[6].ensure(count * 8);
+                dsig = ctx->staging[6].as<uint32_t>();
+            }
+        }
+        launch_clear_u64(*ctx, ctx->stats.as<uint64_t>(), kStatSlots);
+        if (timing) UPBP_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+        pp3d_eval(*ctx, *grid, dq, count, d, flags, dout, dsig);
+        if (timing) UPBP_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
+        if (memory == UPBP_MEM_HOST && count > 0) {
+            if (out) UPBP_HIP(hipMemcpyAsync(out, dout, count * 12, hipMemcpyDeviceToHost, ctx->stream));
+            if (sig) UPBP_HIP(hipMemcpyAsync(sig, dsig, count * 8, hipMemcpyDeviceToHost, ctx->stream));
+        }
+        // the staged inputs are freed on return, so the call always completes here
+        uint64_t st[kStatSlots];
+        UPBP_HIP(hipMemcpyAsync(st, ctx->stats.p, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+        UPBP_HIP(hipStreamSynchronize(ctx->stream));
+        float ms_other = 0.f, ms_k = 0.f;
+        if (timing) {
+            UPBP_HIP(hipEventElapsedTime(&ms_other, ctx->ev[0], ctx->ev[1]));
+            UPBP_HIP(hipEventElapsedTime(&ms_k, ctx->ev[1], ctx->ev[2]));
+        }
+        fill_stats(stats, st, ms_k, 0.f, ms_other, (out ? 1u : 0u) + (sig ? 1u : 0u));
+        const uint64_t e = st[kStatError];
+        UPBP_REQUIRE(!(e & kErrNonFinite), UPBP_ENONFINITE, "non-finite camera vertex position or direction");
+        UPBP_REQUIRE(!(e & kErrBadMedium), UPBP_EINVAL, "camera vertex medium outside the media table");
+        UPBP_REQUIRE(!(e & kBadLightIndex), UPBP_EINVAL,
+                     "PP3D inputs: camera range, light vertex or path index out of range");
     });
 }
 

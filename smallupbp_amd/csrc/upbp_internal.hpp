@@ -229,6 +229,23 @@ void launch_clear_u64(Ctx& ctx, uint64_t* p, int64_t n);
 void launch_clear_f32(Ctx& ctx, float* p, int64_t n);
 void launch_combine_media_grads(Ctx& ctx, const float* gm, int n_media, float* gs, float* ga);
 
+// ---- pp3d.hip: PP3D hashed grid ----------------------------------------------
+struct Pp3d {
+    Ctx* ctx = nullptr;
+    int64_t n_src = 0;  // light vertices given to the build
+    int64_t n = 0;      // vertices in the grid
+    int64_t cells = 0;  // hash buckets
+    float radius = 0.f, r2 = 0.f, inv_cell = 0.f;
+    float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    DevBuf slots;  // float4 [n]: position, light-vertex index (bits); bucket order
+    DevBuf ends;   // uint32 [cells]: one past each bucket's last slot
+};
+// flags[i * stride] holds the UPBP_LV_* word of light vertex i (device pointers).
+void pp3d_build(Ctx& ctx, const float* pos, const uint32_t* flags, int64_t stride, int64_t n, float radius,
+                int64_t cells, Pp3d& out);
+void pp3d_eval(Ctx& ctx, const Pp3d& grid, const upbp_pp3d_query* q, int64_t count, const MisDev& mis,
+               uint32_t flags, float* out, uint32_t* sig);
+
 // ---- batched persistent kernels --------------------------------------------
 struct InstanceDev {
     const NodeDev* ph_nodes;

@@ -462,7 +462,7 @@ MIS_DTYPE = np.dtype(
 assert MIS_DTYPE.itemsize == 64
 LIGHT_VERTEX_DTYPE = np.dtype(
     [("throughput", "<f4", (3,)), ("dir_fix", "<f4", (3,)), ("continuation_prob", "<f4"), ("path_idx", "<i4"),
-     ("path_length", "<u4"), ("is_finite", "<u4"), ("pad", "<u4", (2,)), ("mis", MIS_DTYPE)])
+     ("path_length", "<u4"), ("is_finite", "<u4"), ("flags", "<u4"), ("pad", "<u4"), ("mis", MIS_DTYPE)])
 assert LIGHT_VERTEX_DTYPE.itemsize == 112
 BEAM_MIS_DTYPE = np.dtype([("light_vertex", "<i4"), ("ray_sampling_flags", "<u4"), ("ray_sample_pdf", "<f4"),
                            ("ray_sample_rev_pdf", "<f4"), ("last_pdf_w_inv", "<f4"), ("pad", "<u4", (3,))])
@@ -471,7 +471,13 @@ QUERY_MIS_DTYPE = np.dtype([("camera_offset", "<i8"), ("camera_path_length", "<u
                             ("last_pdf_w_inv", "<f4"), ("pad", "<u4")])
 assert QUERY_MIS_DTYPE.itemsize == 24
 
+PP3D_QUERY_DTYPE = np.dtype([("position", "<f4", (3,)), ("medium", "<i4"), ("dir_fix", "<f4", (3,)),
+                             ("camera_path_length", "<u4"), ("camera_offset", "<i8"), ("pad", "<i8")])
+assert PP3D_QUERY_DTYPE.itemsize == 48
+
 MIS_DELTA, MIS_ON_LIGHT_SOURCE, MIS_SPECULAR, MIS_IN_MEDIUM_WITH_BEAMS = 1, 2, 4, 8
+LV_CONNECTABLE, LV_IN_MEDIUM = 1, 2
+PP3D_NO_MIS = 1
 BPT, SURF, PP3D, PB2D, BB1D = 4, 8, 16, 32, 64
 NO_SINE_IN_WEIGHTS, PREVIOUS, COMPATIBLE, BB1D_PREVIOUS = 128, 256, 512, 1024
 
@@ -559,3 +565,69 @@ def upbp_scene(width=32, height=18, n_paths=2_000, n_beams=500, photon_radius=0.
                   bb1d_factor=1.3, bb1d_min_mfp=0.5, min_dist_to_med=0.0, has_bb1d_beams=1)
     w = Workload("upbp_%dx%d" % (width, height), media, photons, bm, q, SPHERE, 0, 1, seed)
     return w, in_medium, MisInputs(qm, cam, lv, ends, bmis, params)
+
+
+@dataclasses.dataclass
+class Pp3dScene:
+    media: np.ndarray
+    positions: np.ndarray  # [n][3] float32, light-vertex positions (mHitpoint)
+    radius: float
+    cell_count: int  # hash buckets (the reference reserves one per light path)
+    queries: np.ndarray  # PP3D_QUERY_DTYPE
+    mis: MisInputs  # light vertices (with flags), camera MisData, path ends, params; queries unused
+
+
+def medium_at(p: np.ndarray, sphere=SPHERE, box_medium=0, outside_medium=1) -> np.ndarray:
+    """Medium id by position: the sphere inside the unit box, the box medium elsewhere in it."""
+    c, r, sid = sphere
+    inside_box = np.all((p >= 0) & (p <= 1), axis=-1)
+    d2 = ((p.astype(np.float64) - np.asarray(c)) ** 2).sum(-1)
+    return np.where(d2 <= r * r, sid, np.where(inside_box, box_medium, outside_medium)).astype(np.int32)
+
+
+def pp3d_scene(n_paths=2_000, n_queries=4_000, radius=0.05, cell_count=None, seed=61, max_light_length=4,
+               max_camera_length=3, techniques=PB2D | BB1D | BPT | PP3D | SURF, query_beam_type=LONG_BEAM,
+               photon_beam_type=SHORT_BEAM, mean_cosine=0.0, connectable_p=0.9, query_margin=0.05) -> Pp3dScene:
+    """Light sub-paths whose non-source vertices lie in the box + sphere media (UPBP_LV_IN_MEDIUM), most of
+    them connectable; camera vertices uniform in a box slightly larger than the unit box (so some lie
+    outside the grid's bounding box), each with a camera sub-path of random MIS data."""
+    rng = rng_for(seed)
+    rows = _medium_rows_box_sphere()
+    if mean_cosine:
+        rows = [(sa, ss, mean_cosine if i != 1 else 0.0, cp) for i, (sa, ss, _, cp) in enumerate(rows)]
+    media = make_media(rows)
+    lens = rng.integers(1, max_light_length + 1, n_paths)
+    counts = lens + 1
+    ends = np.cumsum(counts).astype(np.int32)
+    nv = int(ends[-1])
+    starts = np.concatenate([[0], ends[:-1]])
+    lv = np.zeros(nv, LIGHT_VERTEX_DTYPE)
+    path_len = np.arange(nv) - np.repeat(starts, counts)
+    lv["path_idx"] = np.repeat(np.arange(n_paths), counts)
+    lv["path_length"] = path_len
+    lv["throughput"] = rng.uniform(0, 1e-3, (nv, 3))
+    lv["dir_fix"] = _isotropic_dirs(rng, nv)
+    lv["continuation_prob"] = rng.uniform(0.5, 1.0, nv)
+    lv["is_finite"] = 1
+    lv["mis"] = random_mis(rng, nv)
+    src = path_len == 0
+    lv["mis"]["flags"][src] = MIS_ON_LIGHT_SOURCE
+    for f in ("surf_factor", "pp3d_factor", "pb2d_factor", "bb1d_factor"):
+        lv["mis"][f][src] = 0.0
+    conn = rng.uniform(size=nv) < connectable_p
+    lv["flags"] = np.where(~src, LV_IN_MEDIUM, 0) | np.where(conn & ~src, LV_CONNECTABLE, 0)
+    pos = rng.uniform(0, 1, (nv, 3)).astype(np.float32)
+    qp = rng.uniform(-query_margin, 1 + query_margin, (n_queries, 3)).astype(np.float32)
+    clen = rng.integers(1, max_camera_length + 1, n_queries)
+    q = np.zeros(n_queries, PP3D_QUERY_DTYPE)
+    q["position"] = qp
+    q["medium"] = medium_at(qp)
+    q["dir_fix"] = _isotropic_dirs(rng, n_queries)
+    q["camera_path_length"] = clen
+    q["camera_offset"] = np.concatenate([[0], np.cumsum(clen + 1)[:-1]])
+    cam = random_mis(rng, int((clen + 1).sum()))
+    params = dict(techniques=techniques, query_beam_type=query_beam_type, photon_beam_type=photon_beam_type,
+                  min_path_length=0, max_path_length=20, surf_factor=0.7, pp3d_factor=0.9, pb2d_factor=0.6,
+                  bb1d_factor=1.3, bb1d_min_mfp=0.5, min_dist_to_med=0.0, has_bb1d_beams=1)
+    mi = MisInputs(None, cam, lv, ends, None, params)
+    return Pp3dScene(media, pos, float(radius), int(cell_count or n_paths), q, mi)
