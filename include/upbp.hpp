@@ -3,6 +3,8 @@
 // few edits:
 //   upbp::EmbreeBre            <-> EmbreeBre            (SmallUPBP/src/Bre/Bre.hxx:40-184)
 //   upbp::PhotonBeamsEvaluator <-> PhotonBeamsEvaluator (SmallUPBP/src/Beams/PhBeams.hxx:46-160)
+//   upbp::HashGridPP3D         <-> HashGrid + RangeQuery for PP3D merging
+//                                  (SmallUPBP/src/Misc/HashGrid.hxx:38-230, SmallUPBP/src/Renderers/UPBP.hxx:63-133)
 //
 // Two ways to evaluate:
 //   evalBre(beamType, origin, dir, segments, n)  one query, synchronous (the reference call shape;
@@ -197,6 +199,45 @@ public:
 private:
     Context& ctx_;
     upbp_beams* h_ = nullptr;
+};
+
+// PP3D vertex merging: mPP3DHashGrid (UPBP.hxx:701-703) and the RangeQuery gather
+// (UPBP.hxx:1110-1120) for a batch of camera vertices.
+class HashGridPP3D {
+public:
+    explicit HashGridPP3D(Context& ctx) : ctx_(ctx) {}
+    ~HashGridPP3D() { destroy(); }
+    HashGridPP3D(const HashGridPP3D&) = delete;
+    HashGridPP3D& operator=(const HashGridPP3D&) = delete;
+
+    // HashGrid::Reserve (HashGrid.hxx:41-44): number of hash buckets.
+    void Reserve(int64_t cellCount) { cells_ = cellCount; }
+    // HashGrid::Build(mLightVertices, radius, PP3D) (HashGrid.hxx:46-128): positions [n][3]
+    // (mHitpoint) and the light-vertex records (UPBP_LV_IN_MEDIUM selects the grid's vertices).
+    void Build(const float* positions, const upbp_light_vertex* vertices, int64_t n, float radius,
+               upbp_memory memory = UPBP_MEM_HOST) {
+        destroy();
+        check(upbp_pp3d_build(ctx_.get(), positions, vertices, n, radius, cells_, memory, &h_), ctx_.get());
+    }
+    // RangeQuery::GetContrib() per camera vertex (UPBP.hxx:1113-1117, before the caller's
+    // cameraState.mThroughput * mPP3DNormalization); ppm = mAlgorithm == kPPM (weights 1).
+    void Process(const upbp_pp3d_query* queries, int64_t n, const upbp_mis_inputs& mis, bool ppm, float* outRgb,
+                 upbp_memory memory = UPBP_MEM_HOST, upbp_eval_stats* stats = nullptr) {
+        check(upbp_pp3d_eval(ctx_.get(), h_, queries, n, &mis, ppm ? UPBP_PP3D_NO_MIS : 0u, outRgb, nullptr, memory,
+                             stats),
+              ctx_.get());
+    }
+    int64_t count() const { return upbp_pp3d_count(h_); }
+    void destroy() {
+        if (h_) upbp_pp3d_destroy(h_);
+        h_ = nullptr;
+    }
+    upbp_pp3d* get() const { return h_; }
+
+private:
+    Context& ctx_;
+    upbp_pp3d* h_ = nullptr;
+    int64_t cells_ = 1;
 };
 
 }  // namespace upbp
