@@ -221,10 +221,10 @@ def pp3d_measure(steps: int, cpu_budget: float, threads: int):
     ctx.set_media(sc.media)
     pos = torch.from_numpy(np.ascontiguousarray(sc.positions)).cuda()
     lvt = torch.from_numpy(np.ascontiguousarray(sc.mis.light_vertices).view(np.uint8)).cuda()
-    grid = native.HashGridPP3D(ctx)
+    grid = native.HashGrid(ctx)
     torch.cuda.synchronize()
     t = time.perf_counter()
-    grid.build(pos, lvt, sc.radius, sc.cell_count)
+    grid.build(pos, lvt, sc.radius, sc.cell_count, scenes.PP3D)
     t_build = time.perf_counter() - t
     ms = native.device_mis(sc.mis)
     q = torch.from_numpy(np.ascontiguousarray(sc.queries).view(np.uint8)).cuda()
@@ -250,16 +250,16 @@ def pp3d_measure(steps: int, cpu_budget: float, threads: int):
     if cpu_budget > 0:
         from oracle import pyoracle as po
 
-        og = po.OraclePp3d(sc)
+        og = po.OracleHashGrid(sc)
         rng = np.random.default_rng(7)
         probe = np.sort(rng.choice(Q, 20000, replace=False))
         t = time.time()
-        po.pp3d_eval(og, sc, queries=sc.queries[probe], threads=threads)
+        po.vm_eval(og, sc, queries=sc.queries[probe], threads=threads)
         per_q = (time.time() - t) / len(probe)
         n = int(min(Q, max(20000, cpu_budget / max(per_q, 1e-9))))
         idx = np.sort(rng.choice(Q, n, replace=False))
         t = time.time()
-        po.pp3d_eval(og, sc, queries=sc.queries[idx], threads=threads)
+        po.vm_eval(og, sc, queries=sc.queries[idx], threads=threads)
         el_cpu = time.time() - t
         rec["cpu_baseline"] = {"value": round(n / el_cpu / 1e6, 5), "unit": "Mqueries/s", "cores": threads,
                                "kind": "port", "sample": "%d random camera vertices of the same workload through "

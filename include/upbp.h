@@ -344,14 +344,14 @@ typedef struct upbp_light_vertex {
     int32_t path_idx;           /* mPathIdx */
     uint32_t path_length;       /* mPathLength */
     uint32_t is_finite;         /* mIsFinite */
-    uint32_t flags;             /* UPBP_LV_*: read by PP3D merging only (0 for PB2D / BB1D use) */
+    uint32_t flags;             /* UPBP_LV_*: read by PP3D / SURF merging only (0 for PB2D / BB1D use) */
     uint32_t pad;
     upbp_mis_data mis;          /* mMisData */
 } upbp_light_vertex;
 
 /* upbp_light_vertex.flags (UPBPLightVertex.hxx:64-66) */
 #define UPBP_LV_CONNECTABLE 1u  /* mConnectable: the vertex can be merged with */
-#define UPBP_LV_IN_MEDIUM 2u    /* mInMedium: the vertex enters the PP3D grid (MatchesType(PP3D)) */
+#define UPBP_LV_IN_MEDIUM 2u    /* mInMedium: the vertex enters a PP3D grid, else a SURF grid (MatchesType) */
 
 /* PhotonBeam MIS fields (PhotonBeam.hxx:48-54), one per beam. 32 bytes. */
 typedef struct upbp_beam_mis {
@@ -412,30 +412,33 @@ upbp_status upbp_eval_upbp(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams*
                            upbp_eval_stats* stats);
 
 /* ---------------------------------------------------------------------------
- * Point-point 3D vertex merging in media (PP3D). A camera vertex inside a medium
- * gathers the light vertices within a fixed radius through the reference's hashed
- * uniform grid (HashGrid, SmallUPBP/src/Misc/HashGrid.hxx:38-230) and weights each
- * one as RangeQuery::Process does (SmallUPBP/src/Renderers/UPBP.hxx:63-133, called
- * at UPBP.hxx:1110-1120). The grid is reproduced exactly, including two of its
- * properties: a camera vertex outside the bounding box of the grid's vertices finds
- * nothing (HashGrid.hxx:137-143), and when two of the eight neighbour cells hash to
- * the same bucket that bucket is visited twice (HashGrid.hxx:161-185). */
-typedef struct upbp_pp3d upbp_pp3d;
+ * Vertex merging through the reference's hashed uniform grid (HashGrid,
+ * SmallUPBP/src/Misc/HashGrid.hxx:38-230) with the RangeQuery::Process weight
+ * (SmallUPBP/src/Renderers/UPBP.hxx:63-133):
+ *   PP3D: point-point 3D merging of camera vertices in media (UPBP.hxx:1110-1120);
+ *   SURF: surface photon mapping at camera vertices on surfaces (UPBP.hxx:1097-1107).
+ * A camera vertex gathers the light vertices within a fixed radius. The grid is
+ * reproduced exactly, including two of its properties: a camera vertex outside the
+ * bounding box of the grid's vertices finds nothing (HashGrid.hxx:137-143), and when two
+ * of the eight neighbour cells hash to the same bucket that bucket is visited twice
+ * (HashGrid.hxx:161-185). */
+typedef struct upbp_hash_grid upbp_hash_grid;
 
-/* Replaces HashGrid::Reserve(cell_count) + HashGrid::Build(mLightVertices, radius, PP3D)
- * (UPBP.hxx:701-703, HashGrid.hxx:41-128). positions: [count][3] (mHitpoint of each
- * light vertex); vertices: [count], only `flags` is read: vertices with
- * UPBP_LV_IN_MEDIUM enter the grid. The grid keeps light-vertex indices, so upbp_pp3d_eval
- * takes the same light-vertex array (mis->light_vertices). cell_count >= 1 is the number
- * of hash buckets (the reference reserves one per light path). */
-upbp_status upbp_pp3d_build(upbp_ctx* ctx, const float* positions, const upbp_light_vertex* vertices,
-                            int64_t count, float radius, int64_t cell_count, upbp_memory memory,
-                            upbp_pp3d** out_grid);
-upbp_status upbp_pp3d_destroy(upbp_pp3d* grid);
-int64_t upbp_pp3d_count(const upbp_pp3d* grid);  /* vertices in the grid */
+/* Replaces HashGrid::Reserve(cell_count) + HashGrid::Build(mLightVertices, radius, type)
+ * (UPBP.hxx:689-703, HashGrid.hxx:41-128). type = UPBP_PP3D keeps the vertices with
+ * UPBP_LV_IN_MEDIUM, type = UPBP_SURF the others (UPBPLightVertex::MatchesType,
+ * UPBPLightVertex.hxx:74-77). positions: [count][3] (mHitpoint of each light vertex);
+ * vertices: [count], only `flags` is read. The grid keeps light-vertex indices, so the
+ * evaluation takes the same light-vertex array (mis->light_vertices). cell_count >= 1 is
+ * the number of hash buckets (the reference reserves one per light path). */
+upbp_status upbp_hash_grid_build(upbp_ctx* ctx, const float* positions, const upbp_light_vertex* vertices,
+                                 int64_t count, float radius, int64_t cell_count, uint32_t type,
+                                 upbp_memory memory, upbp_hash_grid** out_grid);
+upbp_status upbp_hash_grid_destroy(upbp_hash_grid* grid);
+int64_t upbp_hash_grid_count(const upbp_hash_grid* grid);  /* vertices in the grid */
 
-/* One camera vertex in a medium: RangeQuery's position, camera BSDF (a medium BSDF:
- * scattering coefficient, mean cosine and continuation probability of `medium`,
+/* One camera vertex in a medium (PP3D): RangeQuery's position, camera BSDF (a medium
+ * BSDF: scattering coefficient, mean cosine and continuation probability of `medium`,
  * Bsdf.hxx:144-156) and SubPathState (path length; the camera MisData of the path is
  * entries camera_offset .. camera_offset + camera_path_length of mis->camera, the
  * last being this vertex's, as mCameraVerticesMisData holds it). 48 bytes. */
@@ -448,16 +451,46 @@ typedef struct upbp_pp3d_query {
     int64_t pad;
 } upbp_pp3d_query;
 
-/* upbp_pp3d_eval flags */
-#define UPBP_PP3D_NO_MIS 1u  /* mAlgorithm == kPPM: every MIS weight is 1 (UPBP.hxx:121) */
+/* One camera vertex on a surface (SURF): position, path length and camera MisData as
+ * for PP3D, and the camera vertex's surface BSDF as the renderer set it up
+ * (Bsdf.hxx:102-140, GetComponentProbabilities :672-706): its shading frame, the fixed
+ * direction in that frame, the lobe probabilities, the continuation probability and
+ * the material's diffuse and Phong parameters. Only non-delta BSDFs merge
+ * (UPBP.hxx:1098). 128 bytes. */
+typedef struct upbp_surf_query {
+    float position[3];            /* hitPoint */
+    uint32_t camera_path_length;  /* cameraState.mPathLength, >= 1 */
+    float frame_x[3];             /* mFrame.mX (Frame.hxx:56-62) */
+    float continuation_prob;      /* mContinuationProb */
+    float frame_y[3];             /* mFrame.mY */
+    float diff_prob;              /* mDiffProb */
+    float frame_z[3];             /* mFrame.mZ, the shading normal */
+    float phong_prob;             /* mPhongProb */
+    float local_dir_fix[3];       /* mLocalDirFix = mFrame.ToLocal(-ray.direction) */
+    float phong_exponent;         /* Material::mPhongExponent */
+    float diffuse[3];             /* Material::mDiffuseReflectance */
+    float pad0;
+    float phong[3];               /* Material::mPhongReflectance */
+    float pad1;
+    int64_t camera_offset;
+    int64_t pad2;
+} upbp_surf_query;
+
+/* upbp_pp3d_eval / upbp_surf_eval flags */
+#define UPBP_VM_NO_MIS 1u  /* mAlgorithm == kPPM: every MIS weight is 1 (UPBP.hxx:121) */
 
 /* RangeQuery::GetContrib() per camera vertex (the caller multiplies by
- * cameraState.mThroughput * mPP3DNormalization, UPBP.hxx:1117). out: [count][3] or NULL;
- * sig: [count][2] or NULL, {number of Process calls, sum of hash32(light vertex index)}
- * over the vertices the grid finds within the radius (duplicates included), as
- * upbp_hit_signatures. mis->camera, light_vertices, path_ends and params are read;
- * mis->queries and mis->beams are not. `memory` is that of queries, out and sig. */
-upbp_status upbp_pp3d_eval(upbp_ctx* ctx, const upbp_pp3d* grid, const upbp_pp3d_query* queries,
+ * cameraState.mThroughput * mPP3DNormalization, UPBP.hxx:1117, or mSurfNormalization,
+ * UPBP.hxx:1104). The grid must have been built with type UPBP_PP3D / UPBP_SURF.
+ * out: [count][3] or NULL; sig: [count][2] or NULL, {number of Process calls, sum of
+ * hash32(light vertex index)} over the vertices the grid finds within the radius
+ * (duplicates included), as upbp_hit_signatures. mis->camera, light_vertices, path_ends
+ * and params are read; mis->queries and mis->beams are not. `memory` is that of
+ * queries, out and sig. */
+upbp_status upbp_pp3d_eval(upbp_ctx* ctx, const upbp_hash_grid* grid, const upbp_pp3d_query* queries,
+                           int64_t count, const upbp_mis_inputs* mis, uint32_t flags, float* out,
+                           uint32_t* sig, upbp_memory memory, upbp_eval_stats* stats);
+upbp_status upbp_surf_eval(upbp_ctx* ctx, const upbp_hash_grid* grid, const upbp_surf_query* queries,
                            int64_t count, const upbp_mis_inputs* mis, uint32_t flags, float* out,
                            uint32_t* sig, upbp_memory memory, upbp_eval_stats* stats);
 

@@ -3,7 +3,7 @@
 // few edits:
 //   upbp::EmbreeBre            <-> EmbreeBre            (SmallUPBP/src/Bre/Bre.hxx:40-184)
 //   upbp::PhotonBeamsEvaluator <-> PhotonBeamsEvaluator (SmallUPBP/src/Beams/PhBeams.hxx:46-160)
-//   upbp::HashGridPP3D         <-> HashGrid + RangeQuery for PP3D merging
+//   upbp::HashGrid             <-> HashGrid + RangeQuery for PP3D and SURF merging
 //                                  (SmallUPBP/src/Misc/HashGrid.hxx:38-230, SmallUPBP/src/Renderers/UPBP.hxx:63-133)
 //
 // Two ways to evaluate:
@@ -201,42 +201,50 @@ private:
     upbp_beams* h_ = nullptr;
 };
 
-// PP3D vertex merging: mPP3DHashGrid (UPBP.hxx:701-703) and the RangeQuery gather
-// (UPBP.hxx:1110-1120) for a batch of camera vertices.
-class HashGridPP3D {
+// Vertex merging: mPP3DHashGrid / mSurfHashGrid (UPBP.hxx:689-703) and the RangeQuery
+// gathers (UPBP.hxx:1097-1120) for a batch of camera vertices.
+class HashGrid {
 public:
-    explicit HashGridPP3D(Context& ctx) : ctx_(ctx) {}
-    ~HashGridPP3D() { destroy(); }
-    HashGridPP3D(const HashGridPP3D&) = delete;
-    HashGridPP3D& operator=(const HashGridPP3D&) = delete;
+    explicit HashGrid(Context& ctx) : ctx_(ctx) {}
+    ~HashGrid() { destroy(); }
+    HashGrid(const HashGrid&) = delete;
+    HashGrid& operator=(const HashGrid&) = delete;
 
     // HashGrid::Reserve (HashGrid.hxx:41-44): number of hash buckets.
     void Reserve(int64_t cellCount) { cells_ = cellCount; }
-    // HashGrid::Build(mLightVertices, radius, PP3D) (HashGrid.hxx:46-128): positions [n][3]
-    // (mHitpoint) and the light-vertex records (UPBP_LV_IN_MEDIUM selects the grid's vertices).
-    void Build(const float* positions, const upbp_light_vertex* vertices, int64_t n, float radius,
+    // HashGrid::Build(mLightVertices, radius, type) (HashGrid.hxx:46-128), type UPBP_PP3D or
+    // UPBP_SURF: positions [n][3] (mHitpoint) and the light-vertex records (their flags select
+    // the grid's vertices as MatchesType does).
+    void Build(const float* positions, const upbp_light_vertex* vertices, int64_t n, float radius, uint32_t type,
                upbp_memory memory = UPBP_MEM_HOST) {
         destroy();
-        check(upbp_pp3d_build(ctx_.get(), positions, vertices, n, radius, cells_, memory, &h_), ctx_.get());
+        check(upbp_hash_grid_build(ctx_.get(), positions, vertices, n, radius, cells_, type, memory, &h_),
+              ctx_.get());
     }
-    // RangeQuery::GetContrib() per camera vertex (UPBP.hxx:1113-1117, before the caller's
-    // cameraState.mThroughput * mPP3DNormalization); ppm = mAlgorithm == kPPM (weights 1).
+    // RangeQuery::GetContrib() per camera vertex, before the caller's cameraState.mThroughput *
+    // mPP3DNormalization (UPBP.hxx:1117) or mSurfNormalization (:1104); ppm = mAlgorithm == kPPM.
     void Process(const upbp_pp3d_query* queries, int64_t n, const upbp_mis_inputs& mis, bool ppm, float* outRgb,
                  upbp_memory memory = UPBP_MEM_HOST, upbp_eval_stats* stats = nullptr) {
-        check(upbp_pp3d_eval(ctx_.get(), h_, queries, n, &mis, ppm ? UPBP_PP3D_NO_MIS : 0u, outRgb, nullptr, memory,
+        check(upbp_pp3d_eval(ctx_.get(), h_, queries, n, &mis, ppm ? UPBP_VM_NO_MIS : 0u, outRgb, nullptr, memory,
                              stats),
               ctx_.get());
     }
-    int64_t count() const { return upbp_pp3d_count(h_); }
+    void Process(const upbp_surf_query* queries, int64_t n, const upbp_mis_inputs& mis, bool ppm, float* outRgb,
+                 upbp_memory memory = UPBP_MEM_HOST, upbp_eval_stats* stats = nullptr) {
+        check(upbp_surf_eval(ctx_.get(), h_, queries, n, &mis, ppm ? UPBP_VM_NO_MIS : 0u, outRgb, nullptr, memory,
+                             stats),
+              ctx_.get());
+    }
+    int64_t count() const { return upbp_hash_grid_count(h_); }
     void destroy() {
-        if (h_) upbp_pp3d_destroy(h_);
+        if (h_) upbp_hash_grid_destroy(h_);
         h_ = nullptr;
     }
-    upbp_pp3d* get() const { return h_; }
+    upbp_hash_grid* get() const { return h_; }
 
 private:
     Context& ctx_;
-    upbp_pp3d* h_ = nullptr;
+    upbp_hash_grid* h_ = nullptr;
     int64_t cells_ = 1;
 };
 

@@ -66,14 +66,20 @@ def lib():
         L.or_upbp_eval.argtypes = [vp, vp, vp, i32, C.POINTER(_abi.QueryBatch), C.POINTER(_abi.MisInputs), C.c_int,
                                    C.c_int, C.c_int, vp, vp, vp]
         L.or_exp_cephes.restype = f32
-        L.or_pp3d_create.restype = vp
-        L.or_pp3d_create.argtypes = [i64, vp, vp, f32, i64]
-        L.or_pp3d_free.argtypes = [vp]
-        L.or_pp3d_count.restype = i64
-        L.or_pp3d_count.argtypes = [vp]
-        L.or_pp3d_grid_info.argtypes = [vp, vp, vp, vp]
+        L.or_hgrid_create.restype = vp
+        L.or_hgrid_create.argtypes = [i64, vp, vp, f32, i64, C.c_uint32]
+        L.or_hgrid_free.argtypes = [vp]
+        L.or_hgrid_count.restype = i64
+        L.or_hgrid_count.argtypes = [vp]
+        L.or_hgrid_info.argtypes = [vp, vp, vp, vp]
         L.or_pp3d_eval.argtypes = [vp, vp, i32, vp, i64, C.POINTER(_abi.MisInputs), C.c_uint32, C.c_int, C.c_int,
                                    C.c_int, vp, vp, vp, vp]
+        L.or_surf_eval.argtypes = [vp, vp, i64, C.POINTER(_abi.MisInputs), C.c_uint32, C.c_int, C.c_int, C.c_int,
+                                   vp, vp, vp, vp]
+        L.or_camera_walk.restype = C.c_double
+        L.or_camera_walk.argtypes = [vp, C.c_int] + [C.c_double] * 5 + [C.c_uint32] * 3
+        L.or_light_walk.restype = C.c_double
+        L.or_light_walk.argtypes = [vp, i64, vp, i64, C.c_int, C.c_int, C.c_double, C.c_double] + [C.c_uint32] * 4 + [vp]
         L.or_knn_d2.argtypes = [vp, i64, vp, i64, i32, C.c_int, vp]
         L.or_knn_d2.restype = None
         L.or_photon_knn_radius.argtypes = [i64, vp, vp, f32, i32, C.c_int, vp]
@@ -223,50 +229,79 @@ def upbp_eval(oph, obm, media, q: Queries, mi, which: int, precision=32, threads
     return out, int(cnt[0])
 
 
-class OraclePp3d:
-    """HashGrid::Build over the in-medium light vertices of a scenes.Pp3dScene (HashGrid.hxx:46-128)."""
+class OracleHashGrid:
+    """HashGrid::Build over the light vertices of a scenes.MergeScene (HashGrid.hxx:46-128): the
+    in-medium vertices for PP3D, the others for SURF."""
 
     def __init__(self, sc):
+        self.technique = int(sc.technique)
         self.pos = np.ascontiguousarray(sc.positions, np.float32)
         self.lv = np.ascontiguousarray(sc.mis.light_vertices)
         self.cell_count = int(sc.cell_count)
-        self.h = lib().or_pp3d_create(len(self.pos), self.pos.ctypes.data, self.lv.ctypes.data, float(sc.radius),
-                                      self.cell_count)
+        self.h = lib().or_hgrid_create(len(self.pos), self.pos.ctypes.data, self.lv.ctypes.data, float(sc.radius),
+                                       self.cell_count, self.technique)
         if not self.h:
-            raise RuntimeError("or_pp3d_create rejected its arguments")
+            raise RuntimeError("or_hgrid_create rejected its arguments")
 
     def count(self):
-        return int(lib().or_pp3d_count(self.h))
+        return int(lib().or_hgrid_count(self.h))
 
     def grid_info(self):
         """(bounding box (6,), mIndices, mCellEnds) of the built grid."""
         bb = np.zeros(6, np.float32)
         idx = np.zeros(max(self.count(), 1), np.int32)
         ends = np.zeros(self.cell_count, np.int32)
-        lib().or_pp3d_grid_info(self.h, bb.ctypes.data, idx.ctypes.data, ends.ctypes.data)
+        lib().or_hgrid_info(self.h, bb.ctypes.data, idx.ctypes.data, ends.ctypes.data)
         return bb, idx[:self.count()], ends
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().or_pp3d_free(self.h)
+            lib().or_hgrid_free(self.h)
             self.h = None
 
 
-def pp3d_eval(og: OraclePp3d, sc, flags=0, accel=1, precision=32, threads=None, queries=None):
-    """RangeQuery::GetContrib per camera vertex (UPBP.hxx:84-129). Returns (out, sig, counters)."""
+def vm_eval(og: OracleHashGrid, sc, flags=0, accel=1, precision=32, threads=None, queries=None):
+    """RangeQuery::GetContrib per camera vertex (UPBP.hxx:84-129), PP3D or SURF by the scene's
+    technique. Returns (out, sig, counters)."""
     threads = threads or _threads()
     q = np.ascontiguousarray(sc.queries if queries is None else queries)
-    m, mp, nm = _media_ptr(sc.media)
     ms = _abi.mis_inputs(sc.mis)
     out = np.zeros((len(q), 3), np.float64 if precision == 64 else np.float32)
     sig = np.zeros((len(q), 2), np.uint32)
     cnt = np.zeros(2, np.uint64)
-    r = lib().or_pp3d_eval(og.h, mp, nm, q.ctypes.data, len(q), C.byref(ms), int(flags), int(accel), int(precision),
-                           threads, out.ctypes.data if precision != 64 else None,
-                           out.ctypes.data if precision == 64 else None, sig.ctypes.data, cnt.ctypes.data)
+    o32 = out.ctypes.data if precision != 64 else None
+    o64 = out.ctypes.data if precision == 64 else None
+    if sc.technique == _abi.PP3D:
+        m, mp, nm = _media_ptr(sc.media)
+        r = lib().or_pp3d_eval(og.h, mp, nm, q.ctypes.data, len(q), C.byref(ms), int(flags), int(accel),
+                               int(precision), threads, o32, o64, sig.ctypes.data, cnt.ctypes.data)
+    else:
+        r = lib().or_surf_eval(og.h, q.ctypes.data, len(q), C.byref(ms), int(flags), int(accel), int(precision),
+                               threads, o32, o64, sig.ctypes.data, cnt.ctypes.data)
     if r:
-        raise RuntimeError("or_pp3d_eval failed: %s" % _abi.STATUS_NAMES.get(r, r))
+        raise RuntimeError("vertex merging oracle failed: %s" % _abi.STATUS_NAMES.get(r, r))
     return out, sig, cnt
+
+
+def camera_walk(cam, length, last_rev, ntl, tech, last_sin=0.0, last_ray_rev_inv=0.0, last_ray_rev_ratio=0.0,
+                qbt=_abi.LONG_BEAM, pbt=_abi.SHORT_BEAM) -> float:
+    """AccumulateCameraPathWeight (PathWeight.hxx:33-142) over cam[0 .. length], FP64."""
+    cam = np.ascontiguousarray(cam)
+    return lib().or_camera_walk(cam.ctypes.data, int(length), float(last_rev), float(last_sin),
+                                float(last_ray_rev_inv), float(last_ray_rev_ratio), float(ntl), qbt, pbt, int(tech))
+
+
+def light_walk(lv, path_ends, path_idx, path_length, last_rev, ntl, evaluated, tech, qbt=_abi.LONG_BEAM,
+               pbt=_abi.SHORT_BEAM) -> float:
+    """AccumulateLightPathWeight (PathWeight.hxx:145-270, camera connection false), FP64."""
+    lv = np.ascontiguousarray(lv)
+    ends = np.ascontiguousarray(path_ends, np.int32)
+    err = C.c_int(0)
+    w = lib().or_light_walk(lv.ctypes.data, len(lv), ends.ctypes.data, len(ends), int(path_idx), int(path_length),
+                            float(last_rev), float(ntl), int(evaluated), qbt, pbt, int(tech), C.byref(err))
+    if err.value:
+        raise RuntimeError("light walk read outside the light vertices")
+    return w
 
 
 def hash32(i: np.ndarray) -> np.ndarray:

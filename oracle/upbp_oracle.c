@@ -614,12 +614,13 @@ void or_beams_grid_info(const or_beams* bm, int64_t* info6) {
 }
 
 /* ------------------------------------------------------------------------- */
-/* PP3D hashed grid (SmallUPBP/src/Misc/HashGrid.hxx:38-230), FP32 throughout: the
- * grid decides which vertices a camera vertex visits, so it is restated in the
+/* Vertex-merging hashed grid (SmallUPBP/src/Misc/HashGrid.hxx:38-230), FP32 throughout:
+ * the grid decides which vertices a camera vertex visits, so it is restated in the
  * reference's operation order. */
-struct or_pp3d {
+struct or_hgrid {
+    uint32_t type;          /* UPBP_PP3D (in-medium vertices) or UPBP_SURF (the others) */
     int64_t n_src;          /* light vertices given to the build */
-    int64_t n;              /* vertices in the grid (mInMedium) */
+    int64_t n;              /* vertices in the grid (MatchesType) */
     float radius, radius_sqr, cell_size, inv_cell_size;
     float bb_min[3], bb_max[3];
     int64_t cells;          /* mCellEnds.size() */
@@ -630,22 +631,30 @@ struct or_pp3d {
 };
 
 /* GetCellIndex(x, y, z), HashGrid.hxx:195-206: unsigned 32-bit products, XOR, modulo. */
-static inline int32_t pp3d_hash(const or_pp3d* g, int32_t x, int32_t y, int32_t z) {
+static inline int32_t pp3d_hash(const or_hgrid* g, int32_t x, int32_t y, int32_t z) {
     const uint32_t ux = (uint32_t)x, uy = (uint32_t)y, uz = (uint32_t)z;
     return (int32_t)(((ux * 73856093u) ^ (uy * 19349663u) ^ (uz * 83492791u)) % (uint32_t)g->cells);
 }
 
 /* GetCellIndex(Pos), HashGrid.hxx:208-218. */
-static inline int32_t pp3d_cell_of(const or_pp3d* g, const float* p) {
+static inline int32_t pp3d_cell_of(const or_hgrid* g, const float* p) {
     const float dx = p[0] - g->bb_min[0], dy = p[1] - g->bb_min[1], dz = p[2] - g->bb_min[2];
     return pp3d_hash(g, (int32_t)floorf(g->inv_cell_size * dx), (int32_t)floorf(g->inv_cell_size * dy),
                      (int32_t)floorf(g->inv_cell_size * dz));
 }
 
-/* HashGrid::Build, HashGrid.hxx:46-128 (type PP3D: MatchesType = mInMedium). */
-or_pp3d* or_pp3d_create(int64_t n, const float* pos, const upbp_light_vertex* lv, float radius, int64_t cell_count) {
-    if (n < 0 || cell_count < 1 || !(radius > 0.f)) return NULL;
-    or_pp3d* g = (or_pp3d*)calloc(1, sizeof(or_pp3d));
+/* UPBPLightVertex::MatchesType, UPBPLightVertex.hxx:74-77. */
+static inline int grid_matches(const or_hgrid* g, const upbp_light_vertex* v) {
+    const int in_medium = (v->flags & UPBP_LV_IN_MEDIUM) != 0;
+    return g->type == UPBP_PP3D ? in_medium : !in_medium;
+}
+
+/* HashGrid::Build, HashGrid.hxx:46-128. */
+or_hgrid* or_hgrid_create(int64_t n, const float* pos, const upbp_light_vertex* lv, float radius, int64_t cell_count,
+                        uint32_t type) {
+    if (n < 0 || cell_count < 1 || !(radius > 0.f) || (type != UPBP_PP3D && type != UPBP_SURF)) return NULL;
+    or_hgrid* g = (or_hgrid*)calloc(1, sizeof(or_hgrid));
+    g->type = type;
     g->n_src = n;
     g->radius = radius;
     g->radius_sqr = radius * radius;
@@ -660,7 +669,7 @@ or_pp3d* or_pp3d_create(int64_t n, const float* pos, const upbp_light_vertex* lv
     if (n > 0) memcpy(g->pos, pos, sizeof(float) * 3 * (size_t)n);
     int64_t matched = 0;
     for (int64_t i = 0; i < n; ++i) {
-        if (!(lv[i].flags & UPBP_LV_IN_MEDIUM)) continue;
+        if (!grid_matches(g, &lv[i])) continue;
         for (int a = 0; a < 3; ++a) {
             /* std::max(a, b) = (a < b) ? b : a; std::min(a, b) = (b < a) ? b : a */
             if (g->bb_max[a] < pos[3 * i + a]) g->bb_max[a] = pos[3 * i + a];
@@ -673,7 +682,7 @@ or_pp3d* or_pp3d_create(int64_t n, const float* pos, const upbp_light_vertex* lv
     g->bucket = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(matched > 0 ? matched : 1));
     g->cell_ends = (int32_t*)calloc((size_t)cell_count, sizeof(int32_t));
     for (int64_t i = 0; i < n; ++i)
-        if (lv[i].flags & UPBP_LV_IN_MEDIUM) g->cell_ends[pp3d_cell_of(g, pos + 3 * i)]++;
+        if (grid_matches(g, &lv[i])) g->cell_ends[pp3d_cell_of(g, pos + 3 * i)]++;
     int32_t sum = 0;
     for (int64_t c = 0; c < cell_count; ++c) {
         const int32_t t = g->cell_ends[c];
@@ -681,7 +690,7 @@ or_pp3d* or_pp3d_create(int64_t n, const float* pos, const upbp_light_vertex* lv
         sum += t;
     }
     for (int64_t i = 0; i < n; ++i) {
-        if (!(lv[i].flags & UPBP_LV_IN_MEDIUM)) continue;
+        if (!grid_matches(g, &lv[i])) continue;
         const int32_t c = pp3d_cell_of(g, pos + 3 * i);
         const int32_t slot = g->cell_ends[c]++;
         g->indices[slot] = (int32_t)i;
@@ -690,7 +699,7 @@ or_pp3d* or_pp3d_create(int64_t n, const float* pos, const upbp_light_vertex* lv
     return g;
 }
 
-void or_pp3d_free(or_pp3d* g) {
+void or_hgrid_free(or_hgrid* g) {
     if (!g) return;
     free(g->indices);
     free(g->bucket);
@@ -699,9 +708,9 @@ void or_pp3d_free(or_pp3d* g) {
     free(g);
 }
 
-int64_t or_pp3d_count(const or_pp3d* g) { return g->n; }
+int64_t or_hgrid_count(const or_hgrid* g) { return g->n; }
 
-void or_pp3d_grid_info(const or_pp3d* g, float* bb6, int32_t* indices, int32_t* cell_ends) {
+void or_hgrid_info(const or_hgrid* g, float* bb6, int32_t* indices, int32_t* cell_ends) {
     if (bb6)
         for (int a = 0; a < 3; ++a) {
             bb6[a] = g->bb_min[a];
@@ -713,7 +722,7 @@ void or_pp3d_grid_info(const or_pp3d* g, float* bb6, int32_t* indices, int32_t* 
 
 /* The eight buckets a query visits, HashGrid::Process, HashGrid.hxx:130-186.
  * Returns 0 when the query lies outside the grid's bounding box. */
-static int pp3d_buckets(const or_pp3d* g, const float* q, int32_t* h8) {
+static int pp3d_buckets(const or_hgrid* g, const float* q, int32_t* h8) {
     float dmin[3], dmax[3];
     for (int a = 0; a < 3; ++a) {
         dmin[a] = q[a] - g->bb_min[a];
@@ -749,7 +758,7 @@ static inline float pp3d_dist_sqr(const float* q, const float* p) {
  * (accel = 1), or, as an independent statement of the same multiset (accel = 0), for
  * every grid vertex within the radius once per neighbour bucket equal to its own. */
 typedef void (*pp3d_visit_fn)(void* ctx, int32_t vertex);
-static int64_t pp3d_query(const or_pp3d* g, const float* q, int accel, pp3d_visit_fn visit, void* ctx) {
+static int64_t pp3d_query(const or_hgrid* g, const float* q, int accel, pp3d_visit_fn visit, void* ctx) {
     int32_t h8[8];
     if (g->n == 0 || !pp3d_buckets(g, q, h8)) return 0;
     int64_t tested = 0;
@@ -784,6 +793,7 @@ static int64_t pp3d_query(const or_pp3d* g, const float* q, int accel, pp3d_visi
 #define EXP1(x) expf(x)
 #define SQRT(x) sqrtf(x)
 #define FABS(x) fabsf(x)
+#define POW(x, y) powf(x, y)
 #include "oracle_core.inc"
 #undef REAL
 #undef SUF
@@ -791,6 +801,7 @@ static int64_t pp3d_query(const or_pp3d* g, const float* q, int accel, pp3d_visi
 #undef EXP1
 #undef SQRT
 #undef FABS
+#undef POW
 
 #define REAL double
 #define SUF(name) name##_f64
@@ -798,6 +809,7 @@ static int64_t pp3d_query(const or_pp3d* g, const float* q, int accel, pp3d_visi
 #define EXP1(x) exp(x)
 #define SQRT(x) sqrt(x)
 #define FABS(x) fabs(x)
+#define POW(x, y) pow(x, y)
 #include "oracle_core.inc"
 #undef REAL
 #undef SUF
@@ -805,6 +817,7 @@ static int64_t pp3d_query(const or_pp3d* g, const float* q, int accel, pp3d_visi
 #undef EXP1
 #undef SQRT
 #undef FABS
+#undef POW
 
 float or_phase_pdf(const float fix[3], const float gen[3], float g, float* sin_theta) {
     if (sin_theta) {
@@ -812,6 +825,32 @@ float or_phase_pdf(const float fix[3], const float gen[3], float g, float* sin_t
         *sin_theta = sqrtf(fmaxf(0.f, 1.f - c * c));
     }
     return phase_f32(fix, gen, g);
+}
+
+double or_camera_walk(const upbp_mis_data* cam, int len, double last_rev, double last_sin, double last_ray_rev_inv,
+                      double last_ray_rev_ratio, double next_to_last_partial, uint32_t qbt, uint32_t pbt, uint32_t tech) {
+    cam_view_f64 cv;
+    cv.base = cam;
+    cv.len = len;
+    cv.last = mis_load_f64(&cam[len]);
+    cv.prev_ray_rev_inv = len >= 1 ? cam[len - 1].ray_sample_rev_pdf_inv : 0.0;
+    return camera_path_weight_f64(&cv, last_rev, last_sin, last_ray_rev_inv, last_ray_rev_ratio, next_to_last_partial,
+                                  qbt, pbt, tech);
+}
+
+double or_light_walk(const upbp_light_vertex* lv, int64_t lv_count, const int32_t* path_ends, int64_t path_count,
+                     int path_idx, int path_length, double last_rev, double next_to_last_partial, uint32_t evaluated,
+                     uint32_t qbt, uint32_t pbt, uint32_t tech, int* err) {
+    light_view_f64 l;
+    l.lv = lv;
+    l.lv_count = lv_count;
+    l.path_ends = path_ends;
+    l.path_count = path_count;
+    l.beam_mis = NULL;
+    l.override_idx = -1;
+    l.override_ray_rev_inv = 0;
+    l.err = err;
+    return light_path_weight_f64(&l, path_idx, path_length, last_rev, next_to_last_partial, evaluated, qbt, pbt, tech);
 }
 
 int or_test_bre(const float o[3], const float d[3], float min_t, float max_t, const float p[3],
@@ -1034,10 +1073,10 @@ int or_upbp_eval(const or_photons* ph, const or_beams* bm, const upbp_medium* me
     return r;
 }
 
-int or_pp3d_eval(const or_pp3d* g, const upbp_medium* media, int32_t n_media, const upbp_pp3d_query* q, int64_t nq,
+int or_pp3d_eval(const or_hgrid* g, const upbp_medium* media, int32_t n_media, const upbp_pp3d_query* q, int64_t nq,
                  const upbp_mis_inputs* mis, uint32_t flags, int accel, int precision, int threads, float* out32,
                  double* out64, uint32_t* sig, uint64_t* counters) {
-    if (!g || nq < 0 || (nq > 0 && !q) || !mis || n_media < 1 || !media) return UPBP_EINVAL;
+    if (!g || g->type != UPBP_PP3D || nq < 0 || (nq > 0 && !q) || !mis || n_media < 1 || !media) return UPBP_EINVAL;
     if (g->n_src > mis->light_vertex_count) return UPBP_EINVAL;
     or_medium* m = (or_medium*)malloc(sizeof(or_medium) * (size_t)n_media);
     or_prepare_media(media, n_media, m);
@@ -1049,6 +1088,16 @@ int or_pp3d_eval(const or_pp3d* g, const upbp_medium* media, int32_t n_media, co
         r = pp3d_eval_all_f32(g, m, n_media, q, nq, mis, flags, accel, threads, out32, sig, counters);
     free(m);
     return r;
+}
+
+int or_surf_eval(const or_hgrid* g, const upbp_surf_query* q, int64_t nq, const upbp_mis_inputs* mis, uint32_t flags,
+                 int accel, int precision, int threads, float* out32, double* out64, uint32_t* sig,
+                 uint64_t* counters) {
+    if (!g || g->type != UPBP_SURF || nq < 0 || (nq > 0 && !q) || !mis) return UPBP_EINVAL;
+    if (g->n_src > mis->light_vertex_count) return UPBP_EINVAL;
+    if (threads <= 0) threads = 1;
+    if (precision == 64) return surf_eval_all_f64(g, q, nq, mis, flags, accel, threads, out64, sig, counters);
+    return surf_eval_all_f32(g, q, nq, mis, flags, accel, threads, out32, sig, counters);
 }
 
 /* ------------------------------------------------------------------------- */

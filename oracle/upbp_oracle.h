@@ -53,6 +53,16 @@ int or_test_beam_beam(const float o1[3], const float d1[3], float min_t1, float 
                       const float o2[3], const float d2[3], float min_t2, float max_t2,
                       float max_dist_sqr, float* dist, float* sin_theta, float* t1, float* t2);
 
+/* The path-weight walks themselves in FP64 (AccumulateCameraPathWeight /
+ * AccumulateLightPathWeight, PathWeight.hxx:33-270), for first-principles checks. The
+ * camera walk reads cam[0 .. len] as given; the light walk reads the light path of
+ * path_idx through path_ends. */
+double or_camera_walk(const upbp_mis_data* cam, int len, double last_rev, double last_sin, double last_ray_rev_inv,
+                      double last_ray_rev_ratio, double next_to_last_partial, uint32_t qbt, uint32_t pbt, uint32_t tech);
+double or_light_walk(const upbp_light_vertex* lv, int64_t lv_count, const int32_t* path_ends, int64_t path_count,
+                     int path_idx, int path_length, double last_rev, double next_to_last_partial, uint32_t evaluated,
+                     uint32_t qbt, uint32_t pbt, uint32_t tech, int* err);
+
 /* 32-bit hash of an item index used by hit signatures (same as the GPU library). */
 uint32_t or_hash32(uint32_t i);
 
@@ -131,24 +141,29 @@ void or_photon_knn_radius(int64_t n, const float* pos, const uint8_t* in_medium,
 void or_beam_knn_radius(int64_t n, const float* origin, const float* dir, const float* length, float radius,
                         int32_t knn, int threads, float* r0, float* r1);
 
-/* ---- PP3D vertex merging (HashGrid.hxx:38-230, UPBP.hxx:63-133) ----------- */
-typedef struct or_pp3d or_pp3d;
-/* HashGrid::Reserve(cell_count) + Build(vertices, radius, PP3D): vertices whose
- * flags hold UPBP_LV_IN_MEDIUM, FP32 bounding box, cell size 2r, hashed buckets. */
-or_pp3d* or_pp3d_create(int64_t n, const float* pos, const upbp_light_vertex* lv, float radius, int64_t cell_count);
-void or_pp3d_free(or_pp3d* g);
-int64_t or_pp3d_count(const or_pp3d* g);
+/* ---- vertex merging through the hashed grid (HashGrid.hxx:38-230, UPBP.hxx:63-133) ---- */
+typedef struct or_hgrid or_hgrid;
+/* HashGrid::Reserve(cell_count) + Build(vertices, radius, type): type UPBP_PP3D keeps the
+ * vertices whose flags hold UPBP_LV_IN_MEDIUM, UPBP_SURF the others; FP32 bounding box,
+ * cell size 2r, hashed buckets. */
+or_hgrid* or_hgrid_create(int64_t n, const float* pos, const upbp_light_vertex* lv, float radius, int64_t cell_count,
+                        uint32_t type);
+void or_hgrid_free(or_hgrid* g);
+int64_t or_hgrid_count(const or_hgrid* g);
 /* bb6 = (min.xyz, max.xyz); indices [count] (mIndices); cell_ends [cell_count] (mCellEnds). */
-void or_pp3d_grid_info(const or_pp3d* g, float* bb6, int32_t* indices, int32_t* cell_ends);
-/* RangeQuery over HashGrid::Process per camera vertex. accel 1: the grid's own walk
- * (HashGrid.hxx:130-186); accel 0: every grid vertex within the radius, once per
- * neighbour bucket equal to its own bucket (the same multiset, stated without the
- * grid's bucket ranges). out32/out64 [nq][3] by precision, sig [nq][2] optional
- * (Process calls, sum of or_hash32(vertex)); counters[0] = Process calls,
- * counters[1] = distance tests. Returns 0 or an upbp_status. */
-int or_pp3d_eval(const or_pp3d* g, const upbp_medium* media, int32_t n_media, const upbp_pp3d_query* q, int64_t nq,
+void or_hgrid_info(const or_hgrid* g, float* bb6, int32_t* indices, int32_t* cell_ends);
+/* RangeQuery over HashGrid::Process per camera vertex (PP3D: in a medium; SURF: on a
+ * surface). accel 1: the grid's own walk (HashGrid.hxx:130-186); accel 0: every grid
+ * vertex within the radius, once per neighbour bucket equal to its own bucket (the same
+ * multiset, stated without the grid's bucket ranges). out32/out64 [nq][3] by precision,
+ * sig [nq][2] optional (Process calls, sum of or_hash32(vertex)); counters[0] = Process
+ * calls, counters[1] = distance tests. Returns 0 or an upbp_status. */
+int or_pp3d_eval(const or_hgrid* g, const upbp_medium* media, int32_t n_media, const upbp_pp3d_query* q, int64_t nq,
                  const upbp_mis_inputs* mis, uint32_t flags, int accel, int precision, int threads, float* out32,
                  double* out64, uint32_t* sig, uint64_t* counters);
+int or_surf_eval(const or_hgrid* g, const upbp_surf_query* q, int64_t nq, const upbp_mis_inputs* mis, uint32_t flags,
+                 int accel, int precision, int threads, float* out32, double* out64, uint32_t* sig,
+                 uint64_t* counters);
 
 #ifdef __cplusplus
 }

@@ -16,7 +16,11 @@ using namespace upbp;
 struct upbp_ctx : Ctx {};
 struct upbp_bre : Bre {};
 struct upbp_beams : Beams {};
-struct upbp_pp3d : Pp3d {};
+struct upbp_hash_grid : HashGrid {};
+
+static_assert(sizeof(upbp_light_vertex) == 112, "upbp_light_vertex layout");
+static_assert(sizeof(upbp_pp3d_query) == 48, "upbp_pp3d_query layout");
+static_assert(sizeof(upbp_surf_query) == 128, "upbp_surf_query layout");
 
 struct upbp_batch_plan {
     Ctx* ctx = nullptr;
@@ -602,18 +606,19 @@ upbp_status upbp_eval_upbp(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams*
     });
 }
 
-upbp_status upbp_pp3d_build(upbp_ctx* ctx, const float* positions, const upbp_light_vertex* vertices,
-                            int64_t count, float radius, int64_t cell_count, upbp_memory memory,
-                            upbp_pp3d** out_grid) {
+upbp_status upbp_hash_grid_build(upbp_ctx* ctx, const float* positions, const upbp_light_vertex* vertices,
+                                 int64_t count, float radius, int64_t cell_count, uint32_t type,
+                                 upbp_memory memory, upbp_hash_grid** out_grid) {
     if (!ctx) return fail(nullptr, UPBP_EINVAL, "ctx is NULL");
     if (!out_grid) return fail(ctx, UPBP_EINVAL, "out_grid is NULL");
     *out_grid = nullptr;
-    upbp_pp3d* g = new (std::nothrow) upbp_pp3d();
+    upbp_hash_grid* g = new (std::nothrow) upbp_hash_grid();
     if (!g) return fail(ctx, UPBP_ENOMEM, "host allocation failed");
     upbp_status st = guarded(ctx, [&] {
+        UPBP_REQUIRE(type == UPBP_PP3D || type == UPBP_SURF, UPBP_EINVAL, "grid type must be UPBP_PP3D or UPBP_SURF");
         UPBP_REQUIRE(count >= 0 && count < ((int64_t)1 << 31), UPBP_EINVAL, "light vertex count out of range");
         UPBP_REQUIRE(count == 0 || (positions && vertices), UPBP_EINVAL, "positions and vertices are required");
-        UPBP_REQUIRE(std::isfinite(radius) && radius > 0.f, UPBP_EINVAL, "PP3D radius must be finite and > 0");
+        UPBP_REQUIRE(std::isfinite(radius) && radius > 0.f, UPBP_EINVAL, "merging radius must be finite and > 0");
         UPBP_REQUIRE(cell_count >= 1 && cell_count <= ((int64_t)1 << 30), UPBP_EINVAL,
                      "cell_count must be in [1, 2^30]");
         UPBP_REQUIRE(memory == UPBP_MEM_HOST || memory == UPBP_MEM_DEVICE, UPBP_EINVAL, "bad memory kind");
@@ -639,7 +644,7 @@ upbp_status upbp_pp3d_build(upbp_ctx* ctx, const float* positions, const upbp_li
                 stride = (int64_t)(sizeof(upbp_light_vertex) / sizeof(uint32_t));
             }
         }
-        pp3d_build(*ctx, pos, flags, stride, count, radius, cell_count, *g);
+        hash_grid_build(*ctx, pos, flags, stride, count, radius, cell_count, type, *g);
     });
     if (st != UPBP_OK) {
         delete g;
@@ -650,7 +655,7 @@ upbp_status upbp_pp3d_build(upbp_ctx* ctx, const float* positions, const upbp_li
     return UPBP_OK;
 }
 
-upbp_status upbp_pp3d_destroy(upbp_pp3d* grid) {
+upbp_status upbp_hash_grid_destroy(upbp_hash_grid* grid) {
     if (!grid) return UPBP_OK;
     Ctx* ctx = grid->ctx;
     (void)hipSetDevice(ctx->device);
@@ -659,86 +664,119 @@ upbp_status upbp_pp3d_destroy(upbp_pp3d* grid) {
     return UPBP_OK;
 }
 
-int64_t upbp_pp3d_count(const upbp_pp3d* grid) { return grid ? grid->n : 0; }
+int64_t upbp_hash_grid_count(const upbp_hash_grid* grid) { return grid ? grid->n : 0; }
 
-upbp_status upbp_pp3d_eval(upbp_ctx* ctx, const upbp_pp3d* grid, const upbp_pp3d_query* queries, int64_t count,
+}  // extern "C"
+
+namespace {
+
+// Shared driver of upbp_pp3d_eval and upbp_surf_eval.
+template <class Query, class Launch>
+void run_vm_eval(Ctx* ctx, const HashGrid* grid, uint32_t type, const Query* queries, int64_t count,
+                 const upbp_mis_inputs* mis, uint32_t flags, float* out, uint32_t* sig, upbp_memory memory,
+                 upbp_eval_stats* stats, Launch&& launch) {
+    UPBP_REQUIRE(grid != nullptr, UPBP_EINVAL, "grid is NULL");
+    UPBP_REQUIRE(grid->ctx == ctx, UPBP_EINVAL, "grid belongs to another context");
+    UPBP_REQUIRE(grid->type == type, UPBP_EINVAL,
+                 type == UPBP_PP3D ? "PP3D evaluation needs a grid built with type UPBP_PP3D"
+                                   : "SURF evaluation needs a grid built with type UPBP_SURF");
+    UPBP_REQUIRE(count >= 0 && count < ((int64_t)1 << 32), UPBP_EINVAL, "query count out of range");
+    UPBP_REQUIRE(count == 0 || queries, UPBP_EINVAL, "queries is NULL");
+    UPBP_REQUIRE(out || sig, UPBP_EINVAL, "out and sig are both NULL");
+    UPBP_REQUIRE((flags & ~UPBP_VM_NO_MIS) == 0, UPBP_EINVAL, "unknown vertex-merging flags");
+    UPBP_REQUIRE(memory == UPBP_MEM_HOST || memory == UPBP_MEM_DEVICE, UPBP_EINVAL, "bad memory kind");
+    UPBP_REQUIRE(type != UPBP_PP3D || ctx->n_media > 0, UPBP_EINVAL,
+                 "upbp_set_media must be called before PP3D evaluation");
+    UPBP_REQUIRE(mis != nullptr, UPBP_EINVAL, "mis inputs are required (light vertices)");
+    UPBP_REQUIRE(mis->light_vertex_count >= grid->n_src && (mis->light_vertex_count == 0 || mis->light_vertices),
+                 UPBP_EINVAL, "mis->light_vertices must hold the light vertices the grid was built from");
+    const bool use_mis = !(flags & UPBP_VM_NO_MIS);
+    UPBP_REQUIRE(!use_mis || (mis->camera_count > 0 && mis->camera), UPBP_EINVAL, "mis->camera is required");
+    UPBP_REQUIRE(mis->path_count >= 0 && (mis->path_count == 0 || mis->path_ends), UPBP_EINVAL,
+                 "mis->path_ends is required");
+    const bool timing = stats != nullptr;
+    if (timing) UPBP_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+    MisDev d = {};
+    d.params = mis->params;
+    DevBuf sc, sl, sp, sq;
+    auto stage = [&](DevBuf& buf, const void* src, size_t bytes, upbp_memory mem) -> const void* {
+        if (!src || mem == UPBP_MEM_DEVICE) return src;
+        buf.ensure(std::max<size_t>(bytes, 16));
+        UPBP_HIP(hipMemcpyAsync(buf.p, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        return buf.p;
+    };
+    d.camera = use_mis ? (const upbp_mis_data*)stage(sc, mis->camera,
+                                                     (size_t)mis->camera_count * sizeof(upbp_mis_data), mis->memory)
+                       : nullptr;
+    d.camera_count = use_mis ? mis->camera_count : 0;
+    d.lv = (const upbp_light_vertex*)stage(sl, mis->light_vertices,
+                                           (size_t)mis->light_vertex_count * sizeof(upbp_light_vertex), mis->memory);
+    d.lv_count = mis->light_vertex_count;
+    d.path_ends = (const int32_t*)stage(sp, mis->path_ends, (size_t)mis->path_count * 4, mis->memory);
+    d.path_count = mis->path_count;
+    const Query* dq = (const Query*)stage(sq, queries, (size_t)count * sizeof(Query), memory);
+    float* dout = out;
+    uint32_t* dsig = sig;
+    if (memory == UPBP_MEM_HOST && count > 0) {
+        if (out) {
+            ctx->staging[4].ensure(count * 12);
+            dout = ctx->staging[4].as<float>();
+        }
+        if (sig) {
+            ctx->staging[6].ensure(count * 8);
+            dsig = ctx->staging[6].as<uint32_t>();
+        }
+    }
+    launch_clear_u64(*ctx, ctx->stats.as<uint64_t>(), kStatSlots);
+    if (timing) UPBP_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+    launch(dq, d, dout, dsig);
+    if (timing) UPBP_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
+    if (memory == UPBP_MEM_HOST && count > 0) {
+        if (out) UPBP_HIP(hipMemcpyAsync(out, dout, count * 12, hipMemcpyDeviceToHost, ctx->stream));
+        if (sig) UPBP_HIP(hipMemcpyAsync(sig, dsig, count * 8, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    // the staged inputs are freed on return, so the call always completes here
+    uint64_t st[kStatSlots];
+    UPBP_HIP(hipMemcpyAsync(st, ctx->stats.p, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+    UPBP_HIP(hipStreamSynchronize(ctx->stream));
+    float ms_other = 0.f, ms_k = 0.f;
+    if (timing) {
+        UPBP_HIP(hipEventElapsedTime(&ms_other, ctx->ev[0], ctx->ev[1]));
+        UPBP_HIP(hipEventElapsedTime(&ms_k, ctx->ev[1], ctx->ev[2]));
+    }
+    fill_stats(stats, st, ms_k, 0.f, ms_other, (out ? 1u : 0u) + (sig ? 1u : 0u));
+    const uint64_t e = st[kStatError];
+    UPBP_REQUIRE(!(e & kErrNonFinite), UPBP_ENONFINITE, "non-finite camera vertex position, direction or frame");
+    UPBP_REQUIRE(!(e & kErrBadMedium), UPBP_EINVAL, "camera vertex medium outside the media table");
+    UPBP_REQUIRE(!(e & kBadLightIndex), UPBP_EINVAL,
+                 "vertex merging inputs: camera range, light vertex or path index out of range");
+}
+
+}  // namespace
+
+extern "C" {
+
+upbp_status upbp_pp3d_eval(upbp_ctx* ctx, const upbp_hash_grid* grid, const upbp_pp3d_query* queries, int64_t count,
                            const upbp_mis_inputs* mis, uint32_t flags, float* out, uint32_t* sig,
                            upbp_memory memory, upbp_eval_stats* stats) {
     if (!ctx) return fail(nullptr, UPBP_EINVAL, "ctx is NULL");
     return guarded(ctx, [&] {
-        UPBP_REQUIRE(grid != nullptr, UPBP_EINVAL, "grid is NULL");
-        UPBP_REQUIRE(grid->ctx == ctx, UPBP_EINVAL, "grid belongs to another context");
-        UPBP_REQUIRE(count >= 0 && count < ((int64_t)1 << 32), UPBP_EINVAL, "query count out of range");
-        UPBP_REQUIRE(count == 0 || queries, UPBP_EINVAL, "queries is NULL");
-        UPBP_REQUIRE(out || sig, UPBP_EINVAL, "out and sig are both NULL");
-        UPBP_REQUIRE((flags & ~UPBP_PP3D_NO_MIS) == 0, UPBP_EINVAL, "unknown PP3D flags");
-        UPBP_REQUIRE(memory == UPBP_MEM_HOST || memory == UPBP_MEM_DEVICE, UPBP_EINVAL, "bad memory kind");
-        UPBP_REQUIRE(ctx->n_media > 0, UPBP_EINVAL, "upbp_set_media must be called before evaluation");
-        UPBP_REQUIRE(mis != nullptr, UPBP_EINVAL, "mis inputs are required (light vertices)");
-        UPBP_REQUIRE(mis->light_vertex_count >= grid->n_src && (mis->light_vertex_count == 0 || mis->light_vertices),
-                     UPBP_EINVAL, "mis->light_vertices must hold the light vertices the grid was built from");
-        const bool use_mis = !(flags & UPBP_PP3D_NO_MIS);
-        UPBP_REQUIRE(!use_mis || (mis->camera_count > 0 && mis->camera), UPBP_EINVAL, "mis->camera is required");
-        UPBP_REQUIRE(mis->path_count >= 0 && (mis->path_count == 0 || mis->path_ends), UPBP_EINVAL,
-                     "mis->path_ends is required");
-        UPBP_REQUIRE(!use_mis || mis->params.pp3d_factor != 0.f, UPBP_EINVAL, "pp3d_factor must be nonzero");
-        const bool timing = stats != nullptr;
-        if (timing) UPBP_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
-        MisDev d = {};
-        d.params = mis->params;
-        DevBuf sc, sl, sp, sq;
-        auto stage = [&](DevBuf& buf, const void* src, size_t bytes, upbp_memory mem) -> const void* {
-            if (!src || mem == UPBP_MEM_DEVICE) return src;
-            buf.ensure(std::max<size_t>(bytes, 16));
-            UPBP_HIP(hipMemcpyAsync(buf.p, src, bytes, hipMemcpyHostToDevice, ctx->stream));
-            return buf.p;
-        };
-        d.camera = use_mis ? (const upbp_mis_data*)stage(sc, mis->camera,
-                                                         (size_t)mis->camera_count * sizeof(upbp_mis_data), mis->memory)
-                           : nullptr;
-        d.camera_count = use_mis ? mis->camera_count : 0;
-        d.lv = (const upbp_light_vertex*)stage(sl, mis->light_vertices,
-                                               (size_t)mis->light_vertex_count * sizeof(upbp_light_vertex), mis->memory);
-        d.lv_count = mis->light_vertex_count;
-        d.path_ends = (const int32_t*)stage(sp, mis->path_ends, (size_t)mis->path_count * 4, mis->memory);
-        d.path_count = mis->path_count;
-        const upbp_pp3d_query* dq =
-            (const upbp_pp3d_query*)stage(sq, queries, (size_t)count * sizeof(upbp_pp3d_query), memory);
-        float* dout = out;
-        uint32_t* dsig = sig;
-        if (memory == UPBP_MEM_HOST && count > 0) {
-            if (out) {
-                ctx->staging[4].ensure(count * 12);
-                dout = ctx->staging[4].as<float>();
-            }
-            if (sig) {
-                ctx->staging[6].ensure(count * 8);
-                dsig = ctx->staging[6].as<uint32_t>();
-            }
-        }
-        launch_clear_u64(*ctx, ctx->stats.as<uint64_t>(), kStatSlots);
-        if (timing) UPBP_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
-        pp3d_eval(*ctx, *grid, dq, count, d, flags, dout, dsig);
-        if (timing) UPBP_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
-        if (memory == UPBP_MEM_HOST && count > 0) {
-            if (out) UPBP_HIP(hipMemcpyAsync(out, dout, count * 12, hipMemcpyDeviceToHost, ctx->stream));
-            if (sig) UPBP_HIP(hipMemcpyAsync(sig, dsig, count * 8, hipMemcpyDeviceToHost, ctx->stream));
-        }
-        // the staged inputs are freed on return, so the call always completes here
-        uint64_t st[kStatSlots];
-        UPBP_HIP(hipMemcpyAsync(st, ctx->stats.p, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
-        UPBP_HIP(hipStreamSynchronize(ctx->stream));
-        float ms_other = 0.f, ms_k = 0.f;
-        if (timing) {
-            UPBP_HIP(hipEventElapsedTime(&ms_other, ctx->ev[0], ctx->ev[1]));
-            UPBP_HIP(hipEventElapsedTime(&ms_k, ctx->ev[1], ctx->ev[2]));
-        }
-        fill_stats(stats, st, ms_k, 0.f, ms_other, (out ? 1u : 0u) + (sig ? 1u : 0u));
-        const uint64_t e = st[kStatError];
-        UPBP_REQUIRE(!(e & kErrNonFinite), UPBP_ENONFINITE, "non-finite camera vertex position or direction");
-        UPBP_REQUIRE(!(e & kErrBadMedium), UPBP_EINVAL, "camera vertex medium outside the media table");
-        UPBP_REQUIRE(!(e & kBadLightIndex), UPBP_EINVAL,
-                     "PP3D inputs: camera range, light vertex or path index out of range");
+        run_vm_eval(ctx, grid, UPBP_PP3D, queries, count, mis, flags, out, sig, memory, stats,
+                    [&](const upbp_pp3d_query* dq, const MisDev& d, float* o, uint32_t* sg) {
+                        pp3d_eval(*ctx, *grid, dq, count, d, flags, o, sg);
+                    });
+    });
+}
+
+upbp_status upbp_surf_eval(upbp_ctx* ctx, const upbp_hash_grid* grid, const upbp_surf_query* queries, int64_t count,
+                           const upbp_mis_inputs* mis, uint32_t flags, float* out, uint32_t* sig,
+                           upbp_memory memory, upbp_eval_stats* stats) {
+    if (!ctx) return fail(nullptr, UPBP_EINVAL, "ctx is NULL");
+    return guarded(ctx, [&] {
+        run_vm_eval(ctx, grid, UPBP_SURF, queries, count, mis, flags, out, sig, memory, stats,
+                    [&](const upbp_surf_query* dq, const MisDev& d, float* o, uint32_t* sg) {
+                        surf_eval(*ctx, *grid, dq, count, d, flags, o, sg);
+                    });
     });
 }
 

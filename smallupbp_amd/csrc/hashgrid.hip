@@ -1,8 +1,10 @@
-// pp3d.hip -- PP3D vertex merging in media on the GPU: the reference's hashed
-// uniform grid (SmallUPBP/src/Misc/HashGrid.hxx:38-230) and the RangeQuery::Process
-// weight (SmallUPBP/src/Renderers/UPBP.hxx:84-129).
+// hashgrid.hip -- vertex merging on the GPU through the reference's hashed uniform
+// grid (SmallUPBP/src/Misc/HashGrid.hxx:38-230) with the RangeQuery::Process weight
+// (SmallUPBP/src/Renderers/UPBP.hxx:84-129): PP3D for camera vertices in media, SURF
+// for camera vertices on surfaces.
 //
-// Build. One bounds pass over the in-medium vertices, one bucket key per vertex, a
+// Build. One bounds pass over the grid's vertices (in-medium ones for PP3D, the others
+// for SURF), one bucket key per vertex, a
 // stable radix sort of vertex indices by bucket, and the bucket ends from the run
 // boundaries of the sorted keys followed by a running maximum (empty buckets end
 // where the previous one ended). The reference fills its buckets with a counting
@@ -14,8 +16,9 @@
 // position so that a wave's lanes walk nearby buckets. Each lane walks its eight
 // buckets in the reference's order, visiting a bucket twice when two neighbour
 // cells hash to it, and tests every slot with the reference's FP32 distance. The
-// contribution and MIS weight of a vertex found within the radius are computed in
-// registers with the shared path-weight walks.
+// camera BSDF (a medium's phase function for PP3D, the surface's diffuse and Phong
+// lobes for SURF), contribution and MIS weight of a vertex found within the radius
+// are computed in registers with the shared path-weight walks.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -52,17 +55,19 @@ __device__ __forceinline__ uint32_t bucket_of_pos(const GridDev& g, float x, flo
     return bucket_of((int32_t)cx, (int32_t)cy, (int32_t)cz, g.cells);
 }
 
-__device__ __forceinline__ bool in_medium(const uint32_t* flags, int64_t stride, int64_t i) {
-    return (flags[i * stride] & UPBP_LV_IN_MEDIUM) != 0;
+// UPBPLightVertex::MatchesType (UPBPLightVertex.hxx:74-77): PP3D grids keep in-medium
+// vertices, SURF grids the others.
+__device__ __forceinline__ bool in_grid(const uint32_t* flags, int64_t stride, int64_t i, bool want_medium) {
+    return ((flags[i * stride] & UPBP_LV_IN_MEDIUM) != 0) == want_medium;
 }
 
-// Per-block bounds of the in-medium vertices: part[b] = (lo.xyz, hi.xyz, count, non-finite).
+// Per-block bounds of the grid's vertices: part[b] = (lo.xyz, hi.xyz, count, non-finite).
 __global__ void k_grid_bounds(const float* __restrict__ pos, const uint32_t* __restrict__ flags, int64_t stride,
-                              int64_t n, float* __restrict__ part) {
+                              int64_t n, bool want_medium, float* __restrict__ part) {
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     uint32_t cnt = 0, bad = 0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        if (!in_medium(flags, stride, i)) continue;
+        if (!in_grid(flags, stride, i, want_medium)) continue;
         const float x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
         if (!__builtin_isfinite(x) || !__builtin_isfinite(y) || !__builtin_isfinite(z)) {
             bad = 1;
@@ -115,10 +120,12 @@ __global__ void k_grid_bounds(const float* __restrict__ pos, const uint32_t* __r
 
 // Bucket key per vertex (the sentinel `cells` for vertices outside the grid) and its index.
 __global__ void k_grid_keys(const float* __restrict__ pos, const uint32_t* __restrict__ flags, int64_t stride,
-                            int64_t n, GridDev g, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+                            int64_t n, bool want_medium, GridDev g, uint32_t* __restrict__ keys,
+                            uint32_t* __restrict__ vals) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    keys[i] = in_medium(flags, stride, i) ? bucket_of_pos(g, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]) : g.cells;
+    keys[i] = in_grid(flags, stride, i, want_medium) ? bucket_of_pos(g, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2])
+                                                     : g.cells;
     vals[i] = (uint32_t)i;
 }
 
@@ -144,8 +151,9 @@ __device__ __forceinline__ uint32_t spread10(uint32_t v) {
     return v;
 }
 
-__global__ void k_query_morton(const upbp_pp3d_query* __restrict__ q, int64_t n, GridDev g,
-                               uint32_t* __restrict__ keys, uint32_t* __restrict__ idx) {
+template <class Query>
+__global__ void k_query_morton(const Query* __restrict__ q, int64_t n, GridDev g, uint32_t* __restrict__ keys,
+                               uint32_t* __restrict__ idx) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint32_t c[3];
@@ -159,12 +167,11 @@ __global__ void k_query_morton(const upbp_pp3d_query* __restrict__ q, int64_t n,
     idx[i] = (uint32_t)i;
 }
 
-struct Pp3dArgs {
+struct VmArgs {
     const float4* slots;
     const uint32_t* ends;
     GridDev g;
     int64_t n_sel;
-    const upbp_pp3d_query* queries;
     const uint32_t* perm;
     int64_t count;
     const MediumDev* media;
@@ -177,41 +184,38 @@ struct Pp3dArgs {
     int32_t count_stats;  // the launch that adds to the call's counters (one per call)
 };
 
-// RangeQuery::Process (UPBP.hxx:84-129) for light vertex v; adds to acc.
-__device__ void pp3d_process(const Pp3dArgs& A, const upbp_pp3d_query& q, const MediumDev& m, uint32_t v,
-                             float acc[3], uint32_t& err) {
+constexpr float kEpsCosine = 1e-6f;  // EPS_COSINE, Defs.hxx:65
+constexpr float kEpsPhong = 1e-3f;   // EPS_PHONG, Defs.hxx:68
+
+__device__ __forceinline__ bool finite3v(const float* v) {
+    return __builtin_isfinite(v[0]) && __builtin_isfinite(v[1]) && __builtin_isfinite(v[2]);
+}
+
+__device__ __forceinline__ bool camera_range_ok(const VmArgs& A, uint32_t len, int64_t off) {
+    return (A.flags & UPBP_VM_NO_MIS) || (len >= 1 && off >= 0 && off + (int64_t)len < A.mis.camera_count);
+}
+
+// The two path-weight walks of RangeQuery::Process (UPBP.hxx:120-127) and the weighted
+// contribution (:128-129). f = camera BSDF value, dir_pdf / rev_pdf already multiplied by
+// the camera / light continuation probabilities.
+__device__ __forceinline__ void vm_accumulate(const VmArgs& A, const upbp_light_vertex* __restrict__ lv,
+                                              uint32_t cam_len, int64_t cam_off, float factor, float sin_t,
+                                              float dir_pdf, float rev_pdf, uint32_t evaluated, float f0, float f1,
+                                              float f2, float acc[3], uint32_t& err) {
     const MisGlobals& G = A.mis;
     const upbp_mis_params& P = G.p;
-    if ((int64_t)v >= G.lv_count) {
-        err |= kBadLightIndex;
-        return;
-    }
-    const upbp_light_vertex* __restrict__ lv = G.lv + v;
-    if (!(lv->flags & UPBP_LV_CONNECTABLE)) return;
-    const uint32_t path_len = lv->path_length, cam_len = q.camera_path_length;
-    if ((path_len + cam_len > P.max_path_length) || (path_len + cam_len < P.min_path_length)) return;
-    // camera BSDF of a medium vertex (Bsdf.hxx:216-230): sigma_s * phase(fix = camera, gen = light)
-    if (!m.has_scattering) return;
-    const float gx = lv->dir_fix[0], gy = lv->dir_fix[1], gz = lv->dir_fix[2];
-    const float cos_t = -dot3(gx, gy, gz, q.dir_fix[0], q.dir_fix[1], q.dir_fix[2]);
-    const float sin_t = __builtin_sqrtf(fmaxf(0.f, 1.f - cos_t * cos_t));
-    const float pdf = phase_pdf(m.g, m.isotropic != 0, cos_t);
-    const float f0 = m.ss[0] * pdf, f1 = m.ss[1] * pdf, f2 = m.ss[2] * pdf;
-    if (f0 <= 0.f && f1 <= 0.f && f2 <= 0.f) return;
     float w = 1.f;
-    if (!(A.flags & UPBP_PP3D_NO_MIS)) {
-        const float dir_pdf = pdf * m.cont;               // camera continuation (UPBP.hxx:110)
-        const float rev_pdf = pdf * lv->continuation_prob;  // light continuation (UPBP.hxx:116)
-        const float inv = 1.f / lv->mis.pp3d_factor;
+    if (!(A.flags & UPBP_VM_NO_MIS)) {
+        const float inv = 1.f / factor;
         CamView cv;
-        cv.base = G.camera + q.camera_offset;
+        cv.base = G.camera + cam_off;
         cv.len = (int)cam_len;
         cv.last = mis_load(cv.base + cam_len);
         cv.prev_ray_rev_inv = cv.base[cam_len - 1].ray_sample_rev_pdf_inv;
         const float w_cam = camera_path_weight(cv, inv, sin_t, lv->mis.ray_sample_pdf_inv, lv->mis.ray_sample_pdfs_ratio,
                                                rev_pdf, P.query_beam_type, P.photon_beam_type, P.techniques);
         LightView lview{&G, nullptr, -1, 0.f};
-        const float w_light = light_path_weight(lview, lv->path_idx, (int)path_len, inv, dir_pdf, UPBP_PP3D,
+        const float w_light = light_path_weight(lview, lv->path_idx, (int)lv->path_length, inv, dir_pdf, evaluated,
                                                 P.query_beam_type, P.photon_beam_type, P.techniques, err);
         w = 1.f / (w_light + w_cam);
     }
@@ -220,8 +224,133 @@ __device__ void pp3d_process(const Pp3dArgs& A, const upbp_pp3d_query& q, const 
     acc[2] += w * (f2 * lv->throughput[2]);
 }
 
-template <bool SIG>
-__global__ __launch_bounds__(kBlock) void k_pp3d_eval(Pp3dArgs A) {
+// Light vertex v if RangeQuery::Process merges with it (UPBP.hxx:89-98), else null.
+__device__ __forceinline__ const upbp_light_vertex* vm_light_vertex(const VmArgs& A, uint32_t v, uint32_t cam_len,
+                                                                      uint32_t& err) {
+    const MisGlobals& G = A.mis;
+    if ((int64_t)v >= G.lv_count) {
+        err |= kBadLightIndex;
+        return nullptr;
+    }
+    const upbp_light_vertex* __restrict__ lv = G.lv + v;
+    if (!(lv->flags & UPBP_LV_CONNECTABLE)) return nullptr;
+    const uint32_t tot = lv->path_length + cam_len;
+    if (tot > G.p.max_path_length || tot < G.p.min_path_length) return nullptr;
+    return lv;
+}
+
+// ---- PP3D: camera vertex in a medium -------------------------------------
+struct Pp3dCam {
+    MediumDev m;
+};
+
+__device__ __forceinline__ bool query_ok(const VmArgs& A, const upbp_pp3d_query& q, uint32_t& err) {
+    bool ok = true;
+    if (!finite3v(q.position) || !finite3v(q.dir_fix)) {
+        err |= kErrNonFinite;
+        ok = false;
+    }
+    if (q.medium < 0 || q.medium >= A.n_media) {
+        err |= kErrBadMedium;
+        ok = false;
+    }
+    if (!camera_range_ok(A, q.camera_path_length, q.camera_offset)) {
+        err |= kBadLightIndex;
+        ok = false;
+    }
+    return ok;
+}
+
+__device__ __forceinline__ Pp3dCam camera_of(const VmArgs& A, const upbp_pp3d_query& q) {
+    return Pp3dCam{A.media[q.medium]};
+}
+
+// RangeQuery::Process with a medium camera BSDF (Bsdf.hxx:216-230): sigma_s * phase
+// (fix = camera WorldDirFix, gen = light WorldDirFix), zero without scattering.
+__device__ void vm_process(const VmArgs& A, const upbp_pp3d_query& q, const Pp3dCam& c, uint32_t v, float acc[3],
+                           uint32_t& err) {
+    const upbp_light_vertex* __restrict__ lv = vm_light_vertex(A, v, q.camera_path_length, err);
+    if (!lv) return;
+    const MediumDev& m = c.m;
+    if (!m.has_scattering) return;
+    const float cos_t = -dot3(lv->dir_fix[0], lv->dir_fix[1], lv->dir_fix[2], q.dir_fix[0], q.dir_fix[1], q.dir_fix[2]);
+    const float sin_t = __builtin_sqrtf(fmaxf(0.f, 1.f - cos_t * cos_t));
+    const float pdf = phase_pdf(m.g, m.isotropic != 0, cos_t);
+    const float f0 = m.ss[0] * pdf, f1 = m.ss[1] * pdf, f2 = m.ss[2] * pdf;
+    if (f0 <= 0.f && f1 <= 0.f && f2 <= 0.f) return;
+    // camera continuation on the direct pdf (UPBP.hxx:110), light continuation on the reverse (:116)
+    vm_accumulate(A, lv, q.camera_path_length, q.camera_offset, lv->mis.pp3d_factor, sin_t, pdf * m.cont,
+                  pdf * lv->continuation_prob, UPBP_PP3D, f0, f1, f2, acc, err);
+}
+
+// ---- SURF: camera vertex on a surface ------------------------------------
+struct SurfCam {};
+
+__device__ __forceinline__ bool query_ok(const VmArgs& A, const upbp_surf_query& q, uint32_t& err) {
+    bool ok = true;
+    if (!finite3v(q.position) || !finite3v(q.frame_x) || !finite3v(q.frame_y) || !finite3v(q.frame_z) ||
+        !finite3v(q.local_dir_fix)) {
+        err |= kErrNonFinite;
+        ok = false;
+    }
+    if (!camera_range_ok(A, q.camera_path_length, q.camera_offset)) {
+        err |= kBadLightIndex;
+        ok = false;
+    }
+    return ok;
+}
+
+__device__ __forceinline__ SurfCam camera_of(const VmArgs&, const upbp_surf_query&) { return SurfCam{}; }
+
+// RangeQuery::Process with a surface camera BSDF: Bsdf::Evaluate on a surface
+// (Bsdf.hxx:176-215) = EvaluateDiffuse (:542-561) + EvaluatePhong (:563-600), with the
+// light's WorldDirFix as the generated direction; sin(theta) is 0 on surfaces.
+__device__ void vm_process(const VmArgs& A, const upbp_surf_query& q, const SurfCam&, uint32_t v, float acc[3],
+                           uint32_t& err) {
+    const upbp_light_vertex* __restrict__ lv = vm_light_vertex(A, v, q.camera_path_length, err);
+    if (!lv) return;
+    const float gx = lv->dir_fix[0], gy = lv->dir_fix[1], gz = lv->dir_fix[2];
+    // Frame::ToLocal (Frame.hxx:69-72)
+    const float lx = dot3(gx, gy, gz, q.frame_x[0], q.frame_x[1], q.frame_x[2]);
+    const float ly = dot3(gx, gy, gz, q.frame_y[0], q.frame_y[1], q.frame_y[2]);
+    const float lz = dot3(gx, gy, gz, q.frame_z[0], q.frame_z[1], q.frame_z[2]);
+    const float fx = q.local_dir_fix[0], fy = q.local_dir_fix[1], fz = q.local_dir_fix[2];
+    if (fabsf(lz) < kEpsCosine) return;  // too parallel with the tangent plane
+    if (lz * fz < 0.f) return;           // the other side of the surface
+    float f0 = 0.f, f1 = 0.f, f2 = 0.f, dir_pdf = 0.f, rev_pdf = 0.f;
+    const bool lobes = !(fz < kEpsCosine || lz < kEpsCosine);
+    if (q.diff_prob != 0.f && lobes) {
+        dir_pdf += q.diff_prob * fmaxf(0.f, lz * kInvPi);
+        rev_pdf += q.diff_prob * fmaxf(0.f, fz * kInvPi);
+        f0 += q.diffuse[0] * kInvPi;
+        f1 += q.diffuse[1] * kInvPi;
+        f2 += q.diffuse[2] * kInvPi;
+    }
+    if (q.phong_prob != 0.f && lobes) {
+        // ReflectLocal(mLocalDirFix) = (-x, -y, z) (Utils2.hxx:374-377)
+        const float dot_r = dot3(-fx, -fy, fz, lx, ly, lz);
+        if (dot_r > kEpsPhong) {
+            const float e = q.phong_exponent;
+            const float pw = powf(dot_r, e);
+            if (pw > 1e-20f) {  // Float::isPositive (Utils.hxx:109-111)
+                // PowerCosHemispherePdfW (Utils2.hxx:78-86), symmetric
+                const float pdf_w = q.phong_prob * (((e + 1.f) * powf(fmaxf(0.f, dot_r), e)) * (kInvPi * 0.5f));
+                dir_pdf += pdf_w;
+                rev_pdf += pdf_w;
+                f0 += ((q.phong[0] * (e + 2.f)) * 0.5f * kInvPi) * pw;
+                f1 += ((q.phong[1] * (e + 2.f)) * 0.5f * kInvPi) * pw;
+                f2 += ((q.phong[2] * (e + 2.f)) * 0.5f * kInvPi) * pw;
+            }
+        }
+    }
+    if (f0 <= 0.f && f1 <= 0.f && f2 <= 0.f) return;
+    vm_accumulate(A, lv, q.camera_path_length, q.camera_offset, lv->mis.surf_factor, 0.f,
+                  dir_pdf * q.continuation_prob, rev_pdf * lv->continuation_prob, UPBP_SURF, f0, f1, f2, acc, err);
+}
+
+// HashGrid::Process (HashGrid.hxx:130-186) over one lane's camera vertex.
+template <class Query, bool SIG>
+__global__ __launch_bounds__(kBlock) void k_vm_eval(VmArgs A, const Query* __restrict__ queries) {
     const int64_t slot = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const bool valid = slot < A.count;
     uint32_t err = 0, processed = 0, sig_h = 0;
@@ -230,25 +359,8 @@ __global__ __launch_bounds__(kBlock) void k_pp3d_eval(Pp3dArgs A) {
     int64_t qi = 0;
     if (valid) {
         qi = A.perm ? (int64_t)A.perm[slot] : slot;
-        const upbp_pp3d_query q = A.queries[qi];
-        bool ok = true;
-        if (!__builtin_isfinite(q.position[0]) || !__builtin_isfinite(q.position[1]) ||
-            !__builtin_isfinite(q.position[2]) || !__builtin_isfinite(q.dir_fix[0]) ||
-            !__builtin_isfinite(q.dir_fix[1]) || !__builtin_isfinite(q.dir_fix[2])) {
-            err |= kErrNonFinite;
-            ok = false;
-        }
-        if (q.medium < 0 || q.medium >= A.n_media) {
-            err |= kErrBadMedium;
-            ok = false;
-        }
-        if (!(A.flags & UPBP_PP3D_NO_MIS) &&
-            (q.camera_path_length < 1 || q.camera_offset < 0 ||
-             q.camera_offset + (int64_t)q.camera_path_length >= A.mis.camera_count)) {
-            err |= kBadLightIndex;
-            ok = false;
-        }
-        // HashGrid::Process, HashGrid.hxx:130-186
+        const Query q = queries[qi];
+        const bool ok = query_ok(A, q, err);
         float dmin[3], dmax[3];
         for (int a = 0; a < 3; ++a) {
             dmin[a] = q.position[a] - A.g.lo[a];
@@ -256,7 +368,7 @@ __global__ __launch_bounds__(kBlock) void k_pp3d_eval(Pp3dArgs A) {
         }
         if (ok && A.n_sel > 0 && !(dmin[0] < 0.f) && !(dmax[0] < 0.f) && !(dmin[1] < 0.f) && !(dmax[1] < 0.f) &&
             !(dmin[2] < 0.f) && !(dmax[2] < 0.f)) {
-            const MediumDev m = A.media[q.medium];
+            const auto cam = camera_of(A, q);
             int32_t p[3], po[3];
             for (int a = 0; a < 3; ++a) {
                 const float cp = A.g.inv_cell * dmin[a];
@@ -277,11 +389,10 @@ __global__ __launch_bounds__(kBlock) void k_pp3d_eval(Pp3dArgs A) {
                     if ((dx * dx + dy * dy) + dz * dz <= A.g.r2) {
                         const uint32_t v = __float_as_uint(s.w);
                         processed++;
-                        if (SIG) {
+                        if (SIG)
                             sig_h += hash32(v);
-                        } else {
-                            pp3d_process(A, q, m, v, acc, err);
-                        }
+                        else
+                            vm_process(A, q, cam, v, acc, err);
                     }
                 }
             }
@@ -327,12 +438,14 @@ void inclusive_max_u32(Ctx& ctx, uint32_t* data, uint32_t* out, int64_t n) {
 
 }  // namespace
 
-void pp3d_build(Ctx& ctx, const float* pos, const uint32_t* flags, int64_t stride, int64_t n, float radius,
-                int64_t cells, Pp3d& out) {
+void hash_grid_build(Ctx& ctx, const float* pos, const uint32_t* flags, int64_t stride, int64_t n, float radius,
+                     int64_t cells, uint32_t type, HashGrid& out) {
     out.ctx = &ctx;
+    out.type = type;
     out.n_src = n;
     out.cells = cells;
     out.radius = radius;
+    const bool want_medium = type == UPBP_PP3D;
     // HashGrid::Build, HashGrid.hxx:52-55 (FP32 on the host, the same IEEE operations)
     out.r2 = radius * radius;
     const float cell_size = radius * 2.f;
@@ -348,7 +461,7 @@ void pp3d_build(Ctx& ctx, const float* pos, const uint32_t* flags, int64_t strid
     DevBuf part;
     part.ensure(kBoundsBlocks * 8 * sizeof(float));
     hipLaunchKernelGGL(k_grid_bounds, dim3(kBoundsBlocks), dim3(kBlock), 0, ctx.stream, pos, flags, stride, n,
-                       part.as<float>());
+                       want_medium, part.as<float>());
     UPBP_HIP(hipGetLastError());
     std::vector<float> hp(kBoundsBlocks * 8);
     UPBP_HIP(hipMemcpyAsync(hp.data(), part.p, hp.size() * sizeof(float), hipMemcpyDeviceToHost, ctx.stream));
@@ -367,14 +480,14 @@ void pp3d_build(Ctx& ctx, const float* pos, const uint32_t* flags, int64_t strid
             hi[a] = std::max(hi[a], hp[8 * b + 3 + a]);
         }
     }
-    UPBP_REQUIRE(!bad, UPBP_ENONFINITE, "non-finite position of an in-medium light vertex");
+    UPBP_REQUIRE(!bad, UPBP_ENONFINITE, "non-finite position of a light vertex of the grid");
     if (count == 0) return;
     for (int a = 0; a < 3; ++a) {
         out.lo[a] = lo[a];
         out.hi[a] = hi[a];
         // cell coordinates must fit a 32-bit int (the reference converts them with int())
         UPBP_REQUIRE((double)(hi[a] - lo[a]) * (double)out.inv_cell < 1073741824.0, UPBP_EINVAL,
-                     "PP3D radius too small for the extent of the light vertices (cell coordinates overflow)");
+                     "merging radius too small for the extent of the light vertices (cell coordinates overflow)");
     }
     out.n = (int64_t)count;
     GridDev g;
@@ -390,8 +503,8 @@ void pp3d_build(Ctx& ctx, const float* pos, const uint32_t* flags, int64_t strid
     keys2.ensure(n * 4);
     vals.ensure(n * 4);
     vals2.ensure(n * 4);
-    hipLaunchKernelGGL(k_grid_keys, dim3(blocks_for(n)), dim3(kBlock), 0, ctx.stream, pos, flags, stride, n, g,
-                       keys.as<uint32_t>(), vals.as<uint32_t>());
+    hipLaunchKernelGGL(k_grid_keys, dim3(blocks_for(n)), dim3(kBlock), 0, ctx.stream, pos, flags, stride, n,
+                       want_medium, g, keys.as<uint32_t>(), vals.as<uint32_t>());
     UPBP_HIP(hipGetLastError());
     sort_pairs_u32(ctx, keys.as<uint32_t>(), keys2.as<uint32_t>(), vals.as<uint32_t>(), vals2.as<uint32_t>(), n,
                    bits_for((uint64_t)cells));
@@ -406,10 +519,13 @@ void pp3d_build(Ctx& ctx, const float* pos, const uint32_t* flags, int64_t strid
     UPBP_HIP(hipStreamSynchronize(ctx.stream));
 }
 
-void pp3d_eval(Ctx& ctx, const Pp3d& grid, const upbp_pp3d_query* q, int64_t count, const MisDev& mis,
-               uint32_t flags, float* out, uint32_t* sig) {
+namespace {
+
+template <class Query>
+void vm_eval(Ctx& ctx, const HashGrid& grid, const Query* q, int64_t count, const MisDev& mis, uint32_t flags,
+             float* out, uint32_t* sig) {
     if (count == 0) return;
-    Pp3dArgs A = {};
+    VmArgs A = {};
     A.slots = grid.slots.as<float4>();
     A.ends = grid.ends.as<uint32_t>();
     for (int a = 0; a < 3; ++a) {
@@ -420,7 +536,6 @@ void pp3d_eval(Ctx& ctx, const Pp3d& grid, const upbp_pp3d_query* q, int64_t cou
     A.g.inv_cell = grid.inv_cell;
     A.g.cells = (uint32_t)grid.cells;
     A.n_sel = grid.n;
-    A.queries = q;
     A.count = count;
     A.media = ctx.media.as<MediumDev>();
     A.n_media = ctx.n_media;
@@ -440,7 +555,7 @@ void pp3d_eval(Ctx& ctx, const Pp3d& grid, const upbp_pp3d_query* q, int64_t cou
         ctx.qkeys2.ensure(count * 4);
         ctx.qperm.ensure(count * 4);
         ctx.qperm2.ensure(count * 4);
-        hipLaunchKernelGGL(k_query_morton, dim3(blocks_for(count)), dim3(kBlock), 0, ctx.stream, q, count, A.g,
+        hipLaunchKernelGGL(k_query_morton<Query>, dim3(blocks_for(count)), dim3(kBlock), 0, ctx.stream, q, count, A.g,
                            ctx.qkeys.as<uint32_t>(), ctx.qperm.as<uint32_t>());
         UPBP_HIP(hipGetLastError());
         sort_pairs_u32(ctx, ctx.qkeys.as<uint32_t>(), ctx.qkeys2.as<uint32_t>(), ctx.qperm.as<uint32_t>(),
@@ -450,15 +565,27 @@ void pp3d_eval(Ctx& ctx, const Pp3d& grid, const upbp_pp3d_query* q, int64_t cou
     if (out) {
         A.out = out;
         A.count_stats = 1;
-        hipLaunchKernelGGL(k_pp3d_eval<false>, dim3(blocks_for(count)), dim3(kBlock), 0, ctx.stream, A);
+        hipLaunchKernelGGL((k_vm_eval<Query, false>), dim3(blocks_for(count)), dim3(kBlock), 0, ctx.stream, A, q);
         UPBP_HIP(hipGetLastError());
     }
     if (sig) {
         A.sig = sig;
         A.count_stats = out ? 0 : 1;
-        hipLaunchKernelGGL(k_pp3d_eval<true>, dim3(blocks_for(count)), dim3(kBlock), 0, ctx.stream, A);
+        hipLaunchKernelGGL((k_vm_eval<Query, true>), dim3(blocks_for(count)), dim3(kBlock), 0, ctx.stream, A, q);
         UPBP_HIP(hipGetLastError());
     }
+}
+
+}  // namespace
+
+void pp3d_eval(Ctx& ctx, const HashGrid& grid, const upbp_pp3d_query* q, int64_t count, const MisDev& mis,
+               uint32_t flags, float* out, uint32_t* sig) {
+    vm_eval(ctx, grid, q, count, mis, flags, out, sig);
+}
+
+void surf_eval(Ctx& ctx, const HashGrid& grid, const upbp_surf_query* q, int64_t count, const MisDev& mis,
+               uint32_t flags, float* out, uint32_t* sig) {
+    vm_eval(ctx, grid, q, count, mis, flags, out, sig);
 }
 
 }  // namespace upbp

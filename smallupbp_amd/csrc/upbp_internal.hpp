@@ -229,9 +229,10 @@ void launch_clear_u64(Ctx& ctx, uint64_t* p, int64_t n);
 void launch_clear_f32(Ctx& ctx, float* p, int64_t n);
 void launch_combine_media_grads(Ctx& ctx, const float* gm, int n_media, float* gs, float* ga);
 
-// ---- pp3d.hip: PP3D hashed grid ----------------------------------------------
-struct Pp3d {
+// ---- hashgrid.hip: vertex-merging hashed grid (PP3D, SURF) --------------------
+struct HashGrid {
     Ctx* ctx = nullptr;
+    uint32_t type = 0;  // UPBP_PP3D (in-medium vertices) or UPBP_SURF (surface vertices)
     int64_t n_src = 0;  // light vertices given to the build
     int64_t n = 0;      // vertices in the grid
     int64_t cells = 0;  // hash buckets
@@ -241,9 +242,11 @@ struct Pp3d {
     DevBuf ends;   // uint32 [cells]: one past each bucket's last slot
 };
 // flags[i * stride] holds the UPBP_LV_* word of light vertex i (device pointers).
-void pp3d_build(Ctx& ctx, const float* pos, const uint32_t* flags, int64_t stride, int64_t n, float radius,
-                int64_t cells, Pp3d& out);
-void pp3d_eval(Ctx& ctx, const Pp3d& grid, const upbp_pp3d_query* q, int64_t count, const MisDev& mis,
+void hash_grid_build(Ctx& ctx, const float* pos, const uint32_t* flags, int64_t stride, int64_t n, float radius,
+                     int64_t cells, uint32_t type, HashGrid& out);
+void pp3d_eval(Ctx& ctx, const HashGrid& grid, const upbp_pp3d_query* q, int64_t count, const MisDev& mis,
+               uint32_t flags, float* out, uint32_t* sig);
+void surf_eval(Ctx& ctx, const HashGrid& grid, const upbp_surf_query* q, int64_t count, const MisDev& mis,
                uint32_t flags, float* out, uint32_t* sig);
 
 // ---- batched persistent kernels --------------------------------------------

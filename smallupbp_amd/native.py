@@ -31,7 +31,8 @@ EXPORTED = [
     "upbp_beams_piece_count", "upbp_beams_selection_pdf", "upbp_eval", "upbp_hit_signatures",
     "upbp_adjoint", "upbp_batch_plan_create", "upbp_batch_plan_destroy", "upbp_batch_run",
     "upbp_set_tuning", "upbp_get_tuning", "upbp_autotune", "upbp_eval_upbp", "upbp_bre_radii",
-    "upbp_beams_radii", "upbp_pp3d_build", "upbp_pp3d_destroy", "upbp_pp3d_count", "upbp_pp3d_eval",
+    "upbp_beams_radii", "upbp_hash_grid_build", "upbp_hash_grid_destroy", "upbp_hash_grid_count",
+    "upbp_pp3d_eval", "upbp_surf_eval",
 ]
 
 
@@ -120,15 +121,16 @@ def load():
     L.upbp_autotune.restype = st
     L.upbp_autotune.argtypes = [vp, vp, vp, C.POINTER(_abi.QueryBatch), C.POINTER(_abi.Tuning),
                                 C.POINTER(C.c_float)]
-    L.upbp_pp3d_build.restype = st
-    L.upbp_pp3d_build.argtypes = [vp, vp, vp, i64, C.c_float, i64, C.c_int, C.POINTER(vp)]
-    L.upbp_pp3d_destroy.restype = st
-    L.upbp_pp3d_destroy.argtypes = [vp]
-    L.upbp_pp3d_count.restype = i64
-    L.upbp_pp3d_count.argtypes = [vp]
-    L.upbp_pp3d_eval.restype = st
-    L.upbp_pp3d_eval.argtypes = [vp, vp, vp, i64, C.POINTER(_abi.MisInputs), C.c_uint32, vp, vp, C.c_int,
-                                 C.POINTER(_abi.EvalStats)]
+    L.upbp_hash_grid_build.restype = st
+    L.upbp_hash_grid_build.argtypes = [vp, vp, vp, i64, C.c_float, i64, C.c_uint32, C.c_int, C.POINTER(vp)]
+    L.upbp_hash_grid_destroy.restype = st
+    L.upbp_hash_grid_destroy.argtypes = [vp]
+    L.upbp_hash_grid_count.restype = i64
+    L.upbp_hash_grid_count.argtypes = [vp]
+    for name in ("upbp_pp3d_eval", "upbp_surf_eval"):
+        getattr(L, name).restype = st
+        getattr(L, name).argtypes = [vp, vp, vp, i64, C.POINTER(_abi.MisInputs), C.c_uint32, vp, vp, C.c_int,
+                                     C.POINTER(_abi.EvalStats)]
     _lib = L
     return L
 
@@ -339,41 +341,46 @@ class PhotonBeamsEvaluator:
             pass
 
 
-class HashGridPP3D:
-    """PP3D vertex merging: HashGrid over the in-medium light vertices
-    (SmallUPBP/src/Misc/HashGrid.hxx:38-230) queried with RangeQuery::Process
-    (SmallUPBP/src/Renderers/UPBP.hxx:63-133)."""
+class HashGrid:
+    """Vertex merging through the reference's hashed grid (SmallUPBP/src/Misc/HashGrid.hxx:38-230)
+    with RangeQuery::Process (SmallUPBP/src/Renderers/UPBP.hxx:63-133): PP3D for camera vertices in
+    media, SURF for camera vertices on surfaces."""
+
+    QUERY_BYTES = {_abi.PP3D: 48, _abi.SURF: 128}
 
     def __init__(self, ctx: Context):
         self.ctx = ctx
         self.h = None
+        self.technique = None
 
-    def build(self, positions, light_vertices, radius: float, cell_count: int):
-        """HashGrid::Reserve(cell_count) + Build(light vertices, radius, PP3D) (UPBP.hxx:701-703).
+    def build(self, positions, light_vertices, radius: float, cell_count: int, technique: int = _abi.PP3D):
+        """HashGrid::Reserve(cell_count) + Build(light vertices, radius, technique) (UPBP.hxx:689-703).
         positions [n,3] float32 and light_vertices (LIGHT_VERTEX_DTYPE) as host arrays, or torch
         CUDA tensors (the light vertices as a uint8 tensor of n * 112 bytes)."""
         assert self.h is None, "destroy() before building again"
         pos = _arr(positions, np.float32)
         lv = light_vertices if _is_torch(light_vertices) else np.ascontiguousarray(light_vertices)
         dev = _is_torch(pos) and pos.is_cuda
-        n = int(pos.shape[0])
         h = C.c_void_p()
-        self.ctx.check(self.ctx.lib.upbp_pp3d_build(self.ctx.h, _abi.ptr(pos), _abi.ptr(lv), n, float(radius),
-                                                    int(cell_count), _abi.MEM_DEVICE if dev else _abi.MEM_HOST,
-                                                    C.byref(h)))
+        self.ctx.check(self.ctx.lib.upbp_hash_grid_build(self.ctx.h, _abi.ptr(pos), _abi.ptr(lv), int(pos.shape[0]),
+                                                         float(radius), int(cell_count), int(technique),
+                                                         _abi.MEM_DEVICE if dev else _abi.MEM_HOST, C.byref(h)))
         self.h = h
+        self.technique = int(technique)
         return self
 
     @property
     def count(self) -> int:
-        return int(self.ctx.lib.upbp_pp3d_count(self.h))
+        return int(self.ctx.lib.upbp_hash_grid_count(self.h))
 
     def eval(self, queries, mis, flags: int = 0, out=None, signatures: bool = False, stats: Optional[dict] = None):
         """RangeQuery::GetContrib per camera vertex ([Q,3]); with signatures=True, (Process calls,
-        hash sum) per camera vertex ([Q,2] uint32) instead. `queries` is a PP3D_QUERY_DTYPE array (host)
-        or a uint8 CUDA tensor of Q * 48 bytes; `mis` a scenes.MisInputs."""
+        hash sum) per camera vertex ([Q,2] uint32) instead. `queries` is a PP3D_QUERY_DTYPE /
+        SURF_QUERY_DTYPE array (host) or a uint8 CUDA tensor of those records; `mis` a
+        scenes.MisInputs or an _abi.MisInputs (e.g. from device_mis)."""
         dev = _is_torch(queries)
-        q_count = (int(queries.numel()) // 48) if dev else len(queries)
+        qb = self.QUERY_BYTES[self.technique]
+        q_count = (int(queries.numel()) // qb) if dev else len(queries)
         q = queries if dev else np.ascontiguousarray(queries)
         ms = mis if isinstance(mis, _abi.MisInputs) else _abi.mis_inputs(mis)
         if dev:
@@ -386,18 +393,17 @@ class HashGridPP3D:
             res = out if out is not None else np.zeros((q_count, 2 if signatures else 3),
                                                        np.uint32 if signatures else np.float32)
         st = _abi.EvalStats()
-        self.ctx.check(self.ctx.lib.upbp_pp3d_eval(self.ctx.h, self.h, _abi.ptr(q), q_count, C.byref(ms), int(flags),
-                                                   None if signatures else _abi.ptr(res),
-                                                   _abi.ptr(res) if signatures else None,
-                                                   _abi.MEM_DEVICE if dev else _abi.MEM_HOST,
-                                                   C.byref(st) if stats is not None else None))
+        fn = self.ctx.lib.upbp_pp3d_eval if self.technique == _abi.PP3D else self.ctx.lib.upbp_surf_eval
+        self.ctx.check(fn(self.ctx.h, self.h, _abi.ptr(q), q_count, C.byref(ms), int(flags),
+                          None if signatures else _abi.ptr(res), _abi.ptr(res) if signatures else None,
+                          _abi.MEM_DEVICE if dev else _abi.MEM_HOST, C.byref(st) if stats is not None else None))
         if stats is not None:
             stats.update(st.as_dict())
         return res
 
     def destroy(self):
         if self.h is not None:
-            self.ctx.lib.upbp_pp3d_destroy(self.h)
+            self.ctx.lib.upbp_hash_grid_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -474,10 +474,16 @@ assert QUERY_MIS_DTYPE.itemsize == 24
 PP3D_QUERY_DTYPE = np.dtype([("position", "<f4", (3,)), ("medium", "<i4"), ("dir_fix", "<f4", (3,)),
                              ("camera_path_length", "<u4"), ("camera_offset", "<i8"), ("pad", "<i8")])
 assert PP3D_QUERY_DTYPE.itemsize == 48
+SURF_QUERY_DTYPE = np.dtype([("position", "<f4", (3,)), ("camera_path_length", "<u4"), ("frame_x", "<f4", (3,)),
+                             ("continuation_prob", "<f4"), ("frame_y", "<f4", (3,)), ("diff_prob", "<f4"),
+                             ("frame_z", "<f4", (3,)), ("phong_prob", "<f4"), ("local_dir_fix", "<f4", (3,)),
+                             ("phong_exponent", "<f4"), ("diffuse", "<f4", (3,)), ("pad0", "<f4"),
+                             ("phong", "<f4", (3,)), ("pad1", "<f4"), ("camera_offset", "<i8"), ("pad2", "<i8")])
+assert SURF_QUERY_DTYPE.itemsize == 128
 
 MIS_DELTA, MIS_ON_LIGHT_SOURCE, MIS_SPECULAR, MIS_IN_MEDIUM_WITH_BEAMS = 1, 2, 4, 8
 LV_CONNECTABLE, LV_IN_MEDIUM = 1, 2
-PP3D_NO_MIS = 1
+VM_NO_MIS = 1
 BPT, SURF, PP3D, PB2D, BB1D = 4, 8, 16, 32, 64
 NO_SINE_IN_WEIGHTS, PREVIOUS, COMPATIBLE, BB1D_PREVIOUS = 128, 256, 512, 1024
 
@@ -568,12 +574,14 @@ def upbp_scene(width=32, height=18, n_paths=2_000, n_beams=500, photon_radius=0.
 
 
 @dataclasses.dataclass
-class Pp3dScene:
+class MergeScene:
+    """Inputs of PP3D (camera vertices in media) or SURF (camera vertices on surfaces) merging."""
+    technique: int  # PP3D or SURF
     media: np.ndarray
     positions: np.ndarray  # [n][3] float32, light-vertex positions (mHitpoint)
     radius: float
     cell_count: int  # hash buckets (the reference reserves one per light path)
-    queries: np.ndarray  # PP3D_QUERY_DTYPE
+    queries: np.ndarray  # PP3D_QUERY_DTYPE or SURF_QUERY_DTYPE
     mis: MisInputs  # light vertices (with flags), camera MisData, path ends, params; queries unused
 
 
@@ -585,17 +593,9 @@ def medium_at(p: np.ndarray, sphere=SPHERE, box_medium=0, outside_medium=1) -> n
     return np.where(d2 <= r * r, sid, np.where(inside_box, box_medium, outside_medium)).astype(np.int32)
 
 
-def pp3d_scene(n_paths=2_000, n_queries=4_000, radius=0.05, cell_count=None, seed=61, max_light_length=4,
-               max_camera_length=3, techniques=PB2D | BB1D | BPT | PP3D | SURF, query_beam_type=LONG_BEAM,
-               photon_beam_type=SHORT_BEAM, mean_cosine=0.0, connectable_p=0.9, query_margin=0.05) -> Pp3dScene:
-    """Light sub-paths whose non-source vertices lie in the box + sphere media (UPBP_LV_IN_MEDIUM), most of
-    them connectable; camera vertices uniform in a box slightly larger than the unit box (so some lie
-    outside the grid's bounding box), each with a camera sub-path of random MIS data."""
-    rng = rng_for(seed)
-    rows = _medium_rows_box_sphere()
-    if mean_cosine:
-        rows = [(sa, ss, mean_cosine if i != 1 else 0.0, cp) for i, (sa, ss, _, cp) in enumerate(rows)]
-    media = make_media(rows)
+def _light_paths(rng, n_paths, max_light_length):
+    """Light sub-paths of 1..max_light_length vertices after the source, with random MIS data;
+    light-source vertices carry no merging factors."""
     lens = rng.integers(1, max_light_length + 1, n_paths)
     counts = lens + 1
     ends = np.cumsum(counts).astype(np.int32)
@@ -614,20 +614,120 @@ def pp3d_scene(n_paths=2_000, n_queries=4_000, radius=0.05, cell_count=None, see
     lv["mis"]["flags"][src] = MIS_ON_LIGHT_SOURCE
     for f in ("surf_factor", "pp3d_factor", "pb2d_factor", "bb1d_factor"):
         lv["mis"][f][src] = 0.0
+    return lv, ends, src
+
+
+def _camera_paths(rng, n_queries, max_camera_length):
+    clen = rng.integers(1, max_camera_length + 1, n_queries)
+    offsets = np.concatenate([[0], np.cumsum(clen + 1)[:-1]])
+    return clen, offsets, random_mis(rng, int((clen + 1).sum()))
+
+
+def _params(techniques, query_beam_type, photon_beam_type):
+    return dict(techniques=techniques, query_beam_type=query_beam_type, photon_beam_type=photon_beam_type,
+                min_path_length=0, max_path_length=20, surf_factor=0.7, pp3d_factor=0.9, pb2d_factor=0.6,
+                bb1d_factor=1.3, bb1d_min_mfp=0.5, min_dist_to_med=0.0, has_bb1d_beams=1)
+
+
+def pp3d_scene(n_paths=2_000, n_queries=4_000, radius=0.05, cell_count=None, seed=61, max_light_length=4,
+               max_camera_length=3, techniques=PB2D | BB1D | BPT | PP3D | SURF, query_beam_type=LONG_BEAM,
+               photon_beam_type=SHORT_BEAM, mean_cosine=0.0, connectable_p=0.9, query_margin=0.05) -> MergeScene:
+    """Light sub-paths whose non-source vertices lie in the box + sphere media (UPBP_LV_IN_MEDIUM), most of
+    them connectable; camera vertices uniform in a box slightly larger than the unit box (so some lie
+    outside the grid's bounding box), each with a camera sub-path of random MIS data."""
+    rng = rng_for(seed)
+    rows = _medium_rows_box_sphere()
+    if mean_cosine:
+        rows = [(sa, ss, mean_cosine if i != 1 else 0.0, cp) for i, (sa, ss, _, cp) in enumerate(rows)]
+    media = make_media(rows)
+    lv, ends, src = _light_paths(rng, n_paths, max_light_length)
+    nv = len(lv)
     conn = rng.uniform(size=nv) < connectable_p
     lv["flags"] = np.where(~src, LV_IN_MEDIUM, 0) | np.where(conn & ~src, LV_CONNECTABLE, 0)
     pos = rng.uniform(0, 1, (nv, 3)).astype(np.float32)
     qp = rng.uniform(-query_margin, 1 + query_margin, (n_queries, 3)).astype(np.float32)
-    clen = rng.integers(1, max_camera_length + 1, n_queries)
+    clen, coff, cam = _camera_paths(rng, n_queries, max_camera_length)
     q = np.zeros(n_queries, PP3D_QUERY_DTYPE)
     q["position"] = qp
     q["medium"] = medium_at(qp)
     q["dir_fix"] = _isotropic_dirs(rng, n_queries)
     q["camera_path_length"] = clen
-    q["camera_offset"] = np.concatenate([[0], np.cumsum(clen + 1)[:-1]])
-    cam = random_mis(rng, int((clen + 1).sum()))
-    params = dict(techniques=techniques, query_beam_type=query_beam_type, photon_beam_type=photon_beam_type,
-                  min_path_length=0, max_path_length=20, surf_factor=0.7, pp3d_factor=0.9, pb2d_factor=0.6,
-                  bb1d_factor=1.3, bb1d_min_mfp=0.5, min_dist_to_med=0.0, has_bb1d_beams=1)
-    mi = MisInputs(None, cam, lv, ends, None, params)
-    return Pp3dScene(media, pos, float(radius), int(cell_count or n_paths), q, mi)
+    q["camera_offset"] = coff
+    mi = MisInputs(None, cam, lv, ends, None, _params(techniques, query_beam_type, photon_beam_type))
+    return MergeScene(PP3D, media, pos, float(radius), int(cell_count or n_paths), q, mi)
+
+
+# the three planes of the SURF scene: (axis of the normal, its sign, the plane's coordinate)
+_SURF_PLANES = ((1, 1.0, 0.0), (2, -1.0, 1.0), (0, 1.0, 0.0))
+
+
+def _on_planes(rng, n):
+    """Points uniform on three faces of the unit box (floor y = 0, back wall z = 1, left wall x = 0),
+    and the face normals."""
+    k = rng.integers(0, len(_SURF_PLANES), n)
+    p = rng.uniform(0, 1, (n, 3))
+    nrm = np.zeros((n, 3))
+    for i, (ax, sgn, c) in enumerate(_SURF_PLANES):
+        m = k == i
+        p[m, ax] = c
+        nrm[m, ax] = sgn
+    return p.astype(np.float32), nrm
+
+
+def _hemisphere(rng, nrm, flip_p):
+    d = _isotropic_dirs(rng, len(nrm)).astype(np.float64)
+    side = np.sign((d * nrm).sum(-1))
+    side[side == 0] = 1
+    flip = rng.uniform(size=len(nrm)) < flip_p
+    d *= np.where(flip, -side, side)[:, None]
+    return d
+
+
+def _frames(rng, nrm, tilt):
+    """Orthonormal shading frames around the face normals tilted by up to `tilt` radians."""
+    z = nrm + rng.normal(0, tilt, nrm.shape)
+    z /= np.linalg.norm(z, axis=1, keepdims=True)
+    a = np.where(np.abs(z[:, :1]) > 0.99, [[0.0, 1.0, 0.0]], [[1.0, 0.0, 0.0]])
+    y = np.cross(z, a)
+    y /= np.linalg.norm(y, axis=1, keepdims=True)
+    x = np.cross(y, z)
+    return x, y, z
+
+
+def surf_scene(n_paths=2_000, n_queries=4_000, radius=0.05, cell_count=None, seed=63, max_light_length=4,
+               max_camera_length=3, techniques=PB2D | BB1D | BPT | PP3D | SURF, query_beam_type=LONG_BEAM,
+               photon_beam_type=SHORT_BEAM, connectable_p=0.9, lobes="both", flip_p=0.08, tilt=0.05) -> MergeScene:
+    """Light sub-paths whose non-source vertices lie on three faces of the unit box (no
+    UPBP_LV_IN_MEDIUM), with incoming directions mostly on the face's side; camera vertices on the same
+    faces with slightly tilted shading frames and diffuse + Phong materials whose lobe and continuation
+    probabilities follow Bsdf::GetComponentProbabilities for opaque materials. lobes: "both",
+    "diffuse" or "phong"."""
+    rng = rng_for(seed)
+    media = make_media(_medium_rows_box_sphere())
+    lv, ends, src = _light_paths(rng, n_paths, max_light_length)
+    nv = len(lv)
+    conn = rng.uniform(size=nv) < connectable_p
+    lv["flags"] = np.where(conn & ~src, LV_CONNECTABLE, 0)
+    pos, lnrm = _on_planes(rng, nv)
+    lv["dir_fix"] = _hemisphere(rng, lnrm, flip_p).astype(np.float32)
+    qp, qn = _on_planes(rng, n_queries)
+    x, y, z = _frames(rng, qn, tilt)
+    fix_world = _hemisphere(rng, qn, flip_p)
+    clen, coff, cam = _camera_paths(rng, n_queries, max_camera_length)
+    q = np.zeros(n_queries, SURF_QUERY_DTYPE)
+    q["position"] = qp
+    q["frame_x"], q["frame_y"], q["frame_z"] = x, y, z
+    q["local_dir_fix"] = np.stack([(fix_world * x).sum(-1), (fix_world * y).sum(-1), (fix_world * z).sum(-1)], -1)
+    diff = rng.uniform(0.05, 0.8, (n_queries, 3)) * (lobes != "phong")
+    phong = rng.uniform(0.05, 0.5, (n_queries, 3)) * (lobes != "diffuse")
+    lum = np.array([0.212671, 0.715160, 0.072169])
+    ad, ap = diff @ lum, phong @ lum
+    q["diffuse"], q["phong"] = diff, phong
+    q["phong_exponent"] = rng.uniform(1.0, 80.0, n_queries)
+    q["diff_prob"] = ad / (ad + ap)
+    q["phong_prob"] = ap / (ad + ap)
+    q["continuation_prob"] = np.minimum(1.0, (diff + phong).max(axis=1))
+    q["camera_path_length"] = clen
+    q["camera_offset"] = coff
+    mi = MisInputs(None, cam, lv, ends, None, _params(techniques, query_beam_type, photon_beam_type))
+    return MergeScene(SURF, media, pos, float(radius), int(cell_count or n_paths), q, mi)

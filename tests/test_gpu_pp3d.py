@@ -16,20 +16,20 @@ pytestmark = pytest.mark.gpu
 
 def _gpu(sc, flags=0, device=False, signatures=False, stats=None):
     ctx = gpu_context(sc.media)
-    grid = native.HashGridPP3D(ctx)
+    grid = native.HashGrid(ctx)
     if device:
         import torch
 
         pos = torch.from_numpy(np.ascontiguousarray(sc.positions)).cuda()
         lvt = torch.from_numpy(np.ascontiguousarray(sc.mis.light_vertices).view(np.uint8)).cuda()
-        grid.build(pos, lvt, sc.radius, sc.cell_count)
+        grid.build(pos, lvt, sc.radius, sc.cell_count, sc.technique)
         q = torch.from_numpy(np.ascontiguousarray(sc.queries).view(np.uint8)).cuda()
         res = grid.eval(q, sc.mis, flags=flags, signatures=signatures, stats=stats)
         res = res.cpu().numpy()
         if signatures:
             res = res.view(np.uint32)
     else:
-        grid.build(sc.positions, sc.mis.light_vertices, sc.radius, sc.cell_count)
+        grid.build(sc.positions, sc.mis.light_vertices, sc.radius, sc.cell_count, sc.technique)
         res = grid.eval(sc.queries, sc.mis, flags=flags, signatures=signatures, stats=stats)
     n = grid.count
     grid.destroy()
@@ -40,7 +40,7 @@ CASES = [
     # name, scene kwargs, flags
     ("mis_isotropic", dict(n_paths=3000, n_queries=6000, radius=0.05, cell_count=2999, seed=71), 0),
     ("mis_hg", dict(n_paths=3000, n_queries=6000, radius=0.05, cell_count=2999, seed=72, mean_cosine=0.7), 0),
-    ("ppm", dict(n_paths=3000, n_queries=6000, radius=0.05, cell_count=2999, seed=73), scenes.PP3D_NO_MIS),
+    ("ppm", dict(n_paths=3000, n_queries=6000, radius=0.05, cell_count=2999, seed=73), scenes.VM_NO_MIS),
     ("one_bucket", dict(n_paths=800, n_queries=2000, radius=0.08, cell_count=1, seed=74), 0),
     ("few_buckets", dict(n_paths=2000, n_queries=3000, radius=0.05, cell_count=7, seed=75), 0),
     ("many_buckets", dict(n_paths=3000, n_queries=6000, radius=0.04, cell_count=1_000_003, seed=76), 0),
@@ -53,8 +53,8 @@ CASES = [
 def test_pp3d_matches_oracle(name, kw, flags):
     print_case("pp3d", name=name)
     sc = scenes.pp3d_scene(**kw)
-    og = po.OraclePp3d(sc)
-    ref, ref_sig, cnt = po.pp3d_eval(og, sc, flags=flags, precision=64)
+    og = po.OracleHashGrid(sc)
+    ref, ref_sig, cnt = po.vm_eval(og, sc, flags=flags, precision=64)
     assert cnt[0] > 0
     sig, n = _gpu(sc, flags=flags, signatures=True)
     assert n == og.count()
@@ -93,8 +93,8 @@ def test_pp3d_large_hit_sets_match_oracle():
     """1M light vertices, 1M camera vertices: identical hit signatures and 1e-4 contributions."""
     print_case("pp3d", name="large")
     sc = scenes.pp3d_scene(n_paths=330_000, n_queries=1_000_000, radius=0.012, cell_count=330_000, seed=84)
-    og = po.OraclePp3d(sc)
-    ref, ref_sig, cnt = po.pp3d_eval(og, sc, precision=64)
+    og = po.OracleHashGrid(sc)
+    ref, ref_sig, cnt = po.vm_eval(og, sc, precision=64)
     sig, _ = _gpu(sc, signatures=True)
     assert np.array_equal(sig, ref_sig)
     out, _ = _gpu(sc)
@@ -105,8 +105,8 @@ def test_pp3d_input_errors_are_reported():
     print_case("pp3d", name="errors")
     sc = scenes.pp3d_scene(n_paths=200, n_queries=50, seed=85)
     ctx = gpu_context(sc.media)
-    grid = native.HashGridPP3D(ctx)
-    grid.build(sc.positions, sc.mis.light_vertices, sc.radius, sc.cell_count)
+    grid = native.HashGrid(ctx)
+    grid.build(sc.positions, sc.mis.light_vertices, sc.radius, sc.cell_count, sc.technique)
     q = sc.queries.copy()
     q["medium"][4] = 9
     with pytest.raises(native.UpbpError, match="EINVAL"):
@@ -119,12 +119,12 @@ def test_pp3d_input_errors_are_reported():
     q["camera_offset"][0] = len(sc.mis.camera)
     with pytest.raises(native.UpbpError, match="EINVAL"):
         grid.eval(q, sc.mis)
-    grid.eval(q, sc.mis, flags=scenes.PP3D_NO_MIS)  # PPM reads no camera data
-    bad = native.HashGridPP3D(ctx)
+    grid.eval(q, sc.mis, flags=scenes.VM_NO_MIS)  # PPM reads no camera data
+    bad = native.HashGrid(ctx)
     with pytest.raises(native.UpbpError, match="EINVAL"):
-        bad.build(sc.positions, sc.mis.light_vertices, 0.0, 10)
+        bad.build(sc.positions, sc.mis.light_vertices, 0.0, 10, scenes.PP3D)
     pos = sc.positions.copy()
     pos[np.nonzero(sc.mis.light_vertices["flags"] & scenes.LV_IN_MEDIUM)[0][0], 2] = np.nan
     with pytest.raises(native.UpbpError, match="ENONFINITE"):
-        bad.build(pos, sc.mis.light_vertices, sc.radius, 10)
+        bad.build(pos, sc.mis.light_vertices, sc.radius, 10, scenes.PP3D)
     grid.destroy()

@@ -1,0 +1,99 @@
+"""GPU parity of SURF vertex merging (surface photon mapping through the hashed grid) against
+the CPU oracle: identical hit signatures, contributions within 1e-4 per query. Per-query errors
+are taken relative to max(|reference|, 1e-6 x the largest reference value): a Phong lobe raised
+to a high exponent can sit at the 1e-20 cut of Float::isPositive, where one ulp of powf decides
+whether a term of that size enters."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from smallupbp_amd import native, scenes
+
+from gpu_helpers import REL_TOL, gpu_context, print_case, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu(sc, flags=0, device=False, signatures=False, stats=None):
+    ctx = gpu_context(sc.media)
+    grid = native.HashGrid(ctx)
+    if device:
+        import torch
+
+        pos = torch.from_numpy(np.ascontiguousarray(sc.positions)).cuda()
+        lvt = torch.from_numpy(np.ascontiguousarray(sc.mis.light_vertices).view(np.uint8)).cuda()
+        grid.build(pos, lvt, sc.radius, sc.cell_count, sc.technique)
+        q = torch.from_numpy(np.ascontiguousarray(sc.queries).view(np.uint8)).cuda()
+        res = grid.eval(q, native.device_mis(sc.mis), flags=flags, signatures=signatures, stats=stats)
+        res = res.cpu().numpy()
+        if signatures:
+            res = res.view(np.uint32)
+    else:
+        grid.build(sc.positions, sc.mis.light_vertices, sc.radius, sc.cell_count, sc.technique)
+        res = grid.eval(sc.queries, sc.mis, flags=flags, signatures=signatures, stats=stats)
+    n = grid.count
+    grid.destroy()
+    return res, n
+
+
+def _err(out, ref):
+    return rel_err(out, ref, floor=1e-6 * float(np.abs(ref).max()))
+
+
+CASES = [
+    ("both_lobes", dict(n_paths=3000, n_queries=6000, radius=0.05, cell_count=2999, seed=91), 0),
+    ("diffuse", dict(n_paths=3000, n_queries=6000, radius=0.05, cell_count=2999, seed=92, lobes="diffuse"), 0),
+    ("phong", dict(n_paths=3000, n_queries=6000, radius=0.05, cell_count=2999, seed=93, lobes="phong"), 0),
+    ("ppm", dict(n_paths=3000, n_queries=6000, radius=0.05, cell_count=2999, seed=94), scenes.VM_NO_MIS),
+    ("one_bucket", dict(n_paths=800, n_queries=2000, radius=0.08, cell_count=1, seed=95), 0),
+    ("previous", dict(n_paths=3000, n_queries=4000, radius=0.05, cell_count=2999, seed=96,
+                      techniques=scenes.SURF | scenes.BPT | scenes.PP3D | scenes.PREVIOUS), 0),
+]
+
+
+@pytest.mark.parametrize("name,kw,flags", CASES)
+def test_surf_matches_oracle(name, kw, flags):
+    print_case("surf", name=name)
+    sc = scenes.surf_scene(**kw)
+    og = po.OracleHashGrid(sc)
+    ref, ref_sig, cnt = po.vm_eval(og, sc, flags=flags, precision=64)
+    assert cnt[0] > 0 and np.abs(ref).max() > 0
+    sig, n = _gpu(sc, flags=flags, signatures=True)
+    assert n == og.count()
+    assert np.array_equal(sig, ref_sig)
+    stats = {}
+    out, _ = _gpu(sc, flags=flags, stats=stats)
+    assert stats["photon_hits"] == int(cnt[0]) and stats["photon_candidates"] == int(cnt[1])
+    err = _err(out, ref)
+    assert err.max() <= REL_TOL, (name, err.max())
+
+
+def test_surf_device_memory_matches_host():
+    print_case("surf", name="device_memory")
+    sc = scenes.surf_scene(n_paths=2000, n_queries=5000, radius=0.05, cell_count=1999, seed=97)
+    host, _ = _gpu(sc)
+    dev, _ = _gpu(sc, device=True)
+    assert np.array_equal(host, dev)
+
+
+def test_surf_input_errors_are_reported():
+    print_case("surf", name="errors")
+    sc = scenes.surf_scene(n_paths=200, n_queries=50, seed=98)
+    ctx = gpu_context(sc.media)
+    grid = native.HashGrid(ctx)
+    grid.build(sc.positions, sc.mis.light_vertices, sc.radius, sc.cell_count, sc.technique)
+    q = sc.queries.copy()
+    q["frame_z"][3][1] = np.nan
+    with pytest.raises(native.UpbpError, match="ENONFINITE"):
+        grid.eval(q, sc.mis)
+    q = sc.queries.copy()
+    q["camera_path_length"][0] = 0
+    with pytest.raises(native.UpbpError, match="EINVAL"):
+        grid.eval(q, sc.mis)
+    # a SURF grid does not serve PP3D camera vertices
+    pp = scenes.pp3d_scene(n_paths=50, n_queries=10, seed=99)
+    grid.technique = scenes.PP3D
+    with pytest.raises(native.UpbpError, match="EINVAL"):
+        grid.eval(pp.queries, pp.mis)
+    grid.technique = scenes.SURF
+    grid.destroy()

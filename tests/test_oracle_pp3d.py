@@ -46,7 +46,7 @@ def _hash_sum(idx):
 
 def test_grid_bounding_box_buckets_and_order():
     sc = scenes.pp3d_scene(n_paths=1500, n_queries=10, cell_count=997)
-    og = po.OraclePp3d(sc)
+    og = po.OracleHashGrid(sc)
     bb, idx, ends = og.grid_info()
     inm = (sc.mis.light_vertices["flags"] & scenes.LV_IN_MEDIUM) != 0
     assert og.count() == inm.sum()
@@ -60,9 +60,9 @@ def test_grid_bounding_box_buckets_and_order():
 
 def test_grid_finds_exactly_the_vertices_within_the_radius():
     sc = scenes.pp3d_scene(n_paths=1500, n_queries=3000, radius=0.06, cell_count=1_000_003, seed=5)
-    og = po.OraclePp3d(sc)
-    _, sig, _ = po.pp3d_eval(og, sc, flags=scenes.PP3D_NO_MIS)
-    _, sig0, _ = po.pp3d_eval(og, sc, flags=scenes.PP3D_NO_MIS, accel=0)
+    og = po.OracleHashGrid(sc)
+    _, sig, _ = po.vm_eval(og, sc, flags=scenes.VM_NO_MIS)
+    _, sig0, _ = po.vm_eval(og, sc, flags=scenes.VM_NO_MIS, accel=0)
     assert np.array_equal(sig, sig0)  # the grid walk and the bucket-multiplicity statement agree
     inside = _inside_bbox(sc)
     assert inside.sum() > 2000 and (~inside).sum() > 100
@@ -80,8 +80,8 @@ def test_grid_finds_exactly_the_vertices_within_the_radius():
 
 def test_single_bucket_visits_every_vertex_eight_times():
     sc = scenes.pp3d_scene(n_paths=800, n_queries=600, radius=0.08, cell_count=1, seed=9)
-    og = po.OraclePp3d(sc)
-    _, sig, _ = po.pp3d_eval(og, sc, flags=scenes.PP3D_NO_MIS)
+    og = po.OracleHashGrid(sc)
+    _, sig, _ = po.vm_eval(og, sc, flags=scenes.VM_NO_MIS)
     inside = _inside_bbox(sc)
     want = _within(sc)
     for i in np.nonzero(inside)[0]:
@@ -99,8 +99,8 @@ def test_query_just_outside_the_box_finds_nothing():
     q["position"][0] = corner - np.float32([1e-3, 0, 0])  # within r of that vertex, below the box in x
     q["medium"][0] = 0
     assert q["position"][0][0] < lo[0]
-    og = po.OraclePp3d(sc)
-    out, sig, _ = po.pp3d_eval(og, sc, flags=scenes.PP3D_NO_MIS, queries=q)
+    og = po.OracleHashGrid(sc)
+    out, sig, _ = po.vm_eval(og, sc, flags=scenes.VM_NO_MIS, queries=q)
     assert sig[0, 0] == 0 and np.all(out[0] == 0)
 
 
@@ -126,8 +126,8 @@ def _ppm_closed_form(sc, mult, pdf_fn):
 def test_ppm_isotropic_is_sigma_s_over_4pi_times_throughput():
     sc = scenes.pp3d_scene(n_paths=1200, n_queries=800, radius=0.07, cell_count=1, seed=13)
     sc.mis.params.update(min_path_length=2, max_path_length=4)
-    og = po.OraclePp3d(sc)
-    out, sig, _ = po.pp3d_eval(og, sc, flags=scenes.PP3D_NO_MIS, precision=64)
+    og = po.OracleHashGrid(sc)
+    out, sig, _ = po.vm_eval(og, sc, flags=scenes.VM_NO_MIS, precision=64)
     ref = _ppm_closed_form(sc, 8.0, lambda q, v: UNIFORM_PDF) * _inside_bbox(sc)[:, None]
     assert np.abs(ref).max() > 0
     assert np.allclose(out, ref, rtol=1e-12, atol=1e-300)
@@ -141,8 +141,8 @@ def test_ppm_henyey_greenstein():
         c = -float(np.dot(v["dir_fix"].astype(np.float64), q["dir_fix"].astype(np.float64)))
         return UNIFORM_PDF * (1 - g * g) / (1 + g * g - 2 * g * c) ** 1.5
 
-    og = po.OraclePp3d(sc)
-    out, _, _ = po.pp3d_eval(og, sc, flags=scenes.PP3D_NO_MIS, precision=64)
+    og = po.OracleHashGrid(sc)
+    out, _, _ = po.vm_eval(og, sc, flags=scenes.VM_NO_MIS, precision=64)
     ref = _ppm_closed_form(sc, 8.0, hg) * _inside_bbox(sc)[:, None]
     # the oracle's float32 direction products enter cos(theta): 1e-6 relative
     assert np.allclose(out, ref, rtol=1e-5, atol=1e-30)
@@ -163,8 +163,8 @@ def test_pp3d_mis_weight_closed_form():
     rng = np.random.default_rng(1)
     lv["mis"]["pp3d_factor"][src] = rng.uniform(0.2, 2.0, src.sum())  # make the source term visible
     cam = sc.mis.camera
-    og = po.OraclePp3d(sc)
-    out, _, _ = po.pp3d_eval(og, sc, precision=64)
+    og = po.OracleHashGrid(sc)
+    out, _, _ = po.vm_eval(og, sc, precision=64)
     want = _within(sc)
     inside = _inside_bbox(sc)
     ref = np.zeros((len(sc.queries), 3))
@@ -199,9 +199,9 @@ def test_pp3d_mis_weight_closed_form():
 
 def test_fp32_and_fp64_oracles_agree_with_all_techniques():
     sc = scenes.pp3d_scene(n_paths=2000, n_queries=2000, radius=0.06, cell_count=1999, seed=29)
-    og = po.OraclePp3d(sc)
-    o32, s32, _ = po.pp3d_eval(og, sc)
-    o64, s64, _ = po.pp3d_eval(og, sc, precision=64)
+    og = po.OracleHashGrid(sc)
+    o32, s32, _ = po.vm_eval(og, sc)
+    o64, s64, _ = po.vm_eval(og, sc, precision=64)
     assert np.array_equal(s32, s64)
     den = np.maximum(np.abs(o64).max(1), 1e-30)
     assert (np.abs(o32 - o64).max(1) / den).max() < 1e-4
@@ -209,17 +209,17 @@ def test_fp32_and_fp64_oracles_agree_with_all_techniques():
 
 def test_input_errors():
     sc = scenes.pp3d_scene(n_paths=200, n_queries=20, seed=31)
-    og = po.OraclePp3d(sc)
+    og = po.OracleHashGrid(sc)
     q = sc.queries.copy()
     q["medium"][3] = 7
     with pytest.raises(RuntimeError, match="EINVAL"):
-        po.pp3d_eval(og, sc, queries=q)
+        po.vm_eval(og, sc, queries=q)
     q = sc.queries.copy()
     q["position"][2][1] = np.nan
     with pytest.raises(RuntimeError, match="ENONFINITE"):
-        po.pp3d_eval(og, sc, queries=q)
+        po.vm_eval(og, sc, queries=q)
     q = sc.queries.copy()
     q["camera_offset"][0] = len(sc.mis.camera)
     with pytest.raises(RuntimeError, match="EINVAL"):
-        po.pp3d_eval(og, sc, queries=q)
-    po.pp3d_eval(og, sc, queries=q, flags=scenes.PP3D_NO_MIS)  # PPM reads no camera data
+        po.vm_eval(og, sc, queries=q)
+    po.vm_eval(og, sc, queries=q, flags=scenes.VM_NO_MIS)  # PPM reads no camera data
