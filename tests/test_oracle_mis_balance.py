@@ -186,3 +186,84 @@ def test_walk_weights_with_pb2d_long_query_beams(k, seed):
             assert w == pytest.approx(pb2d[i] / S, rel=1e-12)
             total += w
     assert total == pytest.approx(1.0, rel=1e-12)
+
+
+@pytest.mark.parametrize("qbt", [scenes.LONG_BEAM, scenes.SHORT_BEAM])
+@pytest.mark.parametrize("pbt", [scenes.LONG_BEAM, scenes.SHORT_BEAM])
+@pytest.mark.parametrize("no_sine", [False, True])
+def test_walk_weights_with_beam_techniques(qbt, pbt, no_sine):
+    """PB2D and BB1D terms for every beam type. At an in-medium vertex x_j with distance-sampling
+    pdfs rsC[j] (camera direction) and rsL[j] (light direction), whose sampled point the beam
+    techniques replace by a whole segment:
+      PB2D (query beam x photon):  F_pb2d[j] * P[j] * qC[j]
+      BB1D (query beam x photon beam): F_bb1d[j] * sel[j] * sin[j] * P[j] * qC[j] * qL[j]
+    with P[j] = prod(pL[0..j]) prod(pC[j..k-1]), qC = 1 / rsC for a LONG query beam or the short
+    beam's pdf ratio rhoC for a SHORT one, qL likewise for the photon beams, sin the sine between the
+    vertex's two segments (1 with NO_SINE_IN_WEIGHTS). The MisData carry them as the reference's do:
+    camera vertices mRaySamplePdfInv = 1 / rsC, mRaySampleRevPdfInv = 1 / rsL, mRaySamplePdfsRatio
+    = rhoC, mRaySampleRevPdfsRatio = rhoL; light vertices the same quantities with forward and
+    reverse swapped. PP3D, SURF and PB2D merges are evaluated through the walks (PB2D with
+    last = 1 / (qC F_pb2d)); all weights, BB1D included, must be the balance heuristic."""
+    rng = np.random.default_rng(100 + 4 * qbt + 2 * pbt + int(no_sine))
+    k = 7
+    pL, pC, F, surf, spec, beta, gamma = _toy(rng, k, 0.3)
+    rsC, rsL = _pow2(rng, k, -2, 1), _pow2(rng, k, -2, 1)
+    rhoC, rhoL = _pow2(rng, k, -2, 1), _pow2(rng, k, -2, 1)
+    sin = _pow2(rng, k, -3, 0)
+    med = ~surf
+    med[0] = False
+    Fb = np.where(med, _pow2(rng, k, -2, 2), 0.0)
+    Fbb = np.where(med, _pow2(rng, k, -2, 2), 0.0)
+    sel = _pow2(rng, k, -2, 0)
+    beams = med & (rng.uniform(size=k) < 0.8)  # vertices in a medium with photon beams
+    tech = TECH | scenes.PB2D | scenes.BB1D | (scenes.NO_SINE_IN_WEIGHTS if no_sine else 0)
+    qC = 1.0 / rsC if qbt == scenes.LONG_BEAM else rhoC
+    qL = 1.0 / rsL if pbt == scenes.LONG_BEAM else rhoL
+    sn = np.ones(k) if no_sine else sin
+
+    def fill(m, j, cam_side):
+        m["pb2d_factor"] = Fb[j]
+        m["bb1d_factor"] = Fbb[j]
+        m["bb1d_beam_selection_pdf"] = sel[j]
+        m["sin_theta"] = sin[j]
+        m["flags"] = m["flags"] | (scenes.MIS_IN_MEDIUM_WITH_BEAMS if beams[j] else 0)
+        fwd_inv, rev_inv, fwd_ratio, rev_ratio = (1 / rsC[j], 1 / rsL[j], rhoC[j], rhoL[j]) if cam_side else (
+            1 / rsL[j], 1 / rsC[j], rhoL[j], rhoC[j])
+        m["ray_sample_pdf_inv"], m["ray_sample_rev_pdf_inv"] = fwd_inv, rev_inv
+        m["ray_sample_pdfs_ratio"], m["ray_sample_rev_pdfs_ratio"] = fwd_ratio, rev_ratio
+
+    lv, ends = _light_vertices(pL, pC, F, surf, spec, beta)
+    for j in range(k):
+        fill(lv["mis"][j], j, False)
+
+    def cam_for(i):
+        cam, L = _camera(pL, pC, F, surf, spec, gamma, i)
+        for t in range(1, L + 1):
+            fill(cam[t], k - t, True)
+        return cam, L
+
+    merge, bpt, S0 = _pdfs(pL, pC, F, surf, spec)
+    P = np.array([np.prod(pL[:j + 1]) * np.prod(pC[j:]) for j in range(k)])
+    pb2d = Fb * P * qC
+    bb1d = np.where(beams, Fbb * sel * sn * P * qC * qL, 0.0)
+    S = S0 + pb2d.sum() + bb1d.sum()
+    total = (bpt[-1] + bpt[k - 1] + sum(bpt[a] for a in range(0, k - 1)) + bb1d.sum()) / S
+    kw = dict(qbt=qbt, pbt=pbt)
+    for i in range(1, k):
+        cam, L = cam_for(i)
+        # the camera walk's first vertex takes its BB1D data from the merge: the sine between the
+        # camera and light segments and the light side's sampling of x_i (UPBP.hxx:124-126)
+        first = dict(last_sin=sin[i], last_ray_rev_inv=1 / rsL[i], last_ray_rev_ratio=rhoL[i])
+        ev = scenes.SURF if surf[i] else scenes.PP3D
+        inv = 1.0 / F[i]
+        w = 1.0 / (po.camera_walk(cam, L, inv, gamma[i], tech, **first, **kw) +
+                   po.light_walk(lv, ends, 0, i, inv, beta[i], ev, tech, **kw))
+        assert w == pytest.approx(merge[i] / S, rel=1e-12)
+        total += w
+        if Fb[i] > 0:
+            last = 1.0 / (qC[i] * Fb[i])
+            w = 1.0 / (po.camera_walk(cam, L, last, gamma[i], tech, **first, **kw) +
+                       po.light_walk(lv, ends, 0, i, last, beta[i], scenes.PB2D, tech, **kw))
+            assert w == pytest.approx(pb2d[i] / S, rel=1e-12)
+            total += w
+    assert total == pytest.approx(1.0, rel=1e-12)
