@@ -1,5 +1,5 @@
-"""First-principles pin of the PB2D estimator's MIS arguments (breIntersectFuncHomogeneous2,
-SmallUPBP/src/Bre/Bre.cxx:305-463).
+"""First-principles pin of the PB2D and BB1D estimators' MIS arguments (breIntersectFuncHomogeneous2,
+SmallUPBP/src/Bre/Bre.cxx:305-463; PhotonBeam::accumulate2, SmallUPBP/src/Beams/PhotonBeam.hxx:390-605).
 
 One query beam leaves the camera vertex x_{i+1} and finds ONE photon, the light vertex x_i. The
 UPBP estimate divided by the plain estimate of the same hit is that hit's MIS weight. It must be
@@ -172,4 +172,143 @@ def test_pb2d_hit_weight_is_the_balance_heuristic(nseg, origin_in_medium, qbt, p
     bpt = [np.prod(pL[:a + 1]) * np.prod(pC[a + 1:]) for a in range(-1, k)]
     S = techniques.sum() + sum(bpt)
     w_ref = fac["pb2d"][i] * P[i] * qC[i] / S
+    assert w_est[0] == pytest.approx(w_ref, rel=1e-12)
+
+
+@pytest.mark.parametrize("origin_in_medium", [0, 1])
+@pytest.mark.parametrize("beam_origin_in_medium", [0, 1])
+@pytest.mark.parametrize("qbt", [scenes.LONG_BEAM, scenes.SHORT_BEAM])
+@pytest.mark.parametrize("pbt", [scenes.LONG_BEAM, scenes.SHORT_BEAM])
+def test_bb1d_hit_weight_is_the_balance_heuristic(origin_in_medium, beam_origin_in_medium, qbt, pbt):
+    """PhotonBeam::accumulate2 (PhotonBeam.hxx:390-605): ONE query beam (from the camera vertex
+    x_{i+1}) and ONE photon beam (from the light vertex x_b, i = b + 1) meet; the merged point x_i is
+    sampled by neither side. The weight UPBP / plain must be the balance heuristic
+    p_BB1D(x_i) / sum_t p_t with, besides the PB2D test's query-side pdfs:
+      light-direction area pdf of x_i:   beam distance pdf (prefix * sigma_min e^{-sigma_min t2}) /
+                                         (beam last_pdf_w_inv * t2^2),
+      camera-direction area pdf of x_b:  phase * continuation * reverse beam distance pdf * |cos_out(x_b)| / t2^2
+                                         (reverse distance pdf = prefix * sigma_min e^{-sigma_min t2} when
+                                         x_b is in the medium, else e^{-sigma_min t2}),
+      x_b's camera-direction distance pdf for its beam techniques: that reverse distance pdf,
+      p_BB1D(x_i) = F_bb1d * sel * sin(theta) * P_i * qC_i * qL_i (qL_i = 1 / beam distance pdf for a
+      LONG photon beam, 1 / sigma_min for a SHORT one)."""
+    rng = np.random.default_rng(2000 + 8 * origin_in_medium + 4 * beam_origin_in_medium + 2 * qbt + pbt)
+    g = 0.3
+    media = scenes.make_media([((0.1, 0.2, 0.3), (1.0, 1.5, 2.0), g, 0.8)])
+    st = (np.abs(media["absorption"]) + np.abs(media["scattering"])).astype(np.float64)
+    s1 = float(st[0].min())
+    o1 = np.array([0.5, 0.5, 0.125], np.float32)
+    d1 = np.array([0.0, 0.0, 1.0], np.float32)
+    segs = np.zeros(1, scenes.SEGMENT_DTYPE)
+    segs[0]["dist_min"], segs[0]["dist_max"], segs[0]["medium_id"] = 0.0, 0.875, 0
+    segs[0]["ray_sample_pdf"], segs[0]["ray_sample_rev_pdf"] = 1.0, 1.0
+    segs[0]["attenuation"] = np.exp(-st[0] * 0.875)
+    q = scenes.Queries(o1[None], d1[None], np.array([0, 1], np.int64), segs, qbt)
+    # the photon beam: leaves x_b at 0.3 from the query line, crosses it at about z = 0.6 at 60 degrees
+    o2 = np.array([0.25, 0.515625, 0.1875], np.float32)
+    d2 = np.array([0.5, 0.0, np.sqrt(0.75)], np.float32)
+    d2 = (d2 / np.linalg.norm(d2.astype(np.float64))).astype(np.float32)
+
+    b = 2  # beam origin light vertex: x_0 .. x_b light path, x_i = x_{b+1} virtual, x_{i+1} camera vertex
+    i = b + 1
+    nv = b + 2
+    lv = np.zeros(nv, scenes.LIGHT_VERTEX_DTYPE)
+    lv["path_idx"] = 0
+    lv["path_length"] = np.arange(nv)
+    lv["throughput"] = rng.uniform(0.1, 1.0, (nv, 3))
+    lv["continuation_prob"] = _pow2(rng, nv, -1, 0)
+    lv["is_finite"] = 1
+    m = lv["mis"]
+    for f in ("pdf_a_inv", "rev_pdf_a", "rev_pdf_a_without_bsdf", "ray_sample_pdf_inv", "ray_sample_rev_pdf_inv",
+              "ray_sample_pdfs_ratio", "ray_sample_rev_pdfs_ratio", "pp3d_factor", "pb2d_factor", "bb1d_factor",
+              "bb1d_beam_selection_pdf", "cos_theta_out"):
+        m[f] = _pow2(rng, nv)
+    m["cos_theta_out"] = -_pow2(rng, nv, -2, 0)  # a cosine's sign must not matter (|cos|)
+    m["sin_theta"] = _pow2(rng, nv, -3, 0)
+    m["flags"] = scenes.MIS_IN_MEDIUM_WITH_BEAMS
+    m["surf_factor"] = 0.0
+    m["flags"][0] = scenes.MIS_ON_LIGHT_SOURCE
+    for f in ("pp3d_factor", "pb2d_factor", "bb1d_factor"):
+        m[f][0] = 0.0
+    ends = np.array([nv], np.int32)
+    bmis = np.zeros(1, scenes.BEAM_MIS_DTYPE)
+    bmis["light_vertex"] = b
+    bmis["ray_sampling_flags"] = beam_origin_in_medium
+    bmis["ray_sample_pdf"], bmis["ray_sample_rev_pdf"], bmis["last_pdf_w_inv"] = 0.5, 0.25, 2.0
+
+    cam = np.zeros(3, scenes.MIS_DTYPE)
+    for f in ("pdf_a_inv", "rev_pdf_a", "ray_sample_pdf_inv", "ray_sample_rev_pdf_inv", "ray_sample_pdfs_ratio",
+              "ray_sample_rev_pdfs_ratio", "pp3d_factor", "pb2d_factor", "bb1d_factor", "bb1d_beam_selection_pdf",
+              "surf_factor"):
+        cam[f][1] = _pow2(rng, 1)[0]
+    cam["sin_theta"][1] = 0.5
+    cam["flags"][1] = scenes.MIS_IN_MEDIUM_WITH_BEAMS
+    qm = np.zeros(1, scenes.QUERY_MIS_DTYPE)
+    qm["camera_offset"], qm["camera_path_length"] = 0, 2
+    qm["ray_sampling_flags"] = origin_in_medium
+    qm["last_pdf_w_inv"] = 0.5
+    P_ = dict(techniques=ALL, query_beam_type=qbt, photon_beam_type=pbt, min_path_length=0, max_path_length=20,
+              surf_factor=0.7, pp3d_factor=0.75, pb2d_factor=1.25, bb1d_factor=1.5, bb1d_min_mfp=0.01,
+              min_dist_to_med=0.0, has_bb1d_beams=1)
+    mi = scenes.MisInputs(qm, cam, lv, ends, bmis, P_)
+    beams = scenes.Beams(o2[None], d2[None], np.array([1.0], np.float32), np.array([[0.3, 0.5, 0.7]], np.float32),
+                         np.array([pbt], np.uint32), np.array([0], np.int32), 0.05)
+    obm = po.OracleBeams(beams, grid_res=0)
+    plain, _, cnt = po.beams_eval(obm, media, q, accel=0, precision=64)
+    upbp, hits = po.upbp_eval(None, obm, media, q, mi, 1, precision=64)
+    assert cnt[0] == 1 and hits == 1 and (plain > 0).all()
+    w_est = upbp[0] / plain[0]
+    assert np.allclose(w_est, w_est[0], rtol=1e-9)
+
+    # ---- closest approach (double) and the balance heuristic from first principles ----
+    O1, D1, O2, D2 = (x.astype(np.float64) for x in (o1, d1, o2, d2))
+    qd = float(np.dot(D1, D2))
+    w0 = O1 - O2
+    t1 = (qd * float(np.dot(D2, w0)) - float(np.dot(D1, w0))) / (1 - qd * qd)
+    t2 = (float(np.dot(D2, w0)) - qd * float(np.dot(D1, w0))) / (1 - qd * qd)
+    sin_t = float(np.linalg.norm(np.cross(D1, D2)))
+    cos_t = -float(np.dot(D2, D1))  # -dot(gen, fix) with fix = -d1, gen = -d2
+    ph = _hg(float(media["mean_cosine"][0]), cos_t)
+    cont = float(media["continuation_prob"][0])
+    rsp_q = s1 * np.exp(-s1 * t1)
+    rsr_q = rsp_q if origin_in_medium else np.exp(-s1 * t1)
+    rsp_b = s1 * np.exp(-s1 * t2) * float(bmis["ray_sample_pdf"][0])
+    rsr_b = (s1 * np.exp(-s1 * t2) if beam_origin_in_medium else np.exp(-s1 * t2)) * float(bmis["ray_sample_rev_pdf"][0])
+    k = i + 2
+    pL = np.zeros(k)
+    pC = np.zeros(k)
+    for j in range(b + 1):
+        pL[j] = 1.0 / float(m["pdf_a_inv"][j])
+    for j in range(b):
+        pC[j] = float(m["rev_pdf_a"][j])
+    pC[b] = ph * cont * rsr_b * abs(float(m["cos_theta_out"][b])) / (t2 * t2)
+    pL[i] = rsp_b / (float(bmis["last_pdf_w_inv"][0]) * t2 * t2)
+    pC[i] = rsp_q / (float(qm["last_pdf_w_inv"][0]) * t1 * t1)
+    pC[i + 1] = 1.0 / float(cam["pdf_a_inv"][1])
+    pL[i + 1] = float(cam["rev_pdf_a"][1]) * ph * cont * rsr_q / (t1 * t1)
+    P = np.array([np.prod(pL[:j + 1]) * np.prod(pC[j:]) for j in range(k)])
+    LONGQ, LONGB = qbt == scenes.LONG_BEAM, pbt == scenes.LONG_BEAM
+    fac = {f: np.zeros(k) for f in ("pp3d", "pb2d", "bb1d", "sel", "sin", "surf")}
+    qC = np.zeros(k)
+    qL = np.zeros(k)
+    for j in range(1, b + 1):
+        mj = m[j]
+        fac["pp3d"][j], fac["pb2d"][j], fac["bb1d"][j] = mj["pp3d_factor"], mj["pb2d_factor"], mj["bb1d_factor"]
+        fac["sel"][j], fac["sin"][j] = mj["bb1d_beam_selection_pdf"], mj["sin_theta"]
+        qC[j] = float(mj["ray_sample_rev_pdf_inv"]) if LONGQ else float(mj["ray_sample_rev_pdfs_ratio"])
+        qL[j] = float(mj["ray_sample_pdf_inv"]) if LONGB else float(mj["ray_sample_pdfs_ratio"])
+    qC[b] = 1.0 / rsr_b if LONGQ else float(m["ray_sample_rev_pdfs_ratio"][b])  # x_b seen from the merge
+    fac["pp3d"][i], fac["pb2d"][i], fac["bb1d"][i] = P_["pp3d_factor"], P_["pb2d_factor"], P_["bb1d_factor"]
+    fac["sel"][i], fac["sin"][i] = 1.0, sin_t
+    qC[i] = 1.0 / rsp_q if LONGQ else 1.0 / s1
+    qL[i] = 1.0 / rsp_b if LONGB else 1.0 / s1
+    c1 = cam[1]
+    fac["pp3d"][i + 1], fac["pb2d"][i + 1], fac["bb1d"][i + 1] = c1["pp3d_factor"], c1["pb2d_factor"], c1["bb1d_factor"]
+    fac["sel"][i + 1], fac["sin"][i + 1], fac["surf"][i + 1] = c1["bb1d_beam_selection_pdf"], c1["sin_theta"], c1["surf_factor"]
+    qC[i + 1] = float(c1["ray_sample_pdf_inv"]) if LONGQ else float(c1["ray_sample_pdfs_ratio"])
+    qL[i + 1] = 1.0 / rsr_q if LONGB else float(c1["ray_sample_rev_pdfs_ratio"])
+    techniques = (fac["pp3d"] + fac["surf"]) * P + fac["pb2d"] * P * qC + fac["bb1d"] * fac["sel"] * fac["sin"] * P * qC * qL
+    bpt = [np.prod(pL[:a + 1]) * np.prod(pC[a + 1:]) for a in range(-1, k)]
+    S = techniques.sum() + sum(bpt)
+    w_ref = fac["bb1d"][i] * sin_t * P[i] * qC[i] * qL[i] / S
     assert w_est[0] == pytest.approx(w_ref, rel=1e-12)
