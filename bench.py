@@ -13,7 +13,7 @@ Prints ONE JSON line: metric, value (Mqueries/s), roofline of the dominant
 kernel (algorithmic bytes / average HIP-event kernel time vs 8 TB/s), the CPU
 oracle timed on a bounded sample on the host cores, the forward+adjoint vs
 forward cost ratio on the 512x512 differentiable workload, the 64-instance
-batched speedup, and the PP3D vertex-merging rate.
+batched speedup, and the PP3D and SURF vertex-merging rates.
 """
 from __future__ import annotations
 
@@ -203,20 +203,25 @@ def batched_ratio(scale: float, steps: int, n_instances: int = 64):
             "speedup": round(res["plain"] / min(res["batched"], res["batched_graph"]), 3)}
 
 
-def pp3d_measure(steps: int, cpu_budget: float, threads: int):
-    """PP3D vertex merging (the reference's hashed grid + RangeQuery::Process with UPBP MIS weights):
-    1,000,000 camera vertices in the box + sphere media against the in-medium vertices of 1,000,000
-    light paths (radius 0.01, one hash bucket per light path). Inputs resident in HBM; one step =
-    one upbp_pp3d_eval call (lane sort + grid walk + weights). The oracle's restatement of the same
-    hashed grid is timed on a query sample as the CPU comparison."""
+def vm_measure(technique: int, steps: int, cpu_budget: float, threads: int):
+    """Vertex merging through the reference's hashed grid with RangeQuery::Process and UPBP MIS
+    weights: PP3D (1,000,000 camera vertices in the box + sphere media against the in-medium
+    vertices of 1,000,000 light paths) or SURF (1,000,000 camera vertices on three faces of the box
+    against the surface vertices of 1,000,000 light paths); radius 0.01, one hash bucket per light
+    path. Inputs resident in HBM; one step = one evaluation call (lane sort + grid walk + weights).
+    The oracle's restatement of the same hashed grid is timed on a query sample as the CPU
+    comparison."""
     import torch
 
     from smallupbp_amd import native, scenes
 
+    pp3d = technique == scenes.PP3D
+    name = "pp3d" if pp3d else "surf"
     t = time.time()
-    sc = scenes.pp3d_scene(n_paths=1_000_000, n_queries=1_000_000, radius=0.01, cell_count=1_000_000, seed=91)
-    log("pp3d workload: %d light vertices, %d camera vertices, generated in %.1fs"
-        % (len(sc.positions), len(sc.queries), time.time() - t))
+    gen = scenes.pp3d_scene if pp3d else scenes.surf_scene
+    sc = gen(n_paths=1_000_000, n_queries=1_000_000, radius=0.01, cell_count=1_000_000, seed=91 if pp3d else 93)
+    log("%s workload: %d light vertices, %d camera vertices, generated in %.1fs"
+        % (name, len(sc.positions), len(sc.queries), time.time() - t))
     ctx = native.Context(0)
     ctx.set_media(sc.media)
     pos = torch.from_numpy(np.ascontiguousarray(sc.positions)).cuda()
@@ -224,7 +229,7 @@ def pp3d_measure(steps: int, cpu_budget: float, threads: int):
     grid = native.HashGrid(ctx)
     torch.cuda.synchronize()
     t = time.perf_counter()
-    grid.build(pos, lvt, sc.radius, sc.cell_count, scenes.PP3D)
+    grid.build(pos, lvt, sc.radius, sc.cell_count, technique)
     t_build = time.perf_counter() - t
     ms = native.device_mis(sc.mis)
     q = torch.from_numpy(np.ascontiguousarray(sc.queries).view(np.uint8)).cuda()
@@ -242,7 +247,9 @@ def pp3d_measure(steps: int, cpu_budget: float, threads: int):
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     ms_step = el / steps * 1e3
-    rec = {"workload": "pp3d_1m (1M camera vertices x %d in-medium light vertices, r = 0.01, MIS on)" % grid.count,
+    where = "in-medium" if pp3d else "surface"
+    rec = {"workload": "%s_1m (1M camera vertices x %d %s light vertices, r = 0.01, MIS on)"
+                       % (name, grid.count, where),
            "mqueries_per_s": round(Q / (el / steps) / 1e6, 3), "ms_per_step": round(ms_step, 4),
            "kernel_ms": round(float(np.mean([p["kernel_ms_photons"] for p in per])), 4),
            "process_calls": int(per[-1]["photon_hits"]), "distance_tests": int(per[-1]["photon_candidates"]),
@@ -298,7 +305,7 @@ def main():
                     help="ignore smallupbp_amd/tuning_cache.json and use the library defaults")
     ap.add_argument("--batched", type=float, default=0.25,
                     help="also time the 64-instance batched form at this size scale (1 = full size, 0 = skip)")
-    ap.add_argument("--no-pp3d", action="store_true", help="skip the PP3D vertex-merging measurement")
+    ap.add_argument("--no-vm", action="store_true", help="skip the PP3D and SURF vertex-merging measurements")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -430,8 +437,11 @@ def main():
     if args.batched > 0:
         rec["batched"] = batched_ratio(args.batched, max(2, args.steps // 2))
 
-    if not args.no_pp3d:
-        rec["pp3d"] = pp3d_measure(max(3, args.steps // 2), min(args.cpu_budget, 5.0), host_cores())
+    if not args.no_vm:
+        from smallupbp_amd import scenes as _sc
+
+        for tech, key in ((_sc.PP3D, "pp3d"), (_sc.SURF, "surf")):
+            rec[key] = vm_measure(tech, max(3, args.steps // 2), min(args.cpu_budget, 5.0), host_cores())
 
     if args.cpu_budget > 0:
         threads = host_cores()
