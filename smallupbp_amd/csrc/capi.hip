@@ -244,6 +244,13 @@ void run_eval(Ctx& ctx, const Bre* bre, const Beams* bm, const upbp_query_batch*
     scene_bounds(bre, bm, lo, hi);
     QueryView v = make_view(ctx, q, lo, hi);
     launch_clear_u64(ctx, ctx.stats.as<uint64_t>(), kStatSlots);
+    // the UPBP light walks read per-vertex tails, built once here for both estimators
+    MisDev mis_t;
+    if (mis && mode == Mode::Upbp && Q > 0) {
+        mis_t = *mis;
+        launch_light_tails(ctx, mis_t);
+        mis = &mis_t;
+    }
     // device outputs (temporary for host outputs)
     float* dph = o.out_ph;
     float* dbm = o.out_bm;
@@ -729,6 +736,7 @@ void run_vm_eval(Ctx* ctx, const HashGrid* grid, uint32_t type, const Query* que
     }
     launch_clear_u64(*ctx, ctx->stats.as<uint64_t>(), kStatSlots);
     if (timing) UPBP_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+    if (use_mis && count > 0) launch_light_tails(*ctx, d);
     launch(dq, d, dout, dsig);
     if (timing) UPBP_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
     if (memory == UPBP_MEM_HOST && count > 0) {

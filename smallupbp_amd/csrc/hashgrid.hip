@@ -214,9 +214,13 @@ __device__ __forceinline__ void vm_accumulate(const VmArgs& A, const upbp_light_
         cv.prev_ray_rev_inv = cv.base[cam_len - 1].ray_sample_rev_pdf_inv;
         const float w_cam = camera_path_weight(cv, inv, sin_t, lv->mis.ray_sample_pdf_inv, lv->mis.ray_sample_pdfs_ratio,
                                                rev_pdf, P.query_beam_type, P.photon_beam_type, P.techniques);
-        LightView lview{&G, nullptr, -1, 0.f};
-        const float w_light = light_path_weight(lview, lv->path_idx, (int)lv->path_length, inv, dir_pdf, evaluated,
-                                                P.query_beam_type, P.photon_beam_type, P.techniques, err);
+        float w_light;
+        if (!light_weight_tail(G, (int64_t)(lv - G.lv), lv->path_idx, (int)lv->path_length, inv, dir_pdf, P.techniques,
+                               w_light)) {
+            LightView lview{&G, nullptr, -1, 0.f};
+            w_light = light_path_weight(lview, lv->path_idx, (int)lv->path_length, inv, dir_pdf, evaluated,
+                                        P.query_beam_type, P.photon_beam_type, P.techniques, err);
+        }
         w = 1.f / (w_light + w_cam);
     }
     acc[0] += w * (f0 * lv->throughput[0]);
@@ -548,6 +552,7 @@ void vm_eval(Ctx& ctx, const HashGrid& grid, const Query* q, int64_t count, cons
     A.mis.path_count = mis.path_count;
     A.mis.beams = nullptr;
     A.mis.p = mis.params;
+    A.mis.tails = mis.tails;
     A.stats = ctx.stats.as<unsigned long long>();
     A.perm = nullptr;
     if (count > 64 && grid.n > 0) {

@@ -253,9 +253,12 @@ __device__ void photon_hit_mis(const EvalArgs& A, const Ray& r, SegState& s, con
     const MisF lvm = mis_load(&lv->mis);
     const float w_cam = camera_path_weight(cv, last, sin_t, lvm.ray_sample_pdf_inv, lvm.ray_sample_pdfs_ratio,
                                            rev_pdf * ray_rev / dist_sq, P.query_beam_type, P.photon_beam_type, tech);
-    LightView lview{&G, nullptr, -1, 0.f};
-    const float w_light = light_path_weight(lview, lv->path_idx, (int)path_len, last, dir_pdf, UPBP_PB2D,
-                                            P.query_beam_type, P.photon_beam_type, tech, err);
+    float w_light;
+    if (!light_weight_tail(G, (int64_t)lv_idx, lv->path_idx, (int)path_len, last, dir_pdf, tech, w_light)) {
+        LightView lview{&G, nullptr, -1, 0.f};
+        w_light = light_path_weight(lview, lv->path_idx, (int)path_len, last, dir_pdf, UPBP_PB2D, P.query_beam_type,
+                                    P.photon_beam_type, tech, err);
+    }
     const float w = 1.f / (w_light + w_cam);
     s.acc0 += w * u0;
     s.acc1 += w * u1;
@@ -345,12 +348,16 @@ __device__ void beam_hit_mis(const EvalArgs& A, const Ray& r, SegState& s, const
                        : rsp_q * rsp_b / (cv.last.bb1d * sel_pdf * sin_t);
     const float w_cam = camera_path_weight(cv, last, sin_t, rsp_inv_b, ratio_b, rev_pdf * rsr_q / dist_sq_q,
                                            P.query_beam_type, P.photon_beam_type, tech);
-    LightView lview{&G, &bv, (int64_t)bm.light_vertex, 1.0f / rsr_b};
     const uint32_t lvf = lv->mis.flags;
     const bool on_light_delta = (lvf & UPBP_MIS_ON_LIGHT_SOURCE) && (lvf & UPBP_MIS_DELTA);
     const float ntl = on_light_delta ? 0.f : dir_pdf * rsr_b * __builtin_fabsf(lv->mis.cos_theta_out) / dist_sq_b;
-    const float w_light = light_path_weight(lview, lv->path_idx, (int)path_len + 1, last, ntl, UPBP_BB1D,
-                                            P.query_beam_type, P.photon_beam_type, tech, err);
+    float w_light;
+    if (!light_weight_tail_bb1d(G, (int64_t)bm.light_vertex, lv->path_idx, (int)path_len, last, bv.pdf_a_inv, ntl,
+                                1.0f / rsr_b, tech, w_light)) {
+        LightView lview{&G, &bv, (int64_t)bm.light_vertex, 1.0f / rsr_b};
+        w_light = light_path_weight(lview, lv->path_idx, (int)path_len + 1, last, ntl, UPBP_BB1D, P.query_beam_type,
+                                    P.photon_beam_type, tech, err);
+    }
     const float w = 1.f / (w_light + w_cam);
     s.acc0 += w * u0;
     s.acc1 += w * u1;
@@ -869,8 +876,10 @@ __device__ __forceinline__ int64_t xcd_block(int64_t bid, int64_t nblk) {
 
 // Forward kernels ask for 8 resident waves per SIMD (64 VGPRs): a few bytes of
 // register spill cost less than the latency the extra wave hides (measured 3-4 %).
+// UPBP kernels ask for 4 (128 VGPRs), which both fit without spilling.
 template <int EST, int MODE>
-__global__ __launch_bounds__(256, (MODE == (int)Mode::Forward ? 8 : 1)) void k_eval(EvalArgs A) {
+__global__ __launch_bounds__(256, (MODE == (int)Mode::Forward ? 8 : MODE == (int)Mode::Upbp ? 4 : 1)) void k_eval(
+    EvalArgs A) {
     const int wave_in_block = threadIdx.x / kWave;
     const int waves_per_block = blockDim.x / kWave;
     const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
@@ -1050,8 +1059,49 @@ EvalArgs base_args(Ctx& ctx, const QueryView& q) {
 
 }  // namespace
 
+// One thread per light vertex j starting a chain (path length 0): walks up the chain while
+// the stored path lengths count 1, 2, ..., folding each vertex's terms into the tail of the
+// next one. A vertex no chain reaches keeps the NaN below-marker of k_light_tails_fill.
+__global__ void k_light_tails_fill(float4* __restrict__ tails, int64_t n) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) tails[j] = make_float4(0.f, 0.f, 0.f, __builtin_nanf(""));
+}
+
+__global__ void k_light_tails(const upbp_light_vertex* __restrict__ lv, int64_t n, upbp_mis_params p,
+                              float4* __restrict__ tails) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n || lv[s].path_length != 0) return;
+    const uint32_t tech = p.techniques;
+    float below = 0.f;  // tail of the vertex under the current one
+    for (int64_t m = s; m < n && lv[m].path_length == (uint32_t)(m - s); ++m) {
+        const upbp_mis_data& d = lv[m].mis;
+        float t_rest, t_ray;
+        light_tail_terms(d, tech, p.query_beam_type, p.photon_beam_type, t_rest, t_ray);
+        const bool spec = (d.flags & UPBP_MIS_SPECULAR) != 0;
+        const bool next_spec = m > s ? (lv[m - 1].mis.flags & UPBP_MIS_SPECULAR) != 0 : spec;
+        const float b = ((tech & UPBP_BPT) && !spec && !next_spec) ? 1.f : 0.f;
+        tails[m] = make_float4(t_rest, t_ray, b, below);
+        const float ray_rev = d.ray_sample_rev_pdf_inv;
+        below = d.rev_pdf_a == 0.f ? 0.f : d.rev_pdf_a * ((t_rest + ray_rev * t_ray) + d.pdf_a_inv * (b + below));
+    }
+}
+
+void launch_light_tails(Ctx& ctx, MisDev& mis) {
+    mis.tails = nullptr;
+    if (mis.lv_count <= 0) return;
+    ctx.light_tails.ensure((size_t)mis.lv_count * sizeof(float4));
+    float4* t = ctx.light_tails.as<float4>();
+    hipLaunchKernelGGL(k_light_tails_fill, dim3(grid_for(mis.lv_count)), dim3(256), 0, ctx.stream, t, mis.lv_count);
+    UPBP_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_light_tails, dim3(grid_for(mis.lv_count)), dim3(256), 0, ctx.stream, mis.lv, mis.lv_count,
+                       mis.params, t);
+    UPBP_HIP(hipGetLastError());
+    mis.tails = t;
+}
+
 static void set_mis(EvalArgs& A, const MisDev* mis) {
     if (!mis) return;
+    A.mis.tails = mis->tails;
     A.qmis = mis->queries;
     A.mis.camera = mis->camera;
     A.mis.camera_count = mis->camera_count;

@@ -90,6 +90,7 @@ struct Ctx {
     DevBuf stats;       // kStatSlots x uint64
     DevBuf sort_tmp;    // hipcub temporary storage
     DevBuf qkeys, qkeys2, qperm, qperm2;
+    DevBuf light_tails;  // float4 per light vertex of the current UPBP call
     DevBuf staging[8];  // host-input staging copies
     hipEvent_t ev[6] = {};
     // second stream: the photon and beam kernels of one eval call run concurrently
@@ -217,6 +218,7 @@ struct MisDev {
     int64_t path_count;
     const upbp_beam_mis* beams;
     upbp_mis_params params;
+    const float4* tails = nullptr;  // per light vertex, launch_light_tails
 };
 
 void launch_photons(Ctx& ctx, Mode mode, const PhotonArgs& ph, const QueryView& q, float* out,
@@ -225,6 +227,9 @@ void launch_beams(Ctx& ctx, Mode mode, const BeamArgs& bm, const QueryView& q, f
                   uint32_t* sig, const AdjointOut* adj, const MisDev* mis = nullptr);
 void launch_query_keys(Ctx& ctx, const QueryView& q, const float* lo, const float* hi, uint32_t* keys,
                        uint32_t* idx);
+// Folds each light vertex's light walk below it into a float4 (the O(1) light walk of the
+// UPBP weights) into ctx.light_tails and points mis.tails at it; nothing for no vertices.
+void launch_light_tails(Ctx& ctx, MisDev& mis);
 void launch_clear_u64(Ctx& ctx, uint64_t* p, int64_t n);
 void launch_clear_f32(Ctx& ctx, float* p, int64_t n);
 void launch_combine_media_grads(Ctx& ctx, const float* gm, int n_media, float* gs, float* ga);

@@ -52,6 +52,8 @@ struct MisGlobals {
     int64_t path_count;
     const upbp_beam_mis* __restrict__ beams;
     upbp_mis_params p;
+    // per light vertex, the light walk below it folded into four numbers (LightTail), or null
+    const float4* __restrict__ tails;
 };
 
 // Camera sub-path of one query with the per-hit entry [len] and the override of
@@ -177,6 +179,95 @@ __device__ inline float light_path_weight(const LightView& l, int path_idx, int 
         cur = next;
     }
     return weight;
+}
+
+// ---- light walk in O(1) per hit ------------------------------------------
+// The light walk is linear in its running product: from index 2 on, every term is the
+// product so far times constants of the vertices below. So each light vertex j carries
+// LightTail = (t_rest, t_ray, b, below):
+//   terms of j itself when it sits at walk index >= 1 (usable = not specular):
+//       t_rest + ray_rev(j) * t_ray, ray_rev(j) being its mRaySampleRevPdfInv (LONG query beams;
+//       with SHORT ones the ratio terms are folded into t_rest and t_ray is 0);
+//   b = 1 if the BPT term follows j (BPT enabled, j usable, its next vertex not specular);
+//   below = the walk from j's next vertex downwards per unit product:
+//       tail(m) = rev_pdf_a(m) * (T(m) + pdf_a_inv(m) * (b(m) + tail(m-1))), 0 where rev_pdf_a is 0,
+//       and 0 below the light source; NaN when the light path in memory is not well formed
+//       (path lengths not counting down to 0), where callers fall back to the full walk.
+__device__ __forceinline__ void light_tail_terms(const upbp_mis_data& m, uint32_t tech, uint32_t qbt, uint32_t pbt,
+                                                 float& t_rest, float& t_ray) {
+    const bool usable = !(m.flags & UPBP_MIS_SPECULAR);
+    const bool long_q = (qbt & UPBP_LONG_BEAM) != 0;
+    t_rest = 0.f;
+    t_ray = 0.f;
+    if ((tech & UPBP_SURF) && usable) t_rest += m.surf_factor;
+    if (tech & UPBP_PP3D) t_rest += m.pp3d_factor;
+    if (tech & UPBP_PB2D) {
+        if (long_q)
+            t_ray += m.pb2d_factor;
+        else
+            t_rest += m.pb2d_factor * m.ray_sample_rev_pdfs_ratio;
+    }
+    if ((tech & UPBP_BB1D) && !(tech & UPBP_BB1D_PREVIOUS) && (m.flags & UPBP_MIS_IN_MEDIUM_WITH_BEAMS)) {
+        const float sin_t = (tech & UPBP_NO_SINE_IN_WEIGHTS) ? 1.f : m.sin_theta;
+        const float b = (pbt & UPBP_LONG_BEAM) ? m.ray_sample_pdf_inv : m.ray_sample_pdfs_ratio;
+        const float k = m.bb1d_factor * m.bb1d_beam_selection_pdf * sin_t * b;
+        if (long_q)
+            t_ray += k;
+        else
+            t_rest += k * m.ray_sample_rev_pdfs_ratio;
+    }
+}
+
+// Index of the walk's first light vertex (PathWeight.hxx:167), or -1 for a path index
+// outside the path table.
+__device__ __forceinline__ int64_t light_last_index(const MisGlobals& G, int path_idx, int path_length) {
+    if (path_idx == 0) return path_length;
+    if (path_idx - 1 < 0 || path_idx - 1 >= G.path_count) return -1;
+    return (int64_t)G.path_ends[path_idx - 1] + path_length;
+}
+
+// AccumulateLightPathWeight for a merge at light vertex v (PB2D, PP3D, SURF: index 0 is v
+// itself, index 1 its next vertex with rev = rev_pdf_a_without_bsdf * ntl) through the
+// tails. Returns false where the full walk must run instead.
+__device__ __forceinline__ bool light_weight_tail(const MisGlobals& G, int64_t v, int path_idx, int path_length,
+                                                  float last, float ntl, uint32_t tech, float& w) {
+    const bool has1 = path_length > 0;
+    if (!G.tails || light_last_index(G, path_idx, path_length) != v || v < (has1 ? 1 : 0) || v >= G.lv_count)
+        return false;
+    const upbp_light_vertex* __restrict__ lv = G.lv + v;
+    w = 0.f;
+    if (last == 0.f) return true;  // the walk breaks at index 0
+    const float p0 = last * lv->mis.pdf_a_inv;
+    const upbp_light_vertex* __restrict__ nx = has1 ? lv - 1 : lv;
+    if ((tech & UPBP_BPT) && !(nx->mis.flags & UPBP_MIS_SPECULAR)) w = p0;
+    if ((tech & UPBP_PREVIOUS) || !has1) return true;
+    const float4 t = G.tails[v - 1];
+    if (__builtin_isnan(t.w) || (int)nx->path_length != path_length - 1) return false;
+    const float r1 = nx->mis.rev_pdf_a_without_bsdf * ntl;
+    if (r1 == 0.f) return true;
+    w += p0 * r1 * ((t.x + nx->mis.ray_sample_rev_pdf_inv * t.y) + nx->mis.pdf_a_inv * (t.z + t.w));
+    return true;
+}
+
+// The same for BB1D, whose walk starts at the beam's virtual vertex bv (index 0) above the
+// beam's origin light vertex b; index 1 is b with rev = ntl and its mRaySampleRevPdfInv
+// replaced by ray_rev_b (PhotonBeam.hxx:548-591).
+__device__ __forceinline__ bool light_weight_tail_bb1d(const MisGlobals& G, int64_t b, int path_idx,
+                                                       int path_length_b, float last, float bv_pdf_a_inv, float ntl,
+                                                       float ray_rev_b, uint32_t tech, float& w) {
+    if (!G.tails || light_last_index(G, path_idx, path_length_b + 1) != b + 1 || b < 0 || b >= G.lv_count)
+        return false;
+    const upbp_light_vertex* __restrict__ lv = G.lv + b;
+    w = 0.f;
+    if (last == 0.f) return true;
+    const float p0 = last * bv_pdf_a_inv;
+    if ((tech & UPBP_BPT) && !(lv->mis.flags & UPBP_MIS_SPECULAR)) w = p0;
+    if (tech & UPBP_PREVIOUS) return true;
+    const float4 t = G.tails[b];
+    if (__builtin_isnan(t.w)) return false;
+    if (ntl == 0.f) return true;
+    w += p0 * ntl * ((t.x + ray_rev_b * t.y) + lv->mis.pdf_a_inv * (t.z + t.w));
+    return true;
 }
 
 // Per-lane query state of the UPBP callbacks: AdditionalRayDataForMis of the current segment.

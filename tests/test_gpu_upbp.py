@@ -68,3 +68,29 @@ def test_upbp_host_inputs_and_bad_indices():
     bad.queries["camera_offset"][3] = len(mi.camera)  # past the camera array
     with pytest.raises(native.UpbpError):
         _gpu(w, inm, bad)
+
+
+def test_upbp_long_paths_zero_pdfs_and_malformed_lengths():
+    """Light sub-paths up to 8 segments with zero reverse pdfs (the walks stop there) and a few
+    paths whose stored path lengths do not count up by one (the walks then read the vertices
+    below by position, as the reference does)."""
+    print_case("upbp_edge_paths")
+    w, inm, mi = scenes.upbp_scene(40, 24, 3000, 800, seed=63, techniques=ALL, max_light_length=8)
+    lv = mi.light_vertices.copy()
+    rng = np.random.default_rng(5)
+    m = lv["mis"]
+    z = rng.uniform(size=len(lv))
+    m["rev_pdf_a"][z < 0.08] = 0.0
+    m["rev_pdf_a_without_bsdf"][(z >= 0.08) & (z < 0.16)] = 0.0
+    starts = np.concatenate([[0], mi.path_ends[:-1]]).astype(np.int64)
+    lens = mi.path_ends - starts
+    bad = [p for p in rng.choice(len(starts) - 1, 200, replace=False) if lens[p] >= 4]
+    for p in bad:  # a middle vertex claims one more segment (still inside the array)
+        lv["path_length"][starts[p] + 2] += 1
+    mi = scenes.MisInputs(mi.queries, mi.camera, lv, mi.path_ends, mi.beams, mi.params)
+    gp, gb = _gpu(w, inm, mi)
+    rp, rb = _oracle(w, inm, mi)
+    assert len(bad) > 50 and (rp.max(1) > 0).sum() > 100 and (rb.max(1) > 0).sum() > 50
+    ep, eb = rel_err(gp, rp), rel_err(gb, rb)
+    print("max rel err photons %.2e beams %.2e" % (ep.max(), eb.max()))
+    assert ep.max() <= REL_TOL and eb.max() <= REL_TOL
