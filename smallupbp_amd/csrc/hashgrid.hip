@@ -170,6 +170,7 @@ __global__ void k_query_morton(const Query* __restrict__ q, int64_t n, GridDev g
 struct VmArgs {
     const float4* slots;
     const uint32_t* ends;
+    const float4* rec;  // kRecWords float4 per slot, k_vm_records
     GridDev g;
     int64_t n_sel;
     const uint32_t* perm;
@@ -195,52 +196,93 @@ __device__ __forceinline__ bool camera_range_ok(const VmArgs& A, uint32_t len, i
     return (A.flags & UPBP_VM_NO_MIS) || (len >= 1 && off >= 0 && off + (int64_t)len < A.mis.camera_count);
 }
 
+// Per-slot merge record, gathered once per call in slot (bucket) order so that the
+// vertices a query finds are read as contiguous 64-byte records instead of scattered
+// light-vertex gathers:
+//   [0] WorldDirFix xyz, mPathLength (bits)
+//   [1] mThroughput rgb, continuation probability
+//   [2] light-walk constants k0, k1 (light_split; k1 NaN: run the full walk),
+//       mRaySamplePdfInv, mRaySamplePdfsRatio
+//   [3] merging factor (mPP3DMisWeightFactor or mSurfMisWeightFactor), record flags (bits), -, -
+constexpr int kRecWords = 4;
+constexpr uint32_t kRecConnectable = 1u;  // mConnectable
+constexpr uint32_t kRecBadIndex = 2u;     // the slot's vertex index is past the light vertices
+
+__global__ void k_vm_records(VmArgs A, uint32_t type, float4* __restrict__ rec) {
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= A.n_sel) return;
+    const MisGlobals& G = A.mis;
+    const uint32_t v = __float_as_uint(A.slots[k].w);
+    float4* r = rec + kRecWords * k;
+    if ((int64_t)v >= G.lv_count) {
+        r[0] = make_float4(0.f, 0.f, 0.f, 0.f);
+        r[1] = make_float4(0.f, 0.f, 0.f, 0.f);
+        r[2] = make_float4(0.f, 0.f, 0.f, 0.f);
+        r[3] = make_float4(0.f, __uint_as_float(kRecBadIndex), 0.f, 0.f);
+        return;
+    }
+    const upbp_light_vertex* __restrict__ lv = G.lv + v;
+    float k0 = 0.f, k1 = 0.f;
+    if (!(A.flags & UPBP_VM_NO_MIS) &&
+        !light_split(G, (int64_t)v, lv->path_idx, (int)lv->path_length, G.p.techniques, k0, k1))
+        k1 = __builtin_nanf("");
+    r[0] = make_float4(lv->dir_fix[0], lv->dir_fix[1], lv->dir_fix[2], __uint_as_float(lv->path_length));
+    r[1] = make_float4(lv->throughput[0], lv->throughput[1], lv->throughput[2], lv->continuation_prob);
+    r[2] = make_float4(k0, k1, lv->mis.ray_sample_pdf_inv, lv->mis.ray_sample_pdfs_ratio);
+    const float factor = type == UPBP_PP3D ? lv->mis.pp3d_factor : lv->mis.surf_factor;
+    const uint32_t fl = (lv->flags & UPBP_LV_CONNECTABLE) ? kRecConnectable : 0u;
+    r[3] = make_float4(factor, __uint_as_float(fl), 0.f, 0.f);
+}
+
+// The merge record of slot k if RangeQuery::Process merges with its vertex
+// (UPBP.hxx:89-98): dir_fix and path length in rec[0] on success.
+__device__ __forceinline__ bool vm_take(const VmArgs& A, uint32_t k, uint32_t cam_len, float4& r0, float4& r3,
+                                        uint32_t& err) {
+    r3 = A.rec[kRecWords * (int64_t)k + 3];
+    const uint32_t fl = __float_as_uint(r3.y);
+    if (fl & kRecBadIndex) {
+        err |= kBadLightIndex;
+        return false;
+    }
+    if (!(fl & kRecConnectable)) return false;
+    r0 = A.rec[kRecWords * (int64_t)k];
+    const uint32_t tot = __float_as_uint(r0.w) + cam_len;
+    return !(tot > A.mis.p.max_path_length || tot < A.mis.p.min_path_length);
+}
+
 // The two path-weight walks of RangeQuery::Process (UPBP.hxx:120-127) and the weighted
 // contribution (:128-129). f = camera BSDF value, dir_pdf / rev_pdf already multiplied by
-// the camera / light continuation probabilities.
-__device__ __forceinline__ void vm_accumulate(const VmArgs& A, const upbp_light_vertex* __restrict__ lv,
-                                              uint32_t cam_len, int64_t cam_off, float factor, float sin_t,
-                                              float dir_pdf, float rev_pdf, uint32_t evaluated, float f0, float f1,
-                                              float f2, float acc[3], uint32_t& err) {
-    const MisGlobals& G = A.mis;
-    const upbp_mis_params& P = G.p;
+// the camera / light continuation probabilities. Both walks are linear in their first
+// reverse pdf 1 / factor: the light walk is last * (k0 + k1 * dir_pdf) from the slot's
+// record, the camera walk last * ((a0 + bb0 * s * b) + cn * rev_pdf) from the query's split.
+__device__ __forceinline__ void vm_accumulate(const VmArgs& A, uint32_t k, uint32_t v, const CamSplit& cs,
+                                              float factor, float sin_t, float dir_pdf, float rev_pdf,
+                                              uint32_t evaluated, float f0, float f1, float f2, float acc[3],
+                                              uint32_t& err) {
+    const float4 r1 = A.rec[kRecWords * (int64_t)k + 1];
     float w = 1.f;
     if (!(A.flags & UPBP_VM_NO_MIS)) {
+        const MisGlobals& G = A.mis;
+        const upbp_mis_params& P = G.p;
+        const float4 r2 = A.rec[kRecWords * (int64_t)k + 2];
         const float inv = 1.f / factor;
-        CamView cv;
-        cv.base = G.camera + cam_off;
-        cv.len = (int)cam_len;
-        cv.last = mis_load(cv.base + cam_len);
-        cv.prev_ray_rev_inv = cv.base[cam_len - 1].ray_sample_rev_pdf_inv;
-        const float w_cam = camera_path_weight(cv, inv, sin_t, lv->mis.ray_sample_pdf_inv, lv->mis.ray_sample_pdfs_ratio,
-                                               rev_pdf, P.query_beam_type, P.photon_beam_type, P.techniques);
+        const float s = (P.techniques & UPBP_NO_SINE_IN_WEIGHTS) ? 1.f : sin_t;
+        const float b = (P.photon_beam_type & UPBP_LONG_BEAM) ? r2.z : r2.w;
+        const float w_cam = inv * ((cs.a0 + cs.bb0 * (s * b)) + cs.cn * rev_pdf);
         float w_light;
-        if (!light_weight_tail(G, (int64_t)(lv - G.lv), lv->path_idx, (int)lv->path_length, inv, dir_pdf, P.techniques,
-                               w_light)) {
+        if (!__builtin_isnan(r2.y)) {
+            w_light = inv * (r2.x + r2.y * dir_pdf);
+        } else {
+            const upbp_light_vertex* __restrict__ lv = G.lv + v;
             LightView lview{&G, nullptr, -1, 0.f};
             w_light = light_path_weight(lview, lv->path_idx, (int)lv->path_length, inv, dir_pdf, evaluated,
                                         P.query_beam_type, P.photon_beam_type, P.techniques, err);
         }
         w = 1.f / (w_light + w_cam);
     }
-    acc[0] += w * (f0 * lv->throughput[0]);
-    acc[1] += w * (f1 * lv->throughput[1]);
-    acc[2] += w * (f2 * lv->throughput[2]);
-}
-
-// Light vertex v if RangeQuery::Process merges with it (UPBP.hxx:89-98), else null.
-__device__ __forceinline__ const upbp_light_vertex* vm_light_vertex(const VmArgs& A, uint32_t v, uint32_t cam_len,
-                                                                      uint32_t& err) {
-    const MisGlobals& G = A.mis;
-    if ((int64_t)v >= G.lv_count) {
-        err |= kBadLightIndex;
-        return nullptr;
-    }
-    const upbp_light_vertex* __restrict__ lv = G.lv + v;
-    if (!(lv->flags & UPBP_LV_CONNECTABLE)) return nullptr;
-    const uint32_t tot = lv->path_length + cam_len;
-    if (tot > G.p.max_path_length || tot < G.p.min_path_length) return nullptr;
-    return lv;
+    acc[0] += w * (f0 * r1.x);
+    acc[1] += w * (f1 * r1.y);
+    acc[2] += w * (f2 * r1.z);
 }
 
 // ---- PP3D: camera vertex in a medium -------------------------------------
@@ -271,20 +313,20 @@ __device__ __forceinline__ Pp3dCam camera_of(const VmArgs& A, const upbp_pp3d_qu
 
 // RangeQuery::Process with a medium camera BSDF (Bsdf.hxx:216-230): sigma_s * phase
 // (fix = camera WorldDirFix, gen = light WorldDirFix), zero without scattering.
-__device__ void vm_process(const VmArgs& A, const upbp_pp3d_query& q, const Pp3dCam& c, uint32_t v, float acc[3],
-                           uint32_t& err) {
-    const upbp_light_vertex* __restrict__ lv = vm_light_vertex(A, v, q.camera_path_length, err);
-    if (!lv) return;
+__device__ void vm_process(const VmArgs& A, const upbp_pp3d_query& q, const Pp3dCam& c, const CamSplit& cs,
+                           uint32_t k, uint32_t v, float acc[3], uint32_t& err) {
+    float4 r0, r3;
+    if (!vm_take(A, k, q.camera_path_length, r0, r3, err)) return;
     const MediumDev& m = c.m;
     if (!m.has_scattering) return;
-    const float cos_t = -dot3(lv->dir_fix[0], lv->dir_fix[1], lv->dir_fix[2], q.dir_fix[0], q.dir_fix[1], q.dir_fix[2]);
+    const float cos_t = -dot3(r0.x, r0.y, r0.z, q.dir_fix[0], q.dir_fix[1], q.dir_fix[2]);
     const float sin_t = __builtin_sqrtf(fmaxf(0.f, 1.f - cos_t * cos_t));
     const float pdf = phase_pdf(m.g, m.isotropic != 0, cos_t);
     const float f0 = m.ss[0] * pdf, f1 = m.ss[1] * pdf, f2 = m.ss[2] * pdf;
     if (f0 <= 0.f && f1 <= 0.f && f2 <= 0.f) return;
+    const float cont_l = A.rec[kRecWords * (int64_t)k + 1].w;
     // camera continuation on the direct pdf (UPBP.hxx:110), light continuation on the reverse (:116)
-    vm_accumulate(A, lv, q.camera_path_length, q.camera_offset, lv->mis.pp3d_factor, sin_t, pdf * m.cont,
-                  pdf * lv->continuation_prob, UPBP_PP3D, f0, f1, f2, acc, err);
+    vm_accumulate(A, k, v, cs, r3.x, sin_t, pdf * m.cont, pdf * cont_l, UPBP_PP3D, f0, f1, f2, acc, err);
 }
 
 // ---- SURF: camera vertex on a surface ------------------------------------
@@ -309,11 +351,11 @@ __device__ __forceinline__ SurfCam camera_of(const VmArgs&, const upbp_surf_quer
 // RangeQuery::Process with a surface camera BSDF: Bsdf::Evaluate on a surface
 // (Bsdf.hxx:176-215) = EvaluateDiffuse (:542-561) + EvaluatePhong (:563-600), with the
 // light's WorldDirFix as the generated direction; sin(theta) is 0 on surfaces.
-__device__ void vm_process(const VmArgs& A, const upbp_surf_query& q, const SurfCam&, uint32_t v, float acc[3],
-                           uint32_t& err) {
-    const upbp_light_vertex* __restrict__ lv = vm_light_vertex(A, v, q.camera_path_length, err);
-    if (!lv) return;
-    const float gx = lv->dir_fix[0], gy = lv->dir_fix[1], gz = lv->dir_fix[2];
+__device__ void vm_process(const VmArgs& A, const upbp_surf_query& q, const SurfCam&, const CamSplit& cs, uint32_t k,
+                           uint32_t v, float acc[3], uint32_t& err) {
+    float4 r0, r3;
+    if (!vm_take(A, k, q.camera_path_length, r0, r3, err)) return;
+    const float gx = r0.x, gy = r0.y, gz = r0.z;
     // Frame::ToLocal (Frame.hxx:69-72)
     const float lx = dot3(gx, gy, gz, q.frame_x[0], q.frame_x[1], q.frame_x[2]);
     const float ly = dot3(gx, gy, gz, q.frame_y[0], q.frame_y[1], q.frame_y[2]);
@@ -348,8 +390,9 @@ __device__ void vm_process(const VmArgs& A, const upbp_surf_query& q, const Surf
         }
     }
     if (f0 <= 0.f && f1 <= 0.f && f2 <= 0.f) return;
-    vm_accumulate(A, lv, q.camera_path_length, q.camera_offset, lv->mis.surf_factor, 0.f,
-                  dir_pdf * q.continuation_prob, rev_pdf * lv->continuation_prob, UPBP_SURF, f0, f1, f2, acc, err);
+    const float cont_l = A.rec[kRecWords * (int64_t)k + 1].w;
+    vm_accumulate(A, k, v, cs, r3.x, 0.f, dir_pdf * q.continuation_prob, rev_pdf * cont_l, UPBP_SURF, f0, f1, f2,
+                  acc, err);
 }
 
 // HashGrid::Process (HashGrid.hxx:130-186) over one lane's camera vertex.
@@ -373,6 +416,11 @@ __global__ __launch_bounds__(kBlock) void k_vm_eval(VmArgs A, const Query* __res
         if (ok && A.n_sel > 0 && !(dmin[0] < 0.f) && !(dmax[0] < 0.f) && !(dmin[1] < 0.f) && !(dmax[1] < 0.f) &&
             !(dmin[2] < 0.f) && !(dmax[2] < 0.f)) {
             const auto cam = camera_of(A, q);
+            const CamSplit cs =
+                (SIG || (A.flags & UPBP_VM_NO_MIS))
+                    ? CamSplit{0.f, 0.f, 0.f}
+                    : camera_split(A.mis.camera + q.camera_offset, (int)q.camera_path_length,
+                                   A.mis.p.query_beam_type, A.mis.p.photon_beam_type, A.mis.p.techniques);
             int32_t p[3], po[3];
             for (int a = 0; a < 3; ++a) {
                 const float cp = A.g.inv_cell * dmin[a];
@@ -396,7 +444,7 @@ __global__ __launch_bounds__(kBlock) void k_vm_eval(VmArgs A, const Query* __res
                         if (SIG)
                             sig_h += hash32(v);
                         else
-                            vm_process(A, q, cam, v, acc, err);
+                            vm_process(A, q, cam, cs, k, v, acc, err);
                     }
                 }
             }
@@ -555,6 +603,14 @@ void vm_eval(Ctx& ctx, const HashGrid& grid, const Query* q, int64_t count, cons
     A.mis.tails = mis.tails;
     A.stats = ctx.stats.as<unsigned long long>();
     A.perm = nullptr;
+    A.rec = nullptr;
+    if (out && grid.n > 0) {
+        ctx.vm_rec.ensure((size_t)grid.n * kRecWords * sizeof(float4));
+        hipLaunchKernelGGL(k_vm_records, dim3(blocks_for(grid.n)), dim3(kBlock), 0, ctx.stream, A, grid.type,
+                           ctx.vm_rec.as<float4>());
+        UPBP_HIP(hipGetLastError());
+        A.rec = ctx.vm_rec.as<float4>();
+    }
     if (count > 64 && grid.n > 0) {
         ctx.qkeys.ensure(count * 4);
         ctx.qkeys2.ensure(count * 4);

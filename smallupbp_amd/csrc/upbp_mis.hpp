@@ -270,6 +270,79 @@ __device__ __forceinline__ bool light_weight_tail_bb1d(const MisGlobals& G, int6
     return true;
 }
 
+// The light walk of a merge at light vertex v (PP3D, SURF) split into its two per-vertex
+// constants: AccumulateLightPathWeight(last, ntl) = last * (k0 + k1 * ntl). Returns false
+// where the tails do not apply (the full walk must run).
+__device__ __forceinline__ bool light_split(const MisGlobals& G, int64_t v, int path_idx, int path_length,
+                                            uint32_t tech, float& k0, float& k1) {
+    const bool has1 = path_length > 0;
+    if (!G.tails || light_last_index(G, path_idx, path_length) != v || v < (has1 ? 1 : 0) || v >= G.lv_count)
+        return false;
+    const upbp_light_vertex* __restrict__ lv = G.lv + v;
+    const float p0 = lv->mis.pdf_a_inv;
+    const upbp_light_vertex* __restrict__ nx = has1 ? lv - 1 : lv;
+    k0 = ((tech & UPBP_BPT) && !(nx->mis.flags & UPBP_MIS_SPECULAR)) ? p0 : 0.f;
+    k1 = 0.f;
+    if ((tech & UPBP_PREVIOUS) || !has1) return true;
+    const float4 t = G.tails[v - 1];
+    if (__builtin_isnan(t.w) || (int)nx->path_length != path_length - 1) return false;
+    k1 = p0 * nx->mis.rev_pdf_a_without_bsdf *
+         ((t.x + nx->mis.ray_sample_rev_pdf_inv * t.y) + nx->mis.pdf_a_inv * (t.z + t.w));
+    return !__builtin_isnan(k1);
+}
+
+// The camera walk of a merge (PP3D, SURF: the merged vertex is the camera sub-path's
+// entry [len] itself, no override) split into three per-query constants:
+//   AccumulateCameraPathWeight(last_rev, last_sin, ray_rev, ray_rev_ratio, ntl)
+//     = last_rev * ((a0 + bb0 * s * b) + cn * ntl),
+// s = last_sin (1 with NO_SINE_IN_WEIGHTS), b = ray_rev (LONG photon beams) or ray_rev_ratio.
+// Every term past index 0 carries ntl once (through index 1's reverse pdf); with
+// BB1D_PREVIOUS a non-specular vertex past index 0 drops the index-0 BB1D term.
+struct CamSplit {
+    float a0, bb0, cn;
+};
+
+__device__ inline CamSplit camera_split(const upbp_mis_data* __restrict__ base, int len, uint32_t qbt, uint32_t pbt,
+                                        uint32_t tech) {
+    float lin0 = 0.f, bb0 = 0.f, lin = 0.f, bb = 0.f, product = 1.f;
+    bool drop0 = false;
+    const bool long_q = (qbt & UPBP_LONG_BEAM) != 0;
+    MisF cur = mis_load(base + len);
+    for (int index = 0; index < len; ++index) {
+        const MisF next = index < len - 1 ? mis_load(base + len - index - 1) : cur;
+        const bool cur_spec = (cur.flags & UPBP_MIS_SPECULAR) != 0;
+        if (index > 0 && (tech & UPBP_BB1D_PREVIOUS) && !cur_spec) {
+            bb = 0.f;
+            drop0 = true;
+        }
+        if (index > 0) product *= cur.rev_pdf_a;
+        const bool usable = index == 0 || !cur_spec;
+        const float a = long_q ? cur.ray_sample_pdf_inv : cur.ray_sample_pdfs_ratio;
+        float t = 0.f;
+        if ((tech & UPBP_SURF) && usable) t += product * cur.surf;
+        if (tech & UPBP_PP3D) t += product * cur.pp3d;
+        if (tech & UPBP_PB2D) t += product * cur.pb2d * a;
+        if ((tech & UPBP_BB1D) && (cur.flags & UPBP_MIS_IN_MEDIUM_WITH_BEAMS)) {
+            const float k = product * cur.bb1d * cur.bb1d_sel * a;
+            if (index == 0) {
+                bb0 = k;
+            } else {
+                const float sin_t = (tech & UPBP_NO_SINE_IN_WEIGHTS) ? 1.f : cur.sin_theta;
+                const float b = (pbt & UPBP_LONG_BEAM) ? cur.ray_sample_rev_pdf_inv : cur.ray_sample_rev_pdfs_ratio;
+                bb += k * sin_t * b;
+            }
+        }
+        product *= cur.pdf_a_inv;
+        if ((tech & UPBP_BPT) && usable && !(next.flags & UPBP_MIS_SPECULAR)) t += product;
+        if (index == 0)
+            lin0 += t;
+        else
+            lin += t;
+        cur = next;
+    }
+    return CamSplit{lin0, drop0 ? 0.f : bb0, lin + bb};
+}
+
 // Per-lane query state of the UPBP callbacks: AdditionalRayDataForMis of the current segment.
 struct MisLane {
     const upbp_mis_data* cam_base;

@@ -3,6 +3,8 @@
 Hit sets are compared through per-query signatures (number of Process calls, sum of
 hash32(light vertex)), which must be identical; contributions within 1e-4 relative
 per query of the FP64 oracle."""
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -36,6 +38,8 @@ def _gpu(sc, flags=0, device=False, signatures=False, stats=None):
     return res, n
 
 
+ALL = scenes.PB2D | scenes.BB1D | scenes.BPT | scenes.PP3D | scenes.SURF
+
 CASES = [
     # name, scene kwargs, flags
     ("mis_isotropic", dict(n_paths=3000, n_queries=6000, radius=0.05, cell_count=2999, seed=71), 0),
@@ -46,6 +50,13 @@ CASES = [
     ("many_buckets", dict(n_paths=3000, n_queries=6000, radius=0.04, cell_count=1_000_003, seed=76), 0),
     ("previous", dict(n_paths=3000, n_queries=4000, radius=0.05, cell_count=2999, seed=77,
                       techniques=scenes.PP3D | scenes.BPT | scenes.PB2D | scenes.BB1D | scenes.PREVIOUS), 0),
+    ("bb1d_previous_long_photon_beams",
+     dict(n_paths=3000, n_queries=4000, radius=0.05, cell_count=2999, seed=78, max_camera_length=5,
+          max_light_length=7, techniques=ALL | scenes.BB1D_PREVIOUS, photon_beam_type=scenes.LONG_BEAM), 0),
+    ("no_sine_short_queries",
+     dict(n_paths=3000, n_queries=4000, radius=0.05, cell_count=2999, seed=79, max_camera_length=5,
+          techniques=ALL | scenes.NO_SINE_IN_WEIGHTS, query_beam_type=scenes.SHORT_BEAM,
+          photon_beam_type=scenes.LONG_BEAM), 0),
 ]
 
 
@@ -128,3 +139,29 @@ def test_pp3d_input_errors_are_reported():
     with pytest.raises(native.UpbpError, match="ENONFINITE"):
         bad.build(pos, sc.mis.light_vertices, sc.radius, 10, scenes.PP3D)
     grid.destroy()
+
+
+def test_pp3d_zero_pdfs_and_malformed_lengths():
+    """Zero reverse pdfs (the light walk stops there) and light paths whose stored lengths do
+    not count up by one (the walk then reads the vertices below by position, as the
+    reference does)."""
+    print_case("pp3d_edge_paths")
+    sc = scenes.pp3d_scene(n_paths=3000, n_queries=6000, radius=0.05, cell_count=2999, seed=80, max_light_length=8)
+    lv = sc.mis.light_vertices.copy()
+    rng = np.random.default_rng(6)
+    z = rng.uniform(size=len(lv))
+    lv["mis"]["rev_pdf_a"][z < 0.08] = 0.0
+    lv["mis"]["rev_pdf_a_without_bsdf"][(z >= 0.08) & (z < 0.16)] = 0.0
+    ends = np.asarray(sc.mis.path_ends, np.int64)
+    starts = np.concatenate([[0], ends[:-1]])
+    bad = [p for p in rng.choice(len(starts) - 1, 300, replace=False) if ends[p] - starts[p] >= 4]
+    for p in bad:  # a middle vertex claims one more segment (still inside the array)
+        lv["path_length"][starts[p] + 2] += 1
+    mi = dataclasses.replace(sc.mis, light_vertices=lv)
+    sc = dataclasses.replace(sc, mis=mi)
+    og = po.OracleHashGrid(sc)
+    ref, _, cnt = po.vm_eval(og, sc, precision=64)
+    assert len(bad) > 50 and cnt[0] > 0
+    out, _ = _gpu(sc)
+    err = rel_err(out, ref)
+    assert err.max() <= REL_TOL, err.max()

@@ -48,6 +48,10 @@ CASES = [
     ("one_bucket", dict(n_paths=800, n_queries=2000, radius=0.08, cell_count=1, seed=95), 0),
     ("previous", dict(n_paths=3000, n_queries=4000, radius=0.05, cell_count=2999, seed=96,
                       techniques=scenes.SURF | scenes.BPT | scenes.PP3D | scenes.PREVIOUS), 0),
+    ("bb1d_previous_long_camera_paths",
+     dict(n_paths=3000, n_queries=4000, radius=0.05, cell_count=2999, seed=97, max_camera_length=5,
+          max_light_length=7, techniques=scenes.SURF | scenes.BPT | scenes.PP3D | scenes.PB2D | scenes.BB1D
+          | scenes.BB1D_PREVIOUS, photon_beam_type=scenes.LONG_BEAM), 0),
 ]
 
 
