@@ -242,17 +242,14 @@ __device__ void photon_hit_mis(const EvalArgs& A, const Ray& r, SegState& s, con
     if (u0 <= 0.f && u1 <= 0.f && u2 <= 0.f) return;
     const float dist_sq = t * t;
     const float ray_pdf_inv = 1.0f / ray_pdf;
-    CamView cv;
-    cv.base = ml.cam_base;
-    cv.len = ml.cam_len;
-    cv.last = merge_vertex_mis(P, ml.last_pdf_w_inv * dist_sq * ray_pdf_inv, ray_pdf_inv, ratio,
-                               P.has_bb1d_beams ? 1.f : 0.f, s.mfp > P.bb1d_min_mfp ? UPBP_MIS_IN_MEDIUM_WITH_BEAMS : 0u);
-    cv.prev_ray_rev_inv = 1.0f / ray_rev;
-    const float last = short_q ? (float)(1.0 / (double)(ratio * cv.last.pb2d)) : ray_pdf / cv.last.pb2d;
+    const MisF mv = merge_vertex_mis(P, ml.last_pdf_w_inv * dist_sq * ray_pdf_inv, ray_pdf_inv, ratio,
+                                     P.has_bb1d_beams ? 1.f : 0.f,
+                                     s.mfp > P.bb1d_min_mfp ? UPBP_MIS_IN_MEDIUM_WITH_BEAMS : 0u);
+    const float last = short_q ? (float)(1.0 / (double)(ratio * mv.pb2d)) : ray_pdf / mv.pb2d;
     const uint32_t tech = A.short_query ? (UPBP_SHORT_BEAM | P.techniques) : (UPBP_LONG_BEAM | P.techniques);
-    const MisF lvm = mis_load(&lv->mis);
-    const float w_cam = camera_path_weight(cv, last, sin_t, lvm.ray_sample_pdf_inv, lvm.ray_sample_pdfs_ratio,
-                                           rev_pdf * ray_rev / dist_sq, P.query_beam_type, P.photon_beam_type, tech);
+    const float w_cam = camera_weight_tail(ml.tail, mv, last, sin_t, lv->mis.ray_sample_pdf_inv,
+                                           lv->mis.ray_sample_pdfs_ratio, rev_pdf * ray_rev / dist_sq, 1.0f / ray_rev,
+                                           P.query_beam_type, P.photon_beam_type, tech);
     float w_light;
     if (!light_weight_tail(G, (int64_t)lv_idx, lv->path_idx, (int)path_len, last, dir_pdf, tech, w_light)) {
         LightView lview{&G, nullptr, -1, 0.f};
@@ -327,12 +324,8 @@ __device__ void beam_hit_mis(const EvalArgs& A, const Ray& r, SegState& s, const
     if (u0 <= 0.f && u1 <= 0.f && u2 <= 0.f) return;
     const float dist_sq_q = t1 * t1;
     const float rsp_inv_q = 1.0f / rsp_q;
-    CamView cv;
-    cv.base = ml.cam_base;
-    cv.len = ml.cam_len;
-    cv.last = merge_vertex_mis(P, ml.last_pdf_w_inv * dist_sq_q * rsp_inv_q, rsp_inv_q, ratio_q, sel_pdf,
-                               UPBP_MIS_IN_MEDIUM_WITH_BEAMS);
-    cv.prev_ray_rev_inv = 1.0f / rsr_q;
+    const MisF mv = merge_vertex_mis(P, ml.last_pdf_w_inv * dist_sq_q * rsp_inv_q, rsp_inv_q, ratio_q, sel_pdf,
+                                     UPBP_MIS_IN_MEDIUM_WITH_BEAMS);
     const float dist_sq_b = lv->is_finite ? t2 * t2 : 1.0f;
     const float rsp_inv_b = 1.0f / rsp_b;
     const MisF bv = merge_vertex_mis(P, bm.last_pdf_w_inv * dist_sq_b * rsp_inv_b, rsp_inv_b, ratio_b, sel_pdf,
@@ -341,13 +334,13 @@ __device__ void beam_hit_mis(const EvalArgs& A, const Ray& r, SegState& s, const
     const float sin_t = (tech & UPBP_NO_SINE_IN_WEIGHTS) ? 1.0f : sin_theta;
     float last;
     if (short_q)
-        last = short_b ? (float)(1.0 / (double)(ratio_q * ratio_b * cv.last.bb1d * sel_pdf * sin_t))
-                       : rsp_b / (ratio_q * cv.last.bb1d * sel_pdf * sin_t);
+        last = short_b ? (float)(1.0 / (double)(ratio_q * ratio_b * mv.bb1d * sel_pdf * sin_t))
+                       : rsp_b / (ratio_q * mv.bb1d * sel_pdf * sin_t);
     else
-        last = short_b ? rsp_q / (ratio_b * cv.last.bb1d * sel_pdf * sin_t)
-                       : rsp_q * rsp_b / (cv.last.bb1d * sel_pdf * sin_t);
-    const float w_cam = camera_path_weight(cv, last, sin_t, rsp_inv_b, ratio_b, rev_pdf * rsr_q / dist_sq_q,
-                                           P.query_beam_type, P.photon_beam_type, tech);
+        last = short_b ? rsp_q / (ratio_b * mv.bb1d * sel_pdf * sin_t)
+                       : rsp_q * rsp_b / (mv.bb1d * sel_pdf * sin_t);
+    const float w_cam = camera_weight_tail(ml.tail, mv, last, sin_t, rsp_inv_b, ratio_b, rev_pdf * rsr_q / dist_sq_q,
+                                           1.0f / rsr_q, P.query_beam_type, P.photon_beam_type, tech);
     const uint32_t lvf = lv->mis.flags;
     const bool on_light_delta = (lvf & UPBP_MIS_ON_LIGHT_SOURCE) && (lvf & UPBP_MIS_DELTA);
     const float ntl = on_light_delta ? 0.f : dir_pdf * rsr_b * __builtin_fabsf(lv->mis.cos_theta_out) / dist_sq_b;
@@ -655,7 +648,7 @@ __device__ void run_tile(const EvalArgs& A, int64_t slot0, unsigned long long* l
     LaneOut<MODE> lo;
     if (MODE == (int)Mode::Upbp) {
         // per-query AdditionalRayDataForMis (Bre.cxx:842-849 runs it per segment below)
-        lo.ml.cam_base = A.mis.camera;
+        lo.ml.tail = CamTail{0.f, 0.f, 0u};
         lo.ml.cam_len = 1;
         lo.ml.ray_flags = 0;
         lo.ml.last_pdf_w_inv = 1.f;
@@ -670,8 +663,9 @@ __device__ void run_tile(const EvalArgs& A, int64_t slot0, unsigned long long* l
                 alive = false;
                 nseg = 0;
             } else {
-                lo.ml.cam_base = A.mis.camera + qm.camera_offset;
                 lo.ml.cam_len = (int)qm.camera_path_length;
+                lo.ml.tail = camera_tail(A.mis.camera + qm.camera_offset, lo.ml.cam_len, A.mis.p.query_beam_type,
+                                         A.mis.p.photon_beam_type, A.mis.p.techniques);
                 lo.ml.ray_flags = qm.ray_sampling_flags;
                 lo.ml.last_pdf_w_inv = qm.last_pdf_w_inv;
             }

@@ -8,6 +8,12 @@
 // data into the caller's camera array and temporarily into the beam's origin light
 // vertex; here those values are register overrides of the walks, so no input is
 // ever written and concurrent lanes never race.
+//
+// Both walks are linear in their running product, so only their first one or two
+// vertices depend on the hit. The camera walk is split once per query (camera_tail,
+// camera_split) and the light walk once per light vertex (LightTail, light_split);
+// each hit then costs O(1). light_path_weight, the full light walk, remains for light
+// paths whose stored lengths do not count up by one.
 #pragma once
 
 #include "upbp_device.hpp"
@@ -55,62 +61,6 @@ struct MisGlobals {
     // per light vertex, the light walk below it folded into four numbers (LightTail), or null
     const float4* __restrict__ tails;
 };
-
-// Camera sub-path of one query with the per-hit entry [len] and the override of
-// entry [len-1].mRaySampleRevPdfInv (Bre.cxx:396-422, PhotonBeam.hxx:502-521).
-struct CamView {
-    const upbp_mis_data* base;
-    int len;
-    MisF last;
-    float prev_ray_rev_inv;
-};
-
-__device__ __forceinline__ MisF cam_get(const CamView& c, int i) {
-    if (i == c.len) return c.last;
-    MisF m = mis_load(c.base + i);
-    if (i == c.len - 1) m.ray_sample_rev_pdf_inv = c.prev_ray_rev_inv;
-    return m;
-}
-
-// AccumulateCameraPathWeight, PathWeight.hxx:33-142.
-__device__ inline float camera_path_weight(const CamView& cam, float last_rev, float last_sin, float last_ray_rev_inv,
-                                           float last_ray_rev_ratio, float next_to_last_partial, uint32_t qbt,
-                                           uint32_t pbt, uint32_t tech) {
-    float weight = 0.f, product = 1.f, weight_bb1d = 0.f;
-    const int len = cam.len;
-    MisF cur = cam_get(cam, len);
-    for (int index = 0; index < len; ++index) {
-        const MisF next = index < len - 1 ? cam_get(cam, len - index - 1) : cur;
-        const bool cur_spec = (cur.flags & UPBP_MIS_SPECULAR) != 0;
-        if ((tech & UPBP_BB1D_PREVIOUS) && !cur_spec) weight_bb1d = 0.f;
-        float rev = cur.rev_pdf_a, sin_t = cur.sin_theta, ray_rev = cur.ray_sample_rev_pdf_inv,
-              ray_rev_ratio = cur.ray_sample_rev_pdfs_ratio;
-        if (index == 0) {
-            rev = last_rev;
-            sin_t = last_sin;
-            ray_rev = last_ray_rev_inv;
-            ray_rev_ratio = last_ray_rev_ratio;
-        } else if (index == 1) {
-            rev *= next_to_last_partial;
-        }
-        product *= rev;
-        const bool usable = index == 0 || !cur_spec;
-        if ((tech & UPBP_SURF) && usable) weight += product * cur.surf;
-        if (tech & UPBP_PP3D) weight += product * cur.pp3d;
-        if (tech & UPBP_PB2D)
-            weight += product * cur.pb2d * ((qbt & UPBP_LONG_BEAM) ? cur.ray_sample_pdf_inv : cur.ray_sample_pdfs_ratio);
-        if ((tech & UPBP_BB1D) && (cur.flags & UPBP_MIS_IN_MEDIUM_WITH_BEAMS)) {
-            if (tech & UPBP_NO_SINE_IN_WEIGHTS) sin_t = 1.f;
-            const float a = (qbt & UPBP_LONG_BEAM) ? cur.ray_sample_pdf_inv : cur.ray_sample_pdfs_ratio;
-            const float b = (pbt & UPBP_LONG_BEAM) ? ray_rev : ray_rev_ratio;
-            weight_bb1d += product * cur.bb1d * cur.bb1d_sel * a * sin_t * b;
-        }
-        product *= cur.pdf_a_inv;
-        if ((tech & UPBP_BPT) && usable && !(next.flags & UPBP_MIS_SPECULAR)) weight += product;
-        cur = next;
-    }
-    return weight + weight_bb1d;
-}
 
 struct LightView {
     const MisGlobals* g;
@@ -343,9 +293,82 @@ __device__ inline CamSplit camera_split(const upbp_mis_data* __restrict__ base, 
     return CamSplit{lin0, drop0 ? 0.f : bb0, lin + bb};
 }
 
+// The camera walk of a PB2D / BB1D hit (AccumulateCameraPathWeight with the hit's merge
+// vertex as entry [len] and the override of entry [len-1].mRaySampleRevPdfInv) split at
+// index 1: everything from index 1 on is a per-query constant times ntl, except index 1's
+// BB1D term with LONG photon beams, which is linear in the override.
+//   weight = last_rev * (t0 + pdf0 * (b0 + ntl * (c + co * override)))
+// with t0, pdf0 the merge vertex's index-0 terms and forward pdf.
+struct CamTail {
+    float c, co;
+    uint32_t fl;  // kCamB0: the BPT term after index 0; kCamDrop0: index 0's BB1D term is dropped
+};
+constexpr uint32_t kCamB0 = 1u;
+constexpr uint32_t kCamDrop0 = 2u;
+
+__device__ inline CamTail camera_tail(const upbp_mis_data* __restrict__ base, int len, uint32_t qbt, uint32_t pbt,
+                                      uint32_t tech) {
+    CamTail ct = {0.f, 0.f, 0u};
+    // index 0's next: entry [len-1], or the merge vertex itself (never specular) for len 1
+    if ((tech & UPBP_BPT) && (len < 2 || !(base[len - 1].flags & UPBP_MIS_SPECULAR))) ct.fl |= kCamB0;
+    if (len < 2) return ct;
+    const bool long_q = (qbt & UPBP_LONG_BEAM) != 0;
+    const bool long_p = (pbt & UPBP_LONG_BEAM) != 0;
+    float lin = 0.f, bb = 0.f, bb_over = 0.f, product = 1.f;
+    MisF cur = mis_load(base + len - 1);
+    for (int index = 1; index < len; ++index) {
+        const MisF next = index < len - 1 ? mis_load(base + len - index - 1) : cur;
+        const bool cur_spec = (cur.flags & UPBP_MIS_SPECULAR) != 0;
+        if ((tech & UPBP_BB1D_PREVIOUS) && !cur_spec) {
+            bb = 0.f;
+            bb_over = 0.f;
+            ct.fl |= kCamDrop0;
+        }
+        product *= cur.rev_pdf_a;
+        const bool usable = !cur_spec;
+        const float a = long_q ? cur.ray_sample_pdf_inv : cur.ray_sample_pdfs_ratio;
+        if ((tech & UPBP_SURF) && usable) lin += product * cur.surf;
+        if (tech & UPBP_PP3D) lin += product * cur.pp3d;
+        if (tech & UPBP_PB2D) lin += product * cur.pb2d * a;
+        if ((tech & UPBP_BB1D) && (cur.flags & UPBP_MIS_IN_MEDIUM_WITH_BEAMS)) {
+            const float sin_t = (tech & UPBP_NO_SINE_IN_WEIGHTS) ? 1.f : cur.sin_theta;
+            const float k = product * cur.bb1d * cur.bb1d_sel * a * sin_t;
+            if (index == 1 && long_p)
+                bb_over += k;
+            else
+                bb += k * (long_p ? cur.ray_sample_rev_pdf_inv : cur.ray_sample_rev_pdfs_ratio);
+        }
+        product *= cur.pdf_a_inv;
+        if ((tech & UPBP_BPT) && usable && !(next.flags & UPBP_MIS_SPECULAR)) lin += product;
+        cur = next;
+    }
+    ct.c = lin + bb;
+    ct.co = bb_over;
+    return ct;
+}
+
+// camera_path_weight through a CamTail: last = the merge vertex (index 0).
+__device__ __forceinline__ float camera_weight_tail(const CamTail& ct, const MisF& last, float last_rev,
+                                                    float last_sin, float last_ray_rev_inv, float last_ray_rev_ratio,
+                                                    float ntl, float override_ray_rev_inv, uint32_t qbt, uint32_t pbt,
+                                                    uint32_t tech) {
+    const float a0 = (qbt & UPBP_LONG_BEAM) ? last.ray_sample_pdf_inv : last.ray_sample_pdfs_ratio;
+    float t0 = 0.f;
+    if (tech & UPBP_SURF) t0 += last.surf;
+    if (tech & UPBP_PP3D) t0 += last.pp3d;
+    if (tech & UPBP_PB2D) t0 += last.pb2d * a0;
+    if ((tech & UPBP_BB1D) && (last.flags & UPBP_MIS_IN_MEDIUM_WITH_BEAMS) && !(ct.fl & kCamDrop0)) {
+        const float sin_t = (tech & UPBP_NO_SINE_IN_WEIGHTS) ? 1.f : last_sin;
+        const float b = (pbt & UPBP_LONG_BEAM) ? last_ray_rev_inv : last_ray_rev_ratio;
+        t0 += last.bb1d * last.bb1d_sel * a0 * sin_t * b;
+    }
+    const float b0 = (ct.fl & kCamB0) ? 1.f : 0.f;
+    return last_rev * (t0 + last.pdf_a_inv * (b0 + ntl * (ct.c + ct.co * override_ray_rev_inv)));
+}
+
 // Per-lane query state of the UPBP callbacks: AdditionalRayDataForMis of the current segment.
 struct MisLane {
-    const upbp_mis_data* cam_base;
+    CamTail tail;  // camera_tail of the query's camera sub-path
     int cam_len;
     uint32_t ray_flags;    // raySamplingFlags of evalBre (first segment only)
     float last_pdf_w_inv;

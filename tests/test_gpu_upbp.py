@@ -94,3 +94,18 @@ def test_upbp_long_paths_zero_pdfs_and_malformed_lengths():
     ep, eb = rel_err(gp, rp), rel_err(gb, rb)
     print("max rel err photons %.2e beams %.2e" % (ep.max(), eb.max()))
     assert ep.max() <= REL_TOL and eb.max() <= REL_TOL
+
+
+@pytest.mark.parametrize("pbt", [scenes.LONG_BEAM, scenes.SHORT_BEAM])
+def test_upbp_long_camera_paths_bb1d_previous(pbt):
+    """Camera sub-paths of up to 6 vertices (the per-query camera split past index 1) with
+    BB1D_PREVIOUS, whose resets drop earlier BB1D terms."""
+    print_case("upbp_long_camera_paths", pbt=pbt)
+    w, inm, mi = scenes.upbp_scene(40, 24, 3000, 800, seed=64, techniques=ALL | scenes.BB1D_PREVIOUS,
+                                   query_beam_type=scenes.LONG_BEAM, photon_beam_type=pbt, max_camera_length=6)
+    gp, gb = _gpu(w, inm, mi)
+    rp, rb = _oracle(w, inm, mi)
+    assert (rp.max(1) > 0).sum() > 100 and (rb.max(1) > 0).sum() > 50
+    ep, eb = rel_err(gp, rp), rel_err(gb, rb)
+    print("max rel err photons %.2e beams %.2e" % (ep.max(), eb.max()))
+    assert ep.max() <= REL_TOL and eb.max() <= REL_TOL
