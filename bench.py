@@ -13,7 +13,8 @@ Prints ONE JSON line: metric, value (Mqueries/s), roofline of the dominant
 kernel (algorithmic bytes / average HIP-event kernel time vs 8 TB/s), the CPU
 oracle timed on a bounded sample on the host cores, the forward+adjoint vs
 forward cost ratio on the 512x512 differentiable workload, the 64-instance
-batched speedup, and the PP3D and SURF vertex-merging rates.
+batched speedup, the MIS-weighted (UPBP) rate at 1920x1080, and the PP3D and
+SURF vertex-merging rates.
 """
 from __future__ import annotations
 
@@ -277,6 +278,92 @@ def vm_measure(technique: int, steps: int, cpu_budget: float, threads: int):
     return rec
 
 
+def upbp_measure(steps: int, cpu_budget: float, threads: int):
+    """The same estimators with UPBP MIS weights (upbp_eval_upbp) at 1920x1080 LONG query beams:
+    400,000 light paths of 1-4 segments (1.0M in-medium light vertices, the photons) and
+    200,000 SHORT photon beams leaving random light vertices, photon radius 0.005, beam radius
+    0.003; camera sub-paths of 1-3 vertices per query with random MisData. Inputs resident in
+    HBM; one step = one UPBP evaluation of both estimators. The plain (unweighted) evaluation
+    of the same structures is timed beside it, and the oracle's UPBP evaluation on a query
+    sample is the CPU comparison."""
+    import torch
+
+    from smallupbp_amd import native, scenes
+
+    t = time.time()
+    w, inm, mi = scenes.upbp_scene(1920, 1080, 400_000, 200_000, seed=81, photon_radius=0.005, beam_radius=0.003)
+    log("upbp workload: %d light vertices (%d photons), %d beams, %d queries, generated in %.1fs"
+        % (len(mi.light_vertices), int(inm.sum()), w.beams.count, w.queries.count, time.time() - t))
+    ctx = native.Context(0)
+    ctx.set_media(w.media)
+    bre = native.EmbreeBre(ctx)
+    bre.build(w.photons, in_medium=inm)
+    bm = native.PhotonBeamsEvaluator(ctx)
+    bm.build(w.beams)
+    qv = native.QueryBatchView.to_device(w.queries)
+    ms = native.device_mis(mi)
+    Q = w.queries.count
+    op = torch.empty((Q, 3), device="cuda")
+    ob = torch.empty((Q, 3), device="cuda")
+
+    def run(upbp, st=None):
+        if upbp:
+            native.eval_upbp(ctx, bre, bm, qv, ms, stats=st, out_photons=op, out_beams=ob)
+        else:
+            native.eval_batch(ctx, bre, bm, qv, op, ob, stats=st)
+
+    res = {}
+    for name in ("plain", "upbp"):
+        for _ in range(2):
+            run(name == "upbp")
+        torch.cuda.synchronize()
+        per = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st = {}
+            run(name == "upbp", st)
+            per.append(st)
+        torch.cuda.synchronize()
+        res[name] = ((time.perf_counter() - t0) / steps, per)
+    el, per = res["upbp"]
+    rec = {"workload": "upbp_1080p (%d LONG queries x %d photon light vertices + %d SHORT beams, MIS on)"
+                       % (Q, int(inm.sum()), w.beams.count),
+           "mqueries_per_s": round(Q / el / 1e6, 3), "ms_per_step": round(el * 1e3, 4),
+           "kernel_ms": {"photons": round(float(np.mean([p["kernel_ms_photons"] for p in per])), 4),
+                         "beams": round(float(np.mean([p["kernel_ms_beams"] for p in per])), 4)},
+           "hits": {"photons": int(per[-1]["photon_hits"]), "beams": int(per[-1]["beam_hits"])},
+           "plain_ms_per_step": round(res["plain"][0] * 1e3, 4),
+           "mis_cost": round(el / res["plain"][0], 3)}
+    if cpu_budget > 0:
+        from oracle import pyoracle as po
+
+        t = time.time()
+        oph = po.OraclePhotons(w.photons, in_medium=inm)
+        obm = po.OracleBeams(w.beams, grid_res=256)
+        build_s = time.time() - t
+        rng = np.random.default_rng(9)
+
+        def cpu(idx):
+            qs = w.queries.subset(idx)
+            sub = scenes.MisInputs(mi.queries[idx], mi.camera, mi.light_vertices, mi.path_ends, mi.beams, mi.params)
+            t = time.time()
+            po.upbp_eval(oph, None, w.media, qs, sub, 0, threads=threads, accel=1)
+            po.upbp_eval(None, obm, w.media, qs, sub, 1, threads=threads, accel=1)
+            return time.time() - t
+
+        probe = np.sort(rng.choice(Q, 2000, replace=False))
+        per_q = cpu(probe) / len(probe)
+        n = int(min(Q, max(2000, cpu_budget / max(per_q, 1e-9))))
+        idx = np.sort(rng.choice(Q, n, replace=False))
+        el_cpu = cpu(idx)
+        rec["cpu_baseline"] = {"value": round(n / el_cpu / 1e6, 5), "unit": "Mqueries/s", "cores": threads,
+                               "kind": "port", "sample": "%d random queries of the same workload through the oracle's "
+                               "UPBP evaluation (BVH photons, reference beam grid); %.1f s on %d threads, structures "
+                               "built in %.1f s" % (n, el_cpu, threads, build_s)}
+        rec["gpu_vs_cpu"] = round(rec["mqueries_per_s"] / rec["cpu_baseline"]["value"], 1)
+    return rec
+
+
 def calibrate_copy(nbytes: int = 1 << 30):
     """A plain device copy of known size, so PMC runs can scale FETCH_SIZE / WRITE_SIZE."""
     import torch
@@ -306,6 +393,7 @@ def main():
     ap.add_argument("--batched", type=float, default=0.25,
                     help="also time the 64-instance batched form at this size scale (1 = full size, 0 = skip)")
     ap.add_argument("--no-vm", action="store_true", help="skip the PP3D and SURF vertex-merging measurements")
+    ap.add_argument("--no-upbp", action="store_true", help="skip the MIS-weighted (UPBP) measurement")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -436,6 +524,9 @@ def main():
 
     if args.batched > 0:
         rec["batched"] = batched_ratio(args.batched, max(2, args.steps // 2))
+
+    if not args.no_upbp:
+        rec["upbp"] = upbp_measure(max(3, args.steps // 2), min(args.cpu_budget, 5.0), host_cores())
 
     if not args.no_vm:
         from smallupbp_amd import scenes as _sc

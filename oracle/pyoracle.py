@@ -64,7 +64,7 @@ def lib():
                                  vp, vp]
         L.or_upbp_eval.restype = C.c_int
         L.or_upbp_eval.argtypes = [vp, vp, vp, i32, C.POINTER(_abi.QueryBatch), C.POINTER(_abi.MisInputs), C.c_int,
-                                   C.c_int, C.c_int, vp, vp, vp]
+                                   C.c_int, C.c_int, C.c_int, vp, vp, vp]
         L.or_exp_cephes.restype = f32
         L.or_hgrid_create.restype = vp
         L.or_hgrid_create.argtypes = [i64, vp, vp, f32, i64, C.c_uint32]
@@ -213,8 +213,9 @@ def adjoint(oph, obm, media, q: Queries, w_photons, w_beams, threads=None):
     return {"grad_sigma_s": gs, "grad_sigma_a": ga, "grad_photon_power": gp, "grad_beam_power": gb}
 
 
-def upbp_eval(oph, obm, media, q: Queries, mi, which: int, precision=32, threads=None):
-    """UPBP-mode estimate (MIS weights) per query; which = 0 photons, 1 beams. Returns (out, hits)."""
+def upbp_eval(oph, obm, media, q: Queries, mi, which: int, precision=32, threads=None, accel=0):
+    """UPBP-mode estimate (MIS weights) per query; which = 0 photons, 1 beams. accel 0 = brute force
+    over items (exact hit sets), 1 = photon BVH / reference beam grid. Returns (out, hits)."""
     threads = threads or _threads()
     qb = query_batch(q)
     m, mp, nm = _media_ptr(media)
@@ -222,7 +223,7 @@ def upbp_eval(oph, obm, media, q: Queries, mi, which: int, precision=32, threads
     out = np.zeros((q.count, 3), np.float64 if precision == 64 else np.float32)
     cnt = np.zeros(2, np.uint64)
     r = lib().or_upbp_eval(oph.h if oph else None, obm.h if obm else None, mp, nm, C.byref(qb), C.byref(ms), which,
-                           precision, threads, out.ctypes.data if precision != 64 else None,
+                           accel, precision, threads, out.ctypes.data if precision != 64 else None,
                            out.ctypes.data if precision == 64 else None, cnt.ctypes.data)
     if r:
         raise RuntimeError("or_upbp_eval failed: %s" % _abi.STATUS_NAMES.get(r, r))

@@ -1056,8 +1056,8 @@ int or_adjoint(const or_photons* ph, const or_beams* bm, const upbp_medium* medi
 }
 
 int or_upbp_eval(const or_photons* ph, const or_beams* bm, const upbp_medium* media, int32_t n_media,
-                 const upbp_query_batch* q, const upbp_mis_inputs* mis, int which, int precision, int threads,
-                 float* out32, double* out64, uint64_t* counters) {
+                 const upbp_query_batch* q, const upbp_mis_inputs* mis, int which, int accel, int precision,
+                 int threads, float* out32, double* out64, uint64_t* counters) {
     int e = check_batch(q, n_media);
     if (e) return e;
     if (!mis || (which == 0 && !ph) || (which == 1 && (!bm || !mis->beams))) return UPBP_EINVAL;
@@ -1066,9 +1066,9 @@ int or_upbp_eval(const or_photons* ph, const or_beams* bm, const upbp_medium* me
     if (threads <= 0) threads = 1;
     int r;
     if (precision == 64)
-        r = upbp_eval_all_f64(ph, bm, m, n_media, q, mis, which, threads, out64, counters);
+        r = upbp_eval_all_f64(ph, bm, m, n_media, q, mis, which, accel, threads, out64, counters);
     else
-        r = upbp_eval_all_f32(ph, bm, m, n_media, q, mis, which, threads, out32, counters);
+        r = upbp_eval_all_f32(ph, bm, m, n_media, q, mis, which, accel, threads, out32, counters);
     free(m);
     return r;
 }

@@ -115,11 +115,12 @@ int or_beams_eval(const or_beams* bm, const upbp_medium* media, int32_t n_media,
 /* evalBre / evalBeamBeamEstimate with MIS data: breIntersectFuncHomogeneous2
  * (Bre.cxx:293-464), PhotonBeam::accumulate2 (PhotonBeam.hxx:386-607),
  * AccumulateCameraPathWeight / AccumulateLightPathWeight (PathWeight.hxx:33-270),
- * exact path walks, brute force over items. which: 0 photons (photon i = light
- * vertex orig[i]), 1 beams. counters[0] = hits. */
+ * exact path walks. which: 0 photons (photon i = light vertex orig[i]), 1 beams.
+ * accel 0: brute force over items; accel 1: photons through the BVH, beams through the
+ * reference's uniform grid (the timed CPU comparison). counters[0] = hits. */
 int or_upbp_eval(const or_photons* ph, const or_beams* bm, const upbp_medium* media, int32_t n_media,
-                 const upbp_query_batch* q, const upbp_mis_inputs* mis, int which, int precision, int threads,
-                 float* out32, double* out64, uint64_t* counters);
+                 const upbp_query_batch* q, const upbp_mis_inputs* mis, int which, int accel, int precision,
+                 int threads, float* out32, double* out64, uint64_t* counters);
 
 /* ---- adjoint (FP64 analytic gradient of the plain estimators) ----------- */
 /* Gradient of L = sum_q <w_photons[q], out_photons[q]> + <w_beams[q], out_beams[q]>

@@ -212,17 +212,21 @@ __device__ void photon_hit_mis(const EvalArgs& A, const Ray& r, SegState& s, con
     const uint32_t cam_len = (uint32_t)ml.cam_len;
     if ((path_len + cam_len > P.max_path_length) || (path_len + cam_len < P.min_path_length)) return;
     if (cam_len == 1 && t < P.min_dist_to_med) return;
+    // The weights and contributions below need 1e-4 relative accuracy, not the reference's
+    // rounding (hit decisions are all made before this point), so reciprocals and square
+    // roots are the single hardware instructions.
     const float dt = t - s.tmin;
     float a0 = fast_exp(-s.st0 * dt), a1 = fast_exp(-s.st1 * dt), a2 = fast_exp(-s.st2 * dt);
     const float pdf = pick3(s.kidx, a0, a1, a2);
     if (short_q) {
-        a0 = a0 / pdf;
-        a1 = a1 / pdf;
-        a2 = a2 / pdf;
+        const float ip = __builtin_amdgcn_rcpf(pdf);
+        a0 *= ip;
+        a1 *= ip;
+        a2 *= ip;
     }
     float ray_pdf = s.min_comp * pdf;
     float ray_rev = (ml.seg_flags & UPBP_ORIGIN_IN_MEDIUM) ? ray_pdf : pdf;
-    const float ratio = 1.0f / s.min_comp;
+    const float ratio = __builtin_amdgcn_rcpf(s.min_comp);
     if (!(a0 > 1e-20f && a1 > 1e-20f && a2 > 1e-20f)) return;
     ray_pdf *= ml.pref_pdf;
     ray_rev *= ml.pref_rev;
@@ -230,33 +234,35 @@ __device__ void photon_hit_mis(const EvalArgs& A, const Ray& r, SegState& s, con
     const float lx = lv->dir_fix[0], ly = lv->dir_fix[1], lz = lv->dir_fix[2];
     const float cos_t = -dot3(lx, ly, lz, -r.dx, -r.dy, -r.dz);
     const float bsdf = phase_pdf(s.g, s.iso != 0, cos_t);
-    const float sin_t = __builtin_sqrtf(fmaxf(0.f, 1.f - cos_t * cos_t));
+    const float sin_t = __builtin_amdgcn_sqrtf(fmaxf(0.f, 1.f - cos_t * cos_t));
     if (bsdf <= 0.f) return;
     const float dir_pdf = bsdf * s.cont;
     const float rev_pdf = bsdf * lv->continuation_prob;
-    const float kernel = (1.0f - sq / r2) / ((r2 * kPi) * 0.5f);
+    const float kernel = (1.0f - sq * __builtin_amdgcn_rcpf(r2)) * __builtin_amdgcn_rcpf((r2 * kPi) * 0.5f);
     if (!(kernel > 1e-20f)) return;
     const float u0 = (((lv->throughput[0] * a0) * s.ss0) * bsdf) * kernel;
     const float u1 = (((lv->throughput[1] * a1) * s.ss1) * bsdf) * kernel;
     const float u2 = (((lv->throughput[2] * a2) * s.ss2) * bsdf) * kernel;
     if (u0 <= 0.f && u1 <= 0.f && u2 <= 0.f) return;
     const float dist_sq = t * t;
-    const float ray_pdf_inv = 1.0f / ray_pdf;
+    const float ray_pdf_inv = __builtin_amdgcn_rcpf(ray_pdf);
     const MisF mv = merge_vertex_mis(P, ml.last_pdf_w_inv * dist_sq * ray_pdf_inv, ray_pdf_inv, ratio,
                                      P.has_bb1d_beams ? 1.f : 0.f,
                                      s.mfp > P.bb1d_min_mfp ? UPBP_MIS_IN_MEDIUM_WITH_BEAMS : 0u);
-    const float last = short_q ? (float)(1.0 / (double)(ratio * mv.pb2d)) : ray_pdf / mv.pb2d;
+    const float last = short_q ? __builtin_amdgcn_rcpf(ratio * mv.pb2d) : ray_pdf * __builtin_amdgcn_rcpf(mv.pb2d);
     const uint32_t tech = A.short_query ? (UPBP_SHORT_BEAM | P.techniques) : (UPBP_LONG_BEAM | P.techniques);
     const float w_cam = camera_weight_tail(ml.tail, mv, last, sin_t, lv->mis.ray_sample_pdf_inv,
-                                           lv->mis.ray_sample_pdfs_ratio, rev_pdf * ray_rev / dist_sq, 1.0f / ray_rev,
-                                           P.query_beam_type, P.photon_beam_type, tech);
+                                           lv->mis.ray_sample_pdfs_ratio,
+                                           rev_pdf * ray_rev * __builtin_amdgcn_rcpf(dist_sq),
+                                           __builtin_amdgcn_rcpf(ray_rev), P.query_beam_type, P.photon_beam_type,
+                                           tech);
     float w_light;
     if (!light_weight_tail(G, (int64_t)lv_idx, lv->path_idx, (int)path_len, last, dir_pdf, tech, w_light)) {
         LightView lview{&G, nullptr, -1, 0.f};
         w_light = light_path_weight(lview, lv->path_idx, (int)path_len, last, dir_pdf, UPBP_PB2D, P.query_beam_type,
                                     P.photon_beam_type, tech, err);
     }
-    const float w = 1.f / (w_light + w_cam);
+    const float w = __builtin_amdgcn_rcpf(w_light + w_cam);
     s.acc0 += w * u0;
     s.acc1 += w * u1;
     s.acc2 += w * u2;
@@ -279,31 +285,21 @@ __device__ void beam_hit_mis(const EvalArgs& A, const Ray& r, SegState& s, const
     const uint32_t cam_len = (uint32_t)ml.cam_len;
     if ((path_len + 1 + cam_len > P.max_path_length) || (path_len + 1 + cam_len < P.min_path_length)) return;
     if (cam_len == 1 && t1 < P.min_dist_to_med) return;
+    // As in photon_hit_mis, everything below is weight and contribution arithmetic with a
+    // 1e-4 budget: hardware reciprocals, and one exponential per channel for the query and
+    // beam transmittances together (with the SHORT normalisations folded into the exponent).
     const float sel_pdf = 1.0f;  // no per-cell beam reduction
     const float dq = t1 - s.tmin;
-    float a0 = fast_exp(-s.st0 * dq), a1 = fast_exp(-s.st1 * dq), a2 = fast_exp(-s.st2 * dq);
-    const float pq = pick3(s.kidx, a0, a1, a2);
-    if (short_q) {
-        a0 = a0 / pq;
-        a1 = a1 / pq;
-        a2 = a2 / pq;
-    }
+    const float stk = pick3(s.kidx, s.st0, s.st1, s.st2);
+    const float pq = fast_exp(-stk * dq), pb = fast_exp(-stk * t2);
+    const float a0 = fast_exp(neg_tau(s.st0, stk, dq, short_q) + neg_tau(s.st0, stk, t2, short_b));
+    const float a1 = fast_exp(neg_tau(s.st1, stk, dq, short_q) + neg_tau(s.st1, stk, t2, short_b));
+    const float a2 = fast_exp(neg_tau(s.st2, stk, dq, short_q) + neg_tau(s.st2, stk, t2, short_b));
     float rsp_q = s.min_comp * pq;
     float rsr_q = (ml.seg_flags & UPBP_ORIGIN_IN_MEDIUM) ? rsp_q : pq;
-    const float ratio_q = 1.0f / s.min_comp;
-    const float e0 = fast_exp(-s.st0 * t2), e1 = fast_exp(-s.st1 * t2), e2 = fast_exp(-s.st2 * t2);
-    a0 *= e0;
-    a1 *= e1;
-    a2 *= e2;
-    const float pb = pick3(s.kidx, e0, e1, e2);
-    if (short_b) {
-        a0 = a0 / pb;
-        a1 = a1 / pb;
-        a2 = a2 / pb;
-    }
     float rsp_b = s.min_comp * pb;
     float rsr_b = (bm.ray_sampling_flags & UPBP_ORIGIN_IN_MEDIUM) ? rsp_b : pb;
-    const float ratio_b = 1.0f / s.min_comp;
+    const float ratio = __builtin_amdgcn_rcpf(s.min_comp);  // ratio_q = ratio_b
     if (!(a0 > 1e-20f && a1 > 1e-20f && a2 > 1e-20f)) return;
     rsp_q *= ml.pref_pdf;
     rsr_q *= ml.pref_rev;
@@ -315,43 +311,47 @@ __device__ void beam_hit_mis(const EvalArgs& A, const Ray& r, SegState& s, const
     if (bsdf <= 0.f) return;
     const float dir_pdf = bsdf * s.cont;
     const float rev_pdf = bsdf * s.cont;
-    const float k1 = (1.0f - dist * dist / (radius * radius)) * 3.0f;
-    const float den = (4.0f * radius) * sin_theta;
-    const float inv_sel = 1.0f / sel_pdf;
-    const float u0 = ((((((inv_sel * flux.x) * s.ss0) * a0) * bsdf) * k1)) / den;
-    const float u1 = ((((((inv_sel * flux.y) * s.ss1) * a1) * bsdf) * k1)) / den;
-    const float u2 = ((((((inv_sel * flux.z) * s.ss2) * a2) * bsdf) * k1)) / den;
+    // 1D kernel (1 - dist^2/r^2) * 3 / (4 r sin(theta)) / selection pdf (PhotonBeam.hxx:364-365)
+    const float k1 = (1.0f - (dist * dist) * __builtin_amdgcn_rcpf(radius * radius)) * 3.0f;
+    const float kd = k1 * __builtin_amdgcn_rcpf((4.0f * radius) * sin_theta);
+    const float u0 = ((((flux.x * s.ss0) * a0) * bsdf) * kd);
+    const float u1 = ((((flux.y * s.ss1) * a1) * bsdf) * kd);
+    const float u2 = ((((flux.z * s.ss2) * a2) * bsdf) * kd);
     if (u0 <= 0.f && u1 <= 0.f && u2 <= 0.f) return;
     const float dist_sq_q = t1 * t1;
-    const float rsp_inv_q = 1.0f / rsp_q;
-    const MisF mv = merge_vertex_mis(P, ml.last_pdf_w_inv * dist_sq_q * rsp_inv_q, rsp_inv_q, ratio_q, sel_pdf,
+    const float rsp_inv_q = __builtin_amdgcn_rcpf(rsp_q);
+    const MisF mv = merge_vertex_mis(P, ml.last_pdf_w_inv * dist_sq_q * rsp_inv_q, rsp_inv_q, ratio, sel_pdf,
                                      UPBP_MIS_IN_MEDIUM_WITH_BEAMS);
     const float dist_sq_b = lv->is_finite ? t2 * t2 : 1.0f;
-    const float rsp_inv_b = 1.0f / rsp_b;
-    const MisF bv = merge_vertex_mis(P, bm.last_pdf_w_inv * dist_sq_b * rsp_inv_b, rsp_inv_b, ratio_b, sel_pdf,
+    const float rsp_inv_b = __builtin_amdgcn_rcpf(rsp_b);
+    const MisF bv = merge_vertex_mis(P, bm.last_pdf_w_inv * dist_sq_b * rsp_inv_b, rsp_inv_b, ratio, sel_pdf,
                                      UPBP_MIS_IN_MEDIUM_WITH_BEAMS);
     const uint32_t tech = A.short_query ? (UPBP_SHORT_BEAM | P.techniques) : (UPBP_LONG_BEAM | P.techniques);
     const float sin_t = (tech & UPBP_NO_SINE_IN_WEIGHTS) ? 1.0f : sin_theta;
+    // PhotonBeam.hxx:555-564: 1 / (the merged technique's pdf) per beam-type pair
+    const float tp = mv.bb1d * sel_pdf * sin_t;
     float last;
     if (short_q)
-        last = short_b ? (float)(1.0 / (double)(ratio_q * ratio_b * mv.bb1d * sel_pdf * sin_t))
-                       : rsp_b / (ratio_q * mv.bb1d * sel_pdf * sin_t);
+        last = short_b ? __builtin_amdgcn_rcpf(ratio * ratio * tp) : rsp_b * __builtin_amdgcn_rcpf(ratio * tp);
     else
-        last = short_b ? rsp_q / (ratio_b * mv.bb1d * sel_pdf * sin_t)
-                       : rsp_q * rsp_b / (mv.bb1d * sel_pdf * sin_t);
-    const float w_cam = camera_weight_tail(ml.tail, mv, last, sin_t, rsp_inv_b, ratio_b, rev_pdf * rsr_q / dist_sq_q,
-                                           1.0f / rsr_q, P.query_beam_type, P.photon_beam_type, tech);
+        last = short_b ? rsp_q * __builtin_amdgcn_rcpf(ratio * tp) : rsp_q * rsp_b * __builtin_amdgcn_rcpf(tp);
+    const float w_cam = camera_weight_tail(ml.tail, mv, last, sin_t, rsp_inv_b, ratio,
+                                           rev_pdf * rsr_q * __builtin_amdgcn_rcpf(dist_sq_q),
+                                           __builtin_amdgcn_rcpf(rsr_q), P.query_beam_type, P.photon_beam_type, tech);
     const uint32_t lvf = lv->mis.flags;
     const bool on_light_delta = (lvf & UPBP_MIS_ON_LIGHT_SOURCE) && (lvf & UPBP_MIS_DELTA);
-    const float ntl = on_light_delta ? 0.f : dir_pdf * rsr_b * __builtin_fabsf(lv->mis.cos_theta_out) / dist_sq_b;
+    const float ntl = on_light_delta
+                          ? 0.f
+                          : dir_pdf * rsr_b * __builtin_fabsf(lv->mis.cos_theta_out) * __builtin_amdgcn_rcpf(dist_sq_b);
+    const float rsr_inv_b = __builtin_amdgcn_rcpf(rsr_b);
     float w_light;
     if (!light_weight_tail_bb1d(G, (int64_t)bm.light_vertex, lv->path_idx, (int)path_len, last, bv.pdf_a_inv, ntl,
-                                1.0f / rsr_b, tech, w_light)) {
-        LightView lview{&G, &bv, (int64_t)bm.light_vertex, 1.0f / rsr_b};
+                                rsr_inv_b, tech, w_light)) {
+        LightView lview{&G, &bv, (int64_t)bm.light_vertex, rsr_inv_b};
         w_light = light_path_weight(lview, lv->path_idx, (int)path_len + 1, last, ntl, UPBP_BB1D, P.query_beam_type,
                                     P.photon_beam_type, tech, err);
     }
-    const float w = 1.f / (w_light + w_cam);
+    const float w = __builtin_amdgcn_rcpf(w_light + w_cam);
     s.acc0 += w * u0;
     s.acc1 += w * u1;
     s.acc2 += w * u2;
@@ -438,16 +438,12 @@ __device__ __forceinline__ void beam_leaf(const EvalArgs& A, const Ray& r, SegSt
         const PieceDev pc = A.pieces[i];
         const uint32_t word = __float_as_uint(pc.p2.z);
         const int32_t bmed = (int32_t)(word & kMediumMask);
-        float dist, st, t1, t2;
-        bool hit = (bmed == s.medium) &&
-                   test_beam_beam(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, r.d1o1, s.tmin, s.tmax, pc.p0.x, pc.p0.y,
-                                  pc.p0.z, pc.p1.x, pc.p1.y, pc.p1.z, pc.p3.x, pc.p0.w, pc.p2.w, pc.p2.x, pc.p2.y,
-                                  dist, st, t1, t2);
-        float radius = 0.f;
-        if (hit) {
-            radius = pc.p1.w + t2 * pc.p3.y;  // mStartRadius + t2 * mRadiusChange
-            hit = !(radius < dist);
-        }
+        float dist, st, t1, t2, radius = 0.f;
+        // radius = mStartRadius (p1.w) + t2 * mRadiusChange (p3.y)
+        const bool hit = (bmed == s.medium) &&
+                         test_beam_beam(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, r.d1o1, s.tmin, s.tmax, pc.p0.x, pc.p0.y,
+                                        pc.p0.z, pc.p1.x, pc.p1.y, pc.p1.z, pc.p3.x, pc.p0.w, pc.p2.w, pc.p2.x,
+                                        pc.p2.y, pc.p1.w, pc.p3.y, dist, st, t1, t2, radius);
         if (MODE == (int)Mode::Upbp) {
             if (__ballot(hit)) {
                 const float4 flux = A.piece_flux[i];

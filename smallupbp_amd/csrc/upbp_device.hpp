@@ -81,16 +81,25 @@ __device__ __forceinline__ bool test_bre(float ox, float oy, float oz, float dx,
     return false;
 }
 
+// Relative margin inside which the radius test falls back to the reference's exact
+// square root and division: the hardware reciprocal square root is within 1 ulp, so
+// |ad| * rsq(s2) is within a few ulp (< 1e-6) of fl(|ad| / fl(sqrt(s2))).
+constexpr float kRadiusTie = 1e-5f;
+
 // PhotonBeam::testIntersectionBeamBeam (PhotonBeam.hxx:238-296) with
 // minT2 = 0 and maxT2 = length, plus the piece's half-open range [t_lo, t_hi)
-// that makes a beam cut into pieces count once. d1o1 = dot3(d1, o1) and
-// d2o2 = dot3(d2, o2) are passed in precomputed (same operations, same values).
+// that makes a beam cut into pieces count once, and the radius test of
+// PhotonBeam::accumulate (radius = r0 + t2 * dr, hit unless radius < dist,
+// PhotonBeam.hxx:330-334). d1o1 = dot3(d1, o1) and d2o2 = dot3(d2, o2) are passed
+// in precomputed (same operations, same values). The radius test decides with the
+// exact dist = |ad| / sqrt(s2) of the reference; dist and sin_theta are returned from
+// the reciprocal square root unless the test needed the exact values.
 __device__ __forceinline__ bool test_beam_beam(float o1x, float o1y, float o1z, float d1x, float d1y,
                                                float d1z, float d1o1, float min_t1, float max_t1, float o2x,
                                                float o2y, float o2z, float d2x, float d2y, float d2z,
                                                float d2o2, float len, float max_dist_sqr, float t_lo,
-                                               float t_hi, float& dist, float& sin_theta, float& t1_out,
-                                               float& t2_out) {
+                                               float t_hi, float r0, float dr, float& dist, float& sin_theta,
+                                               float& t1_out, float& t2_out, float& radius_out) {
     const float cx = d1y * d2z - d1z * d2y;
     const float cy = d1z * d2x - d1x * d2z;
     const float cz = d1x * d2y - d1y * d2x;
@@ -106,24 +115,36 @@ __device__ __forceinline__ bool test_beam_beam(float o1x, float o1y, float o1z, 
     const float t2 = ((t1 + d1o1) - d1o2) / q;
     if (t2 <= 0.0f || t2 >= len || __builtin_isnan(t2)) return false;
     if (!(t2 >= t_lo && t2 < t_hi)) return false;
-    const float st = __builtin_sqrtf(s2);
-    dist = __builtin_fabsf(ad) / st;
+    const float radius = r0 + t2 * dr;
+    const float aad = __builtin_fabsf(ad);
+    const float rs = __builtin_amdgcn_rsqf(s2);
+    float d = aad * rs, st = s2 * rs;
+    if (!(radius > d * (1.0f + kRadiusTie))) {
+        if (radius < d * (1.0f - kRadiusTie)) return false;
+        st = __builtin_sqrtf(s2);
+        d = aad / st;
+        if (radius < d) return false;
+    }
+    dist = d;
     sin_theta = st;
     t1_out = t1;
     t2_out = t2;
+    radius_out = radius;
     return true;
 }
 
 // exp(x) for x <= 0 through the hardware base-2 exponential.
 __device__ __forceinline__ float fast_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
 
-// PhaseFunction::Pdf (PhaseFunction.hxx:52-80); cos_theta = -dot(gen, fix).
+// PhaseFunction::Pdf (PhaseFunction.hxx:52-80); cos_theta = -dot(gen, fix). It only
+// scales contributions and weights (no hit decision), so dd^(-3/2) is the hardware
+// reciprocal times the hardware reciprocal square root.
 __device__ __forceinline__ float phase_pdf(float g, bool isotropic, float cos_theta) {
     const float uni = kInvPi * 0.25f;
     if (isotropic) return uni;
     const float sq = g * g;
     const float dd = (1.0f + sq) - (g + g) * cos_theta;
-    return dd > 0.0f ? (uni * (1.0f - sq)) / (dd * __builtin_sqrtf(dd)) : 0.0f;
+    return dd > 0.0f ? ((uni * (1.0f - sq)) * __builtin_amdgcn_rcpf(dd)) * __builtin_amdgcn_rsqf(dd) : 0.0f;
 }
 
 __device__ __forceinline__ uint32_t hash32(uint32_t i) {

@@ -237,3 +237,19 @@ def test_inverse_weight_is_affine_in_each_light_field():
                 assert np.array_equal(iw[1], base), (field, index)
             elif field in ("rev_pdf_a_without_bsdf", "pdf_a_inv") and index == 1:
                 assert np.all(iw[1] != base), (field, index)
+
+
+@pytest.mark.parametrize("pbt", [scenes.SHORT_BEAM, scenes.LONG_BEAM])
+def test_upbp_accelerated_traversal_matches_brute_force(pbt):
+    """The oracle's timed UPBP path (photon BVH, reference beam grid with per-cell ray
+    intervals) finds the same hits and weights as its brute-force loop over items."""
+    w, inm, mi = scenes.upbp_scene(64, 36, 4000, 2000, seed=82, photon_radius=0.02, beam_radius=0.01,
+                                   photon_beam_type=pbt)
+    oph = po.OraclePhotons(w.photons, in_medium=inm)
+    obm = po.OracleBeams(w.beams, grid_res=64)
+    for which, ph, bm in ((0, oph, None), (1, None, obm)):
+        a, ha = po.upbp_eval(ph, bm, w.media, w.queries, mi, which, precision=64, accel=0)
+        b, hb = po.upbp_eval(ph, bm, w.media, w.queries, mi, which, precision=64, accel=1)
+        assert ha == hb and ha > 1000
+        den = np.maximum(np.abs(a).max(1), 1e-300)
+        assert (np.abs(a - b).max(1) / den).max() < 1e-12
