@@ -380,7 +380,8 @@ __device__ void vm_process(const VmArgs& A, const upbp_surf_query& q, const Surf
             const float pw = powf(dot_r, e);
             if (pw > 1e-20f) {  // Float::isPositive (Utils.hxx:109-111)
                 // PowerCosHemispherePdfW (Utils2.hxx:78-86), symmetric
-                const float pdf_w = q.phong_prob * (((e + 1.f) * powf(fmaxf(0.f, dot_r), e)) * (kInvPi * 0.5f));
+                // dot_r > EPS_PHONG, so powf(max(0, dot_r), e) is pw itself
+                const float pdf_w = q.phong_prob * (((e + 1.f) * pw) * (kInvPi * 0.5f));
                 dir_pdf += pdf_w;
                 rev_pdf += pdf_w;
                 f0 += ((q.phong[0] * (e + 2.f)) * 0.5f * kInvPi) * pw;
