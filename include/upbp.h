@@ -495,6 +495,76 @@ upbp_status upbp_surf_eval(upbp_ctx* ctx, const upbp_hash_grid* grid, const upbp
                            uint32_t* sig, upbp_memory memory, upbp_eval_stats* stats);
 
 /* ---------------------------------------------------------------------------
+ * Light sub-path tracing in media: the producer of the light vertices, MisData
+ * and photon beams the estimators above consume. Replaces the light-tracing
+ * loop of UPBPRenderer::RunIteration (SmallUPBP/src/Renderers/UPBP.hxx:501-674)
+ * with GenerateLightSample (:1592-1662), AddBeams (:1783-1870) and
+ * SampleScattering (:1908-1969), for scenes of homogeneous media bounded by
+ * imaginary (index-matched) spheres and boxes and lit by point lights
+ * (PointLight::Emit, Scene/Lights.hxx:363-384). Scene::Intersect's media part
+ * (Scene/Scene.hxx:393-531) becomes: the ray is cut at every boundary crossing,
+ * each piece takes the medium of the innermost region containing its midpoint,
+ * and distance sampling follows HomogeneousMedium::SampleRay / RaySamplePdf
+ * (Scene/Media.hxx:240-323). The random numbers come from a counter-based
+ * generator (path index, draw index) instead of the reference's sequential
+ * mt19937, so one path's numbers do not depend on the others. Real surfaces,
+ * area and environment lights, camera connections (ConnectToCamera) and
+ * heterogeneous media are not part of this producer. */
+#define UPBP_MAX_REGIONS 16    /* regions per scene */
+#define UPBP_REGION_SPHERE 0u /* p0 = centre, p1[0] = radius */
+#define UPBP_REGION_BOX 1u    /* p0 = minimum corner, p1 = maximum corner */
+typedef struct upbp_region {
+    uint32_t shape;     /* UPBP_REGION_* */
+    int32_t medium_id;  /* medium inside the region (index into the context's media) */
+    float p0[3];
+    float p1[3];
+} upbp_region;
+
+typedef struct upbp_point_light {
+    float position[3];
+    float intensity[3]; /* mIntensity */
+} upbp_point_light;
+
+typedef struct upbp_light_trace_params {
+    uint64_t seed;
+    uint32_t photon_beam_type;  /* mPhotonBeamType: UPBP_SHORT_BEAM or UPBP_LONG_BEAM */
+    uint32_t max_path_length;   /* mMaxPathLength (>= 1) */
+    int32_t global_medium;      /* medium outside every region (mScene global medium) */
+    int32_t beam_path_count;    /* mBB1DUsedLightSubPathCount: paths below it store beams; 0 = none */
+    int32_t merge_pb2d;         /* mMergeWithLightVerticesPB2D: beams and IN_MEDIUM_WITH_BEAMS need mfp > bb1d_min_mfp */
+    float surf_factor, pp3d_factor, pb2d_factor, bb1d_factor; /* m*MisWeightFactor */
+    float bb1d_min_mfp;         /* mBB1DMinMFP */
+    uint32_t pad;
+} upbp_light_trace_params;
+
+/* Outputs, in caller memory of one kind. Light vertices are grouped per path in
+ * path order (vertex 0 of a path is its light-source vertex); path_ends[p] is
+ * the end of path p, as mPathEnds. Beams follow in path order, each naming its
+ * origin light vertex by global index. counts are written even when a capacity
+ * is exceeded (the call then fails with UPBP_EINVAL and writes nothing else). */
+typedef struct upbp_light_paths {
+    upbp_light_vertex* vertices;  /* [vertex_capacity] */
+    float* positions;             /* [vertex_capacity][3] mHitpoint */
+    int64_t vertex_capacity;
+    int32_t* path_ends;           /* [path_count] */
+    float* beam_origin;           /* [beam_capacity][3] */
+    float* beam_direction;        /* [beam_capacity][3] */
+    float* beam_length;           /* [beam_capacity] */
+    float* beam_throughput;       /* [beam_capacity][3] mThroughputAtOrigin */
+    uint32_t* beam_flags;         /* [beam_capacity] UPBP_SHORT_BEAM / UPBP_LONG_BEAM */
+    int32_t* beam_medium;         /* [beam_capacity] */
+    upbp_beam_mis* beam_mis;      /* [beam_capacity] */
+    int64_t beam_capacity;
+    int64_t vertex_count;         /* out */
+    int64_t beam_count;           /* out */
+} upbp_light_paths;
+
+upbp_status upbp_trace_light_paths(upbp_ctx* ctx, const upbp_region* regions, int32_t region_count,
+                                   const upbp_point_light* lights, int32_t light_count,
+                                   const upbp_light_trace_params* params, int64_t path_count,
+                                   upbp_light_paths* out, upbp_memory memory);
+
+/* ---------------------------------------------------------------------------
  * Batched evaluation of many independent instances in one persistent launch.
  * Each instance names its structures, a device-resident query batch, its
  * media (a range of the context's media table: medium_id + media_base) and

@@ -66,6 +66,9 @@ def lib():
         L.or_upbp_eval.argtypes = [vp, vp, vp, i32, C.POINTER(_abi.QueryBatch), C.POINTER(_abi.MisInputs), C.c_int,
                                    C.c_int, C.c_int, C.c_int, vp, vp, vp]
         L.or_exp_cephes.restype = f32
+        L.or_trace_light_paths.restype = C.c_int
+        L.or_trace_light_paths.argtypes = [vp, i32, vp, i32, vp, i32, C.POINTER(_abi.LightTraceParams), i64,
+                                           C.c_int, C.POINTER(_abi.LightPaths)]
         L.or_hgrid_create.restype = vp
         L.or_hgrid_create.argtypes = [i64, vp, vp, f32, i64, C.c_uint32]
         L.or_hgrid_free.argtypes = [vp]
@@ -345,3 +348,31 @@ def beam_knn_radius(bm: Beams, radius: float, knn: int, threads=None):
     lib().or_beam_knn_radius(bm.count, o.ctypes.data, d.ctypes.data, ln.ctypes.data, float(radius), int(knn),
                              threads or _threads(), r0.ctypes.data, r1.ctypes.data)
     return r0, r1
+
+
+def light_trace_params(scene) -> _abi.LightTraceParams:
+    p = _abi.LightTraceParams()
+    for k, v in scene.params.items():
+        setattr(p, k, v)
+    return p
+
+
+def trace_light_paths(scene, path_count: int, threads=None):
+    """upbp_trace_light_paths restated (or_trace_light_paths): dict of numpy arrays trimmed to
+    the produced vertices and beams."""
+    from smallupbp_amd import scenes
+
+    threads = threads or _threads()
+    cap_v, cap_b = scenes.light_path_capacity(scene, path_count)
+    arr = scenes.light_path_arrays(cap_v, cap_b, path_count)
+    lp = _abi.light_paths(arr, cap_v, cap_b)
+    media = np.ascontiguousarray(scene.media)
+    regions = np.ascontiguousarray(scene.regions)
+    lights = np.ascontiguousarray(scene.lights)
+    par = light_trace_params(scene)
+    r = lib().or_trace_light_paths(media.ctypes.data, len(media), regions.ctypes.data if len(regions) else None,
+                                   len(regions), lights.ctypes.data, len(lights), C.byref(par), path_count, threads,
+                                   C.byref(lp))
+    if r:
+        raise RuntimeError("or_trace_light_paths failed: %s" % _abi.STATUS_NAMES.get(r, r))
+    return scenes.trim_light_paths(arr, lp.vertex_count, lp.beam_count, path_count)

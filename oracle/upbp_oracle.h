@@ -166,6 +166,15 @@ int or_surf_eval(const or_hgrid* g, const upbp_surf_query* q, int64_t nq, const 
                  int accel, int precision, int threads, float* out32, double* out64, uint32_t* sig,
                  uint64_t* counters);
 
+/* ---- light sub-path tracing in media (lighttrace.c) ---------------------- */
+#define LT_MAX_REGIONS UPBP_MAX_REGIONS
+/* upbp_trace_light_paths restated (UPBP.hxx:501-674,1592-1662,1783-1870,1908-1969): the
+ * same outputs, the same counter-based random numbers, FP32. Returns 0 or an upbp_status
+ * (UPBP_EINVAL with out->vertex_count / beam_count set when a capacity is exceeded). */
+int or_trace_light_paths(const upbp_medium* media, int32_t n_media, const upbp_region* regions, int32_t n_regions,
+                         const upbp_point_light* lights, int32_t n_lights, const upbp_light_trace_params* params,
+                         int64_t path_count, int threads, upbp_light_paths* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -113,6 +113,43 @@ class MisParams(C.Structure):
                 ("bb1d_min_mfp", C.c_float), ("min_dist_to_med", C.c_float), ("has_bb1d_beams", C.c_int32)]
 
 
+class Region(C.Structure):
+    _fields_ = [("shape", C.c_uint32), ("medium_id", C.c_int32), ("p0", C.c_float * 3), ("p1", C.c_float * 3)]
+
+
+class PointLight(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("intensity", C.c_float * 3)]
+
+
+class LightTraceParams(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("photon_beam_type", C.c_uint32), ("max_path_length", C.c_uint32),
+                ("global_medium", C.c_int32), ("beam_path_count", C.c_int32), ("merge_pb2d", C.c_int32),
+                ("surf_factor", C.c_float), ("pp3d_factor", C.c_float), ("pb2d_factor", C.c_float),
+                ("bb1d_factor", C.c_float), ("bb1d_min_mfp", C.c_float), ("pad", C.c_uint32)]
+
+
+class LightPaths(C.Structure):
+    _fields_ = [("vertices", C.c_void_p), ("positions", C.c_void_p), ("vertex_capacity", C.c_int64),
+                ("path_ends", C.c_void_p), ("beam_origin", C.c_void_p), ("beam_direction", C.c_void_p),
+                ("beam_length", C.c_void_p), ("beam_throughput", C.c_void_p), ("beam_flags", C.c_void_p),
+                ("beam_medium", C.c_void_p), ("beam_mis", C.c_void_p), ("beam_capacity", C.c_int64),
+                ("vertex_count", C.c_int64), ("beam_count", C.c_int64)]
+
+
+LIGHT_PATH_FIELDS = ("vertices", "positions", "path_ends", "beam_origin", "beam_direction", "beam_length",
+                     "beam_throughput", "beam_flags", "beam_medium", "beam_mis")
+
+
+def light_paths(arrays: dict, vertex_capacity: int, beam_capacity: int) -> LightPaths:
+    """upbp_light_paths over output arrays (numpy or torch, one memory kind)."""
+    lp = LightPaths()
+    for f in LIGHT_PATH_FIELDS:
+        setattr(lp, f, ptr(arrays[f]))
+    lp.vertex_capacity = vertex_capacity
+    lp.beam_capacity = beam_capacity
+    return lp
+
+
 class MisInputs(C.Structure):
     _fields_ = [("queries", C.c_void_p), ("camera", C.c_void_p), ("camera_count", C.c_int64),
                 ("light_vertices", C.c_void_p), ("light_vertex_count", C.c_int64), ("path_ends", C.c_void_p),

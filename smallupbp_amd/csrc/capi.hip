@@ -788,6 +788,105 @@ upbp_status upbp_surf_eval(upbp_ctx* ctx, const upbp_hash_grid* grid, const upbp
     });
 }
 
+upbp_status upbp_trace_light_paths(upbp_ctx* ctx, const upbp_region* regions, int32_t region_count,
+                                   const upbp_point_light* lights, int32_t light_count,
+                                   const upbp_light_trace_params* params, int64_t path_count, upbp_light_paths* out,
+                                   upbp_memory memory) {
+    if (!ctx) return fail(nullptr, UPBP_EINVAL, "ctx is NULL");
+    return guarded(ctx, [&] {
+        UPBP_REQUIRE(params && out, UPBP_EINVAL, "params and out are required");
+        UPBP_REQUIRE(ctx->n_media > 0, UPBP_EINVAL, "upbp_set_media must be called before light tracing");
+        UPBP_REQUIRE(region_count >= 0 && region_count <= UPBP_MAX_REGIONS && (region_count == 0 || regions),
+                     UPBP_EINVAL, "region_count must be 0..UPBP_MAX_REGIONS with regions given");
+        UPBP_REQUIRE(light_count >= 1 && lights, UPBP_EINVAL, "at least one point light is required");
+        UPBP_REQUIRE(path_count >= 0 && path_count < ((int64_t)1 << 31), UPBP_EINVAL, "path count out of range");
+        UPBP_REQUIRE(memory == UPBP_MEM_HOST || memory == UPBP_MEM_DEVICE, UPBP_EINVAL, "bad memory kind");
+        const upbp_light_trace_params& P = *params;
+        UPBP_REQUIRE(P.photon_beam_type == UPBP_SHORT_BEAM || P.photon_beam_type == UPBP_LONG_BEAM, UPBP_EINVAL,
+                     "photon_beam_type must be UPBP_SHORT_BEAM or UPBP_LONG_BEAM");
+        UPBP_REQUIRE(P.max_path_length >= 1 && P.max_path_length <= 1024, UPBP_EINVAL,
+                     "max_path_length must be 1..1024");
+        UPBP_REQUIRE(P.global_medium >= 0 && P.global_medium < ctx->n_media, UPBP_EINVAL,
+                     "global_medium outside the media table");
+        for (int32_t i = 0; i < region_count; ++i) {
+            const upbp_region& g = regions[i];
+            UPBP_REQUIRE(g.shape == UPBP_REGION_SPHERE || g.shape == UPBP_REGION_BOX, UPBP_EINVAL, "bad region shape");
+            UPBP_REQUIRE(g.medium_id >= 0 && g.medium_id < ctx->n_media, UPBP_EINVAL,
+                         "region medium outside the media table");
+            bool fin = true;
+            for (int a = 0; a < 3; ++a) fin = fin && std::isfinite(g.p0[a]) && std::isfinite(g.p1[a]);
+            UPBP_REQUIRE(fin, UPBP_ENONFINITE, "non-finite region parameter");
+            UPBP_REQUIRE(g.shape != UPBP_REGION_SPHERE || g.p1[0] > 0.f, UPBP_EINVAL, "sphere radius must be > 0");
+        }
+        for (int32_t i = 0; i < light_count; ++i) {
+            bool fin = true;
+            for (int a = 0; a < 3; ++a) fin = fin && std::isfinite(lights[i].position[a]) &&
+                                             std::isfinite(lights[i].intensity[a]);
+            UPBP_REQUIRE(fin, UPBP_ENONFINITE, "non-finite point light");
+        }
+        UPBP_REQUIRE(path_count == 0 || out->path_ends, UPBP_EINVAL, "out->path_ends is required");
+        UPBP_REQUIRE(out->vertex_capacity >= 0 && out->beam_capacity >= 0, UPBP_EINVAL, "negative capacity");
+        UPBP_REQUIRE(out->vertex_capacity == 0 || (out->vertices && out->positions), UPBP_EINVAL,
+                     "out->vertices and out->positions are required");
+        UPBP_REQUIRE(out->beam_capacity == 0 || (out->beam_origin && out->beam_direction && out->beam_length &&
+                                                  out->beam_throughput && out->beam_flags && out->beam_medium &&
+                                                  out->beam_mis),
+                     UPBP_EINVAL, "every out->beam_* array is required");
+        if (memory == UPBP_MEM_DEVICE) {
+            trace_light_paths(*ctx, regions, region_count, lights, light_count, P, path_count, *out);
+            return;
+        }
+        // host outputs: trace into device buffers of the same capacities, then copy what was made
+        const int64_t cv = std::max<int64_t>(out->vertex_capacity, 1), cb = std::max<int64_t>(out->beam_capacity, 1);
+        DevBuf v, x, pe, bo, bd, bl, bt, bf, bm, bs;
+        v.ensure(cv * sizeof(upbp_light_vertex));
+        x.ensure(cv * 12);
+        pe.ensure(std::max<int64_t>(path_count, 1) * 4);
+        bo.ensure(cb * 12);
+        bd.ensure(cb * 12);
+        bl.ensure(cb * 4);
+        bt.ensure(cb * 12);
+        bf.ensure(cb * 4);
+        bm.ensure(cb * 4);
+        bs.ensure(cb * sizeof(upbp_beam_mis));
+        upbp_light_paths d = *out;
+        d.vertices = v.as<upbp_light_vertex>();
+        d.positions = x.as<float>();
+        d.path_ends = pe.as<int32_t>();
+        d.beam_origin = bo.as<float>();
+        d.beam_direction = bd.as<float>();
+        d.beam_length = bl.as<float>();
+        d.beam_throughput = bt.as<float>();
+        d.beam_flags = bf.as<uint32_t>();
+        d.beam_medium = bm.as<int32_t>();
+        d.beam_mis = bs.as<upbp_beam_mis>();
+        struct CountsBack {
+            upbp_light_paths& o;
+            const upbp_light_paths& d;
+            ~CountsBack() {
+                o.vertex_count = d.vertex_count;
+                o.beam_count = d.beam_count;
+            }
+        } counts_back{*out, d};
+        trace_light_paths(*ctx, regions, region_count, lights, light_count, P, path_count, d);
+        const int64_t V = d.vertex_count, B = d.beam_count;
+        auto back = [&](void* dst, const DevBuf& src, size_t bytes) {
+            if (bytes) UPBP_HIP(hipMemcpyAsync(dst, src.p, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        };
+        back(out->vertices, v, V * sizeof(upbp_light_vertex));
+        back(out->positions, x, V * 12);
+        back(out->path_ends, pe, path_count * 4);
+        back(out->beam_origin, bo, B * 12);
+        back(out->beam_direction, bd, B * 12);
+        back(out->beam_length, bl, B * 4);
+        back(out->beam_throughput, bt, B * 12);
+        back(out->beam_flags, bf, B * 4);
+        back(out->beam_medium, bm, B * 4);
+        back(out->beam_mis, bs, B * sizeof(upbp_beam_mis));
+        UPBP_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
 upbp_status upbp_hit_signatures(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams* beams,
                                 const upbp_query_batch* queries, uint32_t* photon_sig, uint32_t* beam_sig,
                                 upbp_memory out_memory) {

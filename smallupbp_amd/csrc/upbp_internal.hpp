@@ -304,4 +304,10 @@ void launch_batched(Ctx& ctx, const InstanceDev* inst, int n_inst, int64_t total
                     const MergedDev& merged, int which, int grid_blocks, int waves_per_block);
 void launch_fill_u32(Ctx& ctx, uint32_t* p, uint32_t v, int64_t n);
 
+// ---- lighttrace.hip: light sub-path tracing in media ---------------------------
+// Fills out (device memory) and its counts; UPBP_EINVAL when a capacity is exceeded.
+void trace_light_paths(Ctx& ctx, const upbp_region* regions, int32_t n_regions, const upbp_point_light* lights,
+                       int32_t n_lights, const upbp_light_trace_params& P, int64_t path_count,
+                       upbp_light_paths& out);
+
 }  // namespace upbp

@@ -32,7 +32,7 @@ EXPORTED = [
     "upbp_adjoint", "upbp_batch_plan_create", "upbp_batch_plan_destroy", "upbp_batch_run",
     "upbp_set_tuning", "upbp_get_tuning", "upbp_autotune", "upbp_eval_upbp", "upbp_bre_radii",
     "upbp_beams_radii", "upbp_hash_grid_build", "upbp_hash_grid_destroy", "upbp_hash_grid_count",
-    "upbp_pp3d_eval", "upbp_surf_eval",
+    "upbp_pp3d_eval", "upbp_surf_eval", "upbp_trace_light_paths",
 ]
 
 
@@ -127,6 +127,9 @@ def load():
     L.upbp_hash_grid_destroy.argtypes = [vp]
     L.upbp_hash_grid_count.restype = i64
     L.upbp_hash_grid_count.argtypes = [vp]
+    L.upbp_trace_light_paths.restype = st
+    L.upbp_trace_light_paths.argtypes = [vp, vp, i32, vp, i32, C.POINTER(_abi.LightTraceParams), i64,
+                                         C.POINTER(_abi.LightPaths), C.c_int]
     for name in ("upbp_pp3d_eval", "upbp_surf_eval"):
         getattr(L, name).restype = st
         getattr(L, name).argtypes = [vp, vp, vp, i64, C.POINTER(_abi.MisInputs), C.c_uint32, vp, vp, C.c_int,
@@ -339,6 +342,56 @@ class PhotonBeamsEvaluator:
             self.destroy()
         except Exception:
             pass
+
+
+def trace_light_paths(ctx: Context, scene, path_count: int, device: bool = True, capacity=None):
+    """upbp_trace_light_paths over a scenes.LightScene: light sub-paths traced in the scene's
+    media from its point lights. Returns a dict of arrays (torch tensors on the GPU when
+    `device`, numpy otherwise) trimmed to the vertices and beams produced, with path_ends."""
+    from . import scenes
+
+    cap_v, cap_b = capacity or scenes.light_path_capacity(scene, path_count)
+    if device:
+        import torch
+
+        def zeros(shape, dtype):
+            dt = np.dtype(dtype)
+            n = int(np.prod(shape)) if isinstance(shape, tuple) else int(shape)
+            return torch.zeros(max(n, 1) * dt.itemsize, dtype=torch.uint8, device="cuda")
+
+        arr = {k: zeros(v.shape, v.dtype) for k, v in scenes.light_path_arrays(cap_v, cap_b, path_count).items()}
+    else:
+        arr = scenes.light_path_arrays(cap_v, cap_b, path_count)
+    lp = _abi.light_paths(arr, cap_v, cap_b)
+    regions = np.ascontiguousarray(scene.regions)
+    lights = np.ascontiguousarray(scene.lights)
+    par = _abi.LightTraceParams()
+    for k, v in scene.params.items():
+        setattr(par, k, v)
+    ctx.check(ctx.lib.upbp_trace_light_paths(ctx.h, regions.ctypes.data if len(regions) else None, len(regions),
+                                             lights.ctypes.data, len(lights), C.byref(par), path_count, C.byref(lp),
+                                             _abi.MEM_DEVICE if device else _abi.MEM_HOST))
+    nv, nb = int(lp.vertex_count), int(lp.beam_count)
+    if not device:
+        return scenes.trim_light_paths(arr, nv, nb, path_count)
+    host = scenes.light_path_arrays(1, 1, 1)
+    out = {}
+    for k, t in arr.items():
+        dt = host[k].dtype
+        n = path_count if k == "path_ends" else (nv if k in ("vertices", "positions") else nb)
+        per = dt.itemsize * (3 if host[k].ndim == 2 else 1)
+        out[k] = t[: n * per]
+    out["_dtypes"] = {k: (host[k].dtype, host[k].shape[1:]) for k in host}
+    return out
+
+
+def light_paths_to_host(d: dict) -> dict:
+    """The device result of trace_light_paths as typed numpy arrays."""
+    out = {}
+    for k, (dt, tail) in d["_dtypes"].items():
+        raw = d[k].cpu().numpy()
+        out[k] = raw.view(dt).reshape((-1,) + tuple(tail)) if len(raw) else np.zeros((0,) + tuple(tail), dt)
+    return out
 
 
 class HashGrid:

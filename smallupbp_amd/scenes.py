@@ -517,12 +517,12 @@ def random_mis(rng, n, specular_p=0.1, beams_p=0.7) -> np.ndarray:
 
 def upbp_scene(width=32, height=18, n_paths=2_000, n_beams=500, photon_radius=0.03, beam_radius=0.01, seed=41,
                techniques=PB2D | BB1D | BPT | PP3D | SURF, query_beam_type=LONG_BEAM, photon_beam_type=SHORT_BEAM,
-               max_light_length=4, max_camera_length=3):
+               max_light_length=4, max_camera_length=3, media_rows=None):
     """Box + sphere media; light sub-paths of 1..max_light_length segments whose non-source
     vertices are photons in the media; beams leave random light vertices; each query has a
     camera sub-path of 1..max_camera_length vertices with random MIS data."""
     rng = rng_for(seed)
-    media = make_media(_medium_rows_box_sphere())
+    media = make_media(media_rows or _medium_rows_box_sphere())
     lens = rng.integers(1, max_light_length + 1, n_paths)
     counts = lens + 1
     ends = np.cumsum(counts).astype(np.int32)
@@ -731,3 +731,88 @@ def surf_scene(n_paths=2_000, n_queries=4_000, radius=0.05, cell_count=None, see
     q["camera_offset"] = coff
     mi = MisInputs(None, cam, lv, ends, None, _params(techniques, query_beam_type, photon_beam_type))
     return MergeScene(SURF, media, pos, float(radius), int(cell_count or n_paths), q, mi)
+
+
+# ---- light sub-path tracing (upbp_trace_light_paths) ------------------------------------
+REGION_SPHERE = 0
+REGION_BOX = 1
+REGION_DTYPE = np.dtype([("shape", "<u4"), ("medium_id", "<i4"), ("p0", "<f4", (3,)), ("p1", "<f4", (3,))])
+POINT_LIGHT_DTYPE = np.dtype([("position", "<f4", (3,)), ("intensity", "<f4", (3,))])
+assert REGION_DTYPE.itemsize == 32 and POINT_LIGHT_DTYPE.itemsize == 24
+
+
+@dataclasses.dataclass
+class LightScene:
+    """Inputs of light sub-path tracing: media, nested imaginary regions (outer first; a point
+    takes the medium of the last region containing it, else global_medium), point lights and
+    the upbp_light_trace_params fields."""
+    media: np.ndarray
+    regions: np.ndarray  # REGION_DTYPE
+    lights: np.ndarray  # POINT_LIGHT_DTYPE
+    params: dict
+
+
+def light_scene(photon_beam_type=SHORT_BEAM, max_path_length=6, beam_path_count=1 << 30, merge_pb2d=1, seed=5,
+                n_lights=1, mean_cosine=0.0, bb1d_min_mfp=0.5, media_rows=None) -> LightScene:
+    """The box + sphere media (box medium 0 = [0,1]^3, sphere medium 2 of radius 0.3 at the
+    centre, vacuum medium 1 outside) lit by point lights inside the box."""
+    rows = media_rows or _medium_rows_box_sphere()
+    rows = [(a, s, (mean_cosine if i != 1 else 0.0), c) for i, (a, s, g, c) in enumerate(rows)]
+    media = make_media(rows)
+    c, r, sid = SPHERE
+    regions = np.zeros(2, REGION_DTYPE)
+    regions[0] = (REGION_BOX, 0, (0, 0, 0), (1, 1, 1))
+    regions[1] = (REGION_SPHERE, sid, c, (r, 0, 0))
+    lights = np.zeros(n_lights, POINT_LIGHT_DTYPE)
+    for i in range(n_lights):
+        lights[i] = ((0.5 + 0.3 * np.cos(2.1 * i), 0.5 + 0.3 * np.sin(2.1 * i), 0.1 + 0.05 * i), (1.0, 0.8, 0.6))
+    params = dict(seed=seed, photon_beam_type=photon_beam_type, max_path_length=max_path_length, global_medium=1,
+                  beam_path_count=beam_path_count, merge_pb2d=merge_pb2d, surf_factor=0.7, pp3d_factor=0.9,
+                  pb2d_factor=0.6, bb1d_factor=1.3, bb1d_min_mfp=bb1d_min_mfp)
+    return LightScene(media, regions, lights, params)
+
+
+def light_path_capacity(scene: LightScene, path_count: int):
+    """Upper bounds of the vertex and beam counts: max_path_length vertices and
+    (max_path_length - 1) rays of at most 2 * regions + 1 pieces per path."""
+    L = int(scene.params["max_path_length"])
+    if L < 2:
+        return 0, 0
+    return path_count * L, path_count * (L - 1) * (2 * len(scene.regions) + 1)
+
+
+def light_path_arrays(vertex_capacity: int, beam_capacity: int, path_count: int, zeros=np.zeros) -> dict:
+    """Output arrays of upbp_trace_light_paths (numpy by default)."""
+    return {"vertices": zeros(max(vertex_capacity, 1), LIGHT_VERTEX_DTYPE),
+            "positions": zeros((max(vertex_capacity, 1), 3), np.float32),
+            "path_ends": zeros(max(path_count, 1), np.int32),
+            "beam_origin": zeros((max(beam_capacity, 1), 3), np.float32),
+            "beam_direction": zeros((max(beam_capacity, 1), 3), np.float32),
+            "beam_length": zeros(max(beam_capacity, 1), np.float32),
+            "beam_throughput": zeros((max(beam_capacity, 1), 3), np.float32),
+            "beam_flags": zeros(max(beam_capacity, 1), np.uint32),
+            "beam_medium": zeros(max(beam_capacity, 1), np.int32),
+            "beam_mis": zeros(max(beam_capacity, 1), BEAM_MIS_DTYPE)}
+
+
+def trim_light_paths(arrays: dict, n_vertices: int, n_beams: int, path_count: int) -> dict:
+    out = {k: v[:n_vertices] for k, v in arrays.items() if k in ("vertices", "positions")}
+    out.update({k: v[:n_beams] for k, v in arrays.items() if k.startswith("beam_")})
+    out["path_ends"] = arrays["path_ends"][:path_count]
+    return out
+
+
+def light_paths_to_inputs(lp: dict, media: np.ndarray, beam_radius=0.003, photon_radius=0.005):
+    """The estimators' inputs from traced light paths, as the reference's iteration builds
+    them (UPBP.hxx:676-730): photons = the in-medium light vertices (position, dir_fix as the
+    incoming direction, throughput as power), beams with their MIS records, and the light
+    vertices + path ends of the MIS walks. Returns (Photons, in_medium u8, Beams, beam_mis,
+    light_vertices, path_ends)."""
+    lv = lp["vertices"]
+    ph = Photons(np.ascontiguousarray(lp["positions"], np.float32), np.ascontiguousarray(lv["dir_fix"], np.float32),
+                 np.ascontiguousarray(lv["throughput"], np.float32), None, photon_radius)
+    in_medium = ((lv["flags"] & LV_IN_MEDIUM) != 0).astype(np.uint8)
+    bm = Beams(np.ascontiguousarray(lp["beam_origin"]), np.ascontiguousarray(lp["beam_direction"]),
+               np.ascontiguousarray(lp["beam_length"]), np.ascontiguousarray(lp["beam_throughput"]),
+               np.ascontiguousarray(lp["beam_flags"]), np.ascontiguousarray(lp["beam_medium"]), beam_radius)
+    return ph, in_medium, bm, np.ascontiguousarray(lp["beam_mis"]), lv, np.ascontiguousarray(lp["path_ends"])

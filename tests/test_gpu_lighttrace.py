@@ -1,0 +1,140 @@
+"""GPU parity of light sub-path tracing (upbp_trace_light_paths) against the CPU oracle.
+
+Both draw the same counter-based random numbers, so the same path index gives the same
+path. The math library's last-ulp differences between the GPU and the host can flip a
+distance or roulette decision on a handful of paths (which then diverge); every path whose
+vertex and beam counts agree must match field by field (1e-4 relative, exact integers), and
+at most 1 % of the paths may differ in shape."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from smallupbp_amd import native, scenes
+
+from gpu_helpers import REL_TOL, gpu_context, print_case, rel_err
+
+pytestmark = pytest.mark.gpu
+
+DENSE = [((0.05, 0.1, 0.02), (1.2, 1.0, 0.8), 0.0, 0.9), ((0, 0, 0), (0, 0, 0), 0.0, 1.0),
+         ((0.1, 0.05, 0.2), (3.0, 2.5, 2.0), 0.0, 0.8)]
+
+
+def _split(ends, n):
+    starts = np.concatenate([[0], ends[:-1]]).astype(np.int64)
+    return starts, ends.astype(np.int64)
+
+
+def _compare(sc, g, o, path_count):
+    gs, ge = _split(g["path_ends"], path_count)
+    os_, oe = _split(o["path_ends"], path_count)
+    gbp = g["vertices"]["path_idx"][g["beam_mis"]["light_vertex"]] if len(g["beam_mis"]) else np.zeros(0, int)
+    obp = o["vertices"]["path_idx"][o["beam_mis"]["light_vertex"]] if len(o["beam_mis"]) else np.zeros(0, int)
+    gnb = np.bincount(gbp, minlength=path_count)
+    onb = np.bincount(obp, minlength=path_count)
+    same = ((ge - gs) == (oe - os_)) & (gnb == onb)
+    assert same.mean() >= 0.99, "paths differing in shape: %d of %d" % ((~same).sum(), path_count)
+    vi = np.concatenate([np.arange(gs[p], ge[p]) for p in np.nonzero(same)[0]])
+    wi = np.concatenate([np.arange(os_[p], oe[p]) for p in np.nonzero(same)[0]])
+    gv, ov = g["vertices"][vi], o["vertices"][wi]
+    for f in ("path_idx", "path_length", "is_finite", "flags"):
+        assert np.array_equal(gv[f], ov[f]), f
+    assert np.array_equal(gv["mis"]["flags"], ov["mis"]["flags"])
+
+    def close(a, b, what):
+        a = np.asarray(a, np.float64).reshape(len(a), -1)
+        b = np.asarray(b, np.float64).reshape(len(b), -1)
+        err = rel_err(a, b, floor=1e-30)
+        assert err.max() <= REL_TOL, (what, err.max())
+
+    close(g["positions"][vi], o["positions"][wi], "positions")
+    close(gv["throughput"], ov["throughput"], "throughput")
+    close(gv["dir_fix"], ov["dir_fix"], "dir_fix")
+    for f in ("pdf_a_inv", "rev_pdf_a", "rev_pdf_a_without_bsdf", "ray_sample_pdf_inv", "ray_sample_rev_pdf_inv",
+              "ray_sample_pdfs_ratio", "sin_theta", "cos_theta_out", "pb2d_factor"):
+        close(gv["mis"][f][:, None], ov["mis"][f][:, None], f)
+    # beams of the matching paths, in path order
+    gbi = np.nonzero(same[gbp])[0]
+    obi = np.nonzero(same[obp])[0]
+    assert len(gbi) == len(obi)
+    if len(gbi):
+        assert np.array_equal(g["beam_flags"][gbi], o["beam_flags"][obi])
+        assert np.array_equal(g["beam_medium"][gbi], o["beam_medium"][obi])
+        gm, om = g["beam_mis"][gbi], o["beam_mis"][obi]
+        assert np.array_equal(gm["ray_sampling_flags"], om["ray_sampling_flags"])
+        # light_vertex: the same vertex of the same path
+        assert np.array_equal(g["vertices"]["path_length"][gm["light_vertex"]],
+                              o["vertices"]["path_length"][om["light_vertex"]])
+        for f in ("beam_origin", "beam_direction", "beam_throughput", "beam_length"):
+            close(g[f][gbi], o[f][obi], f)
+        for f in ("ray_sample_pdf", "ray_sample_rev_pdf", "last_pdf_w_inv"):
+            close(gm[f][:, None], om[f][:, None], f)
+    return same
+
+
+CASES = [
+    ("short_iso", dict(photon_beam_type=scenes.SHORT_BEAM, media_rows=DENSE, bb1d_min_mfp=0.1)),
+    ("long_hg", dict(photon_beam_type=scenes.LONG_BEAM, media_rows=DENSE, mean_cosine=0.6, n_lights=3,
+                     bb1d_min_mfp=0.1)),
+    ("default_media_pb2d_filter", dict(max_path_length=10)),
+    ("no_pb2d_merging_some_beams", dict(merge_pb2d=0, beam_path_count=700, media_rows=DENSE)),
+]
+
+
+@pytest.mark.parametrize("name,kw", CASES)
+def test_light_paths_match_oracle(name, kw):
+    print_case("lighttrace", name=name)
+    sc = scenes.light_scene(**kw)
+    n = 4000
+    ctx = gpu_context(sc.media)
+    g = native.light_paths_to_host(native.trace_light_paths(ctx, sc, n, device=True))
+    o = po.trace_light_paths(sc, n)
+    assert len(o["vertices"]) > 1.1 * n and len(o["beam_length"]) > n // 4
+    same = _compare(sc, g, o, n)
+    print("paths matching in shape: %.4f" % same.mean())
+
+
+def test_host_memory_capacity_and_degenerate_calls():
+    print_case("lighttrace_host")
+    sc = scenes.light_scene(media_rows=DENSE)
+    ctx = gpu_context(sc.media)
+    d = native.light_paths_to_host(native.trace_light_paths(ctx, sc, 1000, device=True))
+    h = native.trace_light_paths(ctx, sc, 1000, device=False)
+    for k in ("vertices", "positions", "path_ends", "beam_origin", "beam_length", "beam_mis"):
+        assert np.array_equal(d[k].view(np.uint8), h[k].view(np.uint8)), k
+    with pytest.raises(native.UpbpError):
+        native.trace_light_paths(ctx, sc, 1000, device=False, capacity=(100, 100))
+    empty = native.trace_light_paths(ctx, scenes.light_scene(max_path_length=1), 50, device=False)
+    assert len(empty["vertices"]) == 0 and np.all(empty["path_ends"] == 0)
+    none = native.trace_light_paths(ctx, sc, 0, device=False)
+    assert len(none["vertices"]) == 0
+    bad = scenes.light_scene(media_rows=DENSE)
+    bad.regions[1]["medium_id"] = 7
+    with pytest.raises(native.UpbpError):
+        native.trace_light_paths(ctx, bad, 10, device=False)
+
+
+def test_traced_paths_drive_the_upbp_estimators():
+    """The iteration end to end: GPU-traced light paths become the photons, beams and MIS
+    inputs of upbp_eval_upbp; the oracle evaluates the same traced data."""
+    print_case("lighttrace_to_upbp")
+    sc = scenes.light_scene(media_rows=DENSE, bb1d_min_mfp=0.1)
+    ctx = gpu_context(sc.media)
+    lp = native.light_paths_to_host(native.trace_light_paths(ctx, sc, 3000, device=True))
+    ph, inm, bm, bmis, lv, ends = scenes.light_paths_to_inputs(lp, sc.media, beam_radius=0.01, photon_radius=0.02)
+    w0, _, mi0 = scenes.upbp_scene(40, 24, 10, 10, seed=66, media_rows=DENSE)
+    w = scenes.Workload("traced", sc.media, ph, bm, w0.queries, w0.sphere, 0, 1, 0)
+    mi = scenes.MisInputs(mi0.queries, mi0.camera, lv, ends, bmis, mi0.params)
+    bre = native.EmbreeBre(ctx)
+    bre.build(w.photons, in_medium=inm)
+    b = native.PhotonBeamsEvaluator(ctx)
+    b.build(w.beams)
+    gp, gb = native.eval_upbp(ctx, bre, b, native.QueryBatchView.to_device(w.queries), mi)
+    gp, gb = gp.cpu().numpy(), gb.cpu().numpy()
+    oph = po.OraclePhotons(w.photons, in_medium=inm, build_bvh=False)
+    obm = po.OracleBeams(w.beams, grid_res=0)
+    rp, _ = po.upbp_eval(oph, None, w.media, w.queries, mi, 0)
+    rb, _ = po.upbp_eval(None, obm, w.media, w.queries, mi, 1)
+    assert (rp.max(1) > 0).sum() > 100 and (rb.max(1) > 0).sum() > 100
+    ep, eb = rel_err(gp, rp), rel_err(gb, rb)
+    print("max rel err photons %.2e beams %.2e" % (ep.max(), eb.max()))
+    assert ep.max() <= REL_TOL and eb.max() <= REL_TOL
