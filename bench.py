@@ -13,8 +13,8 @@ Prints ONE JSON line: metric, value (Mqueries/s), roofline of the dominant
 kernel (algorithmic bytes / average HIP-event kernel time vs 8 TB/s), the CPU
 oracle timed on a bounded sample on the host cores, the forward+adjoint vs
 forward cost ratio on the 512x512 differentiable workload, the 64-instance
-batched speedup, the MIS-weighted (UPBP) rate at 1920x1080, and the PP3D and
-SURF vertex-merging rates.
+batched speedup, the MIS-weighted (UPBP) rate at 1920x1080, the light sub-path
+tracing rate, and the PP3D and SURF vertex-merging rates.
 """
 from __future__ import annotations
 
@@ -364,6 +364,73 @@ def upbp_measure(steps: int, cpu_budget: float, threads: int):
     return rec
 
 
+LIGHT_TRACE_MEDIA = [((0.05, 0.1, 0.02), (1.2, 1.0, 0.8), 0.0, 0.9), ((0, 0, 0), (0, 0, 0), 0.0, 1.0),
+                     ((0.1, 0.05, 0.2), (3.0, 2.5, 2.0), 0.0, 0.8)]
+
+
+def light_trace_measure(steps: int, cpu_budget: float, threads: int, paths: int = 1_000_000):
+    """Light sub-path tracing (upbp_trace_light_paths): 1,000,000 paths from a point light in
+    the box + sphere media (sigma_s about 1 and 2.5), up to 10 vertices, SHORT photon beams.
+    Outputs resident in HBM and allocated once; one step = one call (trace, scan, pack). The
+    oracle's restatement with the same random numbers, on a path sample, is the CPU
+    comparison."""
+    import ctypes as C
+
+    import torch
+
+    from smallupbp_amd import _abi, native, scenes
+
+    sc = scenes.light_scene(media_rows=LIGHT_TRACE_MEDIA, max_path_length=10, bb1d_min_mfp=0.1, seed=12)
+    ctx = native.Context(0)
+    ctx.set_media(sc.media)
+    cap_v, cap_b = scenes.light_path_capacity(sc, paths)
+
+    def zeros(shape, dtype):
+        n = int(np.prod(shape)) if isinstance(shape, tuple) else int(shape)
+        return torch.zeros(max(n, 1) * np.dtype(dtype).itemsize, dtype=torch.uint8, device="cuda")
+
+    arr = {k: zeros((cap_v if k in ("vertices", "positions") else paths if k == "path_ends" else cap_b,)
+                    + v.shape[1:], v.dtype) for k, v in scenes.light_path_arrays(1, 1, 1).items()}
+    lp = _abi.light_paths(arr, cap_v, cap_b)
+    regions = np.ascontiguousarray(sc.regions)
+    lights = np.ascontiguousarray(sc.lights)
+    par = _abi.LightTraceParams()
+    for k, v in sc.params.items():
+        setattr(par, k, v)
+
+    def run():
+        ctx.check(ctx.lib.upbp_trace_light_paths(ctx.h, regions.ctypes.data, len(regions), lights.ctypes.data,
+                                                 len(lights), C.byref(par), paths, C.byref(lp), _abi.MEM_DEVICE))
+
+    for _ in range(2):
+        run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run()
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / steps
+    rec = {"workload": "lighttrace_1m (%d light paths, box + sphere media, max path length 10, SHORT beams)" % paths,
+           "mpaths_per_s": round(paths / el / 1e6, 3), "ms_per_step": round(el * 1e3, 4),
+           "vertices": int(lp.vertex_count), "beams": int(lp.beam_count),
+           "mvertices_per_s": round(lp.vertex_count / el / 1e6, 3)}
+    if cpu_budget > 0:
+        from oracle import pyoracle as po
+
+        t = time.time()
+        po.trace_light_paths(sc, 20000, threads=threads)
+        per = (time.time() - t) / 20000
+        n = int(min(paths, max(20000, cpu_budget / max(per, 1e-9))))
+        t = time.time()
+        po.trace_light_paths(sc, n, threads=threads)
+        el_cpu = time.time() - t
+        rec["cpu_baseline"] = {"value": round(n / el_cpu / 1e6, 5), "unit": "Mpaths/s", "cores": threads,
+                               "kind": "port", "sample": "the first %d paths of the same workload through the oracle's "
+                               "restatement (same random numbers), %.1f s on %d threads" % (n, el_cpu, threads)}
+        rec["gpu_vs_cpu"] = round(rec["mpaths_per_s"] / rec["cpu_baseline"]["value"], 1)
+    return rec
+
+
 def calibrate_copy(nbytes: int = 1 << 30):
     """A plain device copy of known size, so PMC runs can scale FETCH_SIZE / WRITE_SIZE."""
     import torch
@@ -394,6 +461,7 @@ def main():
                     help="also time the 64-instance batched form at this size scale (1 = full size, 0 = skip)")
     ap.add_argument("--no-vm", action="store_true", help="skip the PP3D and SURF vertex-merging measurements")
     ap.add_argument("--no-upbp", action="store_true", help="skip the MIS-weighted (UPBP) measurement")
+    ap.add_argument("--no-lighttrace", action="store_true", help="skip the light sub-path tracing measurement")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -527,6 +595,9 @@ def main():
 
     if not args.no_upbp:
         rec["upbp"] = upbp_measure(max(3, args.steps // 2), min(args.cpu_budget, 5.0), host_cores())
+
+    if not args.no_lighttrace:
+        rec["lighttrace"] = light_trace_measure(max(3, args.steps // 2), min(args.cpu_budget, 5.0), host_cores())
 
     if not args.no_vm:
         from smallupbp_amd import scenes as _sc

@@ -17,7 +17,8 @@
 // which keeps the thread's state in registers. A path writes into fixed slots (vertices:
 // max_path_length, beams: (max_path_length - 1) rays of at most 2 R + 1 pieces); an
 // exclusive scan of the per-path counts then packs vertices and beams in path order into
-// the caller's arrays and gives mPathEnds.
+// the caller's arrays and gives mPathEnds. Slots are slot-major across paths (slot k of path p
+// at k * path_count + p), so a wave's accesses to its paths' slot k are contiguous.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -186,13 +187,17 @@ __device__ __forceinline__ float max_beam_length(const MediumDev& m) {
     return mx > 0.f ? (float)(20.723265836946411 / (double)mx) : INFINITY;
 }
 
-struct PathSlots {
-    upbp_light_vertex* v;
-    float* pos;
-    BeamRec* b;
+// The slots of one path, slot-major across paths (slot k of path p at k * path_count + p),
+// so that neighbouring threads' slots are neighbours in memory for the tracer's writes and
+// the packing reads.
+template <class T>
+struct Slots {
+    T* base;
+    int64_t stride;
+    __device__ T& operator[](int64_t k) const { return base[k * stride]; }
 };
 
-__device__ void push_beam(BeamRec* b, int32_t& nb, const float* o, const float* d, float t0, float len,
+__device__ void push_beam(const Slots<BeamRec>& b, int32_t& nb, const float* o, const float* d, float t0, float len,
                           const float* thr, uint32_t flags, int32_t med, int32_t lv, uint32_t rsf, float rsp, float rsr,
                           float lpwi) {
     BeamRec r;
@@ -222,9 +227,9 @@ __global__ __launch_bounds__(kBlock) void k_trace_paths(LtScene S, const upbp_po
                                                          uint32_t* __restrict__ nb_out) {
     const int64_t path = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (path >= path_count) return;
-    upbp_light_vertex* V = sv + path * cap_v;
-    float* X = sp + 3 * path * cap_v;
-    BeamRec* B = sb + path * cap_b;
+    const Slots<upbp_light_vertex> V{sv + path, path_count};
+    const Slots<float3> X{reinterpret_cast<float3*>(sp) + path, path_count};
+    const Slots<BeamRec> B{sb + path, path_count};
     Rng rng{mix64(P.seed ^ ((uint64_t)path * 0xD1B54A32D192ED03ull)), 0};
     int32_t nv = 0, nb = 0;
     // GenerateLightSample
@@ -249,9 +254,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_paths(LtScene S, const upbp_po
         v.mis.cos_theta_out = 1.f;
         v.mis.flags = UPBP_MIS_DELTA | UPBP_MIS_ON_LIGHT_SOURCE;  // delta light: rev pdf 0
         V[0] = v;
-        X[0] = L.position[0];
-        X[1] = L.position[1];
-        X[2] = L.position[2];
+        X[0] = make_float3(L.position[0], L.position[1], L.position[2]);
         nv = 1;
     }
     float thr[3] = {L.intensity[0] / emission_pdf, L.intensity[1] / emission_pdf, L.intensity[2] / emission_pdf};
@@ -368,8 +371,8 @@ __global__ __launch_bounds__(kBlock) void k_trace_paths(LtScene S, const upbp_po
         for (int c = 0; c < 3; ++c) {
             v.throughput[c] = thr[c];
             v.dir_fix[c] = -dir[c];
-            X[3 * nv + c] = hit[c];
         }
+        X[nv] = make_float3(hit[0], hit[1], hit[2]);
         v.continuation_prob = m.cont;
         v.path_idx = (int32_t)path;
         v.path_length = pl;
@@ -427,21 +430,25 @@ __global__ __launch_bounds__(kBlock) void k_trace_paths(LtScene S, const upbp_po
     nb_out[path] = (uint32_t)nb;
 }
 
-// Packs path p's slots at its scanned offsets (vertices, positions, beams in SoA) and
-// writes mPathEnds.
-__global__ void k_pack_paths(int64_t path_count, int32_t cap_v, int32_t cap_b, const upbp_light_vertex* __restrict__ sv,
-                             const float* __restrict__ sp, const BeamRec* __restrict__ sb,
-                             const uint32_t* __restrict__ nv, const uint32_t* __restrict__ nb,
-                             const uint32_t* __restrict__ ov, const uint32_t* __restrict__ ob, upbp_light_paths out) {
+// Packs path p's used slots at its scanned offsets (one thread per path; the slot-major
+// layout makes a wave's reads of slot k contiguous) and writes mPathEnds.
+__global__ void k_pack_paths(int64_t path_count, const upbp_light_vertex* __restrict__ sv, const float3* __restrict__ sp,
+                             const BeamRec* __restrict__ sb, const uint32_t* __restrict__ nv,
+                             const uint32_t* __restrict__ nb, const uint32_t* __restrict__ ov,
+                             const uint32_t* __restrict__ ob, upbp_light_paths out) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= path_count) return;
     const int64_t v0 = ov[p], b0 = ob[p];
-    for (uint32_t k = 0; k < nv[p]; ++k) {
-        out.vertices[v0 + k] = sv[p * cap_v + k];
-        for (int a = 0; a < 3; ++a) out.positions[3 * (v0 + k) + a] = sp[3 * (p * cap_v + k) + a];
+    const uint32_t n_v = nv[p], n_b = nb[p];
+    for (uint32_t k = 0; k < n_v; ++k) {
+        out.vertices[v0 + k] = sv[k * path_count + p];
+        const float3 x = sp[k * path_count + p];
+        out.positions[3 * (v0 + k)] = x.x;
+        out.positions[3 * (v0 + k) + 1] = x.y;
+        out.positions[3 * (v0 + k) + 2] = x.z;
     }
-    for (uint32_t k = 0; k < nb[p]; ++k) {
-        const BeamRec& r = sb[p * cap_b + k];
+    for (uint32_t k = 0; k < n_b; ++k) {
+        const BeamRec r = sb[k * path_count + p];
         const int64_t d = b0 + k;
         for (int a = 0; a < 3; ++a) {
             out.beam_origin[3 * d + a] = r.o[a];
@@ -455,7 +462,7 @@ __global__ void k_pack_paths(int64_t path_count, int32_t cap_v, int32_t cap_b, c
         m.light_vertex += (int32_t)v0;
         out.beam_mis[d] = m;
     }
-    out.path_ends[p] = (int32_t)(v0 + nv[p]);
+    out.path_ends[p] = (int32_t)(v0 + n_v);
 }
 
 __global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
@@ -488,7 +495,9 @@ void trace_light_paths(Ctx& ctx, const upbp_region* regions, int32_t n_regions, 
     const int32_t cap_b = (int32_t)((P.max_path_length - 1) * (2 * n_regions + 1));
     UPBP_REQUIRE(path_count * (int64_t)cap_v < ((int64_t)1 << 31) && path_count * (int64_t)cap_b < ((int64_t)1 << 31),
                  UPBP_EINVAL, "too many light paths for one call (path count x path length slots must fit 2^31)");
-    DevBuf dl, sv, sp, sb, nv, nb, ov, ob;
+    // the slots stay with the context (grow-only), so repeated calls allocate nothing
+    DevBuf &sv = ctx.lt_slots[0], &sp = ctx.lt_slots[1], &sb = ctx.lt_slots[2], &nv = ctx.lt_slots[3],
+           &nb = ctx.lt_slots[4], &ov = ctx.lt_slots[5], &ob = ctx.lt_slots[6], &dl = ctx.lt_slots[7];
     dl.ensure((size_t)n_lights * sizeof(upbp_point_light));
     UPBP_HIP(hipMemcpyAsync(dl.p, lights, (size_t)n_lights * sizeof(upbp_point_light), hipMemcpyHostToDevice,
                             ctx.stream));
@@ -516,11 +525,11 @@ void trace_light_paths(Ctx& ctx, const upbp_region* regions, int32_t n_regions, 
     out.beam_count = (int64_t)tail[2] + tail[3];
     UPBP_REQUIRE(out.vertex_count <= out.vertex_capacity && out.beam_count <= out.beam_capacity, UPBP_EINVAL,
                  "light path output capacity too small (vertex_count / beam_count hold what is needed)");
-    hipLaunchKernelGGL(k_pack_paths, dim3(blocks_for(path_count)), dim3(kBlock), 0, ctx.stream, path_count, cap_v,
-                       cap_b, sv.as<upbp_light_vertex>(), sp.as<float>(), sb.as<BeamRec>(), nv.as<uint32_t>(),
+    hipLaunchKernelGGL(k_pack_paths, dim3(blocks_for(path_count)), dim3(kBlock), 0, ctx.stream, path_count,
+                       sv.as<upbp_light_vertex>(), sp.as<float3>(), sb.as<BeamRec>(), nv.as<uint32_t>(),
                        nb.as<uint32_t>(), ov.as<uint32_t>(), ob.as<uint32_t>(), out);
     UPBP_HIP(hipGetLastError());
-    UPBP_HIP(hipStreamSynchronize(ctx.stream));  // the slots are freed on return
+    UPBP_HIP(hipStreamSynchronize(ctx.stream));
 }
 
 }  // namespace upbp

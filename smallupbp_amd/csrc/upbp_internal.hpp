@@ -92,6 +92,7 @@ struct Ctx {
     DevBuf qkeys, qkeys2, qperm, qperm2;
     DevBuf light_tails;  // float4 per light vertex of the current UPBP call
     DevBuf vm_rec;       // merge records (4 float4 per grid slot) of the current PP3D / SURF call
+    DevBuf lt_slots[8];  // light tracing: per-path vertex, position and beam slots, counts, offsets, lights
     DevBuf staging[8];  // host-input staging copies
     hipEvent_t ev[6] = {};
     // second stream: the photon and beam kernels of one eval call run concurrently
