@@ -124,7 +124,21 @@ typedef struct upbp_build_params {
     int32_t leaf_size;          /* max items per LBVH leaf, 1..32; 0 = library default */
     float beam_piece_factor;    /* beams only: target piece length in units of the max beam radius; 0 = default */
     int32_t max_pieces_per_beam;/* beams only: cap on pieces per beam, 1..256; 0 = default */
+    /* beams only: per-cell beam reduction of the reference's uniform grid (-gridmax / -gridred,
+     * Grid.hxx:167-279,409-631; plain BB1D only, as in the reference). A hit whose query point
+     * lies in a cell holding n > max_beams_in_cell beams is kept as the reduction type decides
+     * and weighted by 1 / (max_beams_in_cell / n). */
+    uint32_t max_beams_in_cell; /* sMaxBeamsInCell; 0 = no reduction (the reference default) */
+    uint32_t reduction_type;    /* sReductionType: UPBP_REDUCE_* */
+    int32_t grid_resolution;    /* cells on the grid's longest axis (sGridSize); 0 = 256 */
+    int32_t reduction_seed;     /* mSeed of the reduction's random numbers */
 } upbp_build_params;
+
+/* BeamReduction, SmallUPBP/src/Misc/Defs.hxx:164-170 */
+#define UPBP_REDUCE_PRESAMPLE 0u      /* a fixed random subset of max beams per cell */
+#define UPBP_REDUCE_OFFSET 1u         /* max consecutive beams of a shuffled cell from a random offset per query */
+#define UPBP_REDUCE_RESAMPLE_FIXED 2u /* max beams drawn with replacement per query */
+#define UPBP_REDUCE_RESAMPLE 3u       /* every beam kept with probability max / n per query */
 
 /* UPBP_KNN_RADIUS (photons: Bre.cxx:562-602; beams: PhBeams.cxx:77-117):
  * d2_knn is the squared distance to the knn-th nearest item of the kd-tree the
@@ -254,8 +268,9 @@ int64_t upbp_beams_piece_count(const upbp_beams* beams);
  * input order; each array holds beams->count floats. */
 upbp_status upbp_beams_radii(const upbp_beams* beams, float* start_radius, float* end_radius,
                              upbp_memory memory);
-/* Replaces PhotonBeamsEvaluator::getBeamSelectionPdf (PhBeams.hxx, Grid.hxx:409-415).
- * Without beam reduction (sMaxBeamsInCell = 0, the reference default) it is 1. */
+/* Replaces PhotonBeamsEvaluator::getBeamSelectionPdf (PhBeams.hxx, Grid.hxx:409-415): the
+ * selection pdf of the reduction grid's cell containing the position (1 without reduction,
+ * sMaxBeamsInCell = 0, the reference default). */
 float upbp_beams_selection_pdf(const upbp_beams* beams, const float position[3]);
 
 /* ---------------------------------------------------------------------------

@@ -161,14 +161,21 @@ public:
 
     // PhotonBeamsEvaluator::build (PhBeams.cxx:66-148) over beams in structure-of-arrays form:
     // mRay.origin, mRay.direction, mLength, mThroughputAtOrigin, mFlags and the medium index.
+    // maxBeamsInCell / reductionType / gridSize / seed: the static sMaxBeamsInCell,
+    // sReductionType and the grid settings of the reference (PhBeams.cxx:51-52,126-136).
     void build(const upbp_beam_input& beams, int radiusCalculation, float beamRadius, int knn,
-               bool verbose = false) {
+               bool verbose = false, uint32_t maxBeamsInCell = 0, uint32_t reductionType = 0, int gridSize = 256,
+               int seed = 1234) {
         (void)verbose;
         destroy();
         upbp_build_params p{};
         p.radius_calculation = radiusCalculation;
         p.radius = beamRadius;
         p.knn = knn;
+        p.max_beams_in_cell = maxBeamsInCell;
+        p.reduction_type = reductionType;
+        p.grid_resolution = gridSize;
+        p.reduction_seed = seed;
         check(upbp_beams_build(ctx_.get(), &beams, &p, &h_), ctx_.get());
     }
 

@@ -56,6 +56,10 @@ def lib():
         L.or_beams_count.restype = i64
         L.or_beams_count.argtypes = [vp]
         L.or_beams_grid_info.argtypes = [vp, vp]
+        L.or_beams_set_reduction.restype = C.c_int
+        L.or_beams_set_reduction.argtypes = [vp, C.c_uint32, C.c_uint32, i32]
+        L.or_beams_selection_pdf.restype = f32
+        L.or_beams_selection_pdf.argtypes = [vp, vp]
         L.or_beams_eval.restype = C.c_int
         L.or_beams_eval.argtypes = [vp, vp, i32, C.POINTER(_abi.QueryBatch), C.c_int, C.c_int,
                                     C.c_int, vp, vp, vp, vp]
@@ -146,6 +150,19 @@ class OracleBeams:
         a = self._arrays
         self.h = lib().or_beams_create(bm.count, *[_abi.ptr(x) for x in a], bm.radius, grid_res)
         self.n = bm.count
+
+    def set_reduction(self, max_beams_in_cell, reduction_type=0, seed=1234):
+        """Per-cell beam reduction (Grid::reduceBeams): or_beams_eval then applies it, with accel 1
+        per crowded cell of the DDA as the reference does, with accel 0 by the cell of each hit's
+        query point."""
+        r = lib().or_beams_set_reduction(self.h, max_beams_in_cell, reduction_type, seed)
+        if r:
+            raise RuntimeError("or_beams_set_reduction failed: %s" % _abi.STATUS_NAMES.get(r, r))
+        return self
+
+    def selection_pdf(self, pos):
+        p = np.ascontiguousarray(pos, np.float32)
+        return float(lib().or_beams_selection_pdf(self.h, p.ctypes.data))
 
     def grid_info(self):
         info = np.zeros(6, np.int64)
@@ -249,6 +266,19 @@ class OracleHashGrid:
 
     def count(self):
         return int(lib().or_hgrid_count(self.h))
+
+    def set_reduction(self, max_beams_in_cell, reduction_type=0, seed=1234):
+        """Per-cell beam reduction (Grid::reduceBeams): or_beams_eval then applies it, with accel 1
+        per crowded cell of the DDA as the reference does, with accel 0 by the cell of each hit's
+        query point."""
+        r = lib().or_beams_set_reduction(self.h, max_beams_in_cell, reduction_type, seed)
+        if r:
+            raise RuntimeError("or_beams_set_reduction failed: %s" % _abi.STATUS_NAMES.get(r, r))
+        return self
+
+    def selection_pdf(self, pos):
+        p = np.ascontiguousarray(pos, np.float32)
+        return float(lib().or_beams_selection_pdf(self.h, p.ctypes.data))
 
     def grid_info(self):
         """(bounding box (6,), mIndices, mCellEnds) of the built grid."""

@@ -147,7 +147,90 @@ struct or_beams {
     float* r2max;
     float* margins;
     or_grid grid;
+    /* per-cell beam reduction (Grid::reduceBeams, Grid.hxx:558-631); red_max 0 = none */
+    uint32_t red_max, red_type;
+    uint64_t red_seed;
 };
+
+/* ---- per-cell beam reduction -------------------------------------------------
+ * The reference draws its selections from a sequential Rng; here every random number is a
+ * counter-based hash of (seed, query ray, cell, draw), so a selection does not depend on
+ * the order queries run in. Permutation of a crowded cell (PRESAMPLE / OFFSET): its entries
+ * sorted by a hashed key of (seed, cell, beam), ties by beam index. */
+static inline uint64_t red_mix(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+static inline uint32_t red_perm_key(uint64_t seed, uint32_t c, uint32_t b) {
+    return (uint32_t)(red_mix(seed ^ red_mix(((uint64_t)c << 32) | b)) >> 32);
+}
+/* uniform in (0, 1): 24 hashed bits, centred */
+static inline float red_uniform(uint64_t seed, uint64_t qkey, uint32_t c, uint32_t t) {
+    uint64_t h = red_mix(seed + red_mix(qkey ^ red_mix(((uint64_t)c << 32) | t)));
+    return ((float)(uint32_t)(h >> 40) + 0.5f) * (1.0f / 16777216.0f);
+}
+/* (uint)(r * n), Grid.hxx:484,520 */
+static inline uint32_t red_index(float u, uint32_t n) {
+    uint32_t i = (uint32_t)(u * (float)n);
+    return i < n ? i : n - 1;
+}
+static uint64_t red_query_key(const float* o, const float* d) {
+    uint32_t bits[6];
+    memcpy(bits, o, 12);
+    memcpy(bits + 3, d, 12);
+    uint64_t k = 0x243F6A8885A308D3ull;
+    for (int i = 0; i < 6; ++i) k = red_mix(k ^ bits[i]);
+    return k;
+}
+/* Grid::pdf's cell of a position: posToVoxel clamp and index (Grid.hxx:71-116,409-415) */
+static uint32_t red_cell(const or_grid* g, const float* p) {
+    int v[3];
+    for (int a = 0; a < 3; ++a) {
+        int i = (int)((p[a] - g->lo[a]) * g->inv_cell[a]);
+        v[a] = i < 0 ? 0 : (i > g->res[a] - 1 ? g->res[a] - 1 : i);
+    }
+    return (uint32_t)(v[0] + g->res[0] * (v[1] + g->res[1] * v[2]));
+}
+/* place of beam b in the cell's permutation */
+static uint32_t red_rank(const or_beams* bm, uint32_t c, uint32_t begin, uint32_t n, uint32_t b) {
+    uint32_t kb = red_perm_key(bm->red_seed, c, b), r = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        uint32_t e = bm->grid.ptrs[begin + k];
+        uint32_t ke = red_perm_key(bm->red_seed, c, e);
+        if (ke < kb || (ke == kb && e < b)) ++r;
+    }
+    return r;
+}
+/* How many times the reduction tests the entry at `place` of crowded cell c's list (the
+ * reference's registration order) for the query: intersectPresampled / intersectOffsetted /
+ * intersectFixedSampled / intersectSampled, Grid.hxx:440-548. */
+static uint32_t red_mult(const or_beams* bm, uint64_t qkey, uint32_t c, uint32_t begin, uint32_t n, uint32_t place,
+                         float pdf) {
+    const uint32_t mx = bm->red_max, b = bm->grid.ptrs[begin + place];
+    if (bm->red_type == UPBP_REDUCE_PRESAMPLE) return red_rank(bm, c, begin, n, b) < mx;
+    if (bm->red_type == UPBP_REDUCE_OFFSET) {
+        uint32_t o = red_index(red_uniform(bm->red_seed, qkey, c, 0), n);
+        return ((red_rank(bm, c, begin, n, b) + n - o) % n) < mx;
+    }
+    if (bm->red_type == UPBP_REDUCE_RESAMPLE_FIXED) {
+        uint32_t m = 0;
+        for (uint32_t t = 0; t < mx; ++t) m += red_index(red_uniform(bm->red_seed, qkey, c, t), n) == place;
+        return m;
+    }
+    return red_uniform(bm->red_seed, qkey, c, b) < pdf;
+}
+/* A hit of beam b whose query point is p belongs to p's cell: times counted, pdf of the cell. */
+static uint32_t red_decide_at(const or_beams* bm, uint64_t qkey, const float* p, uint32_t b, float* pdf) {
+    const or_grid* g = &bm->grid;
+    uint32_t c = red_cell(g, p), begin = g->cells[c], n = g->cells[c + 1] - begin;
+    *pdf = 1.0f;
+    if (n <= bm->red_max) return 1;
+    *pdf = (float)bm->red_max / (float)n;
+    for (uint32_t k = 0; k < n; ++k)
+        if (g->ptrs[begin + k] == b) return red_mult(bm, qkey, c, begin, n, k, *pdf);
+    return 0; /* never tested there */
+}
 
 /* Slab test of a ray parameter interval against a box (culling only). */
 static inline int or_slab(const float* lo, const float* hi, const float* o, const float* inv,
@@ -582,6 +665,20 @@ or_beams* or_beams_create(int64_t n, const float* origin, const float* dir, cons
     }
     if (grid_res > 0 && n > 0) beams_build_grid(bm, grid_res);
     return bm;
+}
+
+int or_beams_set_reduction(or_beams* bm, uint32_t max_beams_in_cell, uint32_t reduction_type, int32_t seed) {
+    if (max_beams_in_cell && (!bm->grid.cells || reduction_type > UPBP_REDUCE_RESAMPLE)) return UPBP_EINVAL;
+    bm->red_max = max_beams_in_cell;
+    bm->red_type = reduction_type;
+    bm->red_seed = (uint64_t)(uint32_t)seed;
+    return 0;
+}
+
+float or_beams_selection_pdf(const or_beams* bm, const float* pos) {
+    if (!bm->red_max || !bm->grid.cells) return 1.0f;
+    uint32_t c = red_cell(&bm->grid, pos), n = bm->grid.cells[c + 1] - bm->grid.cells[c];
+    return n <= bm->red_max ? 1.0f : (float)bm->red_max / (float)n;
 }
 
 void or_beams_free(or_beams* bm) {
@@ -1058,6 +1155,7 @@ int or_adjoint(const or_photons* ph, const or_beams* bm, const upbp_medium* medi
 int or_upbp_eval(const or_photons* ph, const or_beams* bm, const upbp_medium* media, int32_t n_media,
                  const upbp_query_batch* q, const upbp_mis_inputs* mis, int which, int accel, int precision,
                  int threads, float* out32, double* out64, uint64_t* counters) {
+    if (bm && bm->red_max) return UPBP_EUNSUPPORTED; /* the UPBP weights take no reduction here */
     int e = check_batch(q, n_media);
     if (e) return e;
     if (!mis || (which == 0 && !ph) || (which == 1 && (!bm || !mis->beams))) return UPBP_EINVAL;

@@ -101,6 +101,13 @@ or_beams* or_beams_create(int64_t n, const float* origin, const float* dir, cons
                           int32_t grid_res);
 void or_beams_free(or_beams* bm);
 int64_t or_beams_count(const or_beams* bm);
+/* Per-cell beam reduction of the grid (Grid::reduceBeams, Grid.hxx:558-631; needs
+ * grid_res > 0). or_beams_eval then applies it: accel 1 as the reference does, per crowded
+ * cell of the DDA with the cell's ray interval; accel 0 by the cell of each hit's query
+ * point (the GPU's formulation). Random numbers are hashes of (seed, query ray, cell, draw). */
+int or_beams_set_reduction(or_beams* bm, uint32_t max_beams_in_cell, uint32_t reduction_type, int32_t seed);
+/* Grid::pdf (Grid.hxx:409-415): selection pdf of the cell holding pos. */
+float or_beams_selection_pdf(const or_beams* bm, const float* pos);
 /* Grid statistics: res[3], cells, pointers (registered beam references). */
 void or_beams_grid_info(const or_beams* bm, int64_t* info6);
 

@@ -19,6 +19,7 @@ STATUS_NAMES = {
 
 SHORT_BEAM = 1
 LONG_BEAM = 2
+REDUCE_PRESAMPLE, REDUCE_OFFSET, REDUCE_RESAMPLE_FIXED, REDUCE_RESAMPLE = 0, 1, 2, 3
 BPT, SURF, PP3D, PB2D, BB1D = 4, 8, 16, 32, 64
 NO_SINE_IN_WEIGHTS, PREVIOUS, COMPATIBLE, BB1D_PREVIOUS, SPECULAR_ONLY = 128, 256, 512, 1024, 2048
 ORIGIN_IN_MEDIUM, END_IN_MEDIUM = 1, 2
@@ -43,7 +44,8 @@ class VolumeSegment(C.Structure):
 class BuildParams(C.Structure):
     _fields_ = [("radius_calculation", C.c_int32), ("radius", C.c_float), ("knn", C.c_int32),
                 ("leaf_size", C.c_int32), ("beam_piece_factor", C.c_float),
-                ("max_pieces_per_beam", C.c_int32)]
+                ("max_pieces_per_beam", C.c_int32), ("max_beams_in_cell", C.c_uint32),
+                ("reduction_type", C.c_uint32), ("grid_resolution", C.c_int32), ("reduction_seed", C.c_int32)]
 
 
 class PhotonInput(C.Structure):

@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libupbp_hip.so")
-SOURCES = ["capi.hip", "build.hip", "knn.hip", "lbvh.hip", "hashgrid.hip", "traverse.hip", "lighttrace.hip"]
+SOURCES = ["capi.hip", "build.hip", "knn.hip", "lbvh.hip", "hashgrid.hip", "traverse.hip", "lighttrace.hip", "beamgrid.hip"]
 ARCH = os.environ.get("UPBP_OFFLOAD_ARCH", "gfx950")
 
 

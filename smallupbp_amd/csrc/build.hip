@@ -320,7 +320,10 @@ void beams_build(Ctx& ctx, const upbp_beam_input& in, const upbp_build_params& p
     out.ctx = &ctx;
     out.n = in.count;
     out.pieces = 0;
-    if (in.count == 0) return;
+    if (in.count == 0) {
+        beam_reduction_build(ctx, 0, nullptr, nullptr, nullptr, nullptr, nullptr, params, out);
+        return;
+    }
     UPBP_REQUIRE(in.origin && in.direction && in.length && in.throughput && in.flags && in.medium_id,
                  UPBP_EINVAL, "beam origin, direction, length, throughput, flags and medium_id are required");
     const int64_t n = in.count;
@@ -380,6 +383,8 @@ void beams_build(Ctx& ctx, const upbp_beam_input& in, const upbp_build_params& p
                        pieces.as<PieceDev>(), flux.as<float4>(), piece_beam.as<uint32_t>(), out.piece.as<PieceDev>(),
                        out.piece_flux.as<float4>(), out.piece_beam.as<uint32_t>());
     UPBP_HIP(hipGetLastError());
+    beam_reduction_build(ctx, n, b.origin, b.direction, b.length, out.r_start.as<float>(), out.r_end.as<float>(),
+                         params, out);
     UPBP_HIP(hipStreamSynchronize(st));
 }
 

@@ -218,6 +218,7 @@ BeamArgs beam_args(const Beams* b) {
     a.piece_flux = b->piece_flux.as<float4>();
     a.piece_beam = b->piece_beam.as<uint32_t>();
     a.pieces_n = b->pieces;
+    a.red = b->red;
     return a;
 }
 
@@ -542,11 +543,13 @@ upbp_status upbp_beams_radii(const upbp_beams* beams, float* start_radius, float
 }
 
 float upbp_beams_selection_pdf(const upbp_beams* beams, const float position[3]) {
-    // No per-cell beam reduction is performed (the reference default
-    // sMaxBeamsInCell = 0, PhBeams.cxx:51), so every beam is tested: pdf 1.
-    (void)beams;
-    (void)position;
-    return 1.0f;
+    // 1 without reduction (the reference default sMaxBeamsInCell = 0, PhBeams.cxx:51)
+    if (!beams || !position) return 1.0f;
+    try {
+        return beam_selection_pdf(*beams, position);
+    } catch (...) {
+        return 1.0f;
+    }
 }
 
 upbp_status upbp_eval(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams* beams, const upbp_query_batch* queries,
@@ -569,6 +572,8 @@ upbp_status upbp_eval_upbp(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams*
     return guarded(ctx, [&] {
         check_batch(*ctx, queries);
         UPBP_REQUIRE(mis != nullptr, UPBP_EINVAL, "mis inputs are required");
+        UPBP_REQUIRE(!beams || beams->red.max == 0, UPBP_EUNSUPPORTED,
+                     "per-cell beam reduction is supported by upbp_eval and upbp_hit_signatures only");
         UPBP_REQUIRE(!bre || out_photons, UPBP_EINVAL, "out_photons is NULL but a photon structure was given");
         UPBP_REQUIRE(!beams || out_beams, UPBP_EINVAL, "out_beams is NULL but a beam structure was given");
         const int64_t Q = queries->count;
@@ -909,6 +914,8 @@ upbp_status upbp_adjoint(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams* b
         check_batch(*ctx, queries);
         UPBP_REQUIRE(queries->beam_type == UPBP_LONG_BEAM, UPBP_EUNSUPPORTED,
                      "the adjoint supports LONG query beams only");
+        UPBP_REQUIRE(!beams || beams->red.max == 0, UPBP_EUNSUPPORTED,
+                     "per-cell beam reduction is supported by upbp_eval and upbp_hit_signatures only");
         const int64_t Q = queries->count;
         const int M = ctx->n_media;
         const bool do_ph = bre && grad_out_photons && bre->n > 0;
@@ -1057,6 +1064,8 @@ upbp_status upbp_batch_plan_create(upbp_ctx* ctx, const upbp_instance* instances
             UPBP_REQUIRE(in.queries.memory == UPBP_MEM_DEVICE, UPBP_EINVAL, "batched queries must be device memory");
             UPBP_REQUIRE(!in.bre || in.out_photons, UPBP_EINVAL, "instance out_photons is NULL");
             UPBP_REQUIRE(!in.beams || in.out_beams, UPBP_EINVAL, "instance out_beams is NULL");
+            UPBP_REQUIRE(!in.beams || in.beams->red.max == 0, UPBP_EUNSUPPORTED,
+                         "per-cell beam reduction is supported by upbp_eval and upbp_hit_signatures only");
             UPBP_REQUIRE(in.media_base >= 0 && in.media_base < ctx->n_media, UPBP_EINVAL, "media_base out of range");
             const Bre* b = in.bre;
             const Beams* m = in.beams;

@@ -67,6 +67,8 @@ struct EvalArgs {
     MisGlobals mis;
     // traversal starts here (0, or an instance's root in a batched plan's merged tree array)
     uint32_t root;
+    // per-cell beam reduction of the beam estimators (red.max == 0: none)
+    RedDev red;
 };
 
 __device__ __forceinline__ float guard_inv(float d) {
@@ -128,6 +130,10 @@ struct SegState {
     float gs0, gs1, gs2;  // d/d sigma_s direct
     float gt0, gt1, gt2;  // d/d sigma_t
 };
+
+// A MODE of Forward or Signature with this bit set applies the per-cell beam reduction
+// (a separate instantiation: the plain kernels keep their registers).
+constexpr int kReduced = 8;
 
 template <int MODE>
 struct LaneOut {
@@ -493,7 +499,7 @@ __device__ __forceinline__ void photon_leaf(const EvalArgs& A, const Ray& r, Seg
         const float4 p = A.pos_r2[i];  // scalar load, wave-uniform index
         float t, sq;
         const bool hit = test_bre(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, s.tmin, s.tmax, p.x, p.y, p.z, p.w, t, sq);
-        if (MODE == (int)Mode::Upbp) {
+        if ((MODE & 7) == (int)Mode::Upbp) {
             const uint64_t m = __ballot(hit);
             if (m) {
                 const uint32_t lv_idx = A.orig[i];
@@ -535,7 +541,7 @@ __device__ __forceinline__ void photon_leaf(const EvalArgs& A, const Ray& r, Seg
                 // base_c = a_c * scale; contribution_c = power_c * base_c * sigma_s_c
                 const float b0 = a0 * scale, b1 = a1 * scale, b2 = a2 * scale;
                 const float c0 = w.x * b0 * s.ss0, c1 = w.y * b1 * s.ss1, c2 = w.z * b2 * s.ss2;
-                if (MODE == (int)Mode::Adjoint) {
+                if ((MODE & 7) == (int)Mode::Adjoint) {
                     // s.w = dL/dout * P (prefix transmittance of earlier segments)
                     gp0 = s.w0 * b0 * s.ss0;
                     gp1 = s.w1 * b1 * s.ss1;
@@ -555,12 +561,12 @@ __device__ __forceinline__ void photon_leaf(const EvalArgs& A, const Ray& r, Seg
                     s.acc2 += c2;
                 }
                 lo.hits++;
-                if (MODE == (int)Mode::Signature) {
+                if ((MODE & 7) == (int)Mode::Signature) {
                     lo.sig_n++;
                     lo.sig_h += hash32(A.orig[i]);
                 }
             }
-            if (MODE == (int)Mode::Adjoint) add_item_gradient(A.grad_items, A.orig[i], gp0, gp1, gp2);
+            if ((MODE & 7) == (int)Mode::Adjoint) add_item_gradient(A.grad_items, A.orig[i], gp0, gp1, gp2);
         }
     }
 }
@@ -581,7 +587,7 @@ __device__ __forceinline__ void beam_leaf(const EvalArgs& A, const Ray& r, SegSt
                          test_beam_beam(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, r.d1o1, s.tmin, s.tmax, pc.p0.x, pc.p0.y,
                                         pc.p0.z, pc.p1.x, pc.p1.y, pc.p1.z, pc.p3.x, pc.p0.w, pc.p2.w, pc.p2.x,
                                         pc.p2.y, pc.p1.w, pc.p3.y, dist, st, t1, t2, radius);
-        if (MODE == (int)Mode::Upbp) {
+        if ((MODE & 7) == (int)Mode::Upbp) {
             const uint64_t m = __ballot(hit);
             if (m) {
                 if (hit) {
@@ -601,10 +607,20 @@ __device__ __forceinline__ void beam_leaf(const EvalArgs& A, const Ray& r, SegSt
             }
             continue;
         }
-        if (__ballot(hit)) {
+        float red_w = 1.0f;
+        uint32_t red_n = 1;
+        if ((MODE & kReduced) && hit) {
+            // the hit belongs to the reduction grid cell of its query point
+            const uint32_t bits[6] = {__float_as_uint(r.ox), __float_as_uint(r.oy), __float_as_uint(r.oz),
+                                      __float_as_uint(r.dx), __float_as_uint(r.dy), __float_as_uint(r.dz)};
+            red_n = red_decide(A.red, red_query_key(bits), r.ox + r.dx * t1, r.oy + r.dy * t1, r.oz + r.dz * t1,
+                               A.orig[i], red_w);
+            red_w *= (float)red_n;
+        }
+        if (__ballot(hit && red_n != 0)) {
             const float4 flux = A.piece_flux[i];
             float gb0 = 0.f, gb1 = 0.f, gb2 = 0.f;
-            if (hit) {
+            if (hit && red_n != 0) {
                 const bool short_b = (word & kShortBeamBit) != 0;
                 const float dq = t1 - s.tmin;
                 const float stk = pick3(s.kidx, s.st0, s.st1, s.st2);
@@ -617,10 +633,11 @@ __device__ __forceinline__ void beam_leaf(const EvalArgs& A, const Ray& r, SegSt
                                        : phase_pdf(s.g, false, -dot3(pc.p1.x, pc.p1.y, pc.p1.z, r.dx, r.dy, r.dz));
                 // 1D kernel (1 - dist^2/r^2) * 3 / (4 r sin(theta)), PhotonBeam.hxx:364-365
                 const float kern = 1.0f - (dist * dist) * __builtin_amdgcn_rcpf(radius * radius);
-                const float scale = (ph * kern * 3.0f) * __builtin_amdgcn_rcpf((4.0f * radius) * st);
+                float scale = (ph * kern * 3.0f) * __builtin_amdgcn_rcpf((4.0f * radius) * st);
+                if (MODE & kReduced) scale *= red_w;
                 const float b0 = a0 * scale, b1 = a1 * scale, b2 = a2 * scale;
                 const float c0 = flux.x * s.ss0 * b0, c1 = flux.y * s.ss1 * b1, c2 = flux.z * s.ss2 * b2;
-                if (MODE == (int)Mode::Adjoint) {
+                if ((MODE & 7) == (int)Mode::Adjoint) {
                     gb0 = s.w0 * b0 * s.ss0;
                     gb1 = s.w1 * b1 * s.ss1;
                     gb2 = s.w2 * b2 * s.ss2;
@@ -650,12 +667,12 @@ __device__ __forceinline__ void beam_leaf(const EvalArgs& A, const Ray& r, SegSt
                     s.acc2 += c2;
                 }
                 lo.hits++;
-                if (MODE == (int)Mode::Signature) {
-                    lo.sig_n++;
-                    lo.sig_h += hash32(A.orig[i]);
+                if ((MODE & 7) == (int)Mode::Signature) {
+                    lo.sig_n += red_n;
+                    lo.sig_h += red_n * hash32(A.orig[i]);
                 }
             }
-            if (MODE == (int)Mode::Adjoint) add_item_gradient(A.grad_items, A.orig[i], gb0, gb1, gb2);
+            if ((MODE & 7) == (int)Mode::Adjoint) add_item_gradient(A.grad_items, A.orig[i], gb0, gb1, gb2);
         }
     }
 }
@@ -781,13 +798,13 @@ __device__ void run_tile(const EvalArgs& A, int64_t slot0, unsigned long long* l
     float P0 = 1.f, P1 = 1.f, P2 = 1.f;
     float R0 = 0.f, R1 = 0.f, R2 = 0.f;
     float W0 = 0.f, W1 = 0.f, W2 = 0.f;
-    if (MODE == (int)Mode::Adjoint && valid) {
+    if ((MODE & 7) == (int)Mode::Adjoint && valid) {
         W0 = A.grad_out[3 * qi];
         W1 = A.grad_out[3 * qi + 1];
         W2 = A.grad_out[3 * qi + 2];
     }
     LaneOut<MODE> lo;
-    if (MODE == (int)Mode::Upbp) {
+    if ((MODE & 7) == (int)Mode::Upbp) {
         // per-query AdditionalRayDataForMis (Bre.cxx:842-849 runs it per segment below)
         lo.ml.tail = CamTail{0.f, 0.f, 0u};
         lo.ml.cam_len = 1;
@@ -812,7 +829,7 @@ __device__ void run_tile(const EvalArgs& A, int64_t slot0, unsigned long long* l
             }
         }
     }
-    if (MODE == (int)Mode::Upbp) {
+    if ((MODE & 7) == (int)Mode::Upbp) {
         const int w = threadIdx.x / kWave;
         g_hacc[w][0][lane] = 0.f;
         g_hacc[w][1][lane] = 0.f;
@@ -879,7 +896,7 @@ __device__ void run_tile(const EvalArgs& A, int64_t slot0, unsigned long long* l
                     s.mfp = m->mfp;
                     s.cont = m->cont;
                 }
-                if (MODE == (int)Mode::Upbp)
+                if ((MODE & 7) == (int)Mode::Upbp)
                     lo.ml.seg_flags = UPBP_END_IN_MEDIUM | (k == 0 ? lo.ml.ray_flags : 0u);
                 if (A.lite) {
                     const float len = tmax - tmin;
@@ -889,20 +906,20 @@ __device__ void run_tile(const EvalArgs& A, int64_t slot0, unsigned long long* l
                 }
             }
         }
-        if (MODE == (int)Mode::Adjoint) {
+        if ((MODE & 7) == (int)Mode::Adjoint) {
             s.w0 = W0 * P0;
             s.w1 = W1 * P1;
             s.w2 = W2 * P2;
         }
         if (__ballot(scatter)) {
             traverse<EST, MODE>(A, r, s, lo, short_q, nodes_visited, tests, overflow);
-            if (MODE == (int)Mode::Upbp) upbp_segment_end<EST, MODE>(A, r, s, lo, short_q);
+            if ((MODE & 7) == (int)Mode::Upbp) upbp_segment_end<EST, MODE>(A, r, s, lo, short_q);
         }
         if (has) {
             R0 += P0 * s.acc0;
             R1 += P1 * s.acc1;
             R2 += P2 * s.acc2;
-            if (MODE == (int)Mode::Adjoint && scatter) {
+            if ((MODE & 7) == (int)Mode::Adjoint && scatter) {
                 // medium gradients of this segment plus the prefix terms of earlier segments
                 const int mloc = A.media_base + s.medium;
                 float* slab_m = (lds_media && s.medium < A.lds_media_n) ? lds_media : nullptr;
@@ -952,7 +969,7 @@ __device__ void run_tile(const EvalArgs& A, int64_t slot0, unsigned long long* l
                 P2 *= a2 / pdf;
             }
             if (!(P0 > 1e-20f && P1 > 1e-20f && P2 > 1e-20f)) alive = false;
-            if (MODE == (int)Mode::Upbp) {
+            if ((MODE & 7) == (int)Mode::Upbp) {
                 // ray-sampling pdf prefixes of the segment loop (Bre.cxx:871-873, lite :955-958)
                 if (A.lite) {
                     float rev = 1.f, spdf = 1.f;
@@ -973,8 +990,8 @@ __device__ void run_tile(const EvalArgs& A, int64_t slot0, unsigned long long* l
         }
     }
     err |= lo.err;
-    if (valid && MODE != (int)Mode::Adjoint) {
-        if (MODE == (int)Mode::Signature) {
+    if (valid && (MODE & 7) != (int)Mode::Adjoint) {
+        if ((MODE & 7) == (int)Mode::Signature) {
             A.sig[2 * qi] = lo.sig_n;
             A.sig[2 * qi + 1] = lo.sig_h;
         } else {
@@ -1022,7 +1039,7 @@ __device__ __forceinline__ int64_t xcd_block(int64_t bid, int64_t nblk) {
 // register spill cost less than the latency the extra wave hides (measured 3-4 %).
 // UPBP kernels ask for 4 (128 VGPRs), which both fit without spilling.
 template <int EST, int MODE>
-__global__ __launch_bounds__(256, (MODE == (int)Mode::Forward ? 8 : MODE == (int)Mode::Upbp ? 4 : 1)) void k_eval(
+__global__ __launch_bounds__(256, ((MODE & kReduced) ? 6 : (MODE & 7) == (int)Mode::Forward ? 8 : (MODE & 7) == (int)Mode::Upbp ? 4 : 1)) void k_eval(
     EvalArgs A) {
     const int wave_in_block = threadIdx.x / kWave;
     const int waves_per_block = blockDim.x / kWave;
@@ -1032,13 +1049,13 @@ __global__ __launch_bounds__(256, (MODE == (int)Mode::Forward ? 8 : MODE == (int
     // adjoint: per-lane medium gradient slabs, [lds_media_n][6][64] floats per wave, sized at launch
     extern __shared__ float lds_dyn[];
     float* my_lds = nullptr;
-    if (MODE == (int)Mode::Adjoint) {
+    if ((MODE & 7) == (int)Mode::Adjoint) {
         const int slab = A.lds_media_n * 6 * kWave;
         my_lds = lds_dyn + (size_t)wave_in_block * slab;
         for (int k = threadIdx.x & (kWave - 1); k < slab; k += kWave) my_lds[k] = 0.f;
     }
     if (wave * kWave < A.count) run_tile<EST, MODE>(A, wave * kWave, ls, my_lds);
-    if (MODE == (int)Mode::Adjoint) {
+    if ((MODE & 7) == (int)Mode::Adjoint) {
         // reduce the per-lane medium slabs of this wave and add them once
         const int lane = threadIdx.x & (kWave - 1);
         const int nm = A.lds_media_n;
@@ -1165,9 +1182,23 @@ template <int EST>
 void launch_mode(Ctx& ctx, Mode mode, const EvalArgs& A, unsigned blocks, unsigned threads) {
     switch (mode) {
         case Mode::Forward:
+            if constexpr (EST == kEstBeams) {
+                if (A.red.max != 0) {
+                    hipLaunchKernelGGL((k_eval<EST, (int)Mode::Forward | kReduced>), dim3(blocks), dim3(threads), 0,
+                                       ctx.stream, A);
+                    break;
+                }
+            }
             hipLaunchKernelGGL((k_eval<EST, (int)Mode::Forward>), dim3(blocks), dim3(threads), 0, ctx.stream, A);
             break;
         case Mode::Signature:
+            if constexpr (EST == kEstBeams) {
+                if (A.red.max != 0) {
+                    hipLaunchKernelGGL((k_eval<EST, (int)Mode::Signature | kReduced>), dim3(blocks), dim3(threads), 0,
+                                       ctx.stream, A);
+                    break;
+                }
+            }
             hipLaunchKernelGGL((k_eval<EST, (int)Mode::Signature>), dim3(blocks), dim3(threads), 0, ctx.stream, A);
             break;
         case Mode::Adjoint: {
@@ -1288,6 +1319,7 @@ void launch_beams(Ctx& ctx, Mode mode, const BeamArgs& bm, const QueryView& q, f
     A.pieces = bm.pieces;
     A.piece_flux = bm.piece_flux;
     A.orig = bm.piece_beam;
+    A.red = bm.red;
     A.out = out;
     A.sig = sig;
     if (adj) {

@@ -10,6 +10,7 @@
 
 #include "../../include/upbp.h"
 #include "upbp_device.hpp"
+#include "beamgrid.hpp"
 
 namespace upbp {
 
@@ -135,6 +136,10 @@ struct Beams {
     DevBuf piece_flux;    // float4 [pieces] throughput (rgb, 0)
     DevBuf piece_beam;    // uint32 [pieces] original beam index
     DevBuf r_start, r_end;  // float [n] start / end radius the build gave each beam
+    // per-cell beam reduction (max_beams_in_cell > 0): the reference grid's cell lists
+    RedDev red;            // device view; red.max == 0 without reduction
+    int64_t red_cells = 0;
+    DevBuf red_off, red_ids, red_rank;
 };
 
 // Device-resident view of a query batch.
@@ -185,6 +190,14 @@ void bre_build(Ctx& ctx, const upbp_photon_input& in, const upbp_build_params& p
 void bre_set_power(Bre& bre, const float* power_dev);
 void beams_build(Ctx& ctx, const upbp_beam_input& in, const upbp_build_params& params, Beams& out);
 
+// ---- beamgrid.hip ---------------------------------------------------------
+// Registers the beams (device arrays, input order) in the reference's uniform grid and
+// keeps each cell's beam list and reduction ranks in out.red*; a no-op without reduction.
+void beam_reduction_build(Ctx& ctx, int64_t n, const float* origin, const float* dir, const float* length,
+                          const float* r_start, const float* r_end, const upbp_build_params& params, Beams& out);
+// Selection pdf of the cell holding pos (1 without reduction).
+float beam_selection_pdf(const Beams& b, const float pos[3]);
+
 // ---- traverse.hip ---------------------------------------------------------
 enum class Mode : int { Forward = 0, Signature = 1, Adjoint = 2, Upbp = 3 };
 
@@ -202,6 +215,7 @@ struct BeamArgs {
     const float4* piece_flux;
     const uint32_t* piece_beam;
     int64_t pieces_n;
+    RedDev red;
 };
 struct AdjointOut {
     const float* grad_out;  // [Q][3]

@@ -297,8 +297,10 @@ class PhotonBeamsEvaluator:
 
     def build(self, beams: Beams, radius: Optional[float] = None, start_radius=None, end_radius=None,
               leaf_size: int = 0, piece_factor: float = 0.0, max_pieces: int = 0,
-              radius_calculation: int = _abi.CONSTANT_RADIUS, knn: int = 0):
-        """PhotonBeamsEvaluator::build (PhBeams.cxx:66-148)."""
+              radius_calculation: int = _abi.CONSTANT_RADIUS, knn: int = 0, max_beams_in_cell: int = 0,
+              reduction_type: int = 0, grid_resolution: int = 0, reduction_seed: int = 1234):
+        """PhotonBeamsEvaluator::build (PhBeams.cxx:66-148); max_beams_in_cell > 0 turns on the
+        per-cell beam reduction of the reference grid (-gridmax / -gridred)."""
         assert self.h is None, "destroy() before building again"
         arrs = [_arr(beams.origin, np.float32), _arr(beams.direction, np.float32), _arr(beams.length, np.float32),
                 _arr(beams.throughput, np.float32), _arr(beams.flags, np.uint32), _arr(beams.medium_id, np.int32),
@@ -306,7 +308,8 @@ class PhotonBeamsEvaluator:
         dev = any(_is_torch(a) and a.is_cuda for a in arrs if a is not None)
         bin_ = _abi.BeamInput(beams.count, *[_abi.ptr(a) for a in arrs], _abi.MEM_DEVICE if dev else _abi.MEM_HOST)
         r = beams.radius if radius is None else radius
-        params = _abi.BuildParams(radius_calculation, float(r), knn, leaf_size, float(piece_factor), int(max_pieces))
+        params = _abi.BuildParams(radius_calculation, float(r), knn, leaf_size, float(piece_factor), int(max_pieces),
+                                  int(max_beams_in_cell), int(reduction_type), int(grid_resolution), int(reduction_seed))
         h = C.c_void_p()
         self.ctx.check(self.ctx.lib.upbp_beams_build(self.ctx.h, C.byref(bin_), C.byref(params), C.byref(h)))
         self.h = h

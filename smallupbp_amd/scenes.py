@@ -30,6 +30,8 @@ import numpy as np
 
 SHORT_BEAM = 1
 LONG_BEAM = 2
+# per-cell beam reduction types (UPBP_REDUCE_*)
+REDUCE_PRESAMPLE, REDUCE_OFFSET, REDUCE_RESAMPLE_FIXED, REDUCE_RESAMPLE = 0, 1, 2, 3
 
 MEDIUM_DTYPE = np.dtype(
     [
