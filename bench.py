@@ -14,7 +14,8 @@ kernel (algorithmic bytes / average HIP-event kernel time vs 8 TB/s), the CPU
 oracle timed on a bounded sample on the host cores, the forward+adjoint vs
 forward cost ratio on the 512x512 differentiable workload, the 64-instance
 batched speedup, the MIS-weighted (UPBP) rate at 1920x1080, the light sub-path
-tracing rate, and the PP3D and SURF vertex-merging rates.
+tracing rate, the beam estimator under each per-cell beam reduction type, and the
+PP3D and SURF vertex-merging rates.
 """
 from __future__ import annotations
 
@@ -431,6 +432,33 @@ def light_trace_measure(steps: int, cpu_budget: float, threads: int, paths: int 
     return rec
 
 
+def reduction_measure(ctx, w, qv, leaf_bm: int, piece_factor: float, steps: int, max_in_cell: int = 8):
+    """Per-cell beam reduction (-gridmax 8, each -gridred type, grid 256) on the headline
+    1920x1080 workload: beam-estimator time per step and the reduction grid's build time, next
+    to the unreduced structure built the same way."""
+    import torch
+
+    from smallupbp_amd import native
+
+    Q = w.queries.count
+    out_b = torch.empty((Q, 3), device="cuda")
+    rec = {"workload": "bre_beams_1080p beams only, max_beams_in_cell %d, grid 256" % max_in_cell}
+    names = ["none", "presample", "offset", "resample_fixed", "resample"]
+    for rtype in (-1, 0, 1, 2, 3):
+        bm = native.PhotonBeamsEvaluator(ctx)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        bm.build(w.beams, leaf_size=leaf_bm, piece_factor=piece_factor,
+                 max_beams_in_cell=max_in_cell if rtype >= 0 else 0, reduction_type=max(rtype, 0))
+        t_build = time.perf_counter() - t
+        el, per = time_eval(ctx, None, bm, qv, None, out_b, steps, 1)
+        rec[names[rtype + 1]] = {"build_ms": round(t_build * 1e3, 2),
+                                 "kernel_ms": round(float(np.mean([p["kernel_ms_beams"] for p in per])), 3),
+                                 "beam_hits": int(per[-1]["beam_hits"])}
+        bm.destroy()
+    return rec
+
+
 def calibrate_copy(nbytes: int = 1 << 30):
     """A plain device copy of known size, so PMC runs can scale FETCH_SIZE / WRITE_SIZE."""
     import torch
@@ -462,6 +490,7 @@ def main():
     ap.add_argument("--no-vm", action="store_true", help="skip the PP3D and SURF vertex-merging measurements")
     ap.add_argument("--no-upbp", action="store_true", help="skip the MIS-weighted (UPBP) measurement")
     ap.add_argument("--no-lighttrace", action="store_true", help="skip the light sub-path tracing measurement")
+    ap.add_argument("--no-reduction", action="store_true", help="skip the per-cell beam reduction measurement")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -598,6 +627,9 @@ def main():
 
     if not args.no_lighttrace:
         rec["lighttrace"] = light_trace_measure(max(3, args.steps // 2), min(args.cpu_budget, 5.0), host_cores())
+
+    if not args.no_reduction:
+        rec["beam_reduction"] = reduction_measure(ctx, w, qv, leaf_bm, piece_factor, max(3, args.steps // 2))
 
     if not args.no_vm:
         from smallupbp_amd import scenes as _sc
