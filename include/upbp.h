@@ -272,6 +272,11 @@ upbp_status upbp_beams_radii(const upbp_beams* beams, float* start_radius, float
  * selection pdf of the reduction grid's cell containing the position (1 without reduction,
  * sMaxBeamsInCell = 0, the reference default). */
 float upbp_beams_selection_pdf(const upbp_beams* beams, const float position[3]);
+/* getBeamSelectionPdf for many positions at once (xyz [count][3] -> out_pdf [count], both in
+ * `memory`): the loop that sets mBB1DBeamSelectionPdf of every in-medium light vertex after the
+ * beam build (SmallUPBP/src/Renderers/UPBP.hxx:723-728) in one launch. */
+upbp_status upbp_beams_selection_pdfs(const upbp_beams* beams, const float* positions, int64_t count, float* out_pdf,
+                                      upbp_memory memory);
 
 /* ---------------------------------------------------------------------------
  * Evaluation. out_photons receives, per query, EmbreeBre::evalBre

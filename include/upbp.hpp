@@ -196,6 +196,10 @@ public:
 
     // PhotonBeamsEvaluator::getBeamSelectionPdf (1 without per-cell beam reduction).
     float getBeamSelectionPdf(const float position[3]) const { return upbp_beams_selection_pdf(h_, position); }
+    // The same for n positions ([n][3] -> [n]) in one launch.
+    void getBeamSelectionPdfs(const float* positions, int64_t n, float* out, upbp_memory mem = UPBP_MEM_HOST) const {
+        check(upbp_beams_selection_pdfs(h_, positions, n, out, mem), ctx_.get());
+    }
 
     void destroy() {
         if (h_) upbp_beams_destroy(h_);

@@ -166,6 +166,18 @@ __global__ void k_grid_ranks(int64_t m, const uint64_t* sorted_key, const uint32
     rank[sorted_val[k]] = (uint32_t)k - off[c];
 }
 
+__global__ void k_selection_pdfs(int64_t n, RedDev R, const float* pos, float* out) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float pdf = 1.0f;
+    if (R.max != 0) {
+        const uint32_t c = red_cell(R, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+        const uint32_t cnt = R.off[c + 1] - R.off[c];
+        if (cnt > R.max) pdf = (float)R.max / (float)cnt;
+    }
+    out[i] = pdf;
+}
+
 unsigned blocks_for(int64_t n) { return (unsigned)((n + kGridBlock - 1) / kGridBlock); }
 
 int bits_for(uint64_t v) {
@@ -315,6 +327,26 @@ float beam_selection_pdf(const Beams& b, const float pos[3]) {
     UPBP_HIP(hipMemcpy(w, b.red.off + c, 8, hipMemcpyDeviceToHost));
     const uint32_t n = w[1] - w[0];
     return n <= b.red.max ? 1.0f : (float)b.red.max / (float)n;
+}
+
+void beam_selection_pdfs(const Beams& b, const float* pos, int64_t n, float* out, upbp_memory mem) {
+    if (n == 0) return;
+    Ctx& ctx = *b.ctx;
+    hipStream_t st = ctx.stream;
+    DevBuf in_buf, out_buf;
+    const float* p = pos;
+    float* o = out;
+    if (mem == UPBP_MEM_HOST) {
+        in_buf.ensure(n * 12);
+        out_buf.ensure(n * 4);
+        UPBP_HIP(hipMemcpyAsync(in_buf.p, pos, n * 12, hipMemcpyHostToDevice, st));
+        p = in_buf.as<float>();
+        o = out_buf.as<float>();
+    }
+    hipLaunchKernelGGL(k_selection_pdfs, dim3(blocks_for(n)), dim3(kGridBlock), 0, st, n, b.red, p, o);
+    UPBP_HIP(hipGetLastError());
+    if (mem == UPBP_MEM_HOST) UPBP_HIP(hipMemcpyAsync(out, o, n * 4, hipMemcpyDeviceToHost, st));
+    UPBP_HIP(hipStreamSynchronize(st));
 }
 
 }  // namespace upbp

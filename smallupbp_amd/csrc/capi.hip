@@ -552,6 +552,19 @@ float upbp_beams_selection_pdf(const upbp_beams* beams, const float position[3])
     }
 }
 
+upbp_status upbp_beams_selection_pdfs(const upbp_beams* beams, const float* positions, int64_t count, float* out_pdf,
+                                      upbp_memory memory) {
+    if (!beams) return fail(nullptr, UPBP_EINVAL, "beams is NULL");
+    Ctx* ctx = beams->ctx;
+    return guarded(ctx, [&] {
+        UPBP_REQUIRE(count >= 0 && count < ((int64_t)1 << 40), UPBP_EINVAL, "count out of range");
+        UPBP_REQUIRE(count == 0 || (positions && out_pdf), UPBP_EINVAL, "positions and out_pdf are required");
+        UPBP_REQUIRE(memory == UPBP_MEM_HOST || memory == UPBP_MEM_DEVICE, UPBP_EINVAL, "bad memory kind");
+        if (count == 0 || !ctx) return;
+        beam_selection_pdfs(*beams, positions, count, out_pdf, memory);
+    });
+}
+
 upbp_status upbp_eval(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams* beams, const upbp_query_batch* queries,
                       float* out_photons, float* out_beams, upbp_memory out_memory, upbp_eval_stats* stats) {
     if (!ctx) return fail(nullptr, UPBP_EINVAL, "ctx is NULL");

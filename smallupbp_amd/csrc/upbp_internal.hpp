@@ -197,6 +197,8 @@ void beam_reduction_build(Ctx& ctx, int64_t n, const float* origin, const float*
                           const float* r_start, const float* r_end, const upbp_build_params& params, Beams& out);
 // Selection pdf of the cell holding pos (1 without reduction).
 float beam_selection_pdf(const Beams& b, const float pos[3]);
+// The same for n positions (xyz [n][3]) into out [n], both in `mem`.
+void beam_selection_pdfs(const Beams& b, const float* pos, int64_t n, float* out, upbp_memory mem);
 
 // ---- traverse.hip ---------------------------------------------------------
 enum class Mode : int { Forward = 0, Signature = 1, Adjoint = 2, Upbp = 3 };

@@ -28,7 +28,7 @@ EXPORTED = [
     "upbp_ctx_create", "upbp_ctx_destroy", "upbp_ctx_last_error", "upbp_ctx_set_stream",
     "upbp_ctx_get_stream", "upbp_ctx_set_sync", "upbp_set_media", "upbp_bre_build", "upbp_bre_destroy",
     "upbp_bre_set_power", "upbp_bre_count", "upbp_beams_build", "upbp_beams_destroy", "upbp_beams_count",
-    "upbp_beams_piece_count", "upbp_beams_selection_pdf", "upbp_eval", "upbp_hit_signatures",
+    "upbp_beams_piece_count", "upbp_beams_selection_pdf", "upbp_beams_selection_pdfs", "upbp_eval", "upbp_hit_signatures",
     "upbp_adjoint", "upbp_batch_plan_create", "upbp_batch_plan_destroy", "upbp_batch_run",
     "upbp_set_tuning", "upbp_get_tuning", "upbp_autotune", "upbp_eval_upbp", "upbp_bre_radii",
     "upbp_beams_radii", "upbp_hash_grid_build", "upbp_hash_grid_destroy", "upbp_hash_grid_count",
@@ -98,6 +98,8 @@ def load():
     L.upbp_beams_piece_count.argtypes = [vp]
     L.upbp_beams_selection_pdf.restype = C.c_float
     L.upbp_beams_selection_pdf.argtypes = [vp, vp]
+    L.upbp_beams_selection_pdfs.restype = C.c_int
+    L.upbp_beams_selection_pdfs.argtypes = [vp, vp, C.c_int64, vp, C.c_int]
     L.upbp_eval.restype = st
     L.upbp_eval.argtypes = [vp, vp, vp, C.POINTER(_abi.QueryBatch), vp, vp, C.c_int, C.POINTER(_abi.EvalStats)]
     L.upbp_eval_upbp.restype = st
@@ -330,6 +332,21 @@ class PhotonBeamsEvaluator:
     def getBeamSelectionPdf(self, pos) -> float:
         p = np.ascontiguousarray(pos, np.float32)
         return self.ctx.lib.upbp_beams_selection_pdf(self.h, p.ctypes.data)
+
+    def getBeamSelectionPdfs(self, positions):
+        """Selection pdf of the cell holding each position ([n,3] numpy or device tensor -> [n])."""
+        if _is_torch(positions) and positions.is_cuda:
+            import torch
+
+            p = positions.contiguous().float()
+            out = torch.empty(p.shape[0], dtype=torch.float32, device=p.device)
+            mem = _abi.MEM_DEVICE
+        else:
+            p = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+            out = np.empty(p.shape[0], np.float32)
+            mem = _abi.MEM_HOST
+        self.ctx.check(self.ctx.lib.upbp_beams_selection_pdfs(self.h, _abi.ptr(p), p.shape[0], _abi.ptr(out), mem))
+        return out
 
     def evalBeamBeamEstimate(self, queries: QueryBatchView, out=None, stats=None):
         """PhotonBeamsEvaluator::evalBeamBeamEstimate (PhBeams.cxx:177-324) per query -> [Q,3]."""

@@ -77,6 +77,15 @@ def test_selection_pdf_matches_oracle_grid():
     print("cells reduced among the probes: %d" % (o < 1).sum())
     assert (o < 1).sum() > 50
     assert np.array_equal(g, o)
+    # the batched entry point, from host and from device memory
+    assert np.array_equal(bm.getBeamSelectionPdfs(pts), o)
+    import torch
+
+    dev = bm.getBeamSelectionPdfs(torch.from_numpy(pts.astype(np.float32)).cuda())
+    assert np.array_equal(dev.cpu().numpy(), o)
+    plain = native.PhotonBeamsEvaluator(ctx)
+    plain.build(w.beams)
+    assert np.all(plain.getBeamSelectionPdfs(pts) == 1.0)
 
 
 def test_bound_above_occupancy_equals_unreduced():
