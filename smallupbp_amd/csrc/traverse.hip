@@ -81,6 +81,7 @@ struct Ray {
     float ix, iy, iz;     // guarded reciprocal direction
     float oix, oiy, oiz;  // o * inv
     float d1o1;           // dot3(d, o), loop invariant of the beam test
+    uint64_t qkey;        // per-cell beam reduction: key of the query ray (reduced kernels only)
 };
 
 __device__ __forceinline__ bool slab(const Ray& r, float tmin, float tmax, float lx, float ly, float lz,
@@ -611,10 +612,8 @@ __device__ __forceinline__ void beam_leaf(const EvalArgs& A, const Ray& r, SegSt
         uint32_t red_n = 1;
         if ((MODE & kReduced) && hit) {
             // the hit belongs to the reduction grid cell of its query point
-            const uint32_t bits[6] = {__float_as_uint(r.ox), __float_as_uint(r.oy), __float_as_uint(r.oz),
-                                      __float_as_uint(r.dx), __float_as_uint(r.dy), __float_as_uint(r.dz)};
-            red_n = red_decide(A.red, red_query_key(bits), r.ox + r.dx * t1, r.oy + r.dy * t1, r.oz + r.dz * t1,
-                               A.orig[i], red_w);
+            red_n = red_decide(A.red, r.qkey, r.ox + r.dx * t1, r.oy + r.dy * t1, r.oz + r.dz * t1, A.orig[i],
+                               red_w);
             red_w *= (float)red_n;
         }
         if (__ballot(hit && red_n != 0)) {
@@ -789,6 +788,11 @@ __device__ void run_tile(const EvalArgs& A, int64_t slot0, unsigned long long* l
     r.oiy = r.oy * r.iy;
     r.oiz = r.oz * r.iz;
     r.d1o1 = dot3(r.dx, r.dy, r.dz, r.ox, r.oy, r.oz);
+    if (MODE & kReduced) {
+        const uint32_t bits[6] = {__float_as_uint(r.ox), __float_as_uint(r.oy), __float_as_uint(r.oz),
+                                  __float_as_uint(r.dx), __float_as_uint(r.dy), __float_as_uint(r.dz)};
+        r.qkey = red_query_key(bits);
+    }
     int nseg = alive ? (int)(s1 - s0) : 0;
     int maxseg = nseg;
     for (int off = 32; off > 0; off >>= 1) maxseg = max(maxseg, __shfl_xor(maxseg, off, kWave));
