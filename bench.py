@@ -43,12 +43,23 @@ def alg_bytes_queries(q) -> int:
     return 24 * q.count + 12 * nseg + 12 * q.count
 
 
-def host_cores() -> int:
+def host_cpu_info() -> dict:
+    """The host CPUs this run may use: the process's affinity mask, the machine's logical CPU
+    count, and the CPU share the job is given (OMP_NUM_THREADS, which the GPU host sets to the
+    cores it allots per GPU)."""
     try:
-        n = len(os.sched_getaffinity(0))
+        affinity = len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+        affinity = os.cpu_count() or 1
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    share_n = int(share) if share.isdigit() and int(share) > 0 else affinity
+    return {"affinity": affinity, "os_cpu_count": os.cpu_count(), "job_share": share_n,
+            "threads": max(1, min(affinity, share_n)), "model": cpu_name()}
+
+
+def host_cores() -> int:
+    """Oracle threads: every CPU of the affinity mask within the job's CPU share."""
+    return host_cpu_info()["threads"]
 
 
 def cpu_name() -> str:
@@ -110,31 +121,43 @@ def time_eval(ctx, bre, beams, qv, out_p, out_b, steps, warmup):
 
 
 def adjoint_ratio(steps: int):
-    """Forward vs forward+adjoint on the 512x512 differentiable workload (500k photons + 100k beams)."""
+    """Forward vs forward + adjoint on the differentiable workload: 512x512 queries x 500k photons
+    + 100k beams in the box + sphere media, loss = sum_q |L_q - T_q|^2 against a target image T
+    rendered at sigma_s x 1.1. The forward evaluates both estimators; the adjoint step also forms
+    the upstream gradient 2 (L - T) and runs the adjoint kernels for the 18 medium coefficients
+    and the 500k x 3 photon powers (and the beam powers)."""
     import torch
 
     from smallupbp_amd import native, scenes
 
     w = scenes.adjoint_scene()
+    tgt = scenes.with_media(w, scenes.scale_scattering(w.media, 1.1))
     ctx = native.Context(0)
-    ctx.set_media(w.media)
+    ctx.set_media(tgt.media)
     bre = native.EmbreeBre(ctx)
     bre.build(w.photons)
     beams = native.PhotonBeamsEvaluator(ctx)
     beams.build(w.beams)
+    tp, tb = native.eval_batch(ctx, bre, beams, native.QueryBatchView.to_device(tgt.queries))
+    target = (tp + tb).contiguous()
+    ctx.set_media(w.media)
     qv = native.QueryBatchView.to_device(w.queries)
     dev = "cuda"
     op = torch.empty((w.queries.count, 3), device=dev)
     ob = torch.empty((w.queries.count, 3), device=dev)
-    gp = torch.ones((w.queries.count, 3), device=dev)
-    gb = torch.ones((w.queries.count, 3), device=dev)
     gs = torch.zeros((len(w.media), 3), device=dev)
     ga = torch.zeros((len(w.media), 3), device=dev)
     gpp = torch.zeros((w.photons.count, 3), device=dev)
     gbp = torch.zeros((w.beams.count, 3), device=dev)
+
+    def fwd_adj():
+        native.eval_batch(ctx, bre, beams, qv, op, ob)
+        g = 2.0 * (op + ob - target)
+        native.adjoint(ctx, bre, beams, qv, g, g, gs, ga, gpp, gbp)
+
     for _ in range(2):
         native.eval_batch(ctx, bre, beams, qv, op, ob)
-        native.adjoint(ctx, bre, beams, qv, gp, gb, gs, ga, gpp, gbp)
+        fwd_adj()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -143,11 +166,175 @@ def adjoint_ratio(steps: int):
     t_f = (time.perf_counter() - t0) / steps
     t0 = time.perf_counter()
     for _ in range(steps):
-        native.eval_batch(ctx, bre, beams, qv, op, ob)
-        native.adjoint(ctx, bre, beams, qv, gp, gb, gs, ga, gpp, gbp)
+        fwd_adj()
     torch.cuda.synchronize()
     t_fa = (time.perf_counter() - t0) / steps
     return t_fa / t_f, t_f * 1e3, t_fa * 1e3
+
+
+def stress_measure(steps: int, cpu_budget: float, threads: int):
+    """Photon-beam-only stress: 1,000,000 query beams from random camera rays x 2,000,000 photon
+    beams in one medium (sigma_s 1, sigma_a 0.1; half SHORT with Exp(mean 0.05) lengths, half
+    LONG to the box boundary), radius 0.003. Inputs resident in HBM; one step = one evaluation of
+    the beam estimator over all queries (query regrouping included). Roofline of the beam kernel
+    from its HIP-event time; the oracle through the reference's uniform grid on a query sample is
+    the CPU comparison."""
+    import torch
+
+    from smallupbp_amd import autotune, native, scenes
+
+    t = time.time()
+    w = scenes.beams_stress()
+    log("stress workload %s generated in %.1fs" % (w.name, time.time() - t))
+    p = autotune.tuned_params(0, w.beams.count, w.queries.count)
+    ctx = native.Context(0)
+    ctx.set_media(w.media)
+    if p["beam_waves_per_block"]:
+        ctx.set_tuning(p["beam_waves_per_block"])
+    bm = native.PhotonBeamsEvaluator(ctx)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    bm.build(w.beams, leaf_size=p["beam_leaf_size"], piece_factor=p["piece_factor"])
+    t_build = time.perf_counter() - t
+    qv = native.QueryBatchView.to_device(w.queries, order=p["order"])
+    Q = w.queries.count
+    out = torch.empty((Q, 3), device="cuda")
+    el, per = time_eval(ctx, None, bm, qv, None, out, steps, 2)
+    ms_bm = float(np.mean([x["kernel_ms_beams"] for x in per]))
+    H = int(per[-1]["beam_hits"])
+    tests = int(per[-1]["beam_candidates"])
+    dom_bytes = alg_bytes_queries(w.queries) + 48 * H
+    achieved = dom_bytes / (ms_bm * 1e-3) / 1e9
+    traffic = None
+    try:
+        traffic = json.load(open(os.path.join(ROOT, "profiles", "traffic.json"))).get("k_eval<beams> stress")
+    except (OSError, ValueError):
+        traffic = None
+    rec = {"workload": "beams_stress (1M query beams x 2M photon beams, 50 % SHORT / 50 % LONG, r = 0.003)",
+           "mqueries_per_s": round(Q / (el / steps) / 1e6, 3), "ms_per_step": round(el / steps * 1e3, 4),
+           "kernel_ms": round(ms_bm, 4), "beam_hits": H, "hits_per_query": round(H / Q, 1),
+           "beam_tests": tests, "tests_per_hit": round(tests / max(H, 1), 2), "pieces": bm.pieces,
+           "build_ms": round(t_build * 1e3, 2),
+           "tuning": {k: v for k, v in p.items() if k != "cached"} if p["cached"] else "library defaults",
+           "roofline": {"bound": "hbm", "kernel": "k_eval<beams>", "achieved": round(achieved, 2),
+                        "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 5),
+                        "traffic": traffic, "alg_bytes_per_launch": dom_bytes}}
+    if cpu_budget > 0:
+        from oracle import pyoracle as po
+
+        t = time.time()
+        obm = po.OracleBeams(w.beams, grid_res=256)
+        build_s = time.time() - t
+        rng = np.random.default_rng(31)
+        probe = np.sort(rng.choice(Q, 500, replace=False))
+        t = time.time()
+        po.beams_eval(obm, w.media, w.queries.subset(probe), accel=1, threads=threads)
+        per_q = (time.time() - t) / len(probe)
+        n = int(min(Q, max(500, cpu_budget / max(per_q, 1e-9))))
+        idx = np.sort(rng.choice(Q, n, replace=False))
+        t = time.time()
+        po.beams_eval(obm, w.media, w.queries.subset(idx), accel=1, threads=threads)
+        el_cpu = time.time() - t
+        rec["cpu_baseline"] = {"value": round(n / el_cpu / 1e6, 6), "unit": "Mqueries/s", "cores": threads,
+                               "kind": "port", "sample": "%d random queries of the same workload through the oracle's "
+                               "restatement of the reference uniform grid (256 cells, 3D-DDA); %.1f s on %d threads, "
+                               "grid built in %.1f s" % (n, el_cpu, threads, build_s)}
+        rec["gpu_vs_cpu"] = round(rec["mqueries_per_s"] / rec["cpu_baseline"]["value"], 1)
+    bm.destroy()
+    return rec
+
+
+def iteration_measure(steps: int, ctx, qv, leaf_ph: int, leaf_bm: int, piece_factor: float, paths: int = 1_000_000):
+    """One renderer iteration on the GPU with every array device-resident: trace 1,000,000 light
+    paths in the box + sphere media (upbp_trace_light_paths: light vertices and SHORT beams),
+    build the photon structure from the in-medium light vertices (r = 0.005) and the beam
+    structure (r = 0.003) from device memory, then evaluate both estimators for the 1920x1080
+    query beams. The reference rebuilds its structures every iteration
+    (SmallUPBP/src/Renderers/VolLightTracer.hxx:303-410), so this is the per-iteration cost."""
+    import ctypes as C
+
+    import torch
+
+    from smallupbp_amd import _abi, native, scenes
+
+    sc = scenes.light_scene(media_rows=LIGHT_TRACE_MEDIA, max_path_length=10, bb1d_min_mfp=0.1, seed=12)
+    tctx = native.Context(0)
+    tctx.set_media(sc.media)
+    cap_v, cap_b = scenes.light_path_capacity(sc, paths)
+
+    def zeros(n, dtype):
+        return torch.zeros(max(int(n), 1) * np.dtype(dtype).itemsize, dtype=torch.uint8, device="cuda")
+
+    proto = scenes.light_path_arrays(1, 1, 1)
+    arr = {k: zeros((cap_v if k in ("vertices", "positions") else paths if k == "path_ends" else cap_b)
+                    * int(np.prod(v.shape[1:])), v.dtype) for k, v in proto.items()}
+    lp = _abi.light_paths(arr, cap_v, cap_b)
+    regions = np.ascontiguousarray(sc.regions)
+    lights = np.ascontiguousarray(sc.lights)
+    par = _abi.LightTraceParams()
+    for k, v in sc.params.items():
+        setattr(par, k, v)
+    lv_words = scenes.LIGHT_VERTEX_DTYPE.itemsize // 4
+    off_tp = scenes.LIGHT_VERTEX_DTYPE.fields["throughput"][1] // 4
+    off_dir = scenes.LIGHT_VERTEX_DTYPE.fields["dir_fix"][1] // 4
+    off_fl = scenes.LIGHT_VERTEX_DTYPE.fields["flags"][1] // 4
+    out_p = torch.empty((qv.count, 3), device="cuda")
+    out_b = torch.empty((qv.count, 3), device="cuda")
+    ctx.set_media(sc.media)
+
+    def one(times=None):
+        tctx.check(tctx.lib.upbp_trace_light_paths(tctx.h, regions.ctypes.data, len(regions), lights.ctypes.data,
+                                                   len(lights), C.byref(par), paths, C.byref(lp), _abi.MEM_DEVICE))
+        t1 = time.perf_counter()
+        nv, nb = int(lp.vertex_count), int(lp.beam_count)
+        # photon inputs from the light vertices: strided views of the device records
+        words = arr["vertices"][: nv * lv_words * 4].view(torch.float32).view(nv, lv_words)
+        power = words[:, off_tp:off_tp + 3].contiguous()
+        inc = words[:, off_dir:off_dir + 3].contiguous()
+        in_med = ((words[:, off_fl].view(torch.int32) & scenes.LV_IN_MEDIUM) != 0).to(torch.uint8)
+        pos = arr["positions"][: nv * 12].view(torch.float32).view(nv, 3)
+        bre = native.EmbreeBre(ctx)
+        bre.build(scenes.Photons(pos, inc, power, None, 0.005), in_medium=in_med, leaf_size=leaf_ph)
+        t2 = time.perf_counter()
+        bmv = {k: arr[k][: nb * int(np.prod(proto[k].shape[1:])) * proto[k].dtype.itemsize].view(
+            {np.dtype(np.float32): torch.float32, np.dtype(np.uint32): torch.int32,
+             np.dtype(np.int32): torch.int32}[proto[k].dtype]) for k in
+            ("beam_origin", "beam_direction", "beam_length", "beam_throughput", "beam_flags", "beam_medium")}
+        bm = native.PhotonBeamsEvaluator(ctx)
+        bm.build(scenes.Beams(bmv["beam_origin"].view(nb, 3), bmv["beam_direction"].view(nb, 3),
+                              bmv["beam_length"], bmv["beam_throughput"].view(nb, 3), bmv["beam_flags"],
+                              bmv["beam_medium"], 0.003), leaf_size=leaf_bm, piece_factor=piece_factor)
+        t3 = time.perf_counter()
+        native.eval_batch(ctx, bre, bm, qv, out_p, out_b)
+        torch.cuda.synchronize()
+        t4 = time.perf_counter()
+        if times is not None:
+            times.append((t1, t2, t3, t4))
+        kept = bre.num_photons
+        bre.destroy()
+        bm.destroy()
+        return nv, nb, kept
+
+    for _ in range(2):
+        one()
+    torch.cuda.synchronize()
+    tms = []
+    starts = []
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        starts.append(time.perf_counter())
+        nv, nb, kept = one(tms)
+    trace = [t[0] - s for t, s in zip(tms, starts)]
+    build_ph = [t[1] - t[0] for t in tms]
+    build_bm = [t[2] - t[1] for t in tms]
+    ev = [t[3] - t[2] for t in tms]
+    total = [t[3] - s for t, s in zip(tms, starts)]
+    ms = lambda xs: round(float(np.mean(xs)) * 1e3, 3)
+    return {"workload": "iteration (trace %d light paths -> %d light vertices (%d in media, the photons) + %d SHORT "
+                        "beams -> photon and beam builds from device memory -> eval of 1920x1080 queries)"
+                        % (paths, nv, kept, nb),
+            "iteration_ms": ms(total), "trace_ms": ms(trace), "photon_build_ms": ms(build_ph),
+            "beam_build_ms": ms(build_bm), "eval_ms": ms(ev)}
 
 
 def batched_ratio(scale: float, steps: int, n_instances: int = 64):
@@ -485,12 +672,14 @@ def main():
     ap.add_argument("--waves-per-block", type=int, default=0)
     ap.add_argument("--no-tuning", action="store_true",
                     help="ignore smallupbp_amd/tuning_cache.json and use the library defaults")
-    ap.add_argument("--batched", type=float, default=0.25,
+    ap.add_argument("--batched", type=float, default=1.0,
                     help="also time the 64-instance batched form at this size scale (1 = full size, 0 = skip)")
     ap.add_argument("--no-vm", action="store_true", help="skip the PP3D and SURF vertex-merging measurements")
     ap.add_argument("--no-upbp", action="store_true", help="skip the MIS-weighted (UPBP) measurement")
     ap.add_argument("--no-lighttrace", action="store_true", help="skip the light sub-path tracing measurement")
     ap.add_argument("--no-reduction", action="store_true", help="skip the per-cell beam reduction measurement")
+    ap.add_argument("--no-stress", action="store_true", help="skip the 1M x 2M photon-beam stress measurement")
+    ap.add_argument("--no-iteration", action="store_true", help="skip the trace + build + eval iteration measurement")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -511,19 +700,19 @@ def main():
     # launch / build parameters: command line, else the autotuner's cached best for this size, else defaults
     from smallupbp_amd import autotune
 
-    tuned = None if args.no_tuning else autotune.lookup(w.photons.count, w.beams.count, w.queries.count)
-    best_ph = (tuned or {}).get("best", {}).get("photons") or {}
-    best_bm = (tuned or {}).get("best", {}).get("beams") or {}
-    leaf_ph = args.leaf_size or int(best_ph.get("leaf_size", 0))
-    leaf_bm = args.leaf_size or int(best_bm.get("leaf_size", 0))
-    piece_factor = args.piece_factor or float(best_bm.get("piece_factor", 0.0))
-    waves = args.waves_per_block or int(best_bm.get("waves_per_block", 0))
+    tp = autotune.tuned_params(w.photons.count, w.beams.count, w.queries.count)
+    tuned = None if args.no_tuning or not tp["cached"] else tp
+    leaf_ph = args.leaf_size or (tuned["photon_leaf_size"] if tuned else 0)
+    leaf_bm = args.leaf_size or (tuned["beam_leaf_size"] if tuned else 0)
+    piece_factor = args.piece_factor or (tuned["piece_factor"] if tuned else 0.0)
+    waves = args.waves_per_block or (tuned["beam_waves_per_block"] if tuned else 0)
+    waves_ph = args.waves_per_block or (tuned["photon_waves_per_block"] if tuned else 0)
     if args.order == "tuned":
-        args.order = "as_given" if best_bm.get("order") == _abi.ORDER_AS_GIVEN else "sorted"
+        args.order = "as_given" if tuned and tuned["order"] == _abi.ORDER_AS_GIVEN else "sorted"
     ctx = native.Context(0)
     ctx.set_media(w.media)
-    if waves:
-        ctx.set_tuning(waves)
+    if waves or waves_ph:
+        ctx.set_tuning(waves or 4, 0, waves_ph)
     bre = native.EmbreeBre(ctx)
     beams = native.PhotonBeamsEvaluator(ctx)
     torch.cuda.synchronize()
@@ -596,6 +785,7 @@ def main():
             "beam_pieces": beams.pieces,
             "tuning": {"photon_leaf_size": leaf_ph or "default", "beam_leaf_size": leaf_bm or "default",
                        "beam_piece_factor": piece_factor or "default", "waves_per_block": waves or "default",
+                       "photon_waves_per_block": waves_ph or "default",
                        "source": "command line / tuning_cache.json" if tuned else "command line / defaults"},
             "parallelism": "single GPU",
         },
@@ -615,7 +805,9 @@ def main():
 
     if not args.no_adjoint:
         ratio, tf, tfa = adjoint_ratio(max(3, args.steps // 2))
-        rec["adjoint"] = {"workload": "adjoint_512 (512x512 x 500k photons + 100k beams)",
+        rec["adjoint"] = {"workload": "adjoint_512 (512x512 x 500k photons + 100k beams, L2 loss against the "
+                                      "image at sigma_s x 1.1; gradients of 18 medium coefficients, 500k photon "
+                                      "and 100k beam powers)",
                           "fwd_ms": round(tf, 3), "fwd_plus_adjoint_ms": round(tfa, 3),
                           "ratio": round(ratio, 3)}
 
@@ -631,6 +823,13 @@ def main():
     if not args.no_reduction:
         rec["beam_reduction"] = reduction_measure(ctx, w, qv, leaf_bm, piece_factor, max(3, args.steps // 2))
 
+    if not args.no_stress:
+        rec["stress"] = stress_measure(max(3, args.steps // 2), min(args.cpu_budget, 10.0), host_cores())
+
+    if not args.no_iteration:
+        rec["iteration"] = iteration_measure(max(3, args.steps // 2), ctx, qv, leaf_ph, leaf_bm, piece_factor)
+        ctx.set_media(w.media)
+
     if not args.no_vm:
         from smallupbp_amd import scenes as _sc
 
@@ -638,13 +837,18 @@ def main():
             rec[key] = vm_measure(tech, max(3, args.steps // 2), min(args.cpu_budget, 5.0), host_cores())
 
     if args.cpu_budget > 0:
-        threads = host_cores()
+        info = host_cpu_info()
+        threads = info["threads"]
         mq, n, el_cpu, bs = run_cpu_baseline(w, args.cpu_budget, threads)
         rec["cpu_baseline"] = {
             "value": round(mq, 5), "unit": "Mqueries/s", "cores": threads, "kind": "port",
             "sample": "%d random queries of the same 1920x1080 workload, photons via binned-SAH BVH, beams via "
                       "the reference uniform grid (256 cells, 3D-DDA); %.1f s of eval on %d threads (%s)"
-                      % (n, el_cpu, threads, cpu_name()),
+                      % (n, el_cpu, threads, info["model"]),
+            "host": {"affinity_cpus": info["affinity"], "os_cpu_count": info["os_cpu_count"],
+                     "job_cpu_share": info["job_share"], "model": info["model"],
+                     "note": "threads = every CPU of the process's affinity mask within the CPU share the GPU host "
+                             "gives this job (OMP_NUM_THREADS)"},
         }
         rec["gpu_vs_cpu"] = round(value / mq, 1)
     print(json.dumps(rec), flush=True)

@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define UPBP_ABI_VERSION 1
+#define UPBP_ABI_VERSION 2
 
 /* ---------------------------------------------------------------------------
  * Status codes. The reference has none: its assertions print and exit(2)
@@ -270,7 +270,8 @@ upbp_status upbp_beams_radii(const upbp_beams* beams, float* start_radius, float
                              upbp_memory memory);
 /* Replaces PhotonBeamsEvaluator::getBeamSelectionPdf (PhBeams.hxx, Grid.hxx:409-415): the
  * selection pdf of the reduction grid's cell containing the position (1 without reduction,
- * sMaxBeamsInCell = 0, the reference default). */
+ * sMaxBeamsInCell = 0, the reference default). NaN when the device read fails; the reason is
+ * then in upbp_ctx_last_error of the structure's context. */
 float upbp_beams_selection_pdf(const upbp_beams* beams, const float position[3]);
 /* getBeamSelectionPdf for many positions at once (xyz [count][3] -> out_pdf [count], both in
  * `memory`): the loop that sets mBB1DBeamSelectionPdf of every in-medium light vertex after the
@@ -309,7 +310,10 @@ upbp_status upbp_hit_signatures(upbp_ctx* ctx, const upbp_bre* bre, const upbp_b
  * Segment attenuations are taken as exp(-sigma_t * (dist_max - dist_min)) of the
  * segment's medium (what Scene::Intersect stores, Scene.hxx:542), so their
  * derivative is -(dist_max - dist_min) * attenuation. Transmittance is
- * recomputed, never stored. LONG query beams only (UPBP_EUNSUPPORTED for SHORT). */
+ * recomputed, never stored. The media use |sigma| (Media.hxx:136-140), so
+ * the medium gradients are taken with respect to the values given to
+ * upbp_set_media: an entry of a negative coefficient carries its sign.
+ * LONG query beams only (UPBP_EUNSUPPORTED for SHORT). */
 upbp_status upbp_adjoint(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams* beams,
                          const upbp_query_batch* queries, const float* grad_out_photons,
                          const float* grad_out_beams, float* grad_scattering,
@@ -589,7 +593,11 @@ upbp_status upbp_trace_light_paths(upbp_ctx* ctx, const upbp_region* regions, in
  * Each instance names its structures, a device-resident query batch, its
  * media (a range of the context's media table: medium_id + media_base) and
  * device outputs. Waves take (instance, query tile) items from a device work
- * queue until it drains. */
+ * queue until it drains. An instance's media are the table entries from
+ * media_base to the end: a segment medium_id outside [0, count - media_base)
+ * fails the run with UPBP_EINVAL. upbp_set_media between runs is allowed: the
+ * next run checks every media_base against the new table and captures its
+ * graph again. */
 typedef struct upbp_instance {
     const upbp_bre* bre;          /* may be NULL */
     const upbp_beams* beams;      /* may be NULL */
@@ -624,8 +632,10 @@ upbp_status upbp_batch_run(upbp_ctx* ctx, upbp_batch_plan* plan, int use_graph,
  * wave-per-block and waves-per-CU settings and keeps the fastest per
  * (structure size, query count) bucket. */
 typedef struct upbp_tuning {
-    int32_t waves_per_block;  /* 1..16 */
-    int32_t blocks_per_cu;    /* persistent kernels: resident blocks per CU; 0 = occupancy-derived */
+    int32_t waves_per_block;        /* 1..4: waves per workgroup of the traversal kernels */
+    int32_t blocks_per_cu;          /* persistent kernels: resident blocks per CU; 0 = occupancy-derived */
+    int32_t photon_waves_per_block; /* 1..4: the photon kernel's own count; 0 = waves_per_block */
+    int32_t pad;
 } upbp_tuning;
 upbp_status upbp_set_tuning(upbp_ctx* ctx, const upbp_tuning* tuning);
 upbp_status upbp_get_tuning(const upbp_ctx* ctx, upbp_tuning* tuning);

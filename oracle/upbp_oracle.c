@@ -1132,9 +1132,13 @@ int or_adjoint(const or_photons* ph, const or_beams* bm, const upbp_medium* medi
         free(ls);
         free(lt);
     }
+    /* the media use |sigma| (Media.hxx:136-140): d|x|/dx = sign(x) maps the gradient back to the
+     * caller's coefficients */
     for (int64_t k = 0; k < (int64_t)n_media * 3; ++k) {
-        if (grad_sigma_s) grad_sigma_s[k] = gs[k] + gt[k];
-        if (grad_sigma_a) grad_sigma_a[k] = gt[k];
+        const double sgn_s = copysign(1.0, (double)media[k / 3].scattering[k % 3]);
+        const double sgn_a = copysign(1.0, (double)media[k / 3].absorption[k % 3]);
+        if (grad_sigma_s) grad_sigma_s[k] = sgn_s * (gs[k] + gt[k]);
+        if (grad_sigma_a) grad_sigma_a[k] = sgn_a * gt[k];
     }
     if (grad_photon_power && ph) {
         memset(grad_photon_power, 0, (size_t)ph->n_in * 3 * sizeof(double));

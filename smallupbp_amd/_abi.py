@@ -85,7 +85,8 @@ class Instance(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("waves_per_block", C.c_int32), ("blocks_per_cu", C.c_int32)]
+    _fields_ = [("waves_per_block", C.c_int32), ("blocks_per_cu", C.c_int32),
+                ("photon_waves_per_block", C.c_int32), ("pad", C.c_int32)]
 
 
 assert C.sizeof(Medium) == 32

@@ -50,6 +50,20 @@ def lookup(n_photons: int, n_beams: int, n_queries: int, path: str = DEFAULT_CAC
     return load_cache(path).get(bucket(n_photons, n_beams, n_queries))
 
 
+def tuned_params(n_photons: int, n_beams: int, n_queries: int, path: str = DEFAULT_CACHE) -> dict:
+    """Build and launch parameters of the cached best configuration for this size bucket
+    (0 / "sorted" = library default where the bucket has no entry)."""
+    t = lookup(n_photons, n_beams, n_queries, path) or {}
+    ph = t.get("best", {}).get("photons") or {}
+    bm = t.get("best", {}).get("beams") or {}
+    return {"photon_leaf_size": int(ph.get("leaf_size", 0)), "beam_leaf_size": int(bm.get("leaf_size", 0)),
+            "piece_factor": float(bm.get("piece_factor", 0.0)),
+            "photon_waves_per_block": int(ph.get("waves_per_block", 0)),
+            "beam_waves_per_block": int(bm.get("waves_per_block", 0)),
+            "order": _abi.ORDER_AS_GIVEN if bm.get("order") == _abi.ORDER_AS_GIVEN else _abi.ORDER_SORTED,
+            "cached": bool(t)}
+
+
 def _median_kernel_ms(ctx, bre, beams, qv, reps, which):
     ts = []
     for _ in range(reps):

@@ -37,6 +37,9 @@ struct upbp_batch_plan {
     hipGraphExec_t exec = nullptr;
     int graph_blocks = 0;
     int graph_wpb = 0;
+    // the context's media generation the instances were validated against (and the graph captured with)
+    uint64_t media_gen = 0;
+    std::vector<int32_t> media_base;
     ~upbp_batch_plan() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -123,8 +126,28 @@ void check_batch(const Ctx& ctx, const upbp_query_batch* q) {
     UPBP_REQUIRE(ctx.n_media > 0, UPBP_EINVAL, "upbp_set_media must be called before evaluation");
 }
 
+// The context's own workspace: its counters, and for a sorted batch of Q queries its key and
+// permutation buffers and sort storage, grown to fit.
+Workspace context_workspace(Ctx& ctx, const upbp_query_batch* q) {
+    Workspace w;
+    w.stats = ctx.stats.as<unsigned long long>();
+    const int64_t Q = q->count;
+    if (q->order == UPBP_ORDER_SORTED && Q > 64) {
+        for (DevBuf* b : {&ctx.qkeys, &ctx.qkeys2, &ctx.qperm, &ctx.qperm2}) b->ensure((size_t)Q * 4);
+        ctx.sort_tmp.ensure(sort_pairs_u32_temp_bytes(Q, 30));
+        w.keys = ctx.qkeys.as<uint32_t>();
+        w.keys2 = ctx.qkeys2.as<uint32_t>();
+        w.perm = ctx.qperm.as<uint32_t>();
+        w.perm2 = ctx.qperm2.as<uint32_t>();
+        w.max_sorted = Q;
+        w.sort_tmp = ctx.sort_tmp.p;
+        w.sort_tmp_bytes = ctx.sort_tmp.bytes;
+    }
+    return w;
+}
+
 // Device view of a batch (copies host batches), with the sorted permutation when requested.
-QueryView make_view(Ctx& ctx, const upbp_query_batch* q, const float* lo, const float* hi) {
+QueryView make_view(Ctx& ctx, const upbp_query_batch* q, const float* lo, const float* hi, const Workspace& ws) {
     QueryView v;
     v.count = q->count;
     v.beam_type = q->beam_type;
@@ -141,14 +164,11 @@ QueryView make_view(Ctx& ctx, const upbp_query_batch* q, const float* lo, const 
     v.offsets = stage_q(ctx, 2, q->segment_offsets, q->count + 1, q->memory);
     v.segments = q->memory == UPBP_MEM_HOST ? stage_q(ctx, 3, q->segments, nseg, q->memory) : q->segments;
     if (q->order == UPBP_ORDER_SORTED && q->count > 64) {
-        ctx.qkeys.ensure(q->count * 4);
-        ctx.qkeys2.ensure(q->count * 4);
-        ctx.qperm.ensure(q->count * 4);
-        ctx.qperm2.ensure(q->count * 4);
-        launch_query_keys(ctx, v, lo, hi, ctx.qkeys.as<uint32_t>(), ctx.qperm.as<uint32_t>());
-        sort_pairs_u32(ctx, ctx.qkeys.as<uint32_t>(), ctx.qkeys2.as<uint32_t>(), ctx.qperm.as<uint32_t>(),
-                       ctx.qperm2.as<uint32_t>(), q->count, 30);
-        v.perm = ctx.qperm2.as<uint32_t>();
+        UPBP_REQUIRE(q->count <= ws.max_sorted && ws.keys && ws.keys2 && ws.perm && ws.perm2, UPBP_EINVAL,
+                     "more queries to sort than the workspace holds");
+        launch_query_keys(ctx, v, lo, hi, ws.keys, ws.perm);
+        sort_pairs_u32_in(ctx, ws.sort_tmp, ws.sort_tmp_bytes, ws.keys, ws.keys2, ws.perm, ws.perm2, q->count, 30);
+        v.perm = ws.perm2;
     }
     return v;
 }
@@ -230,9 +250,10 @@ struct RunOut {
     uint32_t* sig_bm = nullptr;
 };
 
+// ws: where the call's counters, query keys and sort storage live (null: the context's own buffers).
 void run_eval(Ctx& ctx, const Bre* bre, const Beams* bm, const upbp_query_batch* q, Mode mode, RunOut o,
               upbp_memory out_mem, upbp_eval_stats* stats, const AdjointOut* adj_ph, const AdjointOut* adj_bm,
-              bool force_sync, const MisDev* mis = nullptr) {
+              bool force_sync, const MisDev* mis = nullptr, const Workspace* ws = nullptr) {
     check_batch(ctx, q);
     if (bre) UPBP_REQUIRE(bre->ctx == &ctx, UPBP_EINVAL, "photon structure belongs to another context");
     if (bm) UPBP_REQUIRE(bm->ctx == &ctx, UPBP_EINVAL, "beam structure belongs to another context");
@@ -243,8 +264,13 @@ void run_eval(Ctx& ctx, const Bre* bre, const Beams* bm, const upbp_query_batch*
     if (timing) UPBP_HIP(hipEventRecord(ctx.ev[0], ctx.stream));
     float lo[3], hi[3];
     scene_bounds(bre, bm, lo, hi);
-    QueryView v = make_view(ctx, q, lo, hi);
-    launch_clear_u64(ctx, ctx.stats.as<uint64_t>(), kStatSlots);
+    Workspace own;
+    if (!ws) {
+        own = context_workspace(ctx, q);
+        ws = &own;
+    }
+    QueryView v = make_view(ctx, q, lo, hi, *ws);
+    launch_clear_u64(ctx, (uint64_t*)ws->stats, kStatSlots);
     // the UPBP light walks read per-vertex tails, built once here for both estimators
     MisDev mis_t;
     if (mis && mode == Mode::Upbp && Q > 0) {
@@ -281,7 +307,7 @@ void run_eval(Ctx& ctx, const Bre* bre, const Beams* bm, const upbp_query_batch*
     uint32_t launches = 0;
     if (want_ph) {
         if (bre->n > 0) {
-            launch_photons(ctx, mode, photon_args(bre), v, dph, sph, adj_ph, mis);
+            launch_photons(ctx, mode, photon_args(bre), v, dph, sph, adj_ph, mis, ws->stats);
             launches++;
         } else if (mode != Mode::Adjoint) {
             if (dph) launch_clear_f32(ctx, dph, 3 * Q);
@@ -296,7 +322,7 @@ void run_eval(Ctx& ctx, const Bre* bre, const Beams* bm, const upbp_query_batch*
     if (timing) UPBP_HIP(hipEventRecord(ctx.ev[4], ctx.stream));
     if (want_bm) {
         if (bm->pieces > 0) {
-            launch_beams(ctx, mode, beam_args(bm), v, dbm, sbm, adj_bm, mis);
+            launch_beams(ctx, mode, beam_args(bm), v, dbm, sbm, adj_bm, mis, ws->stats);
             launches++;
         } else if (mode != Mode::Adjoint) {
             if (dbm) launch_clear_f32(ctx, dbm, 3 * Q);
@@ -314,7 +340,7 @@ void run_eval(Ctx& ctx, const Bre* bre, const Beams* bm, const upbp_query_batch*
     const bool do_sync = ctx.sync || timing || force_sync || out_mem == UPBP_MEM_HOST || q->memory == UPBP_MEM_HOST;
     if (!do_sync) return;
     uint64_t st[kStatSlots];
-    UPBP_HIP(hipMemcpyAsync(st, ctx.stats.p, sizeof(st), hipMemcpyDeviceToHost, ctx.stream));
+    UPBP_HIP(hipMemcpyAsync(st, ws->stats, sizeof(st), hipMemcpyDeviceToHost, ctx.stream));
     UPBP_HIP(hipStreamSynchronize(ctx.stream));
     float ms_other = 0.f, ms_ph = 0.f, ms_bm = 0.f;
     if (timing) {
@@ -416,6 +442,7 @@ upbp_status upbp_set_media(upbp_ctx* ctx, const upbp_medium* media, int32_t coun
     return guarded(ctx, [&] {
         UPBP_REQUIRE(count > 0 && count <= (1 << 24) && media, UPBP_EINVAL, "media must be a non-empty array");
         std::vector<MediumDev> m(count);
+        std::vector<float> sign((size_t)count * 6);
         for (int i = 0; i < count; ++i) {
             for (int c = 0; c < 3; ++c)
                 UPBP_REQUIRE(std::isfinite(media[i].absorption[c]) && std::isfinite(media[i].scattering[c]),
@@ -423,11 +450,22 @@ upbp_status upbp_set_media(upbp_ctx* ctx, const upbp_medium* media, int32_t coun
             UPBP_REQUIRE(media[i].mean_cosine >= -1.f && media[i].mean_cosine <= 1.f, UPBP_EINVAL,
                          "mean_cosine outside [-1, 1]");
             m[i] = derive_medium(media[i]);
+            for (int c = 0; c < 3; ++c) {
+                sign[6 * i + c] = std::copysign(1.0f, media[i].absorption[c]);
+                sign[6 * i + 3 + c] = std::copysign(1.0f, media[i].scattering[c]);
+            }
         }
+        // a batched plan's graph may hold the old table's address: it re-captures when media_gen moves
+        ctx->media_gen++;
+        ctx->n_media = 0;
         ctx->media.ensure(count * sizeof(MediumDev));
+        ctx->media_sign.ensure(sign.size() * sizeof(float));
         UPBP_HIP(hipMemcpyAsync(ctx->media.p, m.data(), count * sizeof(MediumDev), hipMemcpyHostToDevice, ctx->stream));
+        UPBP_HIP(hipMemcpyAsync(ctx->media_sign.p, sign.data(), sign.size() * sizeof(float), hipMemcpyHostToDevice,
+                                ctx->stream));
         UPBP_HIP(hipStreamSynchronize(ctx->stream));
         ctx->media_host = m;
+        ctx->media_sign_host = sign;
         ctx->n_media = count;
     });
 }
@@ -545,11 +583,10 @@ upbp_status upbp_beams_radii(const upbp_beams* beams, float* start_radius, float
 float upbp_beams_selection_pdf(const upbp_beams* beams, const float position[3]) {
     // 1 without reduction (the reference default sMaxBeamsInCell = 0, PhBeams.cxx:51)
     if (!beams || !position) return 1.0f;
-    try {
-        return beam_selection_pdf(*beams, position);
-    } catch (...) {
-        return 1.0f;
-    }
+    // a failed device read must not pass for "no reduction": NaN, with the reason in the context
+    float pdf = NAN;
+    const upbp_status st = guarded(beams->ctx, [&] { pdf = beam_selection_pdf(*beams, position); });
+    return st == UPBP_OK ? pdf : NAN;
 }
 
 upbp_status upbp_beams_selection_pdfs(const upbp_beams* beams, const float* positions, int64_t count, float* out_pdf,
@@ -968,17 +1005,21 @@ upbp_status upbp_adjoint(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams* b
         RunOut o;
         run_eval(*ctx, do_ph ? bre : nullptr, do_bm ? beams : nullptr, queries, Mode::Adjoint, o, UPBP_MEM_DEVICE,
                  stats, do_ph ? &aph : nullptr, do_bm ? &abm : nullptr, true);
-        // combine d/d sigma_t into d/d sigma_s and d/d sigma_a, accumulate into the outputs
+        // combine d/d sigma_t into d/d sigma_s and d/d sigma_a, accumulate into the outputs; the
+        // media use |sigma|, so each entry carries the sign of the caller's coefficient
         if (memory == UPBP_MEM_DEVICE) {
-            launch_combine_media_grads(*ctx, gmedia.as<float>(), M, grad_scattering, grad_absorption);
+            launch_combine_media_grads(*ctx, gmedia.as<float>(), ctx->media_sign.as<float>(), M, grad_scattering,
+                                       grad_absorption);
             UPBP_HIP(hipStreamSynchronize(ctx->stream));
         } else {
             std::vector<float> gm((size_t)M * 6);
             UPBP_HIP(hipMemcpy(gm.data(), gmedia.p, gm.size() * 4, hipMemcpyDeviceToHost));
+            const std::vector<float>& sg = ctx->media_sign_host;
             for (int m = 0; m < M; ++m)
                 for (int c = 0; c < 3; ++c) {
-                    if (grad_scattering) grad_scattering[3 * m + c] += gm[6 * m + c] + gm[6 * m + 3 + c];
-                    if (grad_absorption) grad_absorption[3 * m + c] += gm[6 * m + 3 + c];
+                    if (grad_scattering)
+                        grad_scattering[3 * m + c] += sg[6 * m + 3 + c] * (gm[6 * m + c] + gm[6 * m + 3 + c]);
+                    if (grad_absorption) grad_absorption[3 * m + c] += sg[6 * m + c] * gm[6 * m + 3 + c];
                 }
             if (do_ph && grad_photon_power) {
                 std::vector<float> h((size_t)bre->n_in * 3);
@@ -1070,6 +1111,8 @@ upbp_status upbp_batch_plan_create(upbp_ctx* ctx, const upbp_instance* instances
         plan->ctx = ctx;
         plan->n = count;
         plan->host.resize(count);
+        plan->media_gen = ctx->media_gen;
+        plan->media_base.resize(count);
         int64_t tiles = 0;
         for (int i = 0; i < count; ++i) {
             const upbp_instance& in = instances[i];
@@ -1080,6 +1123,7 @@ upbp_status upbp_batch_plan_create(upbp_ctx* ctx, const upbp_instance* instances
             UPBP_REQUIRE(!in.beams || in.beams->red.max == 0, UPBP_EUNSUPPORTED,
                          "per-cell beam reduction is supported by upbp_eval and upbp_hit_signatures only");
             UPBP_REQUIRE(in.media_base >= 0 && in.media_base < ctx->n_media, UPBP_EINVAL, "media_base out of range");
+            plan->media_base[i] = in.media_base;
             const Bre* b = in.bre;
             const Beams* m = in.beams;
             InstanceDev& d = plan->host[i];
@@ -1187,6 +1231,19 @@ upbp_status upbp_batch_run(upbp_ctx* ctx, upbp_batch_plan* plan, int use_graph, 
     if (!ctx) return fail(nullptr, UPBP_EINVAL, "ctx is NULL");
     if (!plan || plan->ctx != ctx) return fail(ctx, UPBP_EINVAL, "plan is NULL or belongs to another context");
     return guarded(ctx, [&] {
+        // upbp_set_media since the last run: the instances' media ranges are checked again, and a
+        // captured graph (which froze the old table's address and size) is dropped
+        if (plan->media_gen != ctx->media_gen) {
+            UPBP_REQUIRE(ctx->n_media > 0, UPBP_EINVAL, "upbp_set_media must be called before evaluation");
+            for (int i = 0; i < plan->n; ++i)
+                UPBP_REQUIRE(plan->media_base[i] < ctx->n_media, UPBP_EINVAL,
+                             "an instance's media_base is outside the context's current media table");
+            if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
+            if (plan->graph) (void)hipGraphDestroy(plan->graph);
+            plan->exec = nullptr;
+            plan->graph = nullptr;
+            plan->media_gen = ctx->media_gen;
+        }
         // blocks_per_cu 0: each persistent kernel fills the GPU to its own occupancy limit
         const int per_cu = std::max(ctx->tuning.blocks_per_cu, 0);
         const int blocks = ctx->num_cus * per_cu;
@@ -1232,10 +1289,11 @@ upbp_status upbp_batch_run(upbp_ctx* ctx, upbp_batch_plan* plan, int use_graph, 
 upbp_status upbp_set_tuning(upbp_ctx* ctx, const upbp_tuning* tuning) {
     if (!ctx || !tuning) return fail(ctx, UPBP_EINVAL, "ctx and tuning are required");
     if (tuning->waves_per_block < 1 || tuning->waves_per_block > 4 || tuning->blocks_per_cu < 0 ||
-        tuning->blocks_per_cu > 64)
-        return fail(ctx, UPBP_EINVAL, "waves_per_block must be 1..4 and blocks_per_cu 0..64");
+        tuning->blocks_per_cu > 64 || tuning->photon_waves_per_block < 0 || tuning->photon_waves_per_block > 4)
+        return fail(ctx, UPBP_EINVAL, "waves_per_block must be 1..4, photon_waves_per_block 0..4 and blocks_per_cu 0..64");
     ctx->tuning.waves_per_block = tuning->waves_per_block;
     ctx->tuning.blocks_per_cu = tuning->blocks_per_cu;
+    ctx->tuning.photon_waves_per_block = tuning->photon_waves_per_block;
     return UPBP_OK;
 }
 
@@ -1243,6 +1301,8 @@ upbp_status upbp_get_tuning(const upbp_ctx* ctx, upbp_tuning* tuning) {
     if (!ctx || !tuning) return UPBP_EINVAL;
     tuning->waves_per_block = ctx->tuning.waves_per_block;
     tuning->blocks_per_cu = ctx->tuning.blocks_per_cu;
+    tuning->photon_waves_per_block = ctx->tuning.photon_waves_per_block;
+    tuning->pad = 0;
     return UPBP_OK;
 }
 
@@ -1281,6 +1341,8 @@ upbp_status upbp_autotune(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams* 
         if (best) {
             best->waves_per_block = best_cfg.waves_per_block;
             best->blocks_per_cu = best_cfg.blocks_per_cu;
+            best->photon_waves_per_block = best_cfg.photon_waves_per_block;
+            best->pad = 0;
         }
         if (best_ms) *best_ms = best_t / 3.f;
     });

@@ -294,6 +294,24 @@ void sort_pairs_u32(Ctx& ctx, uint32_t* keys_in, uint32_t* keys_out, uint32_t* v
                                                 vals_out, (int)n, 0, end_bit, ctx.stream));
 }
 
+size_t sort_pairs_u32_temp_bytes(int64_t n, int end_bit) {
+    size_t bytes = 0;
+    UPBP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                (uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, end_bit,
+                                                (hipStream_t)0));
+    return bytes;
+}
+
+void sort_pairs_u32_in(Ctx& ctx, void* tmp, size_t tmp_bytes, uint32_t* keys_in, uint32_t* keys_out,
+                       uint32_t* vals_in, uint32_t* vals_out, int64_t n, int end_bit) {
+    size_t bytes = 0;
+    UPBP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, keys_in, keys_out, vals_in, vals_out,
+                                                (int)n, 0, end_bit, ctx.stream));
+    UPBP_REQUIRE(bytes <= tmp_bytes, UPBP_EDEVICE, "radix sort needs more temporary storage than the workspace holds");
+    UPBP_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, bytes, keys_in, keys_out, vals_in, vals_out, (int)n, 0,
+                                                end_bit, ctx.stream));
+}
+
 void exclusive_scan_u32(Ctx& ctx, const uint32_t* in, uint32_t* out, int64_t n) {
     size_t bytes = 0;
     UPBP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, ctx.stream));

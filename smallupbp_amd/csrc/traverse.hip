@@ -867,6 +867,7 @@ __device__ void run_tile(const EvalArgs& A, int64_t slot0, unsigned long long* l
             a1 = sg->attenuation[1];
             a2 = sg->attenuation[2];
             pdf = sg->ray_sample_pdf;
+            // A.n_media counts the media from A.media_base on (the whole table outside batched plans)
             if (mid < 0 || mid >= A.n_media) {
                 err |= kErrBadMedium;
                 has = false;
@@ -1107,6 +1108,9 @@ __global__ __launch_bounds__(256, 6) void k_eval_batched(EvalArgs A, const Insta
         B.short_query = (int32_t)uni((uint32_t)(I.beam_type == UPBP_SHORT_BEAM));
         B.lite = (int32_t)uni(I.lite);
         B.media_base = (int32_t)uni((uint32_t)I.media_base);
+        // an instance's media are the table entries from media_base on: run_tile bounds every
+        // instance-local medium id by this count (<= 0 rejects every segment)
+        B.n_media = A.n_media - B.media_base;
         B.out = uni_global(EST == kEstPhotons ? I.out_photons : I.out_beams);
         const int64_t tile_begin = uni64(I.tile_begin);
         run_tile<EST, (int)Mode::Forward>(B, ((int64_t)item - tile_begin) * kWave, ls, nullptr);
@@ -1137,13 +1141,13 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n) {
     if (i < n) p[i] = v;
 }
 
-__global__ void k_combine_media(const float* gm, int n, float* gs, float* ga) {
+__global__ void k_combine_media(const float* gm, const float* sign, int n, float* gs, float* ga) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= 3 * n) return;
     const int m = i / 3, c = i % 3;
     const float direct = gm[6 * m + c], dt = gm[6 * m + 3 + c];
-    if (gs) gs[i] += direct + dt;
-    if (ga) ga[i] += dt;
+    if (gs) gs[i] += sign[6 * m + 3 + c] * (direct + dt);
+    if (ga) ga[i] += sign[6 * m + c] * dt;
 }
 
 // Morton key of the first segment's entry point, for coherent query bundles.
@@ -1293,9 +1297,10 @@ static void set_mis(EvalArgs& A, const MisDev* mis) {
 }
 
 void launch_photons(Ctx& ctx, Mode mode, const PhotonArgs& ph, const QueryView& q, float* out, uint32_t* sig,
-                    const AdjointOut* adj, const MisDev* mis) {
+                    const AdjointOut* adj, const MisDev* mis, unsigned long long* stats) {
     if (q.count == 0) return;
     EvalArgs A = base_args(ctx, q);
+    if (stats) A.stats = stats;
     set_mis(A, mis);
     A.nodes = ph.nodes;
     A.pos_r2 = ph.pos_r2;
@@ -1309,15 +1314,18 @@ void launch_photons(Ctx& ctx, Mode mode, const PhotonArgs& ph, const QueryView& 
         A.grad_media = adj->grad_media;
         A.grad_items = adj->grad_items;
     }
-    const int wpb = mode == Mode::Adjoint ? 4 : std::min(std::max(ctx.tuning.waves_per_block, 1), 4);
+    const int want = ctx.tuning.photon_waves_per_block > 0 ? ctx.tuning.photon_waves_per_block
+                                                           : ctx.tuning.waves_per_block;
+    const int wpb = mode == Mode::Adjoint ? 4 : std::min(std::max(want, 1), 4);
     const int64_t waves = (q.count + kWave - 1) / kWave;
     launch_mode<kEstPhotons>(ctx, mode, A, (unsigned)((waves + wpb - 1) / wpb), (unsigned)(wpb * kWave));
 }
 
 void launch_beams(Ctx& ctx, Mode mode, const BeamArgs& bm, const QueryView& q, float* out, uint32_t* sig,
-                  const AdjointOut* adj, const MisDev* mis) {
+                  const AdjointOut* adj, const MisDev* mis, unsigned long long* stats) {
     if (q.count == 0) return;
     EvalArgs A = base_args(ctx, q);
+    if (stats) A.stats = stats;
     set_mis(A, mis);
     A.nodes = bm.nodes;
     A.pieces = bm.pieces;
@@ -1416,8 +1424,9 @@ void launch_fill_u32(Ctx& ctx, uint32_t* p, uint32_t v, int64_t n) {
     UPBP_HIP(hipGetLastError());
 }
 
-void launch_combine_media_grads(Ctx& ctx, const float* gm, int n_media, float* gs, float* ga) {
-    hipLaunchKernelGGL(k_combine_media, dim3(grid_for(3 * n_media)), dim3(256), 0, ctx.stream, gm, n_media, gs, ga);
+void launch_combine_media_grads(Ctx& ctx, const float* gm, const float* sign, int n_media, float* gs, float* ga) {
+    hipLaunchKernelGGL(k_combine_media, dim3(grid_for(3 * n_media)), dim3(256), 0, ctx.stream, gm, sign, n_media, gs,
+                       ga);
     UPBP_HIP(hipGetLastError());
 }
 

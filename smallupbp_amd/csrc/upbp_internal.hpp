@@ -74,6 +74,7 @@ constexpr uint32_t kErrBadOffsets = 1u, kErrBadMedium = 2u, kErrBadSegment = 4u,
 struct Tuning {
     int waves_per_block = 4;
     int blocks_per_cu = 0;
+    int photon_waves_per_block = 0;  // 0: waves_per_block
 };
 
 struct Ctx {
@@ -88,6 +89,14 @@ struct Ctx {
     int n_media = 0;
     std::vector<MediumDev> media_host;
     DevBuf media;
+    // sign of every input coefficient (+1 / -1) as [M][6] floats (absorption RGB, scattering RGB):
+    // the media use |sigma| (Media.hxx:136-140), so gradients with respect to the caller's
+    // values carry these factors
+    DevBuf media_sign;
+    std::vector<float> media_sign_host;
+    // bumped by every upbp_set_media: a batched plan whose captured graph froze an older table
+    // re-validates its instances and captures again
+    uint64_t media_gen = 0;
     DevBuf stats;       // kStatSlots x uint64
     DevBuf sort_tmp;    // hipcub temporary storage
     DevBuf qkeys, qkeys2, qperm, qperm2;
@@ -142,6 +151,18 @@ struct Beams {
     DevBuf red_off, red_ids, red_rank;
 };
 
+// Device memory one evaluation works in besides its inputs and outputs: the kernels' counters,
+// the Morton keys and permutations of the query sort and the sort's temporary storage. The
+// context's own growable buffers by default; the allocation-free entry point passes a fixed
+// region instead.
+struct Workspace {
+    unsigned long long* stats = nullptr;  // kStatSlots counters
+    uint32_t *keys = nullptr, *keys2 = nullptr, *perm = nullptr, *perm2 = nullptr;  // [max_sorted] each
+    int64_t max_sorted = 0;
+    void* sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+};
+
 // Device-resident view of a query batch.
 struct QueryView {
     int64_t count = 0;
@@ -164,6 +185,11 @@ void sort_pairs_u64(Ctx& ctx, uint64_t* keys_in, uint64_t* keys_out, uint32_t* v
 void sort_pairs_u32(Ctx& ctx, uint32_t* keys_in, uint32_t* keys_out, uint32_t* vals_in,
                     uint32_t* vals_out, int64_t n, int end_bit);
 void exclusive_scan_u32(Ctx& ctx, const uint32_t* in, uint32_t* out, int64_t n);
+// The same sort in caller-provided temporary storage (UPBP_EDEVICE if tmp_bytes is too small);
+// sort_pairs_u32_temp_bytes is what it needs for n keys (no device work, no allocation).
+size_t sort_pairs_u32_temp_bytes(int64_t n, int end_bit);
+void sort_pairs_u32_in(Ctx& ctx, void* tmp, size_t tmp_bytes, uint32_t* keys_in, uint32_t* keys_out,
+                       uint32_t* vals_in, uint32_t* vals_out, int64_t n, int end_bit);
 
 // ---- knn.hip --------------------------------------------------------------
 // Point LBVH answering "squared distance to the k-th nearest point" queries.
@@ -239,10 +265,13 @@ struct MisDev {
     const float4* tails = nullptr;  // per light vertex, launch_light_tails
 };
 
+// stats: the kStatSlots counters the kernels add to (null: the context's own)
 void launch_photons(Ctx& ctx, Mode mode, const PhotonArgs& ph, const QueryView& q, float* out,
-                    uint32_t* sig, const AdjointOut* adj, const MisDev* mis = nullptr);
+                    uint32_t* sig, const AdjointOut* adj, const MisDev* mis = nullptr,
+                    unsigned long long* stats = nullptr);
 void launch_beams(Ctx& ctx, Mode mode, const BeamArgs& bm, const QueryView& q, float* out,
-                  uint32_t* sig, const AdjointOut* adj, const MisDev* mis = nullptr);
+                  uint32_t* sig, const AdjointOut* adj, const MisDev* mis = nullptr,
+                  unsigned long long* stats = nullptr);
 void launch_query_keys(Ctx& ctx, const QueryView& q, const float* lo, const float* hi, uint32_t* keys,
                        uint32_t* idx);
 // Folds each light vertex's light walk below it into a float4 (the O(1) light walk of the
@@ -250,7 +279,9 @@ void launch_query_keys(Ctx& ctx, const QueryView& q, const float* lo, const floa
 void launch_light_tails(Ctx& ctx, MisDev& mis);
 void launch_clear_u64(Ctx& ctx, uint64_t* p, int64_t n);
 void launch_clear_f32(Ctx& ctx, float* p, int64_t n);
-void launch_combine_media_grads(Ctx& ctx, const float* gm, int n_media, float* gs, float* ga);
+// gs += sign_s * (direct + d/d sigma_t), ga += sign_a * d/d sigma_t per medium and channel
+// (sign: [M][6] of ctx.media_sign).
+void launch_combine_media_grads(Ctx& ctx, const float* gm, const float* sign, int n_media, float* gs, float* ga);
 
 // ---- hashgrid.hip: vertex-merging hashed grid (PP3D, SURF) --------------------
 struct HashGrid {

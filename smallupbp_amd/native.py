@@ -175,14 +175,16 @@ class Context:
     def set_sync(self, sync: bool):
         self.check(self.lib.upbp_ctx_set_sync(self.h, 1 if sync else 0))
 
-    def set_tuning(self, waves_per_block: int, blocks_per_cu: int = 0):
-        t = _abi.Tuning(waves_per_block, blocks_per_cu)
+    def set_tuning(self, waves_per_block: int, blocks_per_cu: int = 0, photon_waves_per_block: int = 0):
+        """Waves per workgroup of the traversal kernels (the photon kernel's own count when
+        photon_waves_per_block > 0) and resident blocks per CU of the persistent kernels."""
+        t = _abi.Tuning(waves_per_block, blocks_per_cu, photon_waves_per_block, 0)
         self.check(self.lib.upbp_set_tuning(self.h, C.byref(t)))
 
     def get_tuning(self):
         t = _abi.Tuning()
         self.check(self.lib.upbp_get_tuning(self.h, C.byref(t)))
-        return t.waves_per_block, t.blocks_per_cu
+        return t.waves_per_block, t.blocks_per_cu, t.photon_waves_per_block
 
     def close(self):
         if getattr(self, "h", None):
@@ -331,18 +333,28 @@ class PhotonBeamsEvaluator:
 
     def getBeamSelectionPdf(self, pos) -> float:
         p = np.ascontiguousarray(pos, np.float32)
-        return self.ctx.lib.upbp_beams_selection_pdf(self.h, p.ctypes.data)
+        if p.size != 3:
+            raise ValueError("pos must be one xyz position")
+        pdf = self.ctx.lib.upbp_beams_selection_pdf(self.h, p.ctypes.data)
+        if pdf != pdf:  # NaN: the device read failed
+            raise UpbpError(_abi.UPBP_EDEVICE, self.ctx.lib.upbp_ctx_last_error(self.ctx.h).decode())
+        return pdf
 
     def getBeamSelectionPdfs(self, positions):
         """Selection pdf of the cell holding each position ([n,3] numpy or device tensor -> [n])."""
         if _is_torch(positions) and positions.is_cuda:
             import torch
 
+            if positions.ndim != 2 or positions.shape[1] != 3:
+                raise ValueError("positions must have shape [n, 3], got %s" % (tuple(positions.shape),))
             p = positions.contiguous().float()
             out = torch.empty(p.shape[0], dtype=torch.float32, device=p.device)
             mem = _abi.MEM_DEVICE
         else:
-            p = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+            p = np.ascontiguousarray(positions, np.float32)
+            if p.size % 3 != 0:
+                raise ValueError("positions must hold xyz triples, got %d floats" % p.size)
+            p = p.reshape(-1, 3)
             out = np.empty(p.shape[0], np.float32)
             mem = _abi.MEM_HOST
         self.ctx.check(self.ctx.lib.upbp_beams_selection_pdfs(self.h, _abi.ptr(p), p.shape[0], _abi.ptr(out), mem))

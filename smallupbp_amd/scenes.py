@@ -429,12 +429,15 @@ def with_media(w: Workload, media: np.ndarray) -> Workload:
     return dataclasses.replace(w, media=media, queries=q2)
 
 
-def batched_instances(n_instances=64, seed=4, scale=1.0, max_queries=2_000_000) -> List[Workload]:
+def batched_instances(n_instances=64, seed=4, scale=1.0, max_queries=2_000_000, workers=8) -> List[Workload]:
     """Independent unit-box scenes: sigma_s, sigma_a ~ U[0.05, 3] per channel; photons
     N ~ lognormal(mu=ln 1.4e5, sigma=1) clamped to [1e4, 2e6]; beams N/5; queries ~ U[16k, 2M]
-    random camera rays. `scale` shrinks every count (tests use small scales)."""
-    out = []
-    for i in range(n_instances):
+    random camera rays. `scale` shrinks every count (tests use small scales). Instances are
+    generated on `workers` threads; each has its own seeded generator, so the result does not
+    depend on the thread count."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(i):
         rng = rng_for(seed * 1000 + i)
         sa = rng.uniform(0.05, 3.0, 3)
         ss = rng.uniform(0.05, 3.0, 3)
@@ -448,8 +451,12 @@ def batched_instances(n_instances=64, seed=4, scale=1.0, max_queries=2_000_000) 
         o, d = random_camera_rays(rng, n_q)
         off, seg = ray_segments(media, o, d, (0, 0, 0), (1, 1, 1), 0, 1)
         q = Queries(o, d, off, seg, LONG_BEAM)
-        out.append(Workload("instance_%02d" % i, media, ph, bm, q, None, 0, 1, seed * 1000 + i))
-    return out
+        return Workload("instance_%02d" % i, media, ph, bm, q, None, 0, 1, seed * 1000 + i)
+
+    if workers <= 1 or n_instances <= 1:
+        return [one(i) for i in range(n_instances)]
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        return list(ex.map(one, range(n_instances)))
 
 
 # ---------------------------------------------------------------------------

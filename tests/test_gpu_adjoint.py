@@ -114,3 +114,41 @@ def test_autograd_function_l2_loss():
     assert abs(fd - g) <= 2e-2 * abs(g) + 1e-12
     # photon power: linear, FD exact up to FP32 rounding
     assert torch.isfinite(power.grad).all()
+
+
+def test_adjoint_gradient_of_negative_coefficients_carries_their_sign():
+    """The media use |sigma|: the GPU gradient with respect to a negative input coefficient is
+    the oracle's (whose sign is pinned on the CPU by central differences), sign included."""
+    print_case("adjoint_negative_sigma")
+    w = _small()
+    media = w.media.copy()
+    media[0]["absorption"][1] = -media[0]["absorption"][1]
+    media[2]["scattering"][0] = -media[2]["scattering"][0]
+    w = scenes.with_media(w, media)
+    rng = np.random.default_rng(5)
+    Q = w.queries.count
+    wp = rng.uniform(-1, 1, (Q, 3)).astype(np.float32)
+    wb = rng.uniform(-1, 1, (Q, 3)).astype(np.float32)
+    ctx = gpu_context(media)
+    bre = native.EmbreeBre(ctx)
+    bre.build(w.photons)
+    bm = native.PhotonBeamsEvaluator(ctx)
+    bm.build(w.beams)
+    qv = native.QueryBatchView.to_device(w.queries)
+    M = len(media)
+    for dev in (True, False):
+        gs = torch.zeros((M, 3), device="cuda") if dev else np.zeros((M, 3), np.float32)
+        ga = torch.zeros((M, 3), device="cuda") if dev else np.zeros((M, 3), np.float32)
+        gp = torch.from_numpy(wp).cuda() if dev else wp
+        gb = torch.from_numpy(wb).cuda() if dev else wb
+        q = qv if dev else native.QueryBatchView.host(w.queries)
+        native.adjoint(ctx, bre, bm, q, gp, gb, gs, ga, None, None)
+        if dev:
+            gs, ga = gs.cpu().numpy(), ga.cpu().numpy()
+        ref = po.adjoint(po.OraclePhotons(w.photons, build_bvh=False), po.OracleBeams(w.beams, grid_res=0), media,
+                         w.queries, wp.astype(np.float64), wb.astype(np.float64))
+        for got, want, m, c in ((ga, ref["grad_sigma_a"], 0, 1), (gs, ref["grad_sigma_s"], 2, 0)):
+            assert np.sign(got[m, c]) == np.sign(want[m, c]) != 0
+            assert abs(got[m, c] - want[m, c]) <= 1e-3 * abs(want[m, c]), (dev, m, c, got[m, c], want[m, c])
+        assert _rel(gs, ref["grad_sigma_s"]) < 1e-3
+        assert _rel(ga, ref["grad_sigma_a"]) < 1e-3

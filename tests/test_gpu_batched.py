@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 import torch
 
-from smallupbp_amd import native, scenes
+from smallupbp_amd import _abi, native, scenes
 
 from gpu_helpers import print_case
 
@@ -116,3 +116,75 @@ def test_batched_plan_owns_structure_copies_and_mixed_instances():
                 assert np.array_equal(op.cpu().numpy(), rp)
             if ob is not None:
                 assert np.array_equal(ob.cpu().numpy(), rb)
+
+
+def _two_instance_plan(ws, media, bases):
+    ctx = native.Context(0)
+    ctx.set_media(media)
+    keep, inst = [], []
+    for w, base in zip(ws, bases):
+        bre = native.EmbreeBre(ctx)
+        bre.build(w.photons)
+        bm = native.PhotonBeamsEvaluator(ctx)
+        bm.build(w.beams)
+        qv = native.QueryBatchView.to_device(w.queries)
+        op = torch.full((w.queries.count, 3), -1.0, device="cuda")
+        ob = torch.full((w.queries.count, 3), -1.0, device="cuda")
+        keep.append((bre, bm, qv, op, ob))
+        inst.append((bre, bm, qv, base, op, ob))
+    return ctx, keep, native.BatchPlan(ctx, inst)
+
+
+def test_batched_rejects_a_medium_id_outside_the_instances_range():
+    """An instance's media are the table entries from its media_base on: a segment medium id
+    that is valid for the whole table but not for the instance (media_base + id past the end)
+    fails the run with UPBP_EINVAL instead of reading past the media table."""
+    print_case("batched_medium_range")
+    ws = _instances(2, 0.005, seed=11)
+    media = np.concatenate([w.media for w in ws])  # 2 media per instance, 4 in all
+    w1 = ws[1]
+    seg = w1.queries.segments.copy()
+    seg["medium_id"][0] = 2  # < 4 (the table) but >= 4 - 2 (the instance's range)
+    bad = scenes.Workload(w1.name, w1.media, w1.photons, w1.beams,
+                          scenes.Queries(w1.queries.origin, w1.queries.direction, w1.queries.seg_offsets, seg,
+                                         w1.queries.beam_type))
+    ctx, keep, plan = _two_instance_plan([ws[0], bad], media, [0, 2])
+    for use_graph in (False, True):
+        with pytest.raises(native.UpbpError) as e:
+            plan.run(use_graph=use_graph)
+        assert e.value.status == _abi.UPBP_EINVAL
+    # the same plan with valid ids runs
+    ctx2, keep2, plan2 = _two_instance_plan(ws, media, [0, 2])
+    plan2.run(use_graph=True)
+    torch.cuda.synchronize()
+
+
+def test_batched_graph_follows_a_new_media_table():
+    """upbp_set_media between graph runs: the next run re-validates the instances and captures
+    again, so it reads the new table (grown here, which moves the device buffer) and equals an
+    eager run; a table too small for an instance's media_base is rejected."""
+    print_case("batched_media_regrowth")
+    ws = _instances(2, 0.005, seed=12)
+    media = np.concatenate([w.media for w in ws])
+    ctx, keep, plan = _two_instance_plan(ws, media, [0, 2])
+    plan.run(use_graph=True)
+    torch.cuda.synchronize()
+    first = [(op.cpu().numpy(), ob.cpu().numpy()) for _, _, _, op, ob in keep]
+    # grow the table: instance 1 now reads new coefficients at the same ids
+    grown = np.concatenate([media, media[:2]] * 8)
+    grown[2]["scattering"] = grown[2]["scattering"] * np.float32(1.5)
+    ctx.set_media(grown)
+    plan.run(use_graph=True)
+    torch.cuda.synchronize()
+    graph_out = [(op.cpu().numpy(), ob.cpu().numpy()) for _, _, _, op, ob in keep]
+    plan.run(use_graph=False)
+    torch.cuda.synchronize()
+    eager_out = [(op.cpu().numpy(), ob.cpu().numpy()) for _, _, _, op, ob in keep]
+    for (gp, gb), (ep, eb) in zip(graph_out, eager_out):
+        assert np.array_equal(gp, ep) and np.array_equal(gb, eb)
+    assert np.array_equal(graph_out[0][0], first[0][0])  # instance 0's media did not change
+    assert not np.array_equal(graph_out[1][0], first[1][0])  # instance 1's sigma_s did
+    ctx.set_media(media[:2])
+    with pytest.raises(native.UpbpError) as e:
+        plan.run(use_graph=True)
+    assert e.value.status == _abi.UPBP_EINVAL

@@ -1,0 +1,264 @@
+"""GPU parity at the full sizes of the benchmark configurations.
+
+The GPU evaluates every query of the workload with the same build and launch parameters
+the benchmark uses; a seeded random sample of those queries is then evaluated by the CPU
+oracle (photons through its BVH or by brute force, beams by brute force) and compared:
+hit signatures identical, radiance within REL_TOL = 1e-4 of the FP32 oracle. Properties
+that hold over all queries, where the oracle cannot follow at full size, are checked too:
+the kernels' hit counters equal the sum of the per-query signature counts, and the
+batched plan finds exactly as many hits as the per-instance evaluations.
+
+Workloads (smallupbp_amd/scenes.py):
+  bre_beams_1080p   1920x1080 queries x 1M photons + 200k SHORT beams, box + sphere media
+  beams_stress      1M query beams x 2M beams (50 % SHORT, 50 % LONG), one medium
+  batched_64        64 uneven instances at full size (10k-2M photons, 16k-2M queries each)
+  adjoint_512       512x512 queries x 500k photons + 100k beams, L2 loss against sigma_s x 1.1
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import pyoracle as po
+from smallupbp_amd import autotune, native, scenes
+
+from gpu_helpers import REL_TOL, gpu_context, print_case, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _sample(n, k, seed):
+    return np.sort(np.random.default_rng(seed).choice(n, min(k, n), replace=False))
+
+
+def _tuned_context(w):
+    """Context, structures and device queries with the benchmark's tuned parameters."""
+    p = autotune.tuned_params(w.photons.count if w.photons is not None else 0,
+                              w.beams.count if w.beams is not None else 0, w.queries.count)
+    ctx = gpu_context(w.media)
+    if p["beam_waves_per_block"] or p["photon_waves_per_block"]:
+        ctx.set_tuning(p["beam_waves_per_block"] or 4, 0, p["photon_waves_per_block"])
+    bre = bm = None
+    if w.photons is not None:
+        bre = native.EmbreeBre(ctx)
+        bre.build(w.photons, leaf_size=p["photon_leaf_size"])
+    if w.beams is not None:
+        bm = native.PhotonBeamsEvaluator(ctx)
+        bm.build(w.beams, leaf_size=p["beam_leaf_size"], piece_factor=p["piece_factor"])
+    qv = native.QueryBatchView.to_device(w.queries, order=p["order"])
+    return ctx, bre, bm, qv, p
+
+
+def _check_sample(name, out, sig, ref, rsig, cnt):
+    assert np.array_equal(sig, rsig), "%s hit sets differ from the oracle on the sample" % name
+    assert int(sig[:, 0].astype(np.int64).sum()) == int(cnt[0])
+    e = rel_err(out, ref)
+    print("%s: %d sampled queries, %d hits, max rel err %.3g" % (name, len(sig), int(cnt[0]), e.max()), flush=True)
+    assert e.max() <= REL_TOL
+
+
+@pytest.mark.timeout(600)
+def test_bre_beams_1080p_full_size():
+    """1920x1080 queries x 1M photons + 200k beams: the headline workload with its tuned
+    leaf sizes, piece length and launch shape; 20,000 sampled queries against the oracle's
+    photon BVH (FP32 and FP64), 2,000 against its beam brute force."""
+    print_case("bre_beams_1080p_full")
+    w = scenes.bre_beams()
+    ctx, bre, bm, qv, p = _tuned_context(w)
+    st = {}
+    op, ob = native.eval_batch(ctx, bre, bm, qv, stats=st)
+    op, ob = op.cpu().numpy(), ob.cpu().numpy()
+    sp, sb = native.hit_signatures(ctx, bre, bm, qv)
+    print("hits: %d photon, %d beam; %d beam pieces; params %s" % (st["photon_hits"], st["beam_hits"], bm.pieces, p),
+          flush=True)
+    # over all 2,073,600 queries: the forward kernels' counters equal the signature kernels' counts
+    assert st["photon_hits"] == int(sp[:, 0].astype(np.int64).sum())
+    assert st["beam_hits"] == int(sb[:, 0].astype(np.int64).sum())
+    assert np.isfinite(op).all() and np.isfinite(ob).all()
+    idx = _sample(w.queries.count, 20_000, 101)
+    qs = w.queries.subset(idx)
+    oph = po.OraclePhotons(w.photons)
+    ref, rsig, cnt = po.bre_eval(oph, w.media, qs, accel=1, signatures=True)
+    _check_sample("photons", op[idx], sp[idx], ref, rsig, cnt)
+    ref64, _, _ = po.bre_eval(oph, w.media, qs, accel=1, precision=64)
+    e64 = rel_err(op[idx], ref64)
+    print("photons vs FP64: median %.3g p99.9 %.3g" % (np.median(e64), np.percentile(e64, 99.9)), flush=True)
+    assert np.median(e64) < 1e-5
+    jdx = _sample(w.queries.count, 2_000, 102)
+    ref, rsig, cnt = po.beams_eval(po.OracleBeams(w.beams, grid_res=0), w.media, w.queries.subset(jdx), accel=0,
+                                   signatures=True)
+    _check_sample("beams", ob[jdx], sb[jdx], ref, rsig, cnt)
+
+
+@pytest.mark.timeout(600)
+def test_beams_stress_full_size():
+    """1M query beams x 2M photon beams (half SHORT, half LONG): 2,000 sampled queries
+    against the oracle's brute force; their total hit count equals the oracle's."""
+    print_case("beams_stress_full")
+    w = scenes.beams_stress()
+    ctx, _, bm, qv, p = _tuned_context(w)
+    st = {}
+    _, ob = native.eval_batch(ctx, None, bm, qv, stats=st)
+    ob = ob.cpu().numpy()
+    _, sb = native.hit_signatures(ctx, None, bm, qv)
+    print("beam hits %d (%.0f per query), %d pieces" % (st["beam_hits"], st["beam_hits"] / w.queries.count, bm.pieces),
+          flush=True)
+    assert st["beam_hits"] == int(sb[:, 0].astype(np.int64).sum())
+    assert np.isfinite(ob).all()
+    idx = _sample(w.queries.count, 2_000, 103)
+    ref, rsig, cnt = po.beams_eval(po.OracleBeams(w.beams, grid_res=0), w.media, w.queries.subset(idx), accel=0,
+                                   signatures=True)
+    _check_sample("beams", ob[idx], sb[idx], ref, rsig, cnt)
+
+
+@pytest.mark.timeout(900)
+def test_batched_64_full_size():
+    """64 uneven instances at their stated sizes in one persistent plan, replayed from a
+    hipGraph: per instance 300 sampled queries against the oracle (values; the hit
+    signatures of the same queries through the per-instance path equal the oracle's), and
+    the plan's total hit counts equal the per-instance evaluations' over all queries."""
+    print_case("batched_64_full")
+    ws = scenes.batched_instances(64, seed=4, scale=1.0)
+    Q = sum(w.queries.count for w in ws)
+    print("%d instances, %d photons, %d beams, %d queries" % (
+        len(ws), sum(w.photons.count for w in ws), sum(w.beams.count for w in ws), Q), flush=True)
+    media = np.concatenate([w.media for w in ws])
+    ctx = native.Context(0)
+    ctx.set_media(media)
+    inst, per = [], []
+    for i, w in enumerate(ws):
+        bre = native.EmbreeBre(ctx)
+        bre.build(w.photons)
+        bm = native.PhotonBeamsEvaluator(ctx)
+        bm.build(w.beams)
+        qv = native.QueryBatchView.to_device(w.queries)
+        op = torch.full((w.queries.count, 3), -1.0, device="cuda")
+        ob = torch.full((w.queries.count, 3), -1.0, device="cuda")
+        inst.append((bre, bm, qv, 2 * i, op, ob))
+    plan = native.BatchPlan(ctx, inst)
+    st = {}
+    plan.run(use_graph=True)
+    plan.run(use_graph=True, stats=st)
+    torch.cuda.synchronize()
+    # per-instance evaluations over all queries (own context: media base 0)
+    hp = hb = 0
+    for w, (bre, bm, qv, _, _, _) in zip(ws, inst):
+        pctx = gpu_context(w.media)
+        pb = native.EmbreeBre(pctx)
+        pb.build(w.photons)
+        pm = native.PhotonBeamsEvaluator(pctx)
+        pm.build(w.beams)
+        s1 = {}
+        native.eval_batch(pctx, pb, pm, native.QueryBatchView.to_device(w.queries), stats=s1)
+        hp += s1["photon_hits"]
+        hb += s1["beam_hits"]
+        per.append((pctx, pb, pm))
+    print("plan hits %d + %d, per-instance %d + %d" % (st["photon_hits"], st["beam_hits"], hp, hb), flush=True)
+    assert st["photon_hits"] == hp and st["beam_hits"] == hb
+    worst_p = worst_b = 0.0
+    for i, (w, (_, _, _, _, op, ob), (pctx, pb, pm)) in enumerate(zip(ws, inst, per)):
+        idx = _sample(w.queries.count, 300, 200 + i)
+        qs = w.queries.subset(idx)
+        t = torch.from_numpy(idx).cuda()
+        gp = op.index_select(0, t).cpu().numpy()
+        gb = ob.index_select(0, t).cpu().numpy()
+        sp, sb = native.hit_signatures(pctx, pb, pm, native.QueryBatchView.host(qs))
+        rp, rsp, _ = po.bre_eval(po.OraclePhotons(w.photons, build_bvh=False), w.media, qs, accel=0, signatures=True)
+        rb, rsb, _ = po.beams_eval(po.OracleBeams(w.beams, grid_res=0), w.media, qs, accel=0, signatures=True)
+        assert np.array_equal(sp, rsp), "instance %d: photon hit sets differ" % i
+        assert np.array_equal(sb, rsb), "instance %d: beam hit sets differ" % i
+        ep, eb = rel_err(gp, rp).max(), rel_err(gb, rb).max()
+        worst_p, worst_b = max(worst_p, ep), max(worst_b, eb)
+        assert ep <= REL_TOL and eb <= REL_TOL, (i, ep, eb)
+    print("batched vs oracle on 64 x 300 queries: max rel err %.3g photons, %.3g beams" % (worst_p, worst_b),
+          flush=True)
+
+
+def _l2_setup():
+    """adjoint_512 with chromatic media (a unique minimum sigma_t channel, so central
+    differences do not straddle the switch of the SHORT-beam normalisation channel that grey
+    media sit on) and its target image at sigma_s x 1.1."""
+    w = scenes.adjoint_scene(chromatic=True)
+    tgt = scenes.with_media(w, scenes.scale_scattering(w.media, 1.1))
+    return w, tgt
+
+
+def _forward(w, bre, bm, media):
+    ctx = bre.ctx
+    ctx.set_media(media)
+    qv = native.QueryBatchView.to_device(scenes.with_media(w, media).queries)
+    op, ob = native.eval_batch(ctx, bre, bm, qv)
+    return (op + ob), qv
+
+
+@pytest.mark.timeout(600)
+def test_adjoint_512_full_size():
+    """512x512 x (500k photons + 100k beams), loss = sum_q |L_q - T_q|^2 with T rendered at
+    sigma_s x 1.1. (a) The GPU adjoint on 2,000 sampled queries equals the FP64 oracle
+    gradient of the same queries, each of the 12 medium entries and every photon-power
+    entry to 1e-3 of itself. (b) Central differences (h = 1e-3) of the GPU forward over all
+    262,144 queries equal the GPU adjoint's gradient, each of the 12 entries to 1e-3."""
+    print_case("adjoint_512_full")
+    w, tgt = _l2_setup()
+    ctx = gpu_context(w.media)
+    bre = native.EmbreeBre(ctx)
+    bre.build(w.photons)
+    bm = native.PhotonBeamsEvaluator(ctx)
+    bm.build(w.beams)
+    T, _ = _forward(tgt, bre, bm, tgt.media)
+    L, qv = _forward(w, bre, bm, w.media)
+    g = (2.0 * (L - T)).contiguous()
+    M = len(w.media)
+    gs = torch.zeros((M, 3), device="cuda")
+    ga = torch.zeros((M, 3), device="cuda")
+    gpp = torch.zeros((w.photons.count, 3), device="cuda")
+    native.adjoint(ctx, bre, bm, qv, g, g, gs, ga, gpp, None)
+    gs, ga = gs.cpu().numpy().astype(np.float64), ga.cpu().numpy().astype(np.float64)
+    media_idx = [0, 2]  # box and sphere; medium 1 is the clear medium outside (|sigma| = 0 is a kink)
+
+    # (b) central differences of the GPU forward over every query
+    h = 1e-3
+
+    def loss(media):
+        Lm, _ = _forward(w, bre, bm, media)
+        return float(((Lm.double() - T.double()) ** 2).sum())
+
+    worst = 0.0
+    for m in media_idx:
+        for field, G in (("scattering", gs), ("absorption", ga)):
+            for c in range(3):
+                mp, mm = w.media.copy(), w.media.copy()
+                mp[m][field][c] += h
+                mm[m][field][c] -= h
+                fd = (loss(mp) - loss(mm)) / (2 * h)
+                err = abs(fd - G[m, c]) / abs(G[m, c])
+                worst = max(worst, err)
+                print("d loss / d %s[%d][%d]: adjoint %.7e fd %.7e rel %.2e" % (field, m, c, G[m, c], fd, err),
+                      flush=True)
+                assert err <= 1e-3, (field, m, c, fd, G[m, c])
+    print("worst FD relative error over the 12 entries: %.2e" % worst, flush=True)
+
+    # (a) the FP64 oracle gradient on a query sample, with the GPU adjoint on the same sample
+    ctx.set_media(w.media)
+    idx = _sample(w.queries.count, 2_000, 104)
+    qs = w.queries.subset(idx)
+    gsub = g.index_select(0, torch.from_numpy(idx).cuda()).contiguous()
+    gs2 = torch.zeros((M, 3), device="cuda")
+    ga2 = torch.zeros((M, 3), device="cuda")
+    gpp2 = torch.zeros((w.photons.count, 3), device="cuda")
+    native.adjoint(ctx, bre, bm, native.QueryBatchView.to_device(qs), gsub, gsub, gs2, ga2, gpp2, None)
+    gh = gsub.cpu().numpy().astype(np.float64)
+    ref = po.adjoint(po.OraclePhotons(w.photons, build_bvh=False), po.OracleBeams(w.beams, grid_res=0), w.media, qs,
+                     gh, gh)
+    gs2, ga2, gpp2 = (t.cpu().numpy().astype(np.float64) for t in (gs2, ga2, gpp2))
+    for m in media_idx:
+        for c in range(3):
+            for got, want, name in ((gs2, ref["grad_sigma_s"], "sigma_s"), (ga2, ref["grad_sigma_a"], "sigma_a")):
+                assert abs(got[m, c] - want[m, c]) <= 1e-3 * abs(want[m, c]), (name, m, c, got[m, c], want[m, c])
+    rp = ref["grad_photon_power"]
+    hit = np.abs(rp).max(1) > 0
+    print("photon-power gradient: %d photons hit by the sample" % int(hit.sum()), flush=True)
+    assert hit.sum() > 1000
+    assert np.all(np.abs(gpp2[~hit]) == 0)
+    err = np.abs(gpp2[hit] - rp[hit]) / np.abs(rp[hit]).max(1, keepdims=True)
+    print("photon-power gradient: max rel err %.3g" % err.max(), flush=True)
+    assert err.max() <= 1e-3

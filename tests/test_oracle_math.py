@@ -329,3 +329,36 @@ def test_oracle_adjoint_matches_central_differences():
     pm[k, 1] -= 1e-3
     fd = (loss(w.media, pp) - loss(w.media, pm)) / 2e-3
     assert fd == pytest.approx(g["grad_photon_power"][k, 1], rel=1e-5)
+
+
+def test_oracle_adjoint_carries_the_sign_of_negative_coefficients():
+    """The media use |sigma| (Media.hxx:136-140), so d loss / d sigma for a negative input
+    coefficient is sign(sigma) times the gradient with respect to |sigma|: central
+    differences of the FP64 forward in the caller's (negative) value pin it."""
+    w = scenes.adjoint_scene(12, 12, 3_000, 600, seed=36, chromatic=True)
+    media = w.media.copy()
+    media[0]["absorption"][1] = -media[0]["absorption"][1]
+    media[2]["scattering"][0] = -media[2]["scattering"][0]
+    w = scenes.with_media(w, media)
+    rng = np.random.default_rng(2)
+    wp = rng.uniform(-1, 1, (w.queries.count, 3))
+    wb = rng.uniform(-1, 1, (w.queries.count, 3))
+    oph = po.OraclePhotons(w.photons, build_bvh=False)
+    obm = po.OracleBeams(w.beams, grid_res=0)
+    g = po.adjoint(oph, obm, media, w.queries, wp, wb)
+
+    def loss(m):
+        ww = scenes.with_media(w, m)
+        a = po.bre_eval(oph, m, ww.queries, accel=0, precision=64)[0]
+        b = po.beams_eval(obm, m, ww.queries, accel=0, precision=64)[0]
+        return float((wp * a).sum() + (wb * b).sum())
+
+    h = 1e-3
+    for m, c, field in ((0, 1, "absorption"), (2, 0, "scattering"), (0, 1, "scattering")):
+        mp, mm = media.copy(), media.copy()
+        mp[m][field][c] += h
+        mm[m][field][c] -= h
+        fd = (loss(mp) - loss(mm)) / (2 * h)
+        an = g["grad_sigma_s" if field == "scattering" else "grad_sigma_a"][m, c]
+        assert abs(an) > 0
+        assert fd == pytest.approx(an, rel=1e-3, abs=1e-9 * max(1.0, abs(an)))
