@@ -750,6 +750,18 @@ def main():
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     log("step %.2f ms: photons %.2f ms, beams %.2f ms, sort+clear %.2f ms; hits %d + %d"
         % (ms_step, ms_ph, ms_bm, ms_ot, H_ph, H_bm))
+    # each estimator alone (one kernel per call): the per-kernel times without the other's overlap
+    n_alone = max(3, args.steps // 2)
+    el_p, per_p = time_eval(ctx, bre, None, qv, out_p, None, n_alone, 1)
+    el_b, per_b = time_eval(ctx, None, beams, qv, None, out_b, n_alone, 1)
+    alone = {"photons_ms_per_step": round(el_p / n_alone * 1e3, 4),
+             "photon_kernel_ms": round(float(np.mean([p["kernel_ms_photons"] for p in per_p])), 4),
+             "beams_ms_per_step": round(el_b / n_alone * 1e3, 4),
+             "beam_kernel_ms": round(float(np.mean([p["kernel_ms_beams"] for p in per_b])), 4),
+             "beam_tests_per_hit": round(int(per_b[-1]["beam_candidates"]) / max(1, int(per_b[-1]["beam_hits"])), 3),
+             "photon_tests_per_hit": round(int(per_p[-1]["photon_candidates"]) / max(1, int(per_p[-1]["photon_hits"])),
+                                           3)}
+    log("alone: photons %.2f ms, beams %.2f ms" % (alone["photon_kernel_ms"], alone["beam_kernel_ms"]))
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
@@ -794,6 +806,7 @@ def main():
                               "end on its own stream"},
         "candidates": {"photon_tests": int(per[-1]["photon_candidates"]), "beam_tests": int(per[-1]["beam_candidates"]),
                        "nodes": int(per[-1]["nodes_visited"])},
+        "alone": alone,
         "hits": {"photon": H_ph, "beam": H_bm,
                  "photon_evals_per_s": round(H_ph / (ms_step * 1e-3), 1),
                  "beam_evals_per_s": round(H_bm / (ms_step * 1e-3), 1)},

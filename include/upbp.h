@@ -125,9 +125,9 @@ typedef struct upbp_build_params {
     float beam_piece_factor;    /* beams only: target piece length in units of the max beam radius; 0 = default */
     int32_t max_pieces_per_beam;/* beams only: cap on pieces per beam, 1..256; 0 = default */
     /* beams only: per-cell beam reduction of the reference's uniform grid (-gridmax / -gridred,
-     * Grid.hxx:167-279,409-631; plain BB1D only, as in the reference). A hit whose query point
-     * lies in a cell holding n > max_beams_in_cell beams is kept as the reduction type decides
-     * and weighted by 1 / (max_beams_in_cell / n). */
+     * Grid.hxx:167-279,409-631; upbp_eval, upbp_hit_signatures and upbp_eval_upbp). A hit whose
+     * query point lies in a cell holding n > max_beams_in_cell beams is kept as the reduction
+     * type decides and weighted by 1 / (max_beams_in_cell / n). */
     uint32_t max_beams_in_cell; /* sMaxBeamsInCell; 0 = no reduction (the reference default) */
     uint32_t reduction_type;    /* sReductionType: UPBP_REDUCE_* */
     int32_t grid_resolution;    /* cells on the grid's longest axis (sGridSize); 0 = 256 */
@@ -429,7 +429,14 @@ typedef struct upbp_mis_inputs {
 
 /* UPBP evaluation. The photon structure must have been built from the light
  * vertices themselves (photon i of the build input = light vertex i, filtered by
- * in_medium), as EmbreeBre::build does for UPBPLightVertex (Bre.cxx:649-755). */
+ * in_medium), as EmbreeBre::build does for UPBPLightVertex (Bre.cxx:649-755).
+ * A beam structure built with per-cell reduction (max_beams_in_cell > 0, below 2^23)
+ * weights each beam hit as the reference's reduced grid does: the hit is tested as often
+ * as its query point's cell decides (upbp_eval's rule), and each test divides by the
+ * cell's selection pdf and passes it to the MIS weights as beamSelectionPdf
+ * (PhotonBeam.hxx:487-564). A PB2D hit's camera vertex then takes the pdf of the cell
+ * holding the hit point as its BB1D selection pdf (Bre.cxx:407-415). With out_beams NULL
+ * the beam structure is not evaluated and serves only as that evaluator. */
 upbp_status upbp_eval_upbp(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams* beams,
                            const upbp_query_batch* queries, const upbp_mis_inputs* mis,
                            float* out_photons, float* out_beams, upbp_memory out_memory,

@@ -64,17 +64,30 @@ static int32_t lt_medium_at(const upbp_region* rg, int32_t n, int32_t global, co
     return m;
 }
 
+/* Closest call of a path's branch decisions (test support): each comparison a < b that picks
+ * a branch lowers *margin to |a - b| / scale, the relative distance of the two sides. A path
+ * whose every margin is large takes the same branches under last-ulp differences of the
+ * math library; only a path with a small margin can take another one. */
+static inline void lt_margin(float* margin, float a, float b, float scale) {
+    if (!margin) return;
+    const float m = fabsf(a - b) / fmaxf(scale, 1e-30f);
+    if (m < *margin) *margin = m;
+}
+
 /* positive distances at which the ray crosses the region's boundary */
-static int lt_crossings(const upbp_region* g, const float* o, const float* d, float* t) {
+static int lt_crossings(const upbp_region* g, const float* o, const float* d, float* t, float* margin) {
     int n = 0;
     if (g->shape == UPBP_REGION_SPHERE) {
         const float ox = o[0] - g->p0[0], oy = o[1] - g->p0[1], oz = o[2] - g->p0[2];
         const float b = (ox * d[0] + oy * d[1]) + oz * d[2];
         const float c = ((ox * ox + oy * oy) + oz * oz) - g->p1[0] * g->p1[0];
         const float disc = b * b - c;
+        lt_margin(margin, disc, 0.f, fmaxf(b * b, fabsf(c)));
         if (disc < 0.f) return 0;
         const float s = sqrtf(disc);
         const float t0 = -b - s, t1 = -b + s;
+        lt_margin(margin, t0, 0.f, fabsf(b) + s);
+        lt_margin(margin, t1, 0.f, fabsf(b) + s);
         if (t0 > 0.f) t[n++] = t0;
         if (t1 > 0.f) t[n++] = t1;
         return n;
@@ -94,7 +107,10 @@ static int lt_crossings(const upbp_region* g, const float* o, const float* d, fl
             return 0;
         }
     }
+    lt_margin(margin, tn, tf, fmaxf(fabsf(tn), fabsf(tf)));
     if (!(tn <= tf)) return 0;
+    lt_margin(margin, tn, 0.f, fabsf(tn) + fabsf(tf));
+    lt_margin(margin, tf, 0.f, fabsf(tn) + fabsf(tf));
     if (tn > 0.f) t[n++] = tn;
     if (tf > 0.f) t[n++] = tf;
     return n;
@@ -106,10 +122,11 @@ typedef struct {
 } lt_piece;
 
 /* VolumeSegmentLite list of the ray: [0, c1], [c1, c2], ..., [cn, LT_INF) */
-static int lt_cut(const upbp_region* rg, int32_t nr, int32_t global, const float* o, const float* d, lt_piece* out) {
+static int lt_cut(const upbp_region* rg, int32_t nr, int32_t global, const float* o, const float* d, lt_piece* out,
+                  float* margin) {
     float c[2 * LT_MAX_REGIONS];
     int n = 0;
-    for (int32_t i = 0; i < nr; ++i) n += lt_crossings(&rg[i], o, d, c + n);
+    for (int32_t i = 0; i < nr; ++i) n += lt_crossings(&rg[i], o, d, c + n, margin);
     for (int i = 1; i < n; ++i) { /* insertion sort */
         const float x = c[i];
         int j = i - 1;
@@ -198,6 +215,7 @@ typedef struct {
     int32_t* b_med;
     upbp_beam_mis* b_mis; /* light_vertex = index inside the path */
     int32_t nb;
+    float* margin; /* closest call of the path's branch decisions, or NULL */
 } lt_out;
 
 static void lt_push_beam(lt_out* o, const float* org, const float* dir, float len, const float* thr, uint32_t flags,
@@ -259,7 +277,7 @@ static void lt_trace_path(const or_medium* media, const upbp_region* rg, int32_t
     const int want_beams = P->beam_path_count > 0 && path < P->beam_path_count;
     for (uint32_t pl = 1;; ++pl) {
         lt_piece seg[2 * LT_MAX_REGIONS + 1];
-        const int ns = lt_cut(rg, nr, P->global_medium, org, dir, seg);
+        const int ns = lt_cut(rg, nr, P->global_medium, org, dir, seg, o->margin);
         /* Scene::Intersect, media part: sample distances segment by segment */
         float vs_t0[2 * LT_MAX_REGIONS + 1], vs_t1[2 * LT_MAX_REGIONS + 1], vs_pdf[2 * LT_MAX_REGIONS + 1],
             vs_rev[2 * LT_MAX_REGIONS + 1], vs_att[2 * LT_MAX_REGIONS + 1][3];
@@ -273,6 +291,7 @@ static void lt_trace_path(const or_medium* media, const upbp_region* rg, int32_t
                 const float u = lt_next(&rng);
                 const float s = -logf(u) / m->min_comp;
                 const float len = seg[i].t1 - seg[i].t0;
+                lt_margin(o->margin, s, len, fmaxf(s, len));
                 if (s < len) {
                     const float att = expf(-m->min_comp * s);
                     pdf = m->min_comp * att;
@@ -399,7 +418,12 @@ static void lt_trace_path(const or_medium* media, const upbp_region* rg, int32_t
         if (f[0] <= 0.f && f[1] <= 0.f && f[2] <= 0.f) break;
         float rev_pdf = lt_phase_pdf(m->mean_cosine, (gen[0] * dir[0] + gen[1] * dir[1]) + gen[2] * dir[2]);
         const float cont = m->continuation_prob;
-        if (cont == 0.f || (cont < 1.f && lt_next(&rng) > cont)) break;
+        if (cont == 0.f) break;
+        if (cont < 1.f) {
+            const float u = lt_next(&rng);
+            lt_margin(o->margin, u, cont, fabsf(cont));
+            if (u > cont) break;
+        }
         float dir_pdf = pdf * cont;
         rev_pdf *= cont;
         const float dir_pdf_inv = 1.f / dir_pdf;
@@ -420,6 +444,14 @@ static void lt_trace_path(const or_medium* media, const upbp_region* rg, int32_t
 int or_trace_light_paths(const upbp_medium* media, int32_t n_media, const upbp_region* regions, int32_t n_regions,
                          const upbp_point_light* lights, int32_t n_lights, const upbp_light_trace_params* P,
                          int64_t path_count, int threads, upbp_light_paths* out) {
+    return or_trace_light_paths_margins(media, n_media, regions, n_regions, lights, n_lights, P, path_count, threads,
+                                        out, NULL);
+}
+
+int or_trace_light_paths_margins(const upbp_medium* media, int32_t n_media, const upbp_region* regions,
+                                 int32_t n_regions, const upbp_point_light* lights, int32_t n_lights,
+                                 const upbp_light_trace_params* P, int64_t path_count, int threads,
+                                 upbp_light_paths* out, float* margins) {
     if (!media || n_media < 1 || n_regions < 0 || n_regions > LT_MAX_REGIONS || (n_regions && !regions) ||
         !lights || n_lights < 1 || !P || path_count < 0 || !out || (path_count > 0 && !out->path_ends) ||
         P->global_medium < 0 || P->global_medium >= n_media)
@@ -430,6 +462,8 @@ int or_trace_light_paths(const upbp_medium* media, int32_t n_media, const upbp_r
         out->vertex_count = 0;
         out->beam_count = 0;
         for (int64_t p = 0; p < path_count; ++p) out->path_ends[p] = 0;
+        if (margins)
+            for (int64_t p = 0; p < path_count; ++p) margins[p] = INFINITY;
         return 0;
     }
     or_medium* m = (or_medium*)malloc(sizeof(or_medium) * (size_t)n_media);
@@ -458,6 +492,8 @@ int or_trace_light_paths(const upbp_medium* media, int32_t n_media, const upbp_r
         o.b_flags = sf + bb;
         o.b_med = smed + bb;
         o.b_mis = smis + bb;
+        o.margin = margins ? &margins[p] : NULL;
+        if (margins) margins[p] = INFINITY;
         lt_trace_path(m, regions, n_regions, lights, n_lights, P, p, &o);
         nv[p] = o.nv;
         nb[p] = o.nb;

@@ -15,7 +15,8 @@ from smallupbp_amd import _abi
 from smallupbp_amd.scenes import Beams, Photons, Queries
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# UPBP_ORACLE_LIB selects another build of the same sources (the sanitized one of `make asan`)
+LIB_PATH = os.environ.get("UPBP_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 _lib = None
 
 
@@ -387,9 +388,10 @@ def light_trace_params(scene) -> _abi.LightTraceParams:
     return p
 
 
-def trace_light_paths(scene, path_count: int, threads=None):
+def trace_light_paths(scene, path_count: int, threads=None, margins=False):
     """upbp_trace_light_paths restated (or_trace_light_paths): dict of numpy arrays trimmed to
-    the produced vertices and beams."""
+    the produced vertices and beams. margins=True adds "margins" [path_count] float32: per path
+    the closest call of its branch decisions (relative distance of the compared values)."""
     from smallupbp_amd import scenes
 
     threads = threads or _threads()
@@ -400,9 +402,21 @@ def trace_light_paths(scene, path_count: int, threads=None):
     regions = np.ascontiguousarray(scene.regions)
     lights = np.ascontiguousarray(scene.lights)
     par = light_trace_params(scene)
-    r = lib().or_trace_light_paths(media.ctypes.data, len(media), regions.ctypes.data if len(regions) else None,
-                                   len(regions), lights.ctypes.data, len(lights), C.byref(par), path_count, threads,
-                                   C.byref(lp))
+    args = (media.ctypes.data, len(media), regions.ctypes.data if len(regions) else None, len(regions),
+            lights.ctypes.data, len(lights), C.byref(par), path_count, threads, C.byref(lp))
+    mg = None
+    if margins:
+        mg = np.empty(max(path_count, 1), np.float32)
+        f = lib().or_trace_light_paths_margins
+        f.restype = C.c_int
+        f.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
+                      C.POINTER(_abi.LightTraceParams), C.c_int64, C.c_int, C.POINTER(_abi.LightPaths), C.c_void_p]
+        r = f(*args, mg.ctypes.data)
+    else:
+        r = lib().or_trace_light_paths(*args)
     if r:
         raise RuntimeError("or_trace_light_paths failed: %s" % _abi.STATUS_NAMES.get(r, r))
-    return scenes.trim_light_paths(arr, lp.vertex_count, lp.beam_count, path_count)
+    out = scenes.trim_light_paths(arr, lp.vertex_count, lp.beam_count, path_count)
+    if margins:
+        out["margins"] = mg[:path_count]
+    return out

@@ -220,6 +220,14 @@ static uint32_t red_mult(const or_beams* bm, uint64_t qkey, uint32_t c, uint32_t
     }
     return red_uniform(bm->red_seed, qkey, c, b) < pdf;
 }
+/* The selection pdf of the cell holding p (Grid::pdf, Grid.hxx:409-415): max / n in a cell
+ * of n > max beams, 1 otherwise and without reduction. */
+static float red_cell_pdf(const or_beams* bm, const float* p) {
+    if (!bm->red_max || !bm->grid.cells) return 1.0f;
+    uint32_t c = red_cell(&bm->grid, p), n = bm->grid.cells[c + 1] - bm->grid.cells[c];
+    return n <= bm->red_max ? 1.0f : (float)bm->red_max / (float)n;
+}
+
 /* A hit of beam b whose query point is p belongs to p's cell: times counted, pdf of the cell. */
 static uint32_t red_decide_at(const or_beams* bm, uint64_t qkey, const float* p, uint32_t b, float* pdf) {
     const or_grid* g = &bm->grid;
@@ -675,11 +683,7 @@ int or_beams_set_reduction(or_beams* bm, uint32_t max_beams_in_cell, uint32_t re
     return 0;
 }
 
-float or_beams_selection_pdf(const or_beams* bm, const float* pos) {
-    if (!bm->red_max || !bm->grid.cells) return 1.0f;
-    uint32_t c = red_cell(&bm->grid, pos), n = bm->grid.cells[c + 1] - bm->grid.cells[c];
-    return n <= bm->red_max ? 1.0f : (float)bm->red_max / (float)n;
-}
+float or_beams_selection_pdf(const or_beams* bm, const float* pos) { return red_cell_pdf(bm, pos); }
 
 void or_beams_free(or_beams* bm) {
     if (!bm) return;
@@ -967,6 +971,15 @@ static int check_batch(const upbp_query_batch* q, int32_t n_media) {
     if (q->count > 0 && (!q->origin || !q->direction || !q->segment_offsets)) return UPBP_EINVAL;
     if (q->lite_segments && q->beam_type != UPBP_LONG_BEAM) return UPBP_EINVAL;
     if (n_media <= 0) return UPBP_EINVAL;
+    /* the library's CSR rule: offsets non-decreasing inside [0, offsets[count]] */
+    if (q->count > 0) {
+        const int64_t end = q->segment_offsets[q->count];
+        if (end < 0 || (end > 0 && !q->segments)) return UPBP_EINVAL;
+        for (int64_t i = 0; i < q->count; ++i) {
+            const int64_t s0 = q->segment_offsets[i], s1 = q->segment_offsets[i + 1];
+            if (!(s0 >= 0 && s0 <= s1 && s1 <= end)) return UPBP_EINVAL;
+        }
+    }
     return 0;
 }
 
@@ -1159,7 +1172,6 @@ int or_adjoint(const or_photons* ph, const or_beams* bm, const upbp_medium* medi
 int or_upbp_eval(const or_photons* ph, const or_beams* bm, const upbp_medium* media, int32_t n_media,
                  const upbp_query_batch* q, const upbp_mis_inputs* mis, int which, int accel, int precision,
                  int threads, float* out32, double* out64, uint64_t* counters) {
-    if (bm && bm->red_max) return UPBP_EUNSUPPORTED; /* the UPBP weights take no reduction here */
     int e = check_batch(q, n_media);
     if (e) return e;
     if (!mis || (which == 0 && !ph) || (which == 1 && (!bm || !mis->beams))) return UPBP_EINVAL;

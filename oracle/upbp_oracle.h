@@ -124,7 +124,12 @@ int or_beams_eval(const or_beams* bm, const upbp_medium* media, int32_t n_media,
  * AccumulateCameraPathWeight / AccumulateLightPathWeight (PathWeight.hxx:33-270),
  * exact path walks. which: 0 photons (photon i = light vertex orig[i]), 1 beams.
  * accel 0: brute force over items; accel 1: photons through the BVH, beams through the
- * reference's uniform grid (the timed CPU comparison). counters[0] = hits. */
+ * reference's uniform grid (the timed CPU comparison). counters[0] = hits (beam hits
+ * counted once per test). When bm reduces its cells (or_beams_set_reduction), a beam hit
+ * passes its cell's pdf as beamSelectionPdf and is counted as often as the cell tests it
+ * (accel 0: the cell of the hit's query point, as or_beams_eval), and a photon hit's
+ * camera vertex takes the pdf of the cell holding the hit point (Bre.cxx:407-415);
+ * bm is read for that even when which = 0. */
 int or_upbp_eval(const or_photons* ph, const or_beams* bm, const upbp_medium* media, int32_t n_media,
                  const upbp_query_batch* q, const upbp_mis_inputs* mis, int which, int accel, int precision,
                  int threads, float* out32, double* out64, uint64_t* counters);
@@ -181,6 +186,13 @@ int or_surf_eval(const or_hgrid* g, const upbp_surf_query* q, int64_t nq, const 
 int or_trace_light_paths(const upbp_medium* media, int32_t n_media, const upbp_region* regions, int32_t n_regions,
                          const upbp_point_light* lights, int32_t n_lights, const upbp_light_trace_params* params,
                          int64_t path_count, int threads, upbp_light_paths* out);
+/* The same, and per path the closest call of its branch decisions (distance sample against
+ * the segment end, roulette draw against the continuation probability, sphere and box
+ * crossings), as the relative distance of the two compared values (INFINITY: no decision). */
+int or_trace_light_paths_margins(const upbp_medium* media, int32_t n_media, const upbp_region* regions,
+                                 int32_t n_regions, const upbp_point_light* lights, int32_t n_lights,
+                                 const upbp_light_trace_params* P, int64_t path_count, int threads,
+                                 upbp_light_paths* out, float* margins);
 
 #ifdef __cplusplus
 }

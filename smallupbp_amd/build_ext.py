@@ -38,15 +38,24 @@ def flags():
 
 def build(verbose: bool = False) -> str:
     """Compile every source (always: this is the build check) and link the shared library."""
-    objs = []
+    from concurrent.futures import ThreadPoolExecutor
+
+    cmds, objs = [], []
     for s in SOURCES:
         src = os.path.join(CSRC, s)
         obj = os.path.join(CSRC, s.replace(".hip", ".o"))
-        cmd = [hipcc()] + flags() + ["-c", src, "-o", obj]
+        cmds.append([hipcc()] + flags() + ["-c", src, "-o", obj])
+        objs.append(obj)
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "0") or 0) or os.cpu_count() or 1, 8))
+
+    def compile_one(cmd):
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.check_call(cmd)
-        objs.append(obj)
+
+    with ThreadPoolExecutor(jobs) as pool:
+        for f in [pool.submit(compile_one, c) for c in cmds]:
+            f.result()
     tmp = LIB + ".tmp"
     cmd = [hipcc(), "--offload-arch=%s" % ARCH, "-shared", "-fPIC", "-o", tmp] + objs
     if verbose:

@@ -77,16 +77,23 @@ __host__ __device__ inline uint32_t red_cell(const RedDev& R, float x, float y, 
     return (uint32_t)(v[0] + R.res[0] * (v[1] + R.res[1] * v[2]));
 }
 
+// Selection pdf of the cell holding (x, y, z): Grid::pdf (Grid.hxx:409-415), max / n in a cell
+// of n > max beams, else 1.
+__device__ inline float red_cell_pdf(const RedDev& R, float x, float y, float z) {
+    const uint32_t c = red_cell(R, x, y, z);
+    const uint32_t n = R.off[c + 1] - R.off[c];
+    return n <= R.max ? 1.0f : (float)R.max / (float)n;
+}
+
 // Times (0 = dropped) a hit of beam b whose query point is (x, y, z) is counted, and the
-// weight 1 / pdf of each count.
+// selection pdf of its cell (each count weighs 1 / pdf).
 __device__ inline uint32_t red_decide(const RedDev& R, uint64_t qkey, float x, float y, float z, uint32_t b,
-                                      float& inv_pdf) {
-    inv_pdf = 1.0f;
+                                      float& pdf) {
+    pdf = 1.0f;
     const uint32_t c = red_cell(R, x, y, z);
     const uint32_t b0 = R.off[c], n = R.off[c + 1] - b0;
     if (n <= R.max) return 1u;
-    const float pdf = (float)R.max / (float)n;
-    inv_pdf = 1.0f / pdf;
+    pdf = (float)R.max / (float)n;
     uint32_t lo = 0, hi = n;  // the beam's entry among the cell's ascending ids
     while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;

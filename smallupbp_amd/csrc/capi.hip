@@ -221,7 +221,7 @@ void fill_stats(upbp_eval_stats* out, const uint64_t* st, float ms_ph, float ms_
 }
 
 PhotonArgs photon_args(const Bre* b) {
-    PhotonArgs a;
+    PhotonArgs a = {};
     a.nodes = b->bvh.nodes.as<NodeDev>();
     a.pos_r2 = b->pos_r2.as<float4>();
     a.power = b->power.as<float4>();
@@ -307,7 +307,9 @@ void run_eval(Ctx& ctx, const Bre* bre, const Beams* bm, const upbp_query_batch*
     uint32_t launches = 0;
     if (want_ph) {
         if (bre->n > 0) {
-            launch_photons(ctx, mode, photon_args(bre), v, dph, sph, adj_ph, mis, ws->stats);
+            PhotonArgs pa = photon_args(bre);
+            if (mode == Mode::Upbp && bm) pa.sel = bm->red;
+            launch_photons(ctx, mode, pa, v, dph, sph, adj_ph, mis, ws->stats);
             launches++;
         } else if (mode != Mode::Adjoint) {
             if (dph) launch_clear_f32(ctx, dph, 3 * Q);
@@ -622,10 +624,12 @@ upbp_status upbp_eval_upbp(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams*
     return guarded(ctx, [&] {
         check_batch(*ctx, queries);
         UPBP_REQUIRE(mis != nullptr, UPBP_EINVAL, "mis inputs are required");
-        UPBP_REQUIRE(!beams || beams->red.max == 0, UPBP_EUNSUPPORTED,
-                     "per-cell beam reduction is supported by upbp_eval and upbp_hit_signatures only");
+        // the hit queue packs a reduced hit's test count into 23 bits; a cell tests at most
+        // max_beams_in_cell beams, and reduction happens only in cells holding more
+        UPBP_REQUIRE(!beams || beams->red.max < (1u << 23), UPBP_EINVAL,
+                     "max_beams_in_cell must be below 2^23 for UPBP evaluation");
         UPBP_REQUIRE(!bre || out_photons, UPBP_EINVAL, "out_photons is NULL but a photon structure was given");
-        UPBP_REQUIRE(!beams || out_beams, UPBP_EINVAL, "out_beams is NULL but a beam structure was given");
+        // beams with out_beams NULL: the structure only gives PB2D hits their BB1D selection pdf
         const int64_t Q = queries->count;
         UPBP_REQUIRE(Q == 0 || mis->queries, UPBP_EINVAL, "mis->queries is required");
         UPBP_REQUIRE(mis->camera_count > 0 && mis->camera, UPBP_EINVAL, "mis->camera is required");
@@ -633,7 +637,8 @@ upbp_status upbp_eval_upbp(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams*
                      UPBP_EINVAL, "mis->light_vertices is required");
         UPBP_REQUIRE(mis->path_count >= 0 && (mis->path_count == 0 || mis->path_ends), UPBP_EINVAL,
                      "mis->path_ends is required");
-        UPBP_REQUIRE(!beams || beams->n == 0 || mis->beams, UPBP_EINVAL, "mis->beams is required with beams");
+        UPBP_REQUIRE(!beams || !out_beams || beams->n == 0 || mis->beams, UPBP_EINVAL,
+                     "mis->beams is required to evaluate beams");
         UPBP_REQUIRE(!bre || bre->n_in <= mis->light_vertex_count, UPBP_EINVAL,
                      "the photon structure must be built from the light vertices (photon i = light vertex i)");
         const upbp_mis_params& P = mis->params;
@@ -659,7 +664,7 @@ upbp_status upbp_eval_upbp(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams*
         d.lv_count = mis->light_vertex_count;
         d.path_ends = (const int32_t*)stage_mis(sp, mis->path_ends, (size_t)mis->path_count * 4);
         d.path_count = mis->path_count;
-        d.beams = (const upbp_beam_mis*)stage_mis(sb, mis->beams,
+        d.beams = (const upbp_beam_mis*)stage_mis(sb, out_beams ? mis->beams : nullptr,
                                                   beams ? (size_t)beams->n * sizeof(upbp_beam_mis) : 0);
         RunOut o;
         o.out_ph = bre ? out_photons : nullptr;

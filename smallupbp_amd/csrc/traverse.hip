@@ -254,8 +254,11 @@ __device__ void photon_hit_mis(const EvalArgs& A, const Ray& r, SegState& s, con
     if (u0 <= 0.f && u1 <= 0.f && u2 <= 0.f) return;
     const float dist_sq = t * t;
     const float ray_pdf_inv = __builtin_amdgcn_rcpf(ray_pdf);
-    const MisF mv = merge_vertex_mis(P, ml.last_pdf_w_inv * dist_sq * ray_pdf_inv, ray_pdf_inv, ratio,
-                                     P.has_bb1d_beams ? 1.f : 0.f,
+    // Bre.cxx:407-415: the photon-beam evaluator's getBeamSelectionPdf(isectPt) when it reduces
+    // its cells (A.red is then its grid), 1 when it does not, 0 without photon beams
+    float sel = P.has_bb1d_beams ? 1.f : 0.f;
+    if (P.has_bb1d_beams && A.red.max) sel = red_cell_pdf(A.red, r.ox + t * r.dx, r.oy + t * r.dy, r.oz + t * r.dz);
+    const MisF mv = merge_vertex_mis(P, ml.last_pdf_w_inv * dist_sq * ray_pdf_inv, ray_pdf_inv, ratio, sel,
                                      s.mfp > P.bb1d_min_mfp ? UPBP_MIS_IN_MEDIUM_WITH_BEAMS : 0u);
     const float last = short_q ? __builtin_amdgcn_rcpf(ratio * mv.pb2d) : ray_pdf * __builtin_amdgcn_rcpf(mv.pb2d);
     const uint32_t tech = A.short_query ? (UPBP_SHORT_BEAM | P.techniques) : (UPBP_LONG_BEAM | P.techniques);
@@ -278,9 +281,12 @@ __device__ void photon_hit_mis(const EvalArgs& A, const Ray& r, SegState& s, con
 
 // PhotonBeam::accumulate2 after the beam-beam test (PhotonBeam.hxx:393-605). The beam,
 // its MIS record and its origin light vertex are wave-uniform (scalar loads).
+// sel_pdf: the beam selection pdf of the hit's reduction cell (1 without reduction); mult: the
+// times that cell tests the beam, each test adding the weighted contribution.
 __device__ void beam_hit_mis(const EvalArgs& A, const Ray& r, SegState& s, const MisLane& ml, float dist,
                              float sin_theta, float t1, float t2, float radius, bool short_b, float4 flux,
-                             float d2x, float d2y, float d2z, uint32_t beam_idx, bool short_q, uint32_t& err) {
+                             float d2x, float d2y, float d2z, uint32_t beam_idx, float sel_pdf, uint32_t mult,
+                             bool short_q, uint32_t& err) {
     const MisGlobals& G = A.mis;
     const upbp_mis_params& P = G.p;
     const upbp_beam_mis bm = G.beams[beam_idx];
@@ -296,7 +302,6 @@ __device__ void beam_hit_mis(const EvalArgs& A, const Ray& r, SegState& s, const
     // As in photon_hit_mis, everything below is weight and contribution arithmetic with a
     // 1e-4 budget: hardware reciprocals, and one exponential per channel for the query and
     // beam transmittances together (with the SHORT normalisations folded into the exponent).
-    const float sel_pdf = 1.0f;  // no per-cell beam reduction
     const float dq = t1 - s.tmin;
     const float stk = pick3(s.kidx, s.st0, s.st1, s.st2);
     const float pq = fast_exp(-stk * dq), pb = fast_exp(-stk * t2);
@@ -319,9 +324,9 @@ __device__ void beam_hit_mis(const EvalArgs& A, const Ray& r, SegState& s, const
     if (bsdf <= 0.f) return;
     const float dir_pdf = bsdf * s.cont;
     const float rev_pdf = bsdf * s.cont;
-    // 1D kernel (1 - dist^2/r^2) * 3 / (4 r sin(theta)) / selection pdf (PhotonBeam.hxx:364-365)
+    // 1D kernel (1 - dist^2/r^2) * 3 / (4 r sin(theta)) / selection pdf (PhotonBeam.hxx:487-494)
     const float k1 = (1.0f - (dist * dist) * __builtin_amdgcn_rcpf(radius * radius)) * 3.0f;
-    const float kd = k1 * __builtin_amdgcn_rcpf((4.0f * radius) * sin_theta);
+    const float kd = k1 * __builtin_amdgcn_rcpf((4.0f * radius) * sin_theta) * __builtin_amdgcn_rcpf(sel_pdf);
     const float u0 = ((((flux.x * s.ss0) * a0) * bsdf) * kd);
     const float u1 = ((((flux.y * s.ss1) * a1) * bsdf) * kd);
     const float u2 = ((((flux.z * s.ss2) * a2) * bsdf) * kd);
@@ -359,7 +364,7 @@ __device__ void beam_hit_mis(const EvalArgs& A, const Ray& r, SegState& s, const
         w_light = light_path_weight(lview, lv->path_idx, (int)path_len + 1, last, ntl, UPBP_BB1D, P.query_beam_type,
                                     P.photon_beam_type, tech, err);
     }
-    const float w = __builtin_amdgcn_rcpf(w_light + w_cam);
+    const float w = __builtin_amdgcn_rcpf(w_light + w_cam) * (float)mult;
     s.acc0 += w * u0;
     s.acc1 += w * u1;
     s.acc2 += w * u2;
@@ -438,6 +443,11 @@ __device__ __forceinline__ void drain_photons(const EvalArgs& A, const Ray& r, c
     SegState so;
     MisLane mo;
     owner_state(r, s, lo.ml, owner, ro, so, mo);
+    if (A.red.max) {  // the hit point's reduction cell needs the owner's origin (wave-uniform branch)
+        ro.ox = from_lane(r.ox, owner);
+        ro.oy = from_lane(r.oy, owner);
+        ro.oz = from_lane(r.oz, owner);
+    }
     if (act) {
         photon_hit_mis(A, ro, so, mo, __uint_as_float(g_hq[w][2][pos]), __uint_as_float(g_hq[w][3][pos]),
                        __uint_as_float(g_hq[w][4][pos]), g_hq[w][0][pos], short_q, lo.err);
@@ -446,8 +456,10 @@ __device__ __forceinline__ void drain_photons(const EvalArgs& A, const Ray& r, c
     lo.qh += n;
 }
 
-// The same for beam hits: [0] piece, [1] owner lane | SHORT beam << 8, [2] dist,
-// [3] sin(theta), [4] t1, [5] t2, [6] radius.
+// The same for beam hits: [0] piece, [1] owner lane | SHORT beam << 8 | times counted << 9,
+// [2] dist, [3] sin(theta), [4] t1, [5] t2, [6] radius (reduced kernels: the cell's selection
+// pdf; the radius r0 + t2 dr is then recomputed from the piece, the same operations as the
+// beam test's).
 template <int MODE>
 __device__ __forceinline__ void drain_beams(const EvalArgs& A, const Ray& r, const SegState& s, LaneOut<MODE>& lo,
                                             uint32_t n, bool short_q) {
@@ -463,10 +475,16 @@ __device__ __forceinline__ void drain_beams(const EvalArgs& A, const Ray& r, con
     if (act) {
         const uint32_t i = g_hq[w][0][pos];
         const float4 d2 = A.pieces[i].p1;
+        const float t2 = __uint_as_float(g_hq[w][5][pos]);
+        const float w6 = __uint_as_float(g_hq[w][6][pos]);
+        float radius = w6, sel_pdf = 1.0f;
+        if (MODE & kReduced) {
+            radius = d2.w + t2 * A.pieces[i].p3.y;
+            sel_pdf = w6;
+        }
         beam_hit_mis(A, ro, so, mo, __uint_as_float(g_hq[w][2][pos]), __uint_as_float(g_hq[w][3][pos]),
-                     __uint_as_float(g_hq[w][4][pos]), __uint_as_float(g_hq[w][5][pos]),
-                     __uint_as_float(g_hq[w][6][pos]), (meta >> 8) != 0u, A.piece_flux[i], d2.x, d2.y, d2.z,
-                     A.orig[i], short_q, lo.err);
+                     __uint_as_float(g_hq[w][4][pos]), t2, radius, ((meta >> 8) & 1u) != 0u, A.piece_flux[i], d2.x,
+                     d2.y, d2.z, A.orig[i], sel_pdf, meta >> 9, short_q, lo.err);
         return_to_owner(w, owner, so);
     }
     lo.qh += n;
@@ -589,19 +607,27 @@ __device__ __forceinline__ void beam_leaf(const EvalArgs& A, const Ray& r, SegSt
                                         pc.p0.z, pc.p1.x, pc.p1.y, pc.p1.z, pc.p3.x, pc.p0.w, pc.p2.w, pc.p2.x,
                                         pc.p2.y, pc.p1.w, pc.p3.y, dist, st, t1, t2, radius);
         if ((MODE & 7) == (int)Mode::Upbp) {
-            const uint64_t m = __ballot(hit);
+            // per-cell reduction: a hit is queued with the times its cell tests it and the
+            // cell's pdf, in place of the radius (recomputed from the piece when drained)
+            uint32_t red_n = 1;
+            float red_pdf = 1.0f;
+            if ((MODE & kReduced) && hit)
+                red_n = red_decide(A.red, r.qkey, r.ox + r.dx * t1, r.oy + r.dy * t1, r.oz + r.dz * t1, A.orig[i],
+                                   red_pdf);
+            const bool queue = hit && red_n != 0u;
+            const uint64_t m = __ballot(queue);
             if (m) {
-                if (hit) {
+                if (queue) {
                     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
                     const uint32_t pos = (lo.qt + lane_rank(m)) & (kHq - 1);
                     g_hq[w][0][pos] = i;
-                    g_hq[w][1][pos] = (uint32_t)lane | ((word & kShortBeamBit) ? 256u : 0u);
+                    g_hq[w][1][pos] = (uint32_t)lane | ((word & kShortBeamBit) ? 256u : 0u) | (red_n << 9);
                     g_hq[w][2][pos] = __float_as_uint(dist);
                     g_hq[w][3][pos] = __float_as_uint(st);
                     g_hq[w][4][pos] = __float_as_uint(t1);
                     g_hq[w][5][pos] = __float_as_uint(t2);
-                    g_hq[w][6][pos] = __float_as_uint(radius);
-                    lo.hits++;
+                    g_hq[w][6][pos] = __float_as_uint((MODE & kReduced) ? red_pdf : radius);
+                    lo.hits += red_n;
                 }
                 lo.qt = uni(lo.qt + (uint32_t)__popcll(m));
                 if (lo.qt - lo.qh >= (uint32_t)kWave) drain_beams<MODE>(A, r, s, lo, kWave, short_q);
@@ -612,9 +638,9 @@ __device__ __forceinline__ void beam_leaf(const EvalArgs& A, const Ray& r, SegSt
         uint32_t red_n = 1;
         if ((MODE & kReduced) && hit) {
             // the hit belongs to the reduction grid cell of its query point
-            red_n = red_decide(A.red, r.qkey, r.ox + r.dx * t1, r.oy + r.dy * t1, r.oz + r.dz * t1, A.orig[i],
-                               red_w);
-            red_w *= (float)red_n;
+            float pdf;
+            red_n = red_decide(A.red, r.qkey, r.ox + r.dx * t1, r.oy + r.dy * t1, r.oz + r.dz * t1, A.orig[i], pdf);
+            red_w = (1.0f / pdf) * (float)red_n;
         }
         if (__ballot(hit && red_n != 0)) {
             const float4 flux = A.piece_flux[i];
@@ -1044,7 +1070,7 @@ __device__ __forceinline__ int64_t xcd_block(int64_t bid, int64_t nblk) {
 // register spill cost less than the latency the extra wave hides (measured 3-4 %).
 // UPBP kernels ask for 4 (128 VGPRs), which both fit without spilling.
 template <int EST, int MODE>
-__global__ __launch_bounds__(256, ((MODE & kReduced) ? 6 : (MODE & 7) == (int)Mode::Forward ? 8 : (MODE & 7) == (int)Mode::Upbp ? 4 : 1)) void k_eval(
+__global__ __launch_bounds__(256, ((MODE & 7) == (int)Mode::Upbp ? 4 : (MODE & kReduced) ? 6 : (MODE & 7) == (int)Mode::Forward ? 8 : 1)) void k_eval(
     EvalArgs A) {
     const int wave_in_block = threadIdx.x / kWave;
     const int waves_per_block = blockDim.x / kWave;
@@ -1216,6 +1242,13 @@ void launch_mode(Ctx& ctx, Mode mode, const EvalArgs& A, unsigned blocks, unsign
             break;
         }
         case Mode::Upbp:
+            if constexpr (EST == kEstBeams) {
+                if (A.red.max != 0) {
+                    hipLaunchKernelGGL((k_eval<EST, (int)Mode::Upbp | kReduced>), dim3(blocks), dim3(threads), 0,
+                                       ctx.stream, A);
+                    break;
+                }
+            }
             hipLaunchKernelGGL((k_eval<EST, (int)Mode::Upbp>), dim3(blocks), dim3(threads), 0, ctx.stream, A);
             break;
     }
@@ -1307,6 +1340,7 @@ void launch_photons(Ctx& ctx, Mode mode, const PhotonArgs& ph, const QueryView& 
     A.power = ph.power;
     A.inc = ph.inc;
     A.orig = ph.orig;
+    A.red = ph.sel;
     A.out = out;
     A.sig = sig;
     if (adj) {

@@ -236,6 +236,9 @@ struct PhotonArgs {
     const float4* inc;
     const uint32_t* orig;
     int64_t n;
+    // UPBP mode: the reduction grid of the photon-beam evaluator of the same call (sel.max == 0:
+    // none), whose cell pdf at a PB2D hit is the camera vertex's BB1D beam selection pdf
+    RedDev sel;
 };
 struct BeamArgs {
     const NodeDev* nodes;

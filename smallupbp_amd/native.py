@@ -538,12 +538,15 @@ def eval_batch(ctx: Context, bre: Optional[EmbreeBre], beams: Optional[PhotonBea
 
 
 def eval_upbp(ctx: Context, bre: Optional[EmbreeBre], beams: Optional[PhotonBeamsEvaluator],
-              queries: QueryBatchView, mis, stats: Optional[dict] = None, out_photons=None, out_beams=None):
+              queries: QueryBatchView, mis, stats: Optional[dict] = None, out_photons=None, out_beams=None,
+              photons_only: bool = False):
     """upbp_eval_upbp: UPBP-weighted estimates. `mis` is a scenes.MisInputs (host arrays) or an
-    _abi.MisInputs already describing its memory (device_mis)."""
+    _abi.MisInputs already describing its memory (device_mis). photons_only: the beam structure is
+    not evaluated (its output is None) and serves only as the photon-beam evaluator whose cell
+    selection pdfs the PB2D hits read."""
     ms = mis if isinstance(mis, _abi.MisInputs) else _abi.mis_inputs(mis)
     op = _out_like(queries, out_photons) if bre is not None else None
-    ob = _out_like(queries, out_beams) if beams is not None else None
+    ob = _out_like(queries, out_beams) if beams is not None and not photons_only else None
     dev = (op is not None and _is_torch(op)) or (ob is not None and _is_torch(ob))
     st = _abi.EvalStats()
     ctx.check(ctx.lib.upbp_eval_upbp(ctx.h, bre.h if bre else None, beams.h if beams else None, C.byref(queries.qb),

@@ -41,3 +41,53 @@ def oracle_beams(w, precision=32, accel=0, signatures=True, queries=None, grid_r
     obm = po.OracleBeams(w.beams, grid_res=grid_res if accel == 1 else 0)
     return po.beams_eval(obm, w.media, queries or w.queries, accel=accel, precision=precision,
                          signatures=signatures)
+
+
+def hash32(i):
+    """The signature hash of an item index (lowbias32 of i + 1), as the kernels and the oracle."""
+    x = (np.asarray(i, np.uint64) + 1) & 0xFFFFFFFF
+    x ^= x >> 16
+    x = (x * 0x7FEB352D) & 0xFFFFFFFF
+    x ^= x >> 15
+    x = (x * 0x846CA68B) & 0xFFFFFFFF
+    x ^= x >> 16
+    return x.astype(np.uint32)
+
+
+def beam_mismatch_report(w, queries, sig_gpu, sig_ref, piece_factor=4.0, max_pieces=64, limit=6):
+    """Names the beams behind differing beam hit signatures: for each differing query, the beam
+    whose hash accounts for the difference (one beam missing or extra), with its closest-approach
+    parameters, the radius margin and the piece holding t2. Diagnostics only."""
+    bad = np.nonzero(np.any(sig_gpu != sig_ref, axis=1))[0]
+    lines = ["%d of %d queries differ" % (len(bad), len(sig_gpu))]
+    bm = w.beams
+    O2 = bm.origin.astype(np.float64)
+    D2 = bm.direction.astype(np.float64)
+    H = hash32(np.arange(bm.count))
+    for j in bad[:limit]:
+        gn, gh = int(sig_gpu[j, 0]), int(sig_gpu[j, 1])
+        rn, rh = int(sig_ref[j, 0]), int(sig_ref[j, 1])
+        dh = (gh - rh) % (1 << 32)
+        line = "query %d: gpu %d hits, oracle %d" % (j, gn, rn)
+        cand = np.nonzero(H == (dh if gn > rn else (-dh) % (1 << 32)))[0] if abs(gn - rn) == 1 else []
+        for b in cand[:2]:
+            o1 = queries.origin[j].astype(np.float64)
+            d1 = queries.direction[j].astype(np.float64)
+            c = np.cross(d1, D2[b])
+            s2 = float(c @ c)
+            ad = float((O2[b] - o1) @ c)
+            qd = float(D2[b] @ d1)
+            w0 = o1 - O2[b]
+            t1 = (qd * float(w0 @ D2[b]) - float(w0 @ d1)) / (1 - qd * qd)
+            t2 = (float(w0 @ D2[b]) - qd * float(w0 @ d1)) / (1 - qd * qd)
+            ln = float(bm.length[b])
+            segs = queries.segments[queries.seg_offsets[j]:queries.seg_offsets[j + 1]]
+            inseg = [(float(s["dist_min"]), float(s["dist_max"]), int(s["medium_id"])) for s in segs
+                     if s["dist_min"] < t1 < s["dist_max"]]
+            npc = int(min(max(np.ceil(ln / (piece_factor * bm.radius)), 1), max_pieces))
+            k = t2 / ln * npc
+            line += ("; beam %d (%s): t1 %.7g in %s, t2 %.7g of length %.7g (piece %.4f of %d), "
+                     "dist/r %.8f, medium %d" % (b, "missing" if rn > gn else "extra", t1, inseg, t2, ln, k, npc,
+                                                 abs(ad) / np.sqrt(s2) / bm.radius, int(bm.medium_id[b])))
+        lines.append(line)
+    return "\n".join(lines)

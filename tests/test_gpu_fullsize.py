@@ -21,7 +21,7 @@ import torch
 from oracle import pyoracle as po
 from smallupbp_amd import autotune, native, scenes
 
-from gpu_helpers import REL_TOL, gpu_context, print_case, rel_err
+from gpu_helpers import REL_TOL, beam_mismatch_report, gpu_context, print_case, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -48,7 +48,9 @@ def _tuned_context(w):
     return ctx, bre, bm, qv, p
 
 
-def _check_sample(name, out, sig, ref, rsig, cnt):
+def _check_sample(name, out, sig, ref, rsig, cnt, explain=None):
+    if not np.array_equal(sig, rsig) and explain is not None:
+        print(explain(sig, rsig), flush=True)
     assert np.array_equal(sig, rsig), "%s hit sets differ from the oracle on the sample" % name
     assert int(sig[:, 0].astype(np.int64).sum()) == int(cnt[0])
     e = rel_err(out, ref)
@@ -84,9 +86,10 @@ def test_bre_beams_1080p_full_size():
     print("photons vs FP64: median %.3g p99.9 %.3g" % (np.median(e64), np.percentile(e64, 99.9)), flush=True)
     assert np.median(e64) < 1e-5
     jdx = _sample(w.queries.count, 2_000, 102)
-    ref, rsig, cnt = po.beams_eval(po.OracleBeams(w.beams, grid_res=0), w.media, w.queries.subset(jdx), accel=0,
-                                   signatures=True)
-    _check_sample("beams", ob[jdx], sb[jdx], ref, rsig, cnt)
+    qs = w.queries.subset(jdx)
+    ref, rsig, cnt = po.beams_eval(po.OracleBeams(w.beams, grid_res=0), w.media, qs, accel=0, signatures=True)
+    _check_sample("beams", ob[jdx], sb[jdx], ref, rsig, cnt,
+                  lambda g, r: beam_mismatch_report(w, qs, g, r, piece_factor=p["piece_factor"] or 4.0))
 
 
 @pytest.mark.timeout(600)
@@ -105,9 +108,10 @@ def test_beams_stress_full_size():
     assert st["beam_hits"] == int(sb[:, 0].astype(np.int64).sum())
     assert np.isfinite(ob).all()
     idx = _sample(w.queries.count, 2_000, 103)
-    ref, rsig, cnt = po.beams_eval(po.OracleBeams(w.beams, grid_res=0), w.media, w.queries.subset(idx), accel=0,
-                                   signatures=True)
-    _check_sample("beams", ob[idx], sb[idx], ref, rsig, cnt)
+    qs = w.queries.subset(idx)
+    ref, rsig, cnt = po.beams_eval(po.OracleBeams(w.beams, grid_res=0), w.media, qs, accel=0, signatures=True)
+    _check_sample("beams", ob[idx], sb[idx], ref, rsig, cnt,
+                  lambda g, r: beam_mismatch_report(w, qs, g, r, piece_factor=p["piece_factor"] or 4.0))
 
 
 @pytest.mark.timeout(900)

@@ -1,10 +1,14 @@
 """GPU parity of light sub-path tracing (upbp_trace_light_paths) against the CPU oracle.
 
 Both draw the same counter-based random numbers, so the same path index gives the same
-path. The math library's last-ulp differences between the GPU and the host can flip a
-distance or roulette decision on a handful of paths (which then diverge); every path whose
+path. The math library's last-ulp differences between the GPU and the host (logarithm of the
+distance sample, sine, cosine and powers of the phase sample) can flip a branch decision of a
+path whose two compared values are nearly equal; such a path then diverges. So every path whose
 vertex and beam counts agree must match field by field (1e-4 relative, exact integers), and
-at most 1 % of the paths may differ in shape."""
+every path that differs in shape must have had a close call in the oracle's trace: a branch
+decision (distance sample against the segment end, roulette draw against the continuation
+probability, a sphere or box crossing) whose two sides were within NEAR_TIE relative of each
+other. Such close calls are rare (checked), so a defect that changes paths away from them fails."""
 import numpy as np
 import pytest
 
@@ -24,6 +28,12 @@ def _split(ends, n):
     return starts, ends.astype(np.int64)
 
 
+# A relative gap this small between a branch decision's two sides is a close call: the
+# accumulated last-ulp differences of a path (a few 1e-7 per transcendental, over up to ten
+# segments) stay far below it, while a typical decision's gap is of order one.
+NEAR_TIE = 1e-4
+
+
 def _compare(sc, g, o, path_count):
     gs, ge = _split(g["path_ends"], path_count)
     os_, oe = _split(o["path_ends"], path_count)
@@ -32,7 +42,13 @@ def _compare(sc, g, o, path_count):
     gnb = np.bincount(gbp, minlength=path_count)
     onb = np.bincount(obp, minlength=path_count)
     same = ((ge - gs) == (oe - os_)) & (gnb == onb)
-    assert same.mean() >= 0.99, "paths differing in shape: %d of %d" % ((~same).sum(), path_count)
+    close_call = o["margins"] < NEAR_TIE
+    print("paths differing in shape: %d; close calls in the oracle: %d of %d"
+          % ((~same).sum(), close_call.sum(), path_count))
+    assert close_call.mean() < 0.01, "close calls are not rare: the check below would pin nothing"
+    far = np.nonzero(~same & ~close_call)[0]
+    assert len(far) == 0, "paths %s differ in shape without a close call (margins %s)" % (
+        far[:10], o["margins"][far[:10]])
     vi = np.concatenate([np.arange(gs[p], ge[p]) for p in np.nonzero(same)[0]])
     wi = np.concatenate([np.arange(os_[p], oe[p]) for p in np.nonzero(same)[0]])
     gv, ov = g["vertices"][vi], o["vertices"][wi]
@@ -87,7 +103,7 @@ def test_light_paths_match_oracle(name, kw):
     n = 4000
     ctx = gpu_context(sc.media)
     g = native.light_paths_to_host(native.trace_light_paths(ctx, sc, n, device=True))
-    o = po.trace_light_paths(sc, n)
+    o = po.trace_light_paths(sc, n, margins=True)
     assert len(o["vertices"]) > 1.1 * n and len(o["beam_length"]) > n // 4
     same = _compare(sc, g, o, n)
     print("paths matching in shape: %.4f" % same.mean())

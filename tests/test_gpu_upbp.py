@@ -109,3 +109,46 @@ def test_upbp_long_camera_paths_bb1d_previous(pbt):
     ep, eb = rel_err(gp, rp), rel_err(gb, rb)
     print("max rel err photons %.2e beams %.2e" % (ep.max(), eb.max()))
     assert ep.max() <= REL_TOL and eb.max() <= REL_TOL
+
+
+@pytest.mark.parametrize("rtype", [scenes.REDUCE_PRESAMPLE, scenes.REDUCE_OFFSET, scenes.REDUCE_RESAMPLE_FIXED,
+                                   scenes.REDUCE_RESAMPLE])
+def test_upbp_with_reduced_beam_cells_matches_oracle(rtype):
+    """The photon-beam evaluator reduces its grid cells (max 2 beams tested per cell of an 8-cell
+    grid): beam hits are kept as the cell of their query point decides and weighted with the
+    cell's selection pdf in the contribution and the MIS weights; PB2D hits take the pdf of the
+    cell holding their hit point as the camera vertex's BB1D selection pdf. Same values as the
+    oracle (relative, floored at the median query for radius-grazing beam hits, as the plain
+    reduction test), identical kept-hit counts."""
+    print_case("upbp_reduced", rtype=rtype)
+    w, inm, mi = scenes.upbp_scene(80, 48, 3000, 800, seed=65, techniques=ALL)
+    ctx = gpu_context(w.media)
+    bre = native.EmbreeBre(ctx)
+    bre.build(w.photons, in_medium=inm)
+    bm = native.PhotonBeamsEvaluator(ctx)
+    bm.build(w.beams, max_beams_in_cell=2, reduction_type=rtype, grid_resolution=8, reduction_seed=41)
+    qv = native.QueryBatchView.to_device(w.queries)
+    st = {}
+    gp, gb = native.eval_upbp(ctx, bre, bm, qv, mi, stats=st)
+    gp, gb = gp.cpu().numpy(), gb.cpu().numpy()
+    oph = po.OraclePhotons(w.photons, in_medium=inm, build_bvh=False)
+    obm = po.OracleBeams(w.beams, grid_res=8).set_reduction(2, rtype, 41)
+    rp, hp = po.upbp_eval(oph, obm, w.media, w.queries, mi, 0)
+    rb, hb = po.upbp_eval(None, obm, w.media, w.queries, mi, 1)
+    plain_b, _ = po.upbp_eval(None, po.OracleBeams(w.beams, grid_res=0), w.media, w.queries, mi, 1)
+    sel = np.array([obm.selection_pdf(p) for p in w.photons.position[inm.astype(bool)][:500]])
+    print("photon hits %d / %d, beam tests %d / %d; reduced cells among 500 photon points: %d"
+          % (st["photon_hits"], hp, st["beam_hits"], hb, (sel < 1).sum()))
+    assert (sel < 1).sum() > 100
+    assert st["beam_hits"] == hb and st["photon_hits"] == hp
+    assert (rb.max(1) > 0).sum() > 50 and np.abs(rb - plain_b).max() > 0
+    ep = rel_err(gp, rp)
+    mag = np.abs(rb).max(axis=1)
+    floor = float(np.median(mag[mag > 0]))
+    eb = rel_err(gb, rb, floor=floor)
+    print("max rel err photons %.2e beams %.2e (strict beams %.2e)" % (ep.max(), eb.max(), rel_err(gb, rb).max()))
+    assert ep.max() <= REL_TOL and eb.max() <= REL_TOL
+    assert (rel_err(gb, rb) > REL_TOL).mean() <= 1e-3
+    # without out_beams the reduced structure only gives PB2D hits their selection pdf
+    gp2, gb2 = native.eval_upbp(ctx, bre, bm, qv, mi, photons_only=True)
+    assert gb2 is None and np.array_equal(gp2.cpu().numpy(), gp)

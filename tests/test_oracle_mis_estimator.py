@@ -34,11 +34,25 @@ def _hg(g, cos_t):
     return INV_PI_F * 0.25 * (1 - g * g) / (1 + g * g - 2 * g * cos_t) ** 1.5
 
 
+def _identical_beams(n, origin, direction, length, pbt, radius=0.05):
+    """n copies of one photon beam: every reduction grid cell the beam crosses holds all n."""
+    return scenes.Beams(np.tile(np.float32(origin), (n, 1)), np.tile(np.float32(direction), (n, 1)),
+                        np.full(n, length, np.float32), np.tile(np.float32([[0.3, 0.5, 0.7]]), (n, 1)),
+                        np.full(n, pbt, np.uint32), np.zeros(n, np.int32), radius)
+
+
+N_COPIES, MAX_IN_CELL = 8, 2  # reduced cells: selection pdf 2 / 8 = 0.25 (dyadic)
+
+
+@pytest.mark.parametrize("reduced", [False, True])
 @pytest.mark.parametrize("nseg", [1, 2])
 @pytest.mark.parametrize("origin_in_medium", [0, 1])
 @pytest.mark.parametrize("qbt", [scenes.LONG_BEAM, scenes.SHORT_BEAM])
 @pytest.mark.parametrize("pbt", [scenes.LONG_BEAM, scenes.SHORT_BEAM])
-def test_pb2d_hit_weight_is_the_balance_heuristic(nseg, origin_in_medium, qbt, pbt):
+def test_pb2d_hit_weight_is_the_balance_heuristic(nseg, origin_in_medium, qbt, pbt, reduced):
+    """reduced: the renderer's photon-beam evaluator reduces its grid cells, and the cell holding
+    the hit point keeps MAX_IN_CELL of N_COPIES beams, so the merged vertex's BB1D technique is
+    sampled with the selection pdf MAX_IN_CELL / N_COPIES (Bre.cxx:407-415)."""
     rng = np.random.default_rng(1000 + 8 * nseg + 4 * origin_in_medium + 2 * qbt + pbt)
     g = 0.3
     # medium 0 (first segment of a two-segment query), medium 1 (where the photon is)
@@ -114,7 +128,16 @@ def test_pb2d_hit_weight_is_the_balance_heuristic(nseg, origin_in_medium, qbt, p
     ph = scenes.Photons(pos, np.ascontiguousarray(lv["dir_fix"]), np.ascontiguousarray(lv["throughput"]), None, r)
     oph = po.OraclePhotons(ph, in_medium=inm, build_bvh=False)
     plain, _, cnt = po.bre_eval(oph, media, q, accel=0, precision=64)
-    upbp, hits = po.upbp_eval(oph, None, media, q, mi, 0, precision=64)
+    sel = 1.0
+    obm = None
+    if reduced:
+        # beams of the evaluator crossing the hit point x_i along x (medium 1, as the photon)
+        bm = _identical_beams(N_COPIES, [[0.25, 0.5, 0.625]], [[1.0, 0.0, 0.0]], 0.5, pbt)
+        bm.medium_id[:] = 1
+        obm = po.OracleBeams(bm, grid_res=64).set_reduction(MAX_IN_CELL, scenes.REDUCE_PRESAMPLE, 5)
+        sel = obm.selection_pdf(pos[i])
+        assert sel == MAX_IN_CELL / N_COPIES
+    upbp, hits = po.upbp_eval(oph, obm, media, q, mi, 0, precision=64)
     assert cnt[0] == 1 and hits == 1 and (plain > 0).all()
     w_est = upbp[0] / plain[0]
     assert np.allclose(w_est, w_est[0], rtol=1e-12)
@@ -159,7 +182,7 @@ def test_pb2d_hit_weight_is_the_balance_heuristic(nseg, origin_in_medium, qbt, p
         qL[j] = mj["ray_sample_pdf_inv"] if pbt == scenes.LONG_BEAM else mj["ray_sample_pdfs_ratio"]
     # the merged vertex x_i: the renderer's factors, the query's sampling, the light vertex's own
     fac["pp3d"][i], fac["pb2d"][i], fac["bb1d"][i] = P_["pp3d_factor"], P_["pb2d_factor"], P_["bb1d_factor"]
-    fac["sel"][i], fac["sin"][i] = 1.0, sin_t
+    fac["sel"][i], fac["sin"][i] = sel, sin_t
     qC[i] = 1.0 / dist_pdf if qbt == scenes.LONG_BEAM else 1.0 / s1
     qL[i] = m["ray_sample_pdf_inv"][i] if pbt == scenes.LONG_BEAM else m["ray_sample_pdfs_ratio"][i]
     # the query origin x_{i+1}: its own data; its light-direction distance pdf is the reverse one
@@ -175,11 +198,12 @@ def test_pb2d_hit_weight_is_the_balance_heuristic(nseg, origin_in_medium, qbt, p
     assert w_est[0] == pytest.approx(w_ref, rel=1e-12)
 
 
+@pytest.mark.parametrize("reduced", [False, True])
 @pytest.mark.parametrize("origin_in_medium", [0, 1])
 @pytest.mark.parametrize("beam_origin_in_medium", [0, 1])
 @pytest.mark.parametrize("qbt", [scenes.LONG_BEAM, scenes.SHORT_BEAM])
 @pytest.mark.parametrize("pbt", [scenes.LONG_BEAM, scenes.SHORT_BEAM])
-def test_bb1d_hit_weight_is_the_balance_heuristic(origin_in_medium, beam_origin_in_medium, qbt, pbt):
+def test_bb1d_hit_weight_is_the_balance_heuristic(origin_in_medium, beam_origin_in_medium, qbt, pbt, reduced):
     """PhotonBeam::accumulate2 (PhotonBeam.hxx:390-605): ONE query beam (from the camera vertex
     x_{i+1}) and ONE photon beam (from the light vertex x_b, i = b + 1) meet; the merged point x_i is
     sampled by neither side. The weight UPBP / plain must be the balance heuristic
@@ -191,7 +215,11 @@ def test_bb1d_hit_weight_is_the_balance_heuristic(origin_in_medium, beam_origin_
                                          x_b is in the medium, else e^{-sigma_min t2}),
       x_b's camera-direction distance pdf for its beam techniques: that reverse distance pdf,
       p_BB1D(x_i) = F_bb1d * sel * sin(theta) * P_i * qC_i * qL_i (qL_i = 1 / beam distance pdf for a
-      LONG photon beam, 1 / sigma_min for a SHORT one)."""
+      LONG photon beam, 1 / sigma_min for a SHORT one).
+    reduced: the beam comes N_COPIES times and the grid keeps MAX_IN_CELL of them per cell
+    (PRESAMPLE), each weighted 1 / sel with sel = MAX_IN_CELL / N_COPIES: the plain estimate is still
+    N_COPIES times one beam's, the UPBP one N_COPIES times one beam's weighted by the balance
+    heuristic at selection pdf sel (PhotonBeam.hxx:487-564), so their ratio is that weight."""
     rng = np.random.default_rng(2000 + 8 * origin_in_medium + 4 * beam_origin_in_medium + 2 * qbt + pbt)
     g = 0.3
     media = scenes.make_media([((0.1, 0.2, 0.3), (1.0, 1.5, 2.0), g, 0.8)])
@@ -250,13 +278,18 @@ def test_bb1d_hit_weight_is_the_balance_heuristic(origin_in_medium, beam_origin_
     P_ = dict(techniques=ALL, query_beam_type=qbt, photon_beam_type=pbt, min_path_length=0, max_path_length=20,
               surf_factor=0.7, pp3d_factor=0.75, pb2d_factor=1.25, bb1d_factor=1.5, bb1d_min_mfp=0.01,
               min_dist_to_med=0.0, has_bb1d_beams=1)
-    mi = scenes.MisInputs(qm, cam, lv, ends, bmis, P_)
-    beams = scenes.Beams(o2[None], d2[None], np.array([1.0], np.float32), np.array([[0.3, 0.5, 0.7]], np.float32),
-                         np.array([pbt], np.uint32), np.array([0], np.int32), 0.05)
-    obm = po.OracleBeams(beams, grid_res=0)
-    plain, _, cnt = po.beams_eval(obm, media, q, accel=0, precision=64)
+    ncopy = N_COPIES if reduced else 1
+    beams = _identical_beams(ncopy, o2[None], d2[None], 1.0, pbt)
+    mi = scenes.MisInputs(qm, cam, lv, ends, np.repeat(bmis, ncopy), P_)
+    sel = 1.0
+    if reduced:
+        obm = po.OracleBeams(beams, grid_res=64).set_reduction(MAX_IN_CELL, scenes.REDUCE_PRESAMPLE, 5)
+    else:
+        obm = po.OracleBeams(beams, grid_res=0)
+    plain, sig, cnt = po.beams_eval(obm, media, q, accel=0, precision=64, signatures=True)
     upbp, hits = po.upbp_eval(None, obm, media, q, mi, 1, precision=64)
-    assert cnt[0] == 1 and hits == 1 and (plain > 0).all()
+    kept = MAX_IN_CELL if reduced else 1
+    assert sig[0, 0] == kept and hits == kept and (plain > 0).all()
     w_est = upbp[0] / plain[0]
     assert np.allclose(w_est, w_est[0], rtol=1e-9)
 
@@ -298,8 +331,11 @@ def test_bb1d_hit_weight_is_the_balance_heuristic(origin_in_medium, beam_origin_
         qC[j] = float(mj["ray_sample_rev_pdf_inv"]) if LONGQ else float(mj["ray_sample_rev_pdfs_ratio"])
         qL[j] = float(mj["ray_sample_pdf_inv"]) if LONGB else float(mj["ray_sample_pdfs_ratio"])
     qC[b] = 1.0 / rsr_b if LONGQ else float(m["ray_sample_rev_pdfs_ratio"][b])  # x_b seen from the merge
+    if reduced:
+        sel = obm.selection_pdf(o1 + d1 * np.float32(t1))
+        assert sel == MAX_IN_CELL / N_COPIES
     fac["pp3d"][i], fac["pb2d"][i], fac["bb1d"][i] = P_["pp3d_factor"], P_["pb2d_factor"], P_["bb1d_factor"]
-    fac["sel"][i], fac["sin"][i] = 1.0, sin_t
+    fac["sel"][i], fac["sin"][i] = sel, sin_t
     qC[i] = 1.0 / rsp_q if LONGQ else 1.0 / s1
     qL[i] = 1.0 / rsp_b if LONGB else 1.0 / s1
     c1 = cam[1]
@@ -310,5 +346,5 @@ def test_bb1d_hit_weight_is_the_balance_heuristic(origin_in_medium, beam_origin_
     techniques = (fac["pp3d"] + fac["surf"]) * P + fac["pb2d"] * P * qC + fac["bb1d"] * fac["sel"] * fac["sin"] * P * qC * qL
     bpt = [np.prod(pL[:a + 1]) * np.prod(pC[a + 1:]) for a in range(-1, k)]
     S = techniques.sum() + sum(bpt)
-    w_ref = fac["bb1d"][i] * sin_t * P[i] * qC[i] * qL[i] / S
+    w_ref = fac["bb1d"][i] * sel * sin_t * P[i] * qC[i] * qL[i] / S
     assert w_est[0] == pytest.approx(w_ref, rel=1e-12)
