@@ -292,6 +292,26 @@ upbp_status upbp_eval(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams* beam
                       const upbp_query_batch* queries, float* out_photons, float* out_beams,
                       upbp_memory out_memory, upbp_eval_stats* stats);
 
+/* Size of the fixed per-device region upbp_eval_noalloc works in, and the largest
+ * UPBP_ORDER_SORTED batch it accepts (1920x1080 fits; UPBP_ORDER_AS_GIVEN has no such limit). */
+#define UPBP_NOALLOC_SCRATCH_BYTES (64u << 20)
+#define UPBP_NOALLOC_MAX_SORTED_QUERIES (1 << 21)
+
+/* No counterpart in the reference. upbp_eval for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * same kernels and results as upbp_eval with out_memory = UPBP_MEM_DEVICE, but the counters,
+ * query keys, permutations and sort storage live in a fixed region of
+ * UPBP_NOALLOC_SCRATCH_BYTES that the library holds on every device it runs on until the
+ * process ends, so the call makes no hipMalloc, hipFree or other allocating call. The queries
+ * and both outputs must be device memory (UPBP_EINVAL otherwise), and a UPBP_ORDER_SORTED
+ * batch may hold at most UPBP_NOALLOC_MAX_SORTED_QUERIES queries (UPBP_EINVAL otherwise).
+ * Every call on one GPU shares that GPU's region, whichever context makes it: calls that
+ * overlap in time on one GPU (from two host threads, or on two streams) race and must be
+ * serialised by the caller. */
+upbp_status upbp_eval_noalloc(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams* beams,
+                              const upbp_query_batch* queries, float* out_photons, float* out_beams,
+                              upbp_eval_stats* stats);
+
 /* Hit-set signatures per query: sig[q] = {number of hits, sum over hits of
  * hash32(original item index)} (wrapping uint32). Identical hit sets give
  * identical signatures; used to compare against a CPU evaluation. Either

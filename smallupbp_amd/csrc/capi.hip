@@ -6,6 +6,7 @@
 #include <cstddef>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -47,9 +48,61 @@ struct upbp_batch_plan {
     }
 };
 
+// Fixed device region of upbp_eval_noalloc: the kernels' counters, the query sort's Morton keys
+// and permutations for up to UPBP_NOALLOC_MAX_SORTED_QUERIES queries, and the radix sort's
+// temporary storage. Every GPU the library is loaded on holds its own copy until the process
+// ends. In host code `scratch` names a host-side stand-in: only the address that
+// hipGetSymbolAddress returns for the current device (kept per device below) is ever used.
+constexpr size_t kScratchBytes = (size_t)UPBP_NOALLOC_SCRATCH_BYTES;
+constexpr size_t kScratchStatsBytes = 256;
+constexpr int64_t kScratchMaxSorted = UPBP_NOALLOC_MAX_SORTED_QUERIES;
+static_assert(kStatSlots * 8 <= kScratchStatsBytes, "counters fit the scratch header");
+static_assert(kScratchStatsBytes + 4 * (size_t)kScratchMaxSorted * 4 < kScratchBytes, "scratch keeps room for the sort");
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
 
 thread_local std::string g_no_ctx_error;
+
+// Device address of `scratch` per device index, looked up once per device with that device current.
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;
+
+// Called by upbp_ctx_create with `device` current. Returns null (and the reason) on failure, so
+// only the allocation-free entry point reports it.
+void* scratch_address(int device, std::string& why) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((size_t)device >= g_scratch_ptr.size()) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            why = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e);
+            return nullptr;
+        }
+        g_scratch_ptr[device] = p;
+    }
+    return g_scratch_ptr[device];
+}
+
+// The workspace carved out of a device's scratch region: counters first, then the four
+// [kScratchMaxSorted] key / permutation arrays, then the sort storage in what remains.
+Workspace scratch_workspace(void* base) {
+    Workspace w;
+    char* p = static_cast<char*>(base);
+    w.stats = reinterpret_cast<unsigned long long*>(p);
+    p += kScratchStatsBytes;
+    uint32_t** arrays[4] = {&w.keys, &w.keys2, &w.perm, &w.perm2};
+    for (uint32_t** a : arrays) {
+        *a = reinterpret_cast<uint32_t*>(p);
+        p += (size_t)kScratchMaxSorted * 4;
+    }
+    w.max_sorted = kScratchMaxSorted;
+    w.sort_tmp = p;
+    w.sort_tmp_bytes = kScratchBytes - (size_t)(p - static_cast<char*>(base));
+    return w;
+}
 
 upbp_status fail(Ctx* ctx, upbp_status st, const std::string& msg) {
     if (ctx)
@@ -383,6 +436,7 @@ upbp_status upbp_ctx_create(int device, upbp_ctx** out_ctx) {
         UPBP_HIP(hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
         UPBP_HIP(hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming));
         ctx->stats.ensure(kStatSlots * 8);
+        ctx->scratch = scratch_address(device, ctx->scratch_error);
         *out_ctx = ctx;
         return UPBP_OK;
     } catch (const Error& e) {
@@ -614,6 +668,31 @@ upbp_status upbp_eval(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams* beam
         o.out_ph = bre ? out_photons : nullptr;
         o.out_bm = beams ? out_beams : nullptr;
         run_eval(*ctx, bre, beams, queries, Mode::Forward, o, out_memory, stats, nullptr, nullptr, false);
+    });
+}
+
+upbp_status upbp_eval_noalloc(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams* beams,
+                              const upbp_query_batch* queries, float* out_photons, float* out_beams,
+                              upbp_eval_stats* stats) {
+    if (!ctx) return fail(nullptr, UPBP_EINVAL, "ctx is NULL");
+    return guarded(ctx, [&] {
+        UPBP_REQUIRE(ctx->scratch != nullptr, UPBP_EDEVICE,
+                     "the allocation-free region is unavailable on this device: " + ctx->scratch_error);
+        UPBP_REQUIRE(queries != nullptr, UPBP_EINVAL, "query batch is NULL");
+        UPBP_REQUIRE(queries->memory == UPBP_MEM_DEVICE, UPBP_EINVAL,
+                     "upbp_eval_noalloc takes device-resident queries only (host queries need staging copies)");
+        UPBP_REQUIRE(!(queries->order == UPBP_ORDER_SORTED && queries->count > kScratchMaxSorted), UPBP_EINVAL,
+                     "upbp_eval_noalloc sorts at most UPBP_NOALLOC_MAX_SORTED_QUERIES (" +
+                         std::to_string(kScratchMaxSorted) + ") queries; got " + std::to_string(queries->count) +
+                         " (pass UPBP_ORDER_AS_GIVEN for larger batches)");
+        UPBP_REQUIRE(!bre || out_photons, UPBP_EINVAL, "out_photons is NULL but a photon structure was given");
+        UPBP_REQUIRE(!beams || out_beams, UPBP_EINVAL, "out_beams is NULL but a beam structure was given");
+        RunOut o;
+        o.out_ph = bre ? out_photons : nullptr;
+        o.out_bm = beams ? out_beams : nullptr;
+        const Workspace ws = scratch_workspace(ctx->scratch);
+        run_eval(*ctx, bre, beams, queries, Mode::Forward, o, UPBP_MEM_DEVICE, stats, nullptr, nullptr, false, nullptr,
+                 &ws);
     });
 }
 

@@ -114,6 +114,9 @@ struct Ctx {
     int batched_per_cu[2] = {0, 0};
     int num_cus = 256;
     Tuning tuning;
+    // this device's copy of the allocation-free entry point's fixed region (null: lookup failed)
+    void* scratch = nullptr;
+    std::string scratch_error;
 };
 
 // A built LBVH: node array plus the item order it implies.

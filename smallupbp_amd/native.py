@@ -28,7 +28,7 @@ EXPORTED = [
     "upbp_ctx_create", "upbp_ctx_destroy", "upbp_ctx_last_error", "upbp_ctx_set_stream",
     "upbp_ctx_get_stream", "upbp_ctx_set_sync", "upbp_set_media", "upbp_bre_build", "upbp_bre_destroy",
     "upbp_bre_set_power", "upbp_bre_count", "upbp_beams_build", "upbp_beams_destroy", "upbp_beams_count",
-    "upbp_beams_piece_count", "upbp_beams_selection_pdf", "upbp_beams_selection_pdfs", "upbp_eval", "upbp_hit_signatures",
+    "upbp_beams_piece_count", "upbp_beams_selection_pdf", "upbp_beams_selection_pdfs", "upbp_eval", "upbp_eval_noalloc", "upbp_hit_signatures",
     "upbp_adjoint", "upbp_batch_plan_create", "upbp_batch_plan_destroy", "upbp_batch_run",
     "upbp_set_tuning", "upbp_get_tuning", "upbp_autotune", "upbp_eval_upbp", "upbp_bre_radii",
     "upbp_beams_radii", "upbp_hash_grid_build", "upbp_hash_grid_destroy", "upbp_hash_grid_count",
@@ -102,6 +102,8 @@ def load():
     L.upbp_beams_selection_pdfs.argtypes = [vp, vp, C.c_int64, vp, C.c_int]
     L.upbp_eval.restype = st
     L.upbp_eval.argtypes = [vp, vp, vp, C.POINTER(_abi.QueryBatch), vp, vp, C.c_int, C.POINTER(_abi.EvalStats)]
+    L.upbp_eval_noalloc.restype = st
+    L.upbp_eval_noalloc.argtypes = [vp, vp, vp, C.POINTER(_abi.QueryBatch), vp, vp, C.POINTER(_abi.EvalStats)]
     L.upbp_eval_upbp.restype = st
     L.upbp_eval_upbp.argtypes = [vp, vp, vp, C.POINTER(_abi.QueryBatch), C.POINTER(_abi.MisInputs), vp, vp, C.c_int,
                                  C.POINTER(_abi.EvalStats)]
@@ -535,6 +537,19 @@ def eval_batch(ctx: Context, bre: Optional[EmbreeBre], beams: Optional[PhotonBea
     if stats is not None:
         stats.update(st.as_dict())
     return op, ob
+
+
+def eval_batch_noalloc(ctx: Context, bre: Optional[EmbreeBre], beams: Optional[PhotonBeamsEvaluator],
+                       queries: QueryBatchView, out_photons, out_beams, stats: Optional[dict] = None):
+    """upbp_eval_noalloc: upbp_eval into caller-provided device outputs ([Q,3] float32 tensors,
+    None where the structure is None) with device-resident queries; the library allocates nothing."""
+    st = _abi.EvalStats()
+    ctx.check(ctx.lib.upbp_eval_noalloc(ctx.h, bre.h if bre else None, beams.h if beams else None,
+                                        C.byref(queries.qb), _abi.ptr(out_photons), _abi.ptr(out_beams),
+                                        C.byref(st) if stats is not None else None))
+    if stats is not None:
+        stats.update(st.as_dict())
+    return out_photons, out_beams
 
 
 def eval_upbp(ctx: Context, bre: Optional[EmbreeBre], beams: Optional[PhotonBeamsEvaluator],
