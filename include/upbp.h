@@ -46,7 +46,7 @@ typedef enum upbp_status {
     UPBP_ENOMEM = 2,        /* device or host allocation failed */
     UPBP_EDEVICE = 3,       /* HIP runtime error, no device, or a kernel reported an internal overflow */
     UPBP_ENONFINITE = 4,    /* NaN/Inf in an input position, direction, radius, power or segment */
-    UPBP_EUNSUPPORTED = 5,  /* valid request this build does not serve (e.g. SHORT query in the adjoint) */
+    UPBP_EUNSUPPORTED = 5,  /* valid request this build does not serve (e.g. a reduced beam grid in a batched plan) */
     UPBP_ENOTBUILT = 6      /* handle is empty (build found no photons/beams) where data is required */
 } upbp_status;
 
@@ -333,7 +333,14 @@ upbp_status upbp_hit_signatures(upbp_ctx* ctx, const upbp_bre* bre, const upbp_b
  * recomputed, never stored. The media use |sigma| (Media.hxx:136-140), so
  * the medium gradients are taken with respect to the values given to
  * upbp_set_media: an entry of a negative coefficient carries its sign.
- * LONG query beams only (UPBP_EUNSUPPORTED for SHORT). */
+ * SHORT query beams: the in-segment transmittance is normalised by its
+ * minimum-sigma_t channel k (Bre.cxx:257-263, PhotonBeam.hxx:336-345), and an
+ * earlier segment's factor attenuation / ray_sample_pdf is taken with the
+ * pass-through pdf exp(-sigma_t_k * (dist_max - dist_min)) of a medium that
+ * samples distances (RaySamplePdf, Media.hxx:294-323); k is held fixed (the
+ * derivative at a tie of the minimum channels is one-sided). A beam structure
+ * with per-cell reduction weights each beam hit as upbp_eval does (its count
+ * over the cell's pdf, neither of which depends on the media). */
 upbp_status upbp_adjoint(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams* beams,
                          const upbp_query_batch* queries, const float* grad_out_photons,
                          const float* grad_out_beams, float* grad_scattering,

@@ -66,7 +66,7 @@ def lib():
                                     C.c_int, vp, vp, vp, vp]
         L.or_adjoint.restype = C.c_int
         L.or_adjoint.argtypes = [vp, vp, vp, i32, C.POINTER(_abi.QueryBatch), vp, vp, C.c_int, vp, vp,
-                                 vp, vp]
+                                 vp, vp, C.c_int]
         L.or_upbp_eval.restype = C.c_int
         L.or_upbp_eval.argtypes = [vp, vp, vp, i32, C.POINTER(_abi.QueryBatch), C.POINTER(_abi.MisInputs), C.c_int,
                                    C.c_int, C.c_int, C.c_int, vp, vp, vp]
@@ -214,9 +214,11 @@ def beams_eval(obm: OracleBeams, media, q: Queries, accel=0, precision=32, threa
     return out, sig, cnt
 
 
-def adjoint(oph, obm, media, q: Queries, w_photons, w_beams, threads=None):
+def adjoint(oph, obm, media, q: Queries, w_photons, w_beams, threads=None, fp32_hits=False):
     """FP64 analytic gradient. Returns dict of grad_sigma_s, grad_sigma_a [M,3],
-    grad_photon_power [N_in,3], grad_beam_power [B,3]."""
+    grad_photon_power [N_in,3], grad_beam_power [B,3]. fp32_hits: the hit sets (and hit
+    parameters) of the reference's FP32 predicates, as the FP32 forward and the GPU have them;
+    else those of the FP64 forward."""
     threads = threads or _threads()
     qb = query_batch(q)
     m, mp, nm = _media_ptr(media)
@@ -228,7 +230,7 @@ def adjoint(oph, obm, media, q: Queries, w_photons, w_beams, threads=None):
     wb = None if w_beams is None else np.ascontiguousarray(w_beams, np.float64)
     r = lib().or_adjoint(oph.h if oph else None, obm.h if obm else None, mp, nm, C.byref(qb),
                          _abi.ptr(wp), _abi.ptr(wb), threads, gs.ctypes.data, ga.ctypes.data,
-                         gp.ctypes.data if oph else None, gb.ctypes.data if obm else None)
+                         gp.ctypes.data if oph else None, gb.ctypes.data if obm else None, 1 if fp32_hits else 0)
     if r:
         raise RuntimeError("or_adjoint failed: %s" % _abi.STATUS_NAMES.get(r, r))
     return {"grad_sigma_s": gs, "grad_sigma_a": ga, "grad_photon_power": gp, "grad_beam_power": gb}

@@ -1030,13 +1030,52 @@ typedef struct adj_acc {
     double* gt; /* [n_media][3] d/d sigma_t */
 } adj_acc;
 
+/* Photon and beam hit decisions of the adjoint: FP64 (the FP64 forward's, for finite
+ * differences of it) or FP32 (the reference's, hit sets identical to the FP32 forward and the GPU;
+ * the hit's parameters are then the FP32 ones, widened to double). */
+static int adj_photon_hit(int fp32, const double* o, const double* d, double tmin, double tmax, const double* p,
+                          float r2, double* t, double* s) {
+    if (!fp32) return test_bre_f64(o, d, tmin, tmax, p, (double)r2, t, s);
+    const float of[3] = {(float)o[0], (float)o[1], (float)o[2]}, df[3] = {(float)d[0], (float)d[1], (float)d[2]};
+    const float pf[3] = {(float)p[0], (float)p[1], (float)p[2]};
+    float tf, sf;
+    if (!test_bre_f32(of, df, (float)tmin, (float)tmax, pf, r2, &tf, &sf)) return 0;
+    *t = tf;
+    *s = sf;
+    return 1;
+}
+
+static int adj_beam_hit(int fp32, const double* o, const double* d, double tmin, double tmax, const or_beams* bm,
+                        int64_t i, double* dist, double* st, double* t1, double* t2, double* radius) {
+    if (!fp32) {
+        const double o2[3] = {bm->o[3 * i], bm->o[3 * i + 1], bm->o[3 * i + 2]};
+        const double d2[3] = {bm->d[3 * i], bm->d[3 * i + 1], bm->d[3 * i + 2]};
+        if (!test_beam_beam_f64(o, d, tmin, tmax, o2, d2, 0.0, bm->len[i], bm->r2max[i], dist, st, t1, t2)) return 0;
+        *radius = (double)bm->r0[i] + *t2 * (double)bm->dr[i];
+        return !(*radius < *dist);
+    }
+    const float of[3] = {(float)o[0], (float)o[1], (float)o[2]}, df[3] = {(float)d[0], (float)d[1], (float)d[2]};
+    float fd, fs, f1, f2;
+    if (!test_beam_beam_f32(of, df, (float)tmin, (float)tmax, &bm->o[3 * i], &bm->d[3 * i], 0.f, bm->len[i],
+                            bm->r2max[i], &fd, &fs, &f1, &f2))
+        return 0;
+    const float fr = bm->r0[i] + f2 * bm->dr[i];
+    if (fr < fd) return 0;
+    *dist = fd;
+    *st = fs;
+    *t1 = f1;
+    *t2 = f2;
+    *radius = fr;
+    return 1;
+}
+
 int or_adjoint(const or_photons* ph, const or_beams* bm, const upbp_medium* media, int32_t n_media,
                const upbp_query_batch* q, const double* w_ph, const double* w_bm, int threads,
                double* grad_sigma_s, double* grad_sigma_a, double* grad_photon_power,
-               double* grad_beam_power) {
+               double* grad_beam_power, int fp32_hits) {
     int e = check_batch(q, n_media);
     if (e) return e;
-    if (q->beam_type != UPBP_LONG_BEAM) return UPBP_EUNSUPPORTED;
+    const int short_q = q->beam_type == UPBP_SHORT_BEAM;
     or_medium* md = (or_medium*)malloc(sizeof(or_medium) * (size_t)n_media);
     or_prepare_media(media, n_media, md);
     if (threads <= 0) threads = 1;
@@ -1058,6 +1097,7 @@ int or_adjoint(const or_photons* ph, const or_beams* bm, const upbp_medium* medi
                 d[a] = q->direction[3 * qi + a];
             }
             double P[3] = {1, 1, 1};
+            const uint64_t qkey = red_query_key(&q->origin[3 * qi], &q->direction[3 * qi]);
             int64_t s0 = q->segment_offsets[qi], s1 = q->segment_offsets[qi + 1];
             for (int64_t si = s0; si < s1; ++si) {
                 const upbp_volume_segment* sg = &q->segments[si];
@@ -1066,6 +1106,7 @@ int or_adjoint(const or_photons* ph, const or_beams* bm, const upbp_medium* medi
                     break;
                 }
                 const or_medium* m = &md[sg->medium_id];
+                const int k = m->min_idx;
                 double tmin = sg->dist_min, tmax = sg->dist_max;
                 double g[3] = {0, 0, 0}; /* sum over hits of w_c * C_c in this segment */
                 if (m->has_scattering) {
@@ -1073,13 +1114,18 @@ int or_adjoint(const or_photons* ph, const or_beams* bm, const upbp_medium* medi
                     for (int64_t i = 0; w && i < nph; ++i) {
                         double p[3] = {ph->pos[3 * i], ph->pos[3 * i + 1], ph->pos[3 * i + 2]};
                         double t, s;
-                        if (!test_bre_f64(o, d, tmin, tmax, p, (double)ph->r2[i], &t, &s)) continue;
+                        if (!adj_photon_hit(fp32_hits, o, d, tmin, tmax, p, ph->r2[i], &t, &s)) continue;
                         double r2 = ph->r2[i];
                         double inc[3] = {-ph->inc[3 * i], -ph->inc[3 * i + 1], -ph->inc[3 * i + 2]};
                         double rho = phase_f64(d, inc, m->mean_cosine);
                         double K = (1.0 - s / r2) / (r2 * (double)PI_F * 0.5);
+                        /* SHORT query: T_c / T_k along t - tmin (Bre.cxx:257-263); the exponent
+                         * -(sigma_t_c - sigma_t_k)(t - tmin) gives channel k the +(t - tmin) terms */
+                        const double Ek = exp(-(double)m->sigma_t[k] * (t - tmin));
+                        double sumC = 0;
                         for (int c = 0; c < 3; ++c) {
                             double E = exp(-(double)m->sigma_t[c] * (t - tmin));
+                            if (short_q) E /= Ek;
                             double base = P[c] * E * rho * K; /* C_c = base * flux_c * ss_c */
                             double Cc = base * ph->power[3 * i + c] * m->sigma_s[c];
 #pragma omp atomic
@@ -1087,28 +1133,35 @@ int or_adjoint(const or_photons* ph, const or_beams* bm, const upbp_medium* medi
                             ls[3 * sg->medium_id + c] += w[c] * base * ph->power[3 * i + c];
                             lt[3 * sg->medium_id + c] += w[c] * Cc * (-(t - tmin));
                             g[c] += w[c] * Cc;
+                            sumC += w[c] * Cc;
                         }
+                        if (short_q) lt[3 * sg->medium_id + k] += sumC * (t - tmin);
                     }
                     const double* wb = w_bm ? &w_bm[3 * qi] : NULL;
                     for (int64_t i = 0; wb && i < nbm; ++i) {
                         if (bm->medium[i] != sg->medium_id) continue;
-                        double o2[3] = {bm->o[3 * i], bm->o[3 * i + 1], bm->o[3 * i + 2]};
                         double d2[3] = {bm->d[3 * i], bm->d[3 * i + 1], bm->d[3 * i + 2]};
-                        double dist, st, t1, t2;
-                        if (!test_beam_beam_f64(o, d, tmin, tmax, o2, d2, 0.0, bm->len[i],
-                                                bm->r2max[i], &dist, &st, &t1, &t2))
-                            continue;
-                        double radius = (double)bm->r0[i] + t2 * (double)bm->dr[i];
-                        if (radius < dist) continue;
+                        double dist, st, t1, t2, radius;
+                        if (!adj_beam_hit(fp32_hits, o, d, tmin, tmax, bm, i, &dist, &st, &t1, &t2, &radius)) continue;
+                        /* per-cell reduction: the hit counts `mult` times, each weighted 1 / pdf */
+                        double red_w = 1;
+                        if (bm->red_max) {
+                            float pq[3], pdf;
+                            for (int a = 0; a < 3; ++a) pq[a] = (float)o[a] + (float)d[a] * (float)t1;
+                            const uint32_t mult = red_decide_at(bm, qkey, pq, (uint32_t)i, &pdf);
+                            if (!mult) continue;
+                            red_w = (double)mult / (double)pdf;
+                        }
                         double rho = phase_f64(d, d2, m->mean_cosine);
-                        double K1 = (1.0 - dist * dist / (radius * radius)) * 3.0 / (4.0 * radius * st);
-                        int k = m->min_idx;
+                        double K1 = (1.0 - dist * dist / (radius * radius)) * 3.0 / (4.0 * radius * st) * red_w;
                         int sb = (bm->flags[i] & UPBP_SHORT_BEAM) != 0;
+                        double Eqk = exp(-(double)m->sigma_t[k] * (t1 - tmin));
                         double Ebk = exp(-(double)m->sigma_t[k] * t2);
-                        double Cs[3];
+                        double Cs[3], sumC = 0;
                         for (int c = 0; c < 3; ++c) {
                             double Eq = exp(-(double)m->sigma_t[c] * (t1 - tmin));
                             double Eb = exp(-(double)m->sigma_t[c] * t2);
+                            if (short_q) Eq /= Eqk;
                             if (sb) Eb /= Ebk;
                             double base = P[c] * Eq * Eb * rho * K1;
                             double Cc = base * bm->flux[3 * i + c] * m->sigma_s[c];
@@ -1116,22 +1169,28 @@ int or_adjoint(const or_photons* ph, const or_beams* bm, const upbp_medium* medi
 #pragma omp atomic
                             gbm[3 * i + c] += wb[c] * base * m->sigma_s[c];
                             ls[3 * sg->medium_id + c] += wb[c] * base * bm->flux[3 * i + c];
-                            double diag = -(t1 - tmin) - ((sb && c == k) ? 0.0 : t2);
-                            lt[3 * sg->medium_id + c] += Cs[c] * diag;
+                            lt[3 * sg->medium_id + c] += Cs[c] * (-(t1 - tmin) - t2);
                             g[c] += Cs[c];
+                            sumC += Cs[c];
                         }
-                        if (sb)
-                            for (int c = 0; c < 3; ++c)
-                                if (c != k) lt[3 * sg->medium_id + k] += Cs[c] * t2;
+                        /* the SHORT normalisations by the channel-k transmittance (PhotonBeam.hxx:336-345) */
+                        lt[3 * sg->medium_id + k] += sumC * ((short_q ? (t1 - tmin) : 0.0) + (sb ? t2 : 0.0));
                     }
                 }
-                /* prefix transmittance of the earlier segments */
+                /* prefix transmittance of the earlier segments: exp(-sigma_t len) each, divided for a
+                 * SHORT query by the pass-through pdf exp(-sigma_t_k len) of a medium that samples
+                 * distances (RaySamplePdf, Media.hxx:294-323) */
                 for (int64_t sj = s0; sj < si; ++sj) {
                     const upbp_volume_segment* sp = &q->segments[sj];
+                    const or_medium* mj = &md[sp->medium_id];
                     double len = (double)sp->dist_max - (double)sp->dist_min;
                     for (int c = 0; c < 3; ++c) lt[3 * sp->medium_id + c] += g[c] * (-len);
+                    if (short_q && mj->min_comp > 0 && mj->has_scattering)
+                        lt[3 * sp->medium_id + mj->min_idx] += (g[0] + g[1] + g[2]) * len;
                 }
-                for (int c = 0; c < 3; ++c) P[c] *= (double)sg->attenuation[c];
+                for (int c = 0; c < 3; ++c)
+                    P[c] *= short_q ? (double)sg->attenuation[c] / (double)sg->ray_sample_pdf
+                                    : (double)sg->attenuation[c];
                 if (!(P[0] > 1e-20 && P[1] > 1e-20 && P[2] > 1e-20)) break;
             }
         }

@@ -136,12 +136,18 @@ int or_upbp_eval(const or_photons* ph, const or_beams* bm, const upbp_medium* me
 
 /* ---- adjoint (FP64 analytic gradient of the plain estimators) ----------- */
 /* Gradient of L = sum_q <w_photons[q], out_photons[q]> + <w_beams[q], out_beams[q]>
- * (w are the upstream gradients). Brute-force traversal, double arithmetic,
- * LONG queries only. Outputs are overwritten (not accumulated). */
+ * (w are the upstream gradients). Brute-force traversal, double arithmetic. Segment
+ * attenuations are exp(-sigma_t len); for SHORT queries the earlier segments' pdfs are the
+ * pass-through RaySamplePdf exp(-sigma_t_k len) of media that sample distances, and the
+ * minimum-sigma_t channel k is held fixed. Reduced beam grids weight each hit by its times
+ * counted over the cell's pdf (held fixed). fp32_hits: hit decisions and hit parameters in
+ * the reference's FP32 (the hit sets of the FP32 forward and the GPU), the gradient arithmetic
+ * in double; else everything in double (the FP64 forward's hit sets). Outputs are overwritten
+ * (not accumulated). */
 int or_adjoint(const or_photons* ph, const or_beams* bm, const upbp_medium* media, int32_t n_media,
                const upbp_query_batch* q, const double* w_photons, const double* w_beams,
                int threads, double* grad_sigma_s, double* grad_sigma_a,
-               double* grad_photon_power, double* grad_beam_power);
+               double* grad_photon_power, double* grad_beam_power, int fp32_hits);
 
 /* ---- k-nearest-neighbour radii (KdTmpl.hxx:600-792 result semantics) ----- */
 /* out[i] = k-th smallest squared distance from q[i] to pts (brute force, bounded

@@ -4,7 +4,9 @@ Differentiable parameters: per-medium scattering and absorption (RGB) and
 per-photon power. Photon/beam geometry and query rays are fixed. Segment
 attenuations are recomputed from the current coefficients on every forward,
 exp(-(sigma_a + sigma_s) * (dist_max - dist_min)), which is what the reference's
-Scene::Intersect stores for a segment (SmallUPBP/src/Scene/Scene.hxx:542).
+Scene::Intersect stores for a segment (SmallUPBP/src/Scene/Scene.hxx:542), and so are
+the pass-through ray-sampling pdfs exp(-sigma_t_min * (dist_max - dist_min)) that SHORT
+query beams divide by (RaySamplePdf, SmallUPBP/src/Scene/Media.hxx:294-323).
 """
 from __future__ import annotations
 
@@ -15,10 +17,9 @@ from . import native, scenes
 
 
 class DiffScene:
-    """Built structures plus device-resident queries of one differentiable workload (LONG queries)."""
+    """Built structures plus device-resident queries of one differentiable workload."""
 
     def __init__(self, w: scenes.Workload, device: int = 0):
-        assert w.queries.beam_type == scenes.LONG_BEAM, "the adjoint supports LONG query beams"
         self.w = w
         self.ctx = native.Context(device)
         self.ctx.set_media(w.media)
@@ -42,9 +43,18 @@ class DiffScene:
         m["scattering"] = sigma_s.detach().float().cpu().numpy()
         m["absorption"] = sigma_a.detach().float().cpu().numpy()
         self.ctx.set_media(m)
-        st = (sigma_s.detach().abs() + sigma_a.detach().abs()).float().to(self.seg.device)
+        ss = sigma_s.detach().abs().float().to(self.seg.device)
+        st = ss + sigma_a.detach().abs().float().to(self.seg.device)
         att = torch.exp(-st[self.seg_mid] * self.seg_len[:, None])
         self.seg[:, 4:7] = att
+        # pass-through pdf exp(-sigma_t_min len) of media that sample distances (min over positive channels)
+        mn = torch.where(st > 0, st, torch.full_like(st, float("inf"))).min(dim=1).values
+        samplable = torch.isfinite(mn) & (ss > 0).any(dim=1)
+        mn = torch.where(samplable, mn, torch.zeros_like(mn))
+        pdf = torch.clamp(torch.exp(-mn[self.seg_mid] * self.seg_len), min=1e-35)
+        pdf = torch.where(samplable[self.seg_mid], pdf, torch.ones_like(pdf))
+        self.seg[:, 2] = pdf
+        self.seg[:, 3] = pdf
         if photon_power is not None and self.bre is not None:
             self.bre.set_power(photon_power.detach().float().contiguous())
         torch.cuda.synchronize()

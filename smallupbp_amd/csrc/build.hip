@@ -164,9 +164,14 @@ __global__ void k_beam_pieces(int64_t n, BeamIn b, const uint32_t* count, const 
     const uint32_t c = count[i];
     const uint32_t base = offset[i];
     const float d2o2 = dot3(dx, dy, dz, ox, oy, oz);  // dot(d2, O2) of PhotonBeam.hxx:278
+    // The box of a piece holds every point within the beam's largest radius (1 % wider, for
+    // the rounding of the closest-approach point) of its part of the beam; ownership of a
+    // (query, beam) pair goes by that point's foot on the beam line (test_beam_beam), with
+    // the first and last pieces owning whatever lies before and after them.
+    const float rb = rmax * 1.01f;
     for (uint32_t k = 0; k < c; ++k) {
         const float t0 = cut_at(len, k, c), t1 = cut_at(len, k + 1, c);
-        const float rs = r0 + t0 * dr, re = r0 + t1 * dr;
+        const float rs = rb, re = rb;
         const float sx = ox + dx * t0, sy = oy + dy * t0, sz = oz + dz * t0;
         const float ex = ox + dx * t1, ey = oy + dy * t1, ez = oz + dz * t1;
         const uint32_t p = base + k;
@@ -177,7 +182,7 @@ __global__ void k_beam_pieces(int64_t n, BeamIn b, const uint32_t* count, const 
         PieceDev pc;
         pc.p0 = make_float4(ox, oy, oz, len);
         pc.p1 = make_float4(dx, dy, dz, r0);
-        pc.p2 = make_float4(t0, t1, __uint_as_float(word), rmax * rmax);
+        pc.p2 = make_float4(k == 0 ? -INFINITY : t0, k + 1 >= c ? INFINITY : t1, __uint_as_float(word), rmax * rmax);
         pc.p3 = make_float4(d2o2, dr, 0.f, 0.f);
         pieces[p] = pc;
         flux[p] = make_float4(b.throughput[3 * i], b.throughput[3 * i + 1], b.throughput[3 * i + 2], 0.f);

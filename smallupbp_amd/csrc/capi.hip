@@ -1046,10 +1046,6 @@ upbp_status upbp_adjoint(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams* b
     if (!ctx) return fail(nullptr, UPBP_EINVAL, "ctx is NULL");
     return guarded(ctx, [&] {
         check_batch(*ctx, queries);
-        UPBP_REQUIRE(queries->beam_type == UPBP_LONG_BEAM, UPBP_EUNSUPPORTED,
-                     "the adjoint supports LONG query beams only");
-        UPBP_REQUIRE(!beams || beams->red.max == 0, UPBP_EUNSUPPORTED,
-                     "per-cell beam reduction is supported by upbp_eval and upbp_hit_signatures only");
         const int64_t Q = queries->count;
         const int M = ctx->n_media;
         const bool do_ph = bre && grad_out_photons && bre->n > 0;

@@ -246,7 +246,7 @@ __device__ void photon_hit_mis(const EvalArgs& A, const Ray& r, SegState& s, con
     if (bsdf <= 0.f) return;
     const float dir_pdf = bsdf * s.cont;
     const float rev_pdf = bsdf * lv->continuation_prob;
-    const float kernel = (1.0f - sq * __builtin_amdgcn_rcpf(r2)) * __builtin_amdgcn_rcpf((r2 * kPi) * 0.5f);
+    const float kernel = radial_kernel(sq, r2) * __builtin_amdgcn_rcpf((r2 * kPi) * 0.5f);
     if (!(kernel > 1e-20f)) return;
     const float u0 = (((lv->throughput[0] * a0) * s.ss0) * bsdf) * kernel;
     const float u1 = (((lv->throughput[1] * a1) * s.ss1) * bsdf) * kernel;
@@ -325,7 +325,7 @@ __device__ void beam_hit_mis(const EvalArgs& A, const Ray& r, SegState& s, const
     const float dir_pdf = bsdf * s.cont;
     const float rev_pdf = bsdf * s.cont;
     // 1D kernel (1 - dist^2/r^2) * 3 / (4 r sin(theta)) / selection pdf (PhotonBeam.hxx:487-494)
-    const float k1 = (1.0f - (dist * dist) * __builtin_amdgcn_rcpf(radius * radius)) * 3.0f;
+    const float k1 = radial_kernel(dist * dist, radius * radius) * 3.0f;
     const float kd = k1 * __builtin_amdgcn_rcpf((4.0f * radius) * sin_theta) * __builtin_amdgcn_rcpf(sel_pdf);
     const float u0 = ((((flux.x * s.ss0) * a0) * bsdf) * kd);
     const float u1 = ((((flux.y * s.ss1) * a1) * bsdf) * kd);
@@ -553,9 +553,9 @@ __device__ __forceinline__ void photon_leaf(const EvalArgs& A, const Ray& r, Seg
                     const float4 in = A.inc[i];
                     ph = phase_pdf(s.g, false, dot3(in.x, in.y, in.z, r.dx, r.dy, r.dz));
                 }
-                // Epanechnikov kernel (1 - s/r^2) / (r^2 pi / 2), Bre.cxx:272-273
-                // the kernel value only scales the contribution: a 1-ulp reciprocal suffices
-                const float kern = 1.0f - sq * __builtin_amdgcn_rcpf(p.w);
+                // Epanechnikov kernel (1 - s/r^2) / (r^2 pi / 2), Bre.cxx:272-273; the hardware
+                // reciprocal except near the radius (radial_kernel)
+                const float kern = radial_kernel(sq, p.w);
                 const float scale = (ph * kern) * w.w;
                 // base_c = a_c * scale; contribution_c = power_c * base_c * sigma_s_c
                 const float b0 = a0 * scale, b1 = a1 * scale, b2 = a2 * scale;
@@ -571,6 +571,13 @@ __device__ __forceinline__ void photon_leaf(const EvalArgs& A, const Ray& r, Seg
                     s.gt0 -= s.w0 * c0 * dt;
                     s.gt1 -= s.w1 * c1 * dt;
                     s.gt2 -= s.w2 * c2 * dt;
+                    // SHORT query: T_c / T_k, so channel k gains +dt from every channel
+                    if (short_q) {
+                        const float sum = (s.w0 * c0 + s.w1 * c1) + s.w2 * c2;
+                        if (s.kidx == 0) s.gt0 += sum * dt;
+                        else if (s.kidx == 1) s.gt1 += sum * dt;
+                        else s.gt2 += sum * dt;
+                    }
                     s.acc0 += s.w0 * c0;
                     s.acc1 += s.w1 * c1;
                     s.acc2 += s.w2 * c2;
@@ -657,7 +664,7 @@ __device__ __forceinline__ void beam_leaf(const EvalArgs& A, const Ray& r, SegSt
                 const float ph = s.iso ? kInvPi * 0.25f
                                        : phase_pdf(s.g, false, -dot3(pc.p1.x, pc.p1.y, pc.p1.z, r.dx, r.dy, r.dz));
                 // 1D kernel (1 - dist^2/r^2) * 3 / (4 r sin(theta)), PhotonBeam.hxx:364-365
-                const float kern = 1.0f - (dist * dist) * __builtin_amdgcn_rcpf(radius * radius);
+                const float kern = radial_kernel(dist * dist, radius * radius);
                 float scale = (ph * kern * 3.0f) * __builtin_amdgcn_rcpf((4.0f * radius) * st);
                 if (MODE & kReduced) scale *= red_w;
                 const float b0 = a0 * scale, b1 = a1 * scale, b2 = a2 * scale;
@@ -670,18 +677,19 @@ __device__ __forceinline__ void beam_leaf(const EvalArgs& A, const Ray& r, SegSt
                     s.gs1 += s.w1 * b1 * flux.y;
                     s.gs2 += s.w2 * b2 * flux.z;
                     const float g0 = s.w0 * c0, g1 = s.w1 * c1, g2 = s.w2 * c2;
-                    // d log C_c / d sigma_t_c = -(t1 - tmin) - t2, except the SHORT-beam
-                    // normalisation channel k (its t2 terms cancel) ...
-                    const int k = s.kidx;
-                    s.gt0 -= g0 * (dq + ((short_b && k == 0) ? 0.f : t2));
-                    s.gt1 -= g1 * (dq + ((short_b && k == 1) ? 0.f : t2));
-                    s.gt2 -= g2 * (dq + ((short_b && k == 2) ? 0.f : t2));
-                    // ... which instead gains +t2 from every other channel of a SHORT beam
-                    if (short_b) {
-                        const float other = (k == 0 ? 0.f : g0) + (k == 1 ? 0.f : g1) + (k == 2 ? 0.f : g2);
-                        if (k == 0) s.gt0 += other * t2;
-                        else if (k == 1) s.gt1 += other * t2;
-                        else s.gt2 += other * t2;
+                    // log C_c holds -sigma_t_c ((t1 - tmin) + t2) plus, for the SHORT
+                    // normalisations by the channel-k transmittance, +sigma_t_k (t1 - tmin)
+                    // (SHORT query) and +sigma_t_k t2 (SHORT beam)
+                    const float L = dq + t2;
+                    s.gt0 -= g0 * L;
+                    s.gt1 -= g1 * L;
+                    s.gt2 -= g2 * L;
+                    const float Lk = (short_q ? dq : 0.f) + (short_b ? t2 : 0.f);
+                    if (Lk != 0.f) {
+                        const float sum = ((g0 + g1) + g2) * Lk;
+                        if (s.kidx == 0) s.gt0 += sum;
+                        else if (s.kidx == 1) s.gt1 += sum;
+                        else s.gt2 += sum;
                     }
                     s.acc0 += g0;
                     s.acc1 += g1;
@@ -976,7 +984,18 @@ __device__ void run_tile(const EvalArgs& A, int64_t slot0, unsigned long long* l
                     const upbp_volume_segment* sp = A.segments + (s0 + j);
                     const float len = sp->dist_max - sp->dist_min;
                     const int32_t mj = sp->medium_id;
-                    const float d0 = -s.acc0 * len, d1 = -s.acc1 * len, d2 = -s.acc2 * len;
+                    float d0 = -s.acc0 * len, d1 = -s.acc1 * len, d2 = -s.acc2 * len;
+                    // SHORT query: the segment's factor is attenuation over its pass-through pdf
+                    // exp(-sigma_t_k len) when the medium samples distances (Media.hxx:294-323)
+                    if (short_q && !A.lite) {
+                        const MediumDev* mm = A.media + (A.media_base + mj);
+                        if (mm->min_comp > 0.f && mm->has_scattering) {
+                            const float sum = ((s.acc0 + s.acc1) + s.acc2) * len;
+                            if (mm->min_idx == 0) d0 += sum;
+                            else if (mm->min_idx == 1) d1 += sum;
+                            else d2 += sum;
+                        }
+                    }
                     if (lds_media && mj < A.lds_media_n) {
                         float* base = lds_media + (size_t)mj * 6 * kWave + lane;
                         base[3 * kWave] += d0;
@@ -1070,7 +1089,7 @@ __device__ __forceinline__ int64_t xcd_block(int64_t bid, int64_t nblk) {
 // register spill cost less than the latency the extra wave hides (measured 3-4 %).
 // UPBP kernels ask for 4 (128 VGPRs), which both fit without spilling.
 template <int EST, int MODE>
-__global__ __launch_bounds__(256, ((MODE & 7) == (int)Mode::Upbp ? 4 : (MODE & kReduced) ? 6 : (MODE & 7) == (int)Mode::Forward ? 8 : 1)) void k_eval(
+__global__ __launch_bounds__(256, ((MODE & 7) == (int)Mode::Upbp ? 4 : (MODE & 7) == (int)Mode::Adjoint ? 1 : (MODE & kReduced) ? 6 : (MODE & 7) == (int)Mode::Forward ? 8 : 1)) void k_eval(
     EvalArgs A) {
     const int wave_in_block = threadIdx.x / kWave;
     const int waves_per_block = blockDim.x / kWave;
@@ -1238,6 +1257,13 @@ void launch_mode(Ctx& ctx, Mode mode, const EvalArgs& A, unsigned blocks, unsign
         case Mode::Adjoint: {
             // LDS gradient slabs for the media actually present: fewer bytes, more resident waves
             const size_t lds = (size_t)(threads / kWave) * A.lds_media_n * 6 * kWave * sizeof(float);
+            if constexpr (EST == kEstBeams) {
+                if (A.red.max != 0) {
+                    hipLaunchKernelGGL((k_eval<EST, (int)Mode::Adjoint | kReduced>), dim3(blocks), dim3(threads), lds,
+                                       ctx.stream, A);
+                    break;
+                }
+            }
             hipLaunchKernelGGL((k_eval<EST, (int)Mode::Adjoint>), dim3(blocks), dim3(threads), lds, ctx.stream, A);
             break;
         }

@@ -86,14 +86,34 @@ __device__ __forceinline__ bool test_bre(float ox, float oy, float oz, float dx,
 // |ad| * rsq(s2) is within a few ulp (< 1e-6) of fl(|ad| / fl(sqrt(s2))).
 constexpr float kRadiusTie = 1e-5f;
 
+// Kernels 1 - x (x = d^2 / r^2 of a hit) closer to 0 than this are evaluated with the
+// reference's exact operations: near the radius the subtraction amplifies the few-ulp
+// difference of the hardware reciprocal (square root) by 1 / (1 - x), which above this
+// bound stays below 3e-5 relative.
+constexpr float kGrazing = 0.02f;
+
+// 1 - num / den, the 2D and 1D kernels' radial factor (Bre.cxx:272, PhotonBeam.hxx:364):
+// through the hardware reciprocal, or the reference's IEEE division near the radius.
+__device__ __forceinline__ float radial_kernel(float num, float den) {
+    const float k = 1.0f - num * __builtin_amdgcn_rcpf(den);
+    return k < kGrazing ? 1.0f - num / den : k;
+}
+
 // PhotonBeam::testIntersectionBeamBeam (PhotonBeam.hxx:238-296) with
-// minT2 = 0 and maxT2 = length, plus the piece's half-open range [t_lo, t_hi)
-// that makes a beam cut into pieces count once, and the radius test of
-// PhotonBeam::accumulate (radius = r0 + t2 * dr, hit unless radius < dist,
-// PhotonBeam.hxx:330-334). d1o1 = dot3(d1, o1) and d2o2 = dot3(d2, o2) are passed
-// in precomputed (same operations, same values). The radius test decides with the
-// exact dist = |ad| / sqrt(s2) of the reference; dist and sin_theta are returned from
-// the reciprocal square root unless the test needed the exact values.
+// minT2 = 0 and maxT2 = length, the radius test of PhotonBeam::accumulate
+// (radius = r0 + t2 * dr, hit unless radius < dist, PhotonBeam.hxx:330-334), and the
+// piece's half-open range [t_lo, t_hi) that makes a beam cut into pieces count once.
+// The range is tested on the foot of the query's closest-approach point on the beam line,
+// tw = d2.(o1 - o2) + t1 q, not on the reference's t2 = ((t1 + d1.o1) - d1.o2) / q: for
+// nearly perpendicular lines (q -> 0) that quotient carries an error of about 1e-7 / |q|
+// of the scene scale, so its piece can lie far from the point the ray actually passes
+// (a piece the traversal rightly culls). tw is well conditioned and the same for every
+// piece of the beam, so exactly one piece owns each pair; the beam's first and last
+// pieces own everything before and after them. The hit decision itself stays the
+// reference's, on its own t2. d1o1 = dot3(d1, o1) and d2o2 = dot3(d2, o2) are passed in
+// precomputed (same operations, same values). The radius test decides with the exact
+// dist = |ad| / sqrt(s2) of the reference; dist and sin_theta are returned from the
+// reciprocal square root unless the test needed the exact values.
 __device__ __forceinline__ bool test_beam_beam(float o1x, float o1y, float o1z, float d1x, float d1y,
                                                float d1z, float d1o1, float min_t1, float max_t1, float o2x,
                                                float o2y, float o2z, float d2x, float d2y, float d2z,
@@ -110,16 +130,20 @@ __device__ __forceinline__ bool test_beam_beam(float o1x, float o1y, float o1z, 
     const float q2m1 = q * q - 1.0f;
     if (q2m1 < 1e-5f && q2m1 > -1e-5f) return false;
     const float d1o2 = dot3(d1x, d1y, d1z, o2x, o2y, o2z);
-    const float t1 = ((d1o1 - d1o2) - q * (dot3(d2x, d2y, d2z, o1x, o1y, o1z) - d2o2)) / q2m1;
+    const float d2oo = dot3(d2x, d2y, d2z, o1x, o1y, o1z) - d2o2;
+    const float t1 = ((d1o1 - d1o2) - q * d2oo) / q2m1;
     if (t1 <= min_t1 || t1 >= max_t1) return false;
+    const float tw = d2oo + t1 * q;
+    if (!(tw >= t_lo && tw < t_hi)) return false;
     const float t2 = ((t1 + d1o1) - d1o2) / q;
     if (t2 <= 0.0f || t2 >= len || __builtin_isnan(t2)) return false;
-    if (!(t2 >= t_lo && t2 < t_hi)) return false;
     const float radius = r0 + t2 * dr;
     const float aad = __builtin_fabsf(ad);
     const float rs = __builtin_amdgcn_rsqf(s2);
     float d = aad * rs, st = s2 * rs;
-    if (!(radius > d * (1.0f + kRadiusTie))) {
+    // within 1 % of the radius (where the kernel 1 - d^2 / r^2 drops below ~kGrazing), or at
+    // the tie, the reference's exact sqrt and division decide and give the returned values
+    if (!(radius > d * 1.0101f)) {
         if (radius < d * (1.0f - kRadiusTie)) return false;
         st = __builtin_sqrtf(s2);
         d = aad / st;

@@ -97,16 +97,23 @@ def test_bound_above_occupancy_equals_unreduced():
 
 
 def test_reduction_rejected_where_unsupported():
+    """A reduced beam grid in a batched plan is rejected (UPBP_EUNSUPPORTED); the adjoint and
+    the UPBP weights accept it (their parity tests are in test_gpu_adjoint.py and
+    test_gpu_upbp.py); an unknown reduction type is rejected at build."""
+    import torch
+
     print_case("beam_reduction_rejects")
     w = scenes.beams_stress(n_queries=64, n_beams=500, seed=84)
     ctx = gpu_context(w.media)
     bm = native.PhotonBeamsEvaluator(ctx)
     bm.build(w.beams, max_beams_in_cell=2, reduction_type=scenes.REDUCE_PRESAMPLE, grid_resolution=8)
     qv = native.QueryBatchView.to_device(w.queries)
+    with pytest.raises(native.UpbpError) as e:
+        native.BatchPlan(ctx, [(None, bm, qv, 0, None, torch.zeros((w.queries.count, 3), device="cuda"))])
+    assert e.value.status == _abi.UPBP_EUNSUPPORTED
     g = np.zeros((w.queries.count, 3), np.float32)
-    with pytest.raises(native.UpbpError):
-        native.adjoint(ctx, None, bm, qv, grad_out_beams=g, grad_scattering=np.zeros((len(w.media), 3), np.float32),
-                       grad_absorption=np.zeros((len(w.media), 3), np.float32))
+    native.adjoint(ctx, None, bm, qv, grad_out_beams=g, grad_scattering=np.zeros((len(w.media), 3), np.float32),
+                   grad_absorption=np.zeros((len(w.media), 3), np.float32))
     with pytest.raises(native.UpbpError):
         bad = native.PhotonBeamsEvaluator(ctx)
         bad.build(w.beams, max_beams_in_cell=2, reduction_type=9)

@@ -6,6 +6,7 @@ normalisation integrals, invariants and brute-force equivalence. Each check is
 chosen so that a plausible slip (dropped term, wrong sign, strict vs inclusive
 bound, transposed operand) fails it.
 """
+import dataclasses
 import math
 
 import numpy as np
@@ -329,6 +330,49 @@ def test_oracle_adjoint_matches_central_differences():
     pm[k, 1] -= 1e-3
     fd = (loss(w.media, pp) - loss(w.media, pm)) / 2e-3
     assert fd == pytest.approx(g["grad_photon_power"][k, 1], rel=1e-5)
+
+
+@pytest.mark.parametrize("reduced", [False, True])
+def test_oracle_adjoint_short_queries_match_central_differences(reduced):
+    """SHORT query beams (Bre.cxx:257-263, PhotonBeam.hxx:336-345): in-segment transmittance
+    T_c / T_k and, across segments, attenuation over the pass-through pdf exp(-sigma_t_k len), with
+    k the minimum-sigma_t channel (unique in these chromatic media, so central differences stay on
+    one side of its switch). reduced: the beam grid keeps 2 beams per crowded cell, each hit
+    weighted by its count over the cell's pdf; the weights do not depend on sigma.
+    The FP64 forward reads float32 segment records (attenuation, pdf), whose rounding puts noise of
+    about 1e-7 L / h into central differences: at h = 1e-3 that exceeds 1e-3 of the smallest of the
+    12 entries here, at h = 1e-2 it is below 1e-5 while the truncation error stays below 2e-4."""
+    w = scenes.adjoint_scene(12, 12, 3_000, 600, seed=37, chromatic=True)
+    q = w.queries
+    w = scenes.with_media(dataclasses.replace(w, queries=scenes.Queries(q.origin, q.direction, q.seg_offsets,
+                                                                      q.segments, scenes.SHORT_BEAM)), w.media)
+    assert w.queries.beam_type == scenes.SHORT_BEAM
+    rng = np.random.default_rng(4)
+    wp = rng.uniform(-1, 1, (w.queries.count, 3))
+    wb = rng.uniform(-1, 1, (w.queries.count, 3))
+    oph = po.OraclePhotons(w.photons, build_bvh=False)
+    if reduced:
+        obm = po.OracleBeams(w.beams, grid_res=8).set_reduction(2, scenes.REDUCE_RESAMPLE_FIXED, 3)
+    else:
+        obm = po.OracleBeams(w.beams, grid_res=0)
+    g = po.adjoint(oph, obm, w.media, w.queries, wp, wb)
+
+    def loss(media):
+        ww = scenes.with_media(w, media)
+        a = po.bre_eval(oph, media, ww.queries, accel=0, precision=64)[0]
+        b = po.beams_eval(obm, media, ww.queries, accel=0, precision=64)[0]
+        return float((wp * a).sum() + (wb * b).sum())
+
+    h = 1e-2
+    for m in (0, 2):
+        for field in ("scattering", "absorption"):
+            for c in range(3):
+                mp, mm = w.media.copy(), w.media.copy()
+                mp[m][field][c] += h
+                mm[m][field][c] -= h
+                fd = (loss(mp) - loss(mm)) / (2 * h)
+                an = g["grad_sigma_s" if field == "scattering" else "grad_sigma_a"][m, c]
+                assert fd == pytest.approx(an, rel=1e-3, abs=1e-9 * max(1.0, abs(an))), (m, field, c)
 
 
 def test_oracle_adjoint_carries_the_sign_of_negative_coefficients():
