@@ -217,8 +217,10 @@ def beams_eval(obm: OracleBeams, media, q: Queries, accel=0, precision=32, threa
 def adjoint(oph, obm, media, q: Queries, w_photons, w_beams, threads=None, fp32_hits=False):
     """FP64 analytic gradient. Returns dict of grad_sigma_s, grad_sigma_a [M,3],
     grad_photon_power [N_in,3], grad_beam_power [B,3]. fp32_hits: the hit sets (and hit
-    parameters) of the reference's FP32 predicates, as the FP32 forward and the GPU have them;
-    else those of the FP64 forward."""
+    parameters) of the reference's FP32 predicates, as the FP32 forward and the GPU have them,
+    and the kernels' radial factors 1 - d^2 / r^2 in the reference's float arithmetic (near the
+    radius their relative accuracy in float is about 2^-24 / (1 - d^2 / r^2)); else those of the
+    FP64 forward."""
     threads = threads or _threads()
     qb = query_batch(q)
     m, mp, nm = _media_ptr(media)

@@ -1118,7 +1118,12 @@ int or_adjoint(const or_photons* ph, const or_beams* bm, const upbp_medium* medi
                         double r2 = ph->r2[i];
                         double inc[3] = {-ph->inc[3 * i], -ph->inc[3 * i + 1], -ph->inc[3 * i + 2]};
                         double rho = phase_f64(d, inc, m->mean_cosine);
-                        double K = (1.0 - s / r2) / (r2 * (double)PI_F * 0.5);
+                        /* fp32_hits: the radial factor is also the reference's float expression
+                         * (Bre.cxx:272-273); near the radius 1 - s / r^2 keeps only about
+                         * 2^-24 / (1 - s / r^2) relative accuracy in float, so a double value
+                         * would not be the reference's */
+                        double radial = fp32_hits ? (double)(1.0f - (float)s / ph->r2[i]) : 1.0 - s / r2;
+                        double K = radial / (r2 * (double)PI_F * 0.5);
                         /* SHORT query: T_c / T_k along t - tmin (Bre.cxx:257-263); the exponent
                          * -(sigma_t_c - sigma_t_k)(t - tmin) gives channel k the +(t - tmin) terms */
                         const double Ek = exp(-(double)m->sigma_t[k] * (t - tmin));
@@ -1153,7 +1158,11 @@ int or_adjoint(const or_photons* ph, const or_beams* bm, const upbp_medium* medi
                             red_w = (double)mult / (double)pdf;
                         }
                         double rho = phase_f64(d, d2, m->mean_cosine);
-                        double K1 = (1.0 - dist * dist / (radius * radius)) * 3.0 / (4.0 * radius * st) * red_w;
+                        /* fp32_hits: the reference's float radial factor (PhotonBeam.hxx:364) */
+                        const float fd = (float)dist, fr = (float)radius;
+                        double radial = fp32_hits ? (double)(1.0f - fd * fd / (fr * fr))
+                                                  : 1.0 - dist * dist / (radius * radius);
+                        double K1 = radial * 3.0 / (4.0 * radius * st) * red_w;
                         int sb = (bm->flags[i] & UPBP_SHORT_BEAM) != 0;
                         double Eqk = exp(-(double)m->sigma_t[k] * (t1 - tmin));
                         double Ebk = exp(-(double)m->sigma_t[k] * t2);
