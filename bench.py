@@ -244,12 +244,14 @@ def stress_measure(steps: int, cpu_budget: float, threads: int):
     return rec
 
 
-def iteration_measure(steps: int, ctx, qv, leaf_ph: int, leaf_bm: int, piece_factor: float, paths: int = 1_000_000):
+def iteration_measure(steps: int, ctx, qv, leaf_ph: int, leaf_bm: int, piece_factor: float, paths: int = 1_000_000,
+                      beam_paths: int = 125_000):
     """One renderer iteration on the GPU with every array device-resident: trace 1,000,000 light
-    paths in the box + sphere media (upbp_trace_light_paths: light vertices and SHORT beams),
-    build the photon structure from the in-medium light vertices (r = 0.005) and the beam
-    structure (r = 0.003) from device memory, then evaluate both estimators for the 1920x1080
-    query beams. The reference rebuilds its structures every iteration
+    paths in the box + sphere media (upbp_trace_light_paths: light vertices, and SHORT beams
+    from the first 125,000 paths, the reference's `-pbc 125000`, about 200k beams as in the
+    1080p workload), build the photon structure from the in-medium light vertices (r = 0.005)
+    and the beam structure (r = 0.003) from device memory, then evaluate both estimators for
+    the 1920x1080 query beams. The reference rebuilds its structures every iteration
     (SmallUPBP/src/Renderers/VolLightTracer.hxx:303-410), so this is the per-iteration cost."""
     import ctypes as C
 
@@ -257,7 +259,8 @@ def iteration_measure(steps: int, ctx, qv, leaf_ph: int, leaf_bm: int, piece_fac
 
     from smallupbp_amd import _abi, native, scenes
 
-    sc = scenes.light_scene(media_rows=LIGHT_TRACE_MEDIA, max_path_length=10, bb1d_min_mfp=0.1, seed=12)
+    sc = scenes.light_scene(media_rows=LIGHT_TRACE_MEDIA, max_path_length=10, bb1d_min_mfp=0.1, seed=12,
+                            beam_path_count=beam_paths)
     tctx = native.Context(0)
     tctx.set_media(sc.media)
     cap_v, cap_b = scenes.light_path_capacity(sc, paths)
@@ -305,16 +308,19 @@ def iteration_measure(steps: int, ctx, qv, leaf_ph: int, leaf_bm: int, piece_fac
                               bmv["beam_length"], bmv["beam_throughput"].view(nb, 3), bmv["beam_flags"],
                               bmv["beam_medium"], 0.003), leaf_size=leaf_bm, piece_factor=piece_factor)
         t3 = time.perf_counter()
-        native.eval_batch(ctx, bre, bm, qv, out_p, out_b)
+        st = {}
+        native.eval_batch(ctx, bre, bm, qv, out_p, out_b, stats=st)
         torch.cuda.synchronize()
         t4 = time.perf_counter()
         if times is not None:
             times.append((t1, t2, t3, t4))
+            hits.append((int(st["photon_hits"]), int(st["beam_hits"])))
         kept = bre.num_photons
         bre.destroy()
         bm.destroy()
         return nv, nb, kept
 
+    hits = []
     for _ in range(2):
         one()
     torch.cuda.synchronize()
@@ -331,10 +337,10 @@ def iteration_measure(steps: int, ctx, qv, leaf_ph: int, leaf_bm: int, piece_fac
     total = [t[3] - s for t, s in zip(tms, starts)]
     ms = lambda xs: round(float(np.mean(xs)) * 1e3, 3)
     return {"workload": "iteration (trace %d light paths -> %d light vertices (%d in media, the photons) + %d SHORT "
-                        "beams -> photon and beam builds from device memory -> eval of 1920x1080 queries)"
-                        % (paths, nv, kept, nb),
+                        "beams from the first %d paths -> photon and beam builds from device memory -> eval of "
+                        "1920x1080 queries)" % (paths, nv, kept, nb, beam_paths),
             "iteration_ms": ms(total), "trace_ms": ms(trace), "photon_build_ms": ms(build_ph),
-            "beam_build_ms": ms(build_bm), "eval_ms": ms(ev)}
+            "beam_build_ms": ms(build_bm), "eval_ms": ms(ev), "photon_hits": hits[-1][0], "beam_hits": hits[-1][1]}
 
 
 def batched_ratio(scale: float, steps: int, n_instances: int = 64):
@@ -568,7 +574,8 @@ def light_trace_measure(steps: int, cpu_budget: float, threads: int, paths: int 
 
     from smallupbp_amd import _abi, native, scenes
 
-    sc = scenes.light_scene(media_rows=LIGHT_TRACE_MEDIA, max_path_length=10, bb1d_min_mfp=0.1, seed=12)
+    sc = scenes.light_scene(media_rows=LIGHT_TRACE_MEDIA, max_path_length=10, bb1d_min_mfp=0.1, seed=12,
+                            beam_path_count=beam_paths)
     ctx = native.Context(0)
     ctx.set_media(sc.media)
     cap_v, cap_b = scenes.light_path_capacity(sc, paths)
