@@ -84,14 +84,19 @@ struct Ray {
     uint64_t qkey;        // per-cell beam reduction: key of the query ray (reduced kernels only)
 };
 
-__device__ __forceinline__ bool slab(const Ray& r, float tmin, float tmax, float lx, float ly, float lz,
-                                     float hx, float hy, float hz) {
+// Wave mask of the lanes whose segment [tmin, tmax] meets the box:
+// max(tn, tmin) <= min(tf, tmax) as three compares, each balloted on its own. min / max of
+// tmin and tmax (values the compiler cannot prove canonical) would put a canonicalising max
+// in front of each use at every node, and a ballot of a combined bool first selects it into
+// a VGPR and compares it back; the slab values are fma results and need neither.
+__device__ __forceinline__ uint64_t slab_mask(const Ray& r, float tmin, float tmax, float lx, float ly, float lz,
+                                              float hx, float hy, float hz) {
     const float ax = __builtin_fmaf(lx, r.ix, -r.oix), bx = __builtin_fmaf(hx, r.ix, -r.oix);
     const float ay = __builtin_fmaf(ly, r.iy, -r.oiy), by = __builtin_fmaf(hy, r.iy, -r.oiy);
     const float az = __builtin_fmaf(lz, r.iz, -r.oiz), bz = __builtin_fmaf(hz, r.iz, -r.oiz);
-    const float tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), tmin));
-    const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tmax));
-    return tn <= tf;
+    const float tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));
+    const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
+    return __ballot(tn <= tf) & __ballot(tn <= tmax) & __ballot(tf >= tmin);
 }
 
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -537,7 +542,8 @@ __device__ __forceinline__ void photon_leaf(const EvalArgs& A, const Ray& r, Seg
             }
             continue;
         }
-        if (__ballot(hit)) {
+        // as in beam_leaf: only the adjoint needs the whole wave inside
+        if ((MODE & 7) == (int)Mode::Adjoint ? __ballot(hit) != 0 : hit) {
             const float4 w = A.power[i];  // (rgb, 1 / (r^2 pi / 2))
             float gp0 = 0.f, gp1 = 0.f, gp2 = 0.f;
             if (hit) {
@@ -607,7 +613,7 @@ __device__ __forceinline__ void beam_leaf(const EvalArgs& A, const Ray& r, SegSt
         const PieceDev pc = A.pieces[i];
         const uint32_t word = __float_as_uint(pc.p2.z);
         const int32_t bmed = (int32_t)(word & kMediumMask);
-        float dist, st, t1, t2, radius = 0.f;
+        float dist, st, t1, t2, radius;
         // radius = mStartRadius (p1.w) + t2 * mRadiusChange (p3.y)
         const bool hit = (bmed == s.medium) &&
                          test_beam_beam(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, r.d1o1, s.tmin, s.tmax, pc.p0.x, pc.p0.y,
@@ -649,7 +655,11 @@ __device__ __forceinline__ void beam_leaf(const EvalArgs& A, const Ray& r, SegSt
             red_n = red_decide(A.red, r.qkey, r.ox + r.dx * t1, r.oy + r.dy * t1, r.oz + r.dz * t1, A.orig[i], pdf);
             red_w = (1.0f / pdf) * (float)red_n;
         }
-        if (__ballot(hit && red_n != 0)) {
+        // the adjoint reduces the item gradient over the wave, so every lane enters when one
+        // hits; elsewhere the lanes that hit enter on their own (a ballot of this divergent
+        // bool would first select it into a VGPR and compare it back)
+        const bool take = hit && red_n != 0;
+        if ((MODE & 7) == (int)Mode::Adjoint ? __ballot(take) != 0 : take) {
             const float4 flux = A.piece_flux[i];
             float gb0 = 0.f, gb1 = 0.f, gb2 = 0.f;
             if (hit && red_n != 0) {
@@ -729,12 +739,11 @@ __device__ __forceinline__ void traverse(const EvalArgs& A, const Ray& r, SegSta
         const float4 a = nd.a, b = nd.b, c = nd.c;
         const uint4 rr = nd.r;
         nodes_visited++;
-        const bool s0 = slab(r, s.tmin, s.tmax, a.x, a.y, a.z, a.w, b.x, b.y);
-        const bool s1 = slab(r, s.tmin, s.tmax, b.z, b.w, c.x, c.y, c.z, c.w);
-        const bool h0 = s0 & (rr.z != kInvalid);
-        const bool h1 = s1 & (rr.w != kInvalid);
-        const bool any0 = __ballot(h0) != 0;
-        const bool any1 = __ballot(h1) != 0;
+        const uint64_t m0 = slab_mask(r, s.tmin, s.tmax, a.x, a.y, a.z, a.w, b.x, b.y);
+        const uint64_t m1 = slab_mask(r, s.tmin, s.tmax, b.z, b.w, c.x, c.y, c.z, c.w);
+        // the wave-uniform child references are tested on the scalar side
+        const bool any0 = m0 != 0 && rr.z != kInvalid;
+        const bool any1 = m1 != 0 && rr.w != kInvalid;
         if (any0 && rr.z != 0) {
             tests += (uint64_t)rr.z * active;
             if (EST == kEstPhotons)

@@ -142,19 +142,22 @@ __device__ __forceinline__ bool test_beam_beam(float o1x, float o1y, float o1z, 
     const float rs = __builtin_amdgcn_rsqf(s2);
     float d = aad * rs, st = s2 * rs;
     // within 1 % of the radius (where the kernel 1 - d^2 / r^2 drops below ~kGrazing), or at
-    // the tie, the reference's exact sqrt and division decide and give the returned values
+    // the tie, the reference's exact sqrt and division decide and give the returned values.
+    // One exit for both outcomes: separate early returns make the compiler copy the five
+    // outputs through every exit block.
+    bool ok = true;
     if (!(radius > d * 1.0101f)) {
-        if (radius < d * (1.0f - kRadiusTie)) return false;
+        ok = !(radius < d * (1.0f - kRadiusTie));
         st = __builtin_sqrtf(s2);
         d = aad / st;
-        if (radius < d) return false;
+        ok = ok && !(radius < d);
     }
     dist = d;
     sin_theta = st;
     t1_out = t1;
     t2_out = t2;
     radius_out = radius;
-    return true;
+    return ok;
 }
 
 // exp(x) for x <= 0 through the hardware base-2 exponential.
