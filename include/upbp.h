@@ -279,6 +279,24 @@ float upbp_beams_selection_pdf(const upbp_beams* beams, const float position[3])
 upbp_status upbp_beams_selection_pdfs(const upbp_beams* beams, const float* positions, int64_t count, float* out_pdf,
                                       upbp_memory memory);
 
+/* Per-query statistics of the reference's uniform beam grid: struct GridStats
+ * (SmallUPBP/src/Beams/GridStats.hxx:37-54) as Grid::intersect gathers it
+ * (SmallUPBP/src/Beams/Grid.hxx:318-399, counts at :365,378-382) over every segment of a
+ * query in a scattering medium, the walk of PhotonBeamsEvaluator::evalBeamBeamEstimate
+ * (SmallUPBP/src/Beams/PhBeams.cxx:177-230; it ends once a channel of the attenuation product
+ * is not above 1e-20). out_stats [count][3], in out_memory, query order as given:
+ *   [0] intersectedBeams: the beams registered in each cell the query crosses, or
+ *       max_beams_in_cell for a cell holding more;
+ *   [1] intersectedCells: the cells crossed;
+ *   [2] overfullCells: crossed cells holding more than max_beams_in_cell beams (0 without
+ *       reduction).
+ * The reference feeds these counts to its beam-density images (BeamDensity.hxx:110-140). The
+ * grid is the one the structure's reduction uses; a structure built without reduction gets a
+ * grid of its build's grid_resolution (0 = 256) for the call. The walk is the reference's FP32
+ * arithmetic, so the counts are those of the reference's own grid. */
+upbp_status upbp_beams_grid_stats(upbp_ctx* ctx, const upbp_beams* beams, const upbp_query_batch* queries,
+                                  uint32_t* out_stats, upbp_memory out_memory);
+
 /* ---------------------------------------------------------------------------
  * Evaluation. out_photons receives, per query, EmbreeBre::evalBre
  * (Bre.cxx:812-879 full segments, :895-964 lite); out_beams receives

@@ -194,6 +194,13 @@ public:
         check(upbp_eval(ctx_.get(), nullptr, h_, &q, nullptr, outRgb, UPBP_MEM_HOST, stats), ctx_.get());
     }
 
+    // GridStats per query of the reference grid's walk (GridStats.hxx:37-54, Grid.hxx:318-399,
+    // PhBeams.cxx:177-230): outStats [n][3] = intersectedBeams, intersectedCells, overfullCells.
+    void gridStats(const QueryBatch& batch, uint32_t beamType, uint32_t* outStats, bool lite = false) {
+        upbp_query_batch q = batch.view(beamType, lite);
+        check(upbp_beams_grid_stats(ctx_.get(), h_, &q, outStats, UPBP_MEM_HOST), ctx_.get());
+    }
+
     // PhotonBeamsEvaluator::getBeamSelectionPdf (1 without per-cell beam reduction).
     float getBeamSelectionPdf(const float position[3]) const { return upbp_beams_selection_pdf(h_, position); }
     // The same for n positions ([n][3] -> [n]) in one launch.

@@ -57,6 +57,11 @@ def lib():
         L.or_beams_count.restype = i64
         L.or_beams_count.argtypes = [vp]
         L.or_beams_grid_info.argtypes = [vp, vp]
+        L.or_beam_grid_stats.restype = C.c_int
+        L.or_beams_grid_geometry.argtypes = [vp, vp]
+        L.or_beams_grid_cell_counts.restype = C.c_int
+        L.or_beams_grid_cell_counts.argtypes = [vp, vp, i64]
+        L.or_beam_grid_stats.argtypes = [vp, vp, i32, C.POINTER(_abi.QueryBatch), C.c_int, vp]
         L.or_beams_set_reduction.restype = C.c_int
         L.or_beams_set_reduction.argtypes = [vp, C.c_uint32, C.c_uint32, i32]
         L.or_beams_selection_pdf.restype = f32
@@ -170,6 +175,20 @@ class OracleBeams:
         lib().or_beams_grid_info(self.h, info.ctypes.data)
         return info
 
+    def grid_geometry(self):
+        """(lo [3], hi [3], cell size [3]) of the reference grid."""
+        g = np.zeros(9, np.float32)
+        lib().or_beams_grid_geometry(self.h, g.ctypes.data)
+        return g[:3], g[3:6], g[6:]
+
+    def grid_cell_counts(self):
+        """Beams registered per cell, [res_z, res_y, res_x]."""
+        res = self.grid_info()[:3]
+        out = np.zeros(int(np.prod(res)), np.uint32)
+        if lib().or_beams_grid_cell_counts(self.h, out.ctypes.data, out.size):
+            raise RuntimeError("or_beams_grid_cell_counts failed")
+        return out.reshape(int(res[2]), int(res[1]), int(res[0]))
+
     def __del__(self):
         if getattr(self, "h", None):
             lib().or_beams_free(self.h)
@@ -196,6 +215,20 @@ def bre_eval(oph: OraclePhotons, media, q: Queries, accel=1, precision=32, threa
     if r:
         raise RuntimeError("or_bre_eval failed: %s" % _abi.STATUS_NAMES.get(r, r))
     return out, sig, cnt
+
+
+def beam_grid_stats(obm: OracleBeams, media, q: Queries, threads=None):
+    """GridStats per query ([Q,3] u32: beams tested, cells crossed, overfull cells crossed) of the
+    reference grid's walk over the query's scattering segments; needs an OracleBeams built with
+    grid_res > 0 (and set_reduction for overfull cells)."""
+    threads = threads or _threads()
+    qb = query_batch(q)
+    m, mp, nm = _media_ptr(media)
+    out = np.zeros((q.count, 3), np.uint32)
+    r = lib().or_beam_grid_stats(obm.h, mp, nm, C.byref(qb), threads, out.ctypes.data)
+    if r:
+        raise RuntimeError("or_beam_grid_stats failed: %s" % _abi.STATUS_NAMES.get(r, r))
+    return out
 
 
 def beams_eval(obm: OracleBeams, media, q: Queries, accel=0, precision=32, threads=None, signatures=False):

@@ -714,6 +714,21 @@ void or_beams_grid_info(const or_beams* bm, int64_t* info6) {
     info6[5] = bm->n;
 }
 
+void or_beams_grid_geometry(const or_beams* bm, float* out9) {
+    for (int a = 0; a < 3; ++a) {
+        out9[a] = bm->grid.lo[a];
+        out9[3 + a] = bm->grid.hi[a];
+        out9[6 + a] = bm->grid.cell[a];
+    }
+}
+
+int or_beams_grid_cell_counts(const or_beams* bm, uint32_t* out, int64_t n) {
+    const or_grid* g = &bm->grid;
+    if (!g->cells || n != (int64_t)g->res[0] * g->res[1] * g->res[2]) return UPBP_EINVAL;
+    for (int64_t c = 0; c < n; ++c) out[c] = g->cells[c + 1] - g->cells[c];
+    return 0;
+}
+
 /* ------------------------------------------------------------------------- */
 /* Vertex-merging hashed grid (SmallUPBP/src/Misc/HashGrid.hxx:38-230), FP32 throughout:
  * the grid decides which vertices a camera vertex visits, so it is restated in the
@@ -1015,6 +1030,42 @@ int or_beams_eval(const or_beams* bm, const upbp_medium* media, int32_t n_media,
         r = beams_eval_all_f32(bm, m, n_media, q, accel, threads, out32, sig, counters);
     free(m);
     return r;
+}
+
+/* GridStats per query (GridStats.hxx:37-54): Grid::intersect's counts (Grid.hxx:365,378-382)
+ * summed over the query's segments in a scattering medium, with the walk over segments of
+ * PhotonBeamsEvaluator::evalBeamBeamEstimate (PhBeams.cxx:177-230): the float attenuation
+ * product (divided by the ray-sample pdf for a SHORT query) ends the walk once a channel is
+ * not above 1e-20 (Rgb::isPositive, Utils.hxx:109-111). */
+int or_beam_grid_stats(const or_beams* bm, const upbp_medium* media, int32_t n_media, const upbp_query_batch* q,
+                       int threads, uint32_t* out) {
+    int e = check_batch(q, n_media);
+    if (e) return e;
+    if (!bm->grid.cells || !out) return UPBP_EINVAL;
+    or_medium* m = (or_medium*)malloc(sizeof(or_medium) * (size_t)n_media);
+    or_prepare_media(media, n_media, m);
+    if (threads <= 0) threads = 1;
+    const int short_q = q->beam_type == UPBP_SHORT_BEAM;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(threads)
+    for (int64_t qi = 0; qi < q->count; ++qi) {
+        uint32_t gs[3] = {0, 0, 0};
+        float att[3] = {1.0f, 1.0f, 1.0f};
+        const float* o = &q->origin[3 * qi];
+        const float* d = &q->direction[3 * qi];
+        for (int64_t si = q->segment_offsets[qi]; si < q->segment_offsets[qi + 1]; ++si) {
+            const upbp_volume_segment* sg = &q->segments[si];
+            if (m[sg->medium_id].has_scattering)
+                grid_walk_f32(bm, o, d, sg->dist_min, sg->dist_max, 0, NULL, NULL, gs);
+            for (int c = 0; c < 3; ++c)
+                att[c] = att[c] * (short_q ? sg->attenuation[c] / sg->ray_sample_pdf : sg->attenuation[c]);
+            if (!(att[0] > 1e-20f && att[1] > 1e-20f && att[2] > 1e-20f)) break;
+        }
+        out[3 * qi] = gs[0];
+        out[3 * qi + 1] = gs[1];
+        out[3 * qi + 2] = gs[2];
+    }
+    free(m);
+    return 0;
 }
 
 /* ------------------------------------------------------------------------- */

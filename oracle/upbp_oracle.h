@@ -110,6 +110,10 @@ int or_beams_set_reduction(or_beams* bm, uint32_t max_beams_in_cell, uint32_t re
 float or_beams_selection_pdf(const or_beams* bm, const float* pos);
 /* Grid statistics: res[3], cells, pointers (registered beam references). */
 void or_beams_grid_info(const or_beams* bm, int64_t* info6);
+/* The grid's box lo[3], hi[3] and cell size[3]; the beams registered in each of its
+ * res[0] res[1] res[2] cells (x fastest; n must be that product). */
+void or_beams_grid_geometry(const or_beams* bm, float* out9);
+int or_beams_grid_cell_counts(const or_beams* bm, uint32_t* out, int64_t n);
 
 /* PhotonBeamsEvaluator::evalBeamBeamEstimate (PhBeams.cxx:177-241, :257-324)
  * with PhotonBeam::accumulate (PhotonBeam.hxx:315-367).
@@ -117,6 +121,13 @@ void or_beams_grid_info(const or_beams* bm, int64_t* info6);
 int or_beams_eval(const or_beams* bm, const upbp_medium* media, int32_t n_media,
                   const upbp_query_batch* q, int accel, int precision, int threads,
                   float* out32, double* out64, uint32_t* sig, uint64_t* counters);
+
+/* GridStats of Grid::intersect per query over its scattering segments (GridStats.hxx:37-54,
+ * Grid.hxx:318-399, PhBeams.cxx:177-230): out[3 q] beams tested (a reduced cell counts
+ * max_beams_in_cell), out[3 q + 1] cells crossed, out[3 q + 2] reduced (overfull) cells
+ * crossed. Needs grid_res > 0. */
+int or_beam_grid_stats(const or_beams* bm, const upbp_medium* media, int32_t n_media, const upbp_query_batch* q,
+                       int threads, uint32_t* out);
 
 /* ---- UPBP mode ----------------------------------------------------------- */
 /* evalBre / evalBeamBeamEstimate with MIS data: breIntersectFuncHomogeneous2

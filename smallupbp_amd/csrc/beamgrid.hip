@@ -178,6 +178,144 @@ __global__ void k_selection_pdfs(int64_t n, RedDev R, const float* pos, float* o
     out[i] = pdf;
 }
 
+// ---- GridStats: the reference grid's walk per query ---------------------------------------
+
+struct GridWalkDev {
+    float lo[3], hi[3], cell[3], inv_cell[3];
+    int32_t res[3];
+    const uint32_t* off;  // [cells + 1]
+    uint32_t max;         // cells holding more beams are reduced (overfull); 0: none
+};
+
+// BoundingBox3::intersect (SmallUPBP/src/Structs/BoundingBox.hxx:218-271): the SSE slab test,
+// whose min / max against +-inf send a NaN slab value to the bound that keeps the box.
+__device__ __forceinline__ bool grid_box_intersect(const GridWalkDev& G, const float o[3], const float inv[3],
+                                                   float& tmin, float& tmax) {
+    float mx[3], mn[3];
+    for (int a = 0; a < 3; ++a) {
+        const float l1 = (G.lo[a] - o[a]) * inv[a];
+        const float l2 = (G.hi[a] - o[a]) * inv[a];
+        const float l1a = (l1 < INFINITY) ? l1 : INFINITY, l2a = (l2 < INFINITY) ? l2 : INFINITY;
+        const float l1b = (l1 > -INFINITY) ? l1 : -INFINITY, l2b = (l2 > -INFINITY) ? l2 : -INFINITY;
+        mx[a] = (l1a > l2a) ? l1a : l2a;
+        mn[a] = (l1b < l2b) ? l1b : l2b;
+    }
+    float lmax = mx[0], lmin = mn[0];
+    if (mx[1] < lmax) lmax = mx[1];
+    if (mn[1] > lmin) lmin = mn[1];
+    if (mx[2] < lmax) lmax = mx[2];
+    if (mn[2] > lmin) lmin = mn[2];
+    if (!(lmax >= lmin)) return false;
+    tmin = lmin;
+    tmax = lmax;
+    return true;
+}
+
+// Grid::intersect's 3D-DDA (SmallUPBP/src/Beams/Grid.hxx:318-399) in its FP32 operations,
+// gathering only its GridStats (:365,378-382): the beams each crossed cell holds (max of an
+// overfull one), the cells, the overfull cells. Every step moves one axis' cell index toward
+// that axis' exit value, so a walk takes at most res[0] + res[1] + res[2] steps.
+__device__ void grid_stats_walk(const GridWalkDev& G, const float o[3], const float d[3], float mint, float maxt,
+                                uint32_t gs[3]) {
+    const float inv[3] = {1.0f / d[0], 1.0f / d[1], 1.0f / d[2]};
+    float t0, t1;
+    if (!grid_box_intersect(G, o, inv, t0, t1)) return;
+    if (t0 < mint) t0 = mint;
+    if (t1 > maxt) t1 = maxt;
+    if (!(t1 > t0)) return;
+    float delta[3], enter[3], l[3];
+    int ipos[3], step[3], check[3];
+    const int64_t stride[3] = {1, (int64_t)G.res[0], (int64_t)G.res[0] * G.res[1]};
+    for (int a = 0; a < 3; ++a) {
+        delta[a] = G.cell[a] * inv[a];
+        enter[a] = ((o[a] + d[a] * t0) - G.lo[a]) * G.inv_cell[a];
+        const int v = (int)enter[a];
+        ipos[a] = v < 0 ? 0 : (v > G.res[a] - 1 ? G.res[a] - 1 : v);
+    }
+    int64_t cell = (int64_t)ipos[0] + (int64_t)G.res[0] * ((int64_t)ipos[1] + (int64_t)G.res[1] * ipos[2]);
+    int shift[3];
+    for (int a = 0; a < 3; ++a) {
+        if (d[a] >= 0.0f) {
+            shift[a] = 1;
+            check[a] = G.res[a];
+            step[a] = 1;
+        } else {
+            shift[a] = 0;
+            check[a] = -1;
+            step[a] = -1;
+            delta[a] = -delta[a];
+        }
+    }
+    for (int a = 0; a < 3; ++a) l[a] = (((float)(ipos[a] + shift[a]) - enter[a]) * inv[a]) * G.cell[a];
+    float t = t0;
+    do {
+        const uint32_t n = G.off[cell + 1] - G.off[cell];
+        const bool over = G.max != 0 && n > G.max;
+        gs[0] += over ? G.max : n;
+        gs[1] += 1;
+        gs[2] += over ? 1u : 0u;
+        const int m = (l[0] < l[1]) ? ((l[0] < l[2]) ? 0 : 2) : ((l[1] < l[2]) ? 1 : 2);
+        t += l[m];
+        const float lm = l[m];
+        for (int a = 0; a < 3; ++a) l[a] -= lm;
+        l[m] = delta[m];
+        cell += stride[m] * step[m];
+        ipos[m] += step[m];
+        if (ipos[m] == check[m]) break;
+    } while (t < t1);
+}
+
+// One query per thread: the walk over its segments in a scattering medium, as
+// PhotonBeamsEvaluator::evalBeamBeamEstimate (SmallUPBP/src/Beams/PhBeams.cxx:177-230) calls
+// Grid::intersect, ending once a channel of the FP32 attenuation product (divided by the
+// ray-sample pdf for SHORT queries) is not above 1e-20 (Rgb::isPositive).
+__global__ void k_grid_stats(int64_t nq, GridWalkDev G, const float* origin, const float* direction,
+                             const int64_t* offsets, const upbp_volume_segment* seg, const MediumDev* media,
+                             int32_t n_media, int32_t short_q, uint32_t* out, uint32_t* err) {
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    uint32_t gs[3] = {0u, 0u, 0u};
+    const int64_t s0 = offsets[q], s1 = offsets[q + 1], sn = offsets[nq];
+    if (!(s0 >= 0 && s0 <= s1 && s1 <= sn)) {
+        atomicOr(err, kErrBadOffsets);
+    } else {
+        const float o[3] = {origin[3 * q], origin[3 * q + 1], origin[3 * q + 2]};
+        const float d[3] = {direction[3 * q], direction[3 * q + 1], direction[3 * q + 2]};
+        float att[3] = {1.0f, 1.0f, 1.0f};
+        for (int64_t i = s0; i < s1; ++i) {
+            const upbp_volume_segment& sg = seg[i];
+            if (sg.medium_id < 0 || sg.medium_id >= n_media) {
+                atomicOr(err, kErrBadMedium);
+                break;
+            }
+            if (media[sg.medium_id].has_scattering) grid_stats_walk(G, o, d, sg.dist_min, sg.dist_max, gs);
+            for (int c = 0; c < 3; ++c)
+                att[c] = att[c] * (short_q ? sg.attenuation[c] / sg.ray_sample_pdf : sg.attenuation[c]);
+            if (!(att[0] > 1e-20f && att[1] > 1e-20f && att[2] > 1e-20f)) break;
+        }
+    }
+    out[3 * q] = gs[0];
+    out[3 * q + 1] = gs[1];
+    out[3 * q + 2] = gs[2];
+}
+
+// Per-beam origin, direction and length from the pieces (every valid beam has at least one,
+// and all of a beam's pieces hold the same values, so concurrent writes agree).
+__global__ void k_beams_from_pieces(int64_t np, const PieceDev* pieces, const uint32_t* piece_beam, float* origin,
+                                    float* dir, float* length) {
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const PieceDev pc = pieces[p];
+    const int64_t b = piece_beam[p];
+    origin[3 * b] = pc.p0.x;
+    origin[3 * b + 1] = pc.p0.y;
+    origin[3 * b + 2] = pc.p0.z;
+    dir[3 * b] = pc.p1.x;
+    dir[3 * b + 1] = pc.p1.y;
+    dir[3 * b + 2] = pc.p1.z;
+    length[b] = pc.p0.w;
+}
+
 unsigned blocks_for(int64_t n) { return (unsigned)((n + kGridBlock - 1) / kGridBlock); }
 
 int bits_for(uint64_t v) {
@@ -189,10 +327,11 @@ int bits_for(uint64_t v) {
 }  // namespace
 
 void beam_reduction_build(Ctx& ctx, int64_t n, const float* origin, const float* dir, const float* length,
-                          const float* r_start, const float* r_end, const upbp_build_params& params, Beams& out) {
+                          const float* r_start, const float* r_end, const upbp_build_params& params, Beams& out,
+                          bool always) {
     out.red = RedDev{};
     out.red_cells = 0;
-    if (params.max_beams_in_cell == 0) return;
+    if (params.max_beams_in_cell == 0 && !always) return;
     UPBP_REQUIRE(params.reduction_type <= UPBP_REDUCE_RESAMPLE, UPBP_EINVAL,
                  "reduction_type must be one of UPBP_REDUCE_*");
     const int res = params.grid_resolution > 0 ? params.grid_resolution : 256;
@@ -225,6 +364,8 @@ void beam_reduction_build(Ctx& ctx, int64_t n, const float* origin, const float*
         G.res[a] = (int)(unsigned)(ext[a] / size) + 1;
         const float cell = ext[a] / (float)G.res[a];
         G.inv_cell[a] = 1.0f / cell;
+        out.grid_cell[a] = cell;
+        out.grid_hi[a] = key_float(words[3 + a]);
     }
     G.n_cells = (int64_t)(G.res[0] + 1) * (G.res[1] + 1) * (G.res[2] + 1) + 1;
     UPBP_REQUIRE(G.n_cells < (int64_t)0x7FFFFFFF, UPBP_EINVAL, "beam reduction grid has too many cells");
@@ -347,6 +488,59 @@ void beam_selection_pdfs(const Beams& b, const float* pos, int64_t n, float* out
     UPBP_HIP(hipGetLastError());
     if (mem == UPBP_MEM_HOST) UPBP_HIP(hipMemcpyAsync(out, o, n * 4, hipMemcpyDeviceToHost, st));
     UPBP_HIP(hipStreamSynchronize(st));
+}
+
+void beam_grid_stats(Ctx& ctx, const Beams& b, const QueryView& q, uint32_t* out) {
+    if (q.count == 0) return;
+    hipStream_t st = ctx.stream;
+    const Beams* g = &b;
+    Beams tmp;
+    if (b.red_cells == 0 && b.n > 0 && b.pieces > 0) {
+        // no grid kept (no reduction): build one of the beams' resolution for this call
+        DevBuf o, d, len;
+        o.ensure(b.n * 12);
+        d.ensure(b.n * 12);
+        len.ensure(b.n * 4);
+        hipLaunchKernelGGL(k_beams_from_pieces, dim3(blocks_for(b.pieces)), dim3(kGridBlock), 0, st, b.pieces,
+                           b.piece.as<PieceDev>(), b.piece_beam.as<uint32_t>(), o.as<float>(), d.as<float>(),
+                           len.as<float>());
+        UPBP_HIP(hipGetLastError());
+        upbp_build_params p{};
+        p.grid_resolution = b.grid_resolution;
+        p.max_beams_in_cell = 0;
+        p.reduction_type = UPBP_REDUCE_RESAMPLE;  // no permutation ranks needed
+        beam_reduction_build(ctx, b.n, o.as<float>(), d.as<float>(), len.as<float>(), b.r_start.as<float>(),
+                             b.r_end.as<float>(), p, tmp, true);
+        g = &tmp;
+    }
+    if (g->red_cells == 0) {
+        // no beams: Grid::intersect misses the empty grid box
+        launch_fill_u32(ctx, out, 0u, 3 * q.count);
+        UPBP_HIP(hipStreamSynchronize(st));
+        return;
+    }
+    GridWalkDev G;
+    for (int a = 0; a < 3; ++a) {
+        G.lo[a] = g->red.lo[a];
+        G.hi[a] = g->grid_hi[a];
+        G.cell[a] = g->grid_cell[a];
+        G.inv_cell[a] = g->red.inv[a];
+        G.res[a] = g->red.res[a];
+    }
+    G.off = g->red.off;
+    G.max = b.red.max;
+    DevBuf err;
+    err.ensure(4);
+    launch_fill_u32(ctx, err.as<uint32_t>(), 0u, 1);
+    hipLaunchKernelGGL(k_grid_stats, dim3(blocks_for(q.count)), dim3(kGridBlock), 0, st, q.count, G, q.origin,
+                       q.direction, q.offsets, q.segments, ctx.media.as<MediumDev>(), (int32_t)ctx.n_media,
+                       (int32_t)(q.beam_type == UPBP_SHORT_BEAM), out, err.as<uint32_t>());
+    UPBP_HIP(hipGetLastError());
+    uint32_t e = 0;
+    UPBP_HIP(hipMemcpyAsync(&e, err.p, 4, hipMemcpyDeviceToHost, st));
+    UPBP_HIP(hipStreamSynchronize(st));
+    UPBP_REQUIRE(!(e & kErrBadOffsets), UPBP_EINVAL, "segment_offsets out of order or out of range");
+    UPBP_REQUIRE(!(e & kErrBadMedium), UPBP_EINVAL, "segment medium outside the media table");
 }
 
 }  // namespace upbp

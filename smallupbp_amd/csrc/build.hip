@@ -325,6 +325,7 @@ void beams_build(Ctx& ctx, const upbp_beam_input& in, const upbp_build_params& p
     out.ctx = &ctx;
     out.n = in.count;
     out.pieces = 0;
+    out.grid_resolution = params.grid_resolution;
     if (in.count == 0) {
         beam_reduction_build(ctx, 0, nullptr, nullptr, nullptr, nullptr, nullptr, params, out);
         return;

@@ -658,6 +658,36 @@ upbp_status upbp_beams_selection_pdfs(const upbp_beams* beams, const float* posi
     });
 }
 
+upbp_status upbp_beams_grid_stats(upbp_ctx* ctx, const upbp_beams* beams, const upbp_query_batch* queries,
+                                  uint32_t* out_stats, upbp_memory out_memory) {
+    if (!ctx) return fail(nullptr, UPBP_EINVAL, "ctx is NULL");
+    return guarded(ctx, [&] {
+        UPBP_REQUIRE(beams != nullptr, UPBP_EINVAL, "beams is NULL");
+        UPBP_REQUIRE(beams->ctx == ctx, UPBP_EINVAL, "beams belong to another context");
+        check_batch(*ctx, queries);
+        const int64_t Q = queries->count;
+        UPBP_REQUIRE(Q == 0 || out_stats, UPBP_EINVAL, "out_stats is NULL");
+        UPBP_REQUIRE(out_memory == UPBP_MEM_HOST || out_memory == UPBP_MEM_DEVICE, UPBP_EINVAL, "bad memory kind");
+        if (Q == 0) return;
+        // one thread per query in the given order: no sort, no workspace
+        upbp_query_batch qb = *queries;
+        qb.order = UPBP_ORDER_AS_GIVEN;
+        const float none[3] = {0.f, 0.f, 0.f};
+        const QueryView v = make_view(*ctx, &qb, none, none, Workspace{});
+        DevBuf tmp;
+        uint32_t* dout = out_stats;
+        if (out_memory == UPBP_MEM_HOST) {
+            tmp.ensure((size_t)Q * 12);
+            dout = tmp.as<uint32_t>();
+        }
+        beam_grid_stats(*ctx, *beams, v, dout);
+        if (out_memory == UPBP_MEM_HOST) {
+            UPBP_HIP(hipMemcpyAsync(out_stats, dout, (size_t)Q * 12, hipMemcpyDeviceToHost, ctx->stream));
+            UPBP_HIP(hipStreamSynchronize(ctx->stream));
+        }
+    });
+}
+
 upbp_status upbp_eval(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams* beams, const upbp_query_batch* queries,
                       float* out_photons, float* out_beams, upbp_memory out_memory, upbp_eval_stats* stats) {
     if (!ctx) return fail(nullptr, UPBP_EINVAL, "ctx is NULL");

@@ -152,6 +152,10 @@ struct Beams {
     RedDev red;            // device view; red.max == 0 without reduction
     int64_t red_cells = 0;
     DevBuf red_off, red_ids, red_rank;
+    // the grid's box top and cell size (with red_cells > 0), and the resolution the build was
+    // given (0 = 256): what a walk of the grid needs besides RedDev
+    float grid_hi[3] = {0.f, 0.f, 0.f}, grid_cell[3] = {0.f, 0.f, 0.f};
+    int32_t grid_resolution = 0;
 };
 
 // Device memory one evaluation works in besides its inputs and outputs: the kernels' counters,
@@ -222,8 +226,14 @@ void beams_build(Ctx& ctx, const upbp_beam_input& in, const upbp_build_params& p
 // ---- beamgrid.hip ---------------------------------------------------------
 // Registers the beams (device arrays, input order) in the reference's uniform grid and
 // keeps each cell's beam list and reduction ranks in out.red*; a no-op without reduction.
+// With `always`, the grid is built even without reduction (red.max stays 0).
 void beam_reduction_build(Ctx& ctx, int64_t n, const float* origin, const float* dir, const float* length,
-                          const float* r_start, const float* r_end, const upbp_build_params& params, Beams& out);
+                          const float* r_start, const float* r_end, const upbp_build_params& params, Beams& out,
+                          bool always = false);
+// Per-query GridStats of the reference grid's walk (beams tested, cells crossed, overfull
+// cells crossed) into out [q.count][3] on the device (query order as given). Beams built
+// without reduction get a grid of their build's resolution for the call.
+void beam_grid_stats(Ctx& ctx, const Beams& b, const QueryView& q, uint32_t* out);
 // Selection pdf of the cell holding pos (1 without reduction).
 float beam_selection_pdf(const Beams& b, const float pos[3]);
 // The same for n positions (xyz [n][3]) into out [n], both in `mem`.

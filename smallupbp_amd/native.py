@@ -28,7 +28,8 @@ EXPORTED = [
     "upbp_ctx_create", "upbp_ctx_destroy", "upbp_ctx_last_error", "upbp_ctx_set_stream",
     "upbp_ctx_get_stream", "upbp_ctx_set_sync", "upbp_set_media", "upbp_bre_build", "upbp_bre_destroy",
     "upbp_bre_set_power", "upbp_bre_count", "upbp_beams_build", "upbp_beams_destroy", "upbp_beams_count",
-    "upbp_beams_piece_count", "upbp_beams_selection_pdf", "upbp_beams_selection_pdfs", "upbp_eval", "upbp_eval_noalloc", "upbp_hit_signatures",
+    "upbp_beams_piece_count", "upbp_beams_selection_pdf", "upbp_beams_selection_pdfs", "upbp_beams_grid_stats",
+    "upbp_eval", "upbp_eval_noalloc", "upbp_hit_signatures",
     "upbp_adjoint", "upbp_batch_plan_create", "upbp_batch_plan_destroy", "upbp_batch_run",
     "upbp_set_tuning", "upbp_get_tuning", "upbp_autotune", "upbp_eval_upbp", "upbp_bre_radii",
     "upbp_beams_radii", "upbp_hash_grid_build", "upbp_hash_grid_destroy", "upbp_hash_grid_count",
@@ -100,6 +101,8 @@ def load():
     L.upbp_beams_selection_pdf.argtypes = [vp, vp]
     L.upbp_beams_selection_pdfs.restype = C.c_int
     L.upbp_beams_selection_pdfs.argtypes = [vp, vp, C.c_int64, vp, C.c_int]
+    L.upbp_beams_grid_stats.restype = C.c_int
+    L.upbp_beams_grid_stats.argtypes = [vp, vp, C.POINTER(_abi.QueryBatch), vp, C.c_int]
     L.upbp_eval.restype = st
     L.upbp_eval.argtypes = [vp, vp, vp, C.POINTER(_abi.QueryBatch), vp, vp, C.c_int, C.POINTER(_abi.EvalStats)]
     L.upbp_eval_noalloc.restype = st
@@ -360,6 +363,14 @@ class PhotonBeamsEvaluator:
             out = np.empty(p.shape[0], np.float32)
             mem = _abi.MEM_HOST
         self.ctx.check(self.ctx.lib.upbp_beams_selection_pdfs(self.h, _abi.ptr(p), p.shape[0], _abi.ptr(out), mem))
+        return out
+
+    def grid_stats(self, queries: QueryBatchView):
+        """GridStats of the reference grid's walk per query (Grid.hxx:318-399, PhBeams.cxx:177-230)
+        -> host [Q,3] u32: beams tested, cells crossed, overfull cells crossed."""
+        out = np.zeros((queries.count, 3), np.uint32)
+        self.ctx.check(self.ctx.lib.upbp_beams_grid_stats(self.ctx.h, self.h, C.byref(queries.qb), _abi.ptr(out),
+                                                          _abi.MEM_HOST))
         return out
 
     def evalBeamBeamEstimate(self, queries: QueryBatchView, out=None, stats=None):
