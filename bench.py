@@ -574,8 +574,7 @@ def light_trace_measure(steps: int, cpu_budget: float, threads: int, paths: int 
 
     from smallupbp_amd import _abi, native, scenes
 
-    sc = scenes.light_scene(media_rows=LIGHT_TRACE_MEDIA, max_path_length=10, bb1d_min_mfp=0.1, seed=12,
-                            beam_path_count=beam_paths)
+    sc = scenes.light_scene(media_rows=LIGHT_TRACE_MEDIA, max_path_length=10, bb1d_min_mfp=0.1, seed=12)
     ctx = native.Context(0)
     ctx.set_media(sc.media)
     cap_v, cap_b = scenes.light_path_capacity(sc, paths)

@@ -121,8 +121,8 @@ typedef struct upbp_build_params {
                                    with UPBP_KNN_RADIUS the multiplier of 2 sqrt(d2_knn) */
     int32_t knn;                /* UPBP_KNN_RADIUS: which nearest neighbour (>= 1, the item itself
                                    counts as the first) sets the radius */
-    int32_t leaf_size;          /* max items per LBVH leaf, 1..32; 0 = library default */
-    float beam_piece_factor;    /* beams only: target piece length in units of the max beam radius; 0 = default */
+    int32_t leaf_size;          /* max items per LBVH leaf, 1..32; 0 = library default (8) */
+    float beam_piece_factor;    /* beams only: target piece length in units of the max beam radius; 0 = default (8) */
     int32_t max_pieces_per_beam;/* beams only: cap on pieces per beam, 1..256; 0 = default */
     /* beams only: per-cell beam reduction of the reference's uniform grid (-gridmax / -gridred,
      * Grid.hxx:167-279,409-631; upbp_eval, upbp_hit_signatures and upbp_eval_upbp). A hit whose

@@ -135,7 +135,7 @@ def check_beam_ties(queries, beams, sig_gpu, sig_ref):
     return ties
 
 
-def beam_mismatch_report(w, queries, sig_gpu, sig_ref, piece_factor=4.0, max_pieces=64, limit=6):
+def beam_mismatch_report(w, queries, sig_gpu, sig_ref, piece_factor=8.0, max_pieces=64, limit=6):
     """Names the beams behind differing beam hit signatures: for each differing query, the beam
     whose hash accounts for the difference (one beam missing or extra), with its closest-approach
     parameters, the radius margin and the piece holding t2. Diagnostics only."""

@@ -100,7 +100,7 @@ def test_bre_beams_1080p_full_size():
     qs = w.queries.subset(jdx)
     ref, rsig, cnt = po.beams_eval(po.OracleBeams(w.beams, grid_res=0), w.media, qs, accel=0, signatures=True)
     _check_sample("beams", ob[jdx], sb[jdx], ref, rsig, cnt,
-                  lambda g, r: beam_mismatch_report(w, qs, g, r, piece_factor=p["piece_factor"] or 4.0),
+                  lambda g, r: beam_mismatch_report(w, qs, g, r, piece_factor=p["piece_factor"] or 8.0),
                   lambda g, r: check_beam_ties(qs, w.beams, g, r))
 
 
@@ -123,7 +123,7 @@ def test_beams_stress_full_size():
     qs = w.queries.subset(idx)
     ref, rsig, cnt = po.beams_eval(po.OracleBeams(w.beams, grid_res=0), w.media, qs, accel=0, signatures=True)
     _check_sample("beams", ob[idx], sb[idx], ref, rsig, cnt,
-                  lambda g, r: beam_mismatch_report(w, qs, g, r, piece_factor=p["piece_factor"] or 4.0),
+                  lambda g, r: beam_mismatch_report(w, qs, g, r, piece_factor=p["piece_factor"] or 8.0),
                   lambda g, r: check_beam_ties(qs, w.beams, g, r))
 
 
