@@ -676,6 +676,8 @@ def main():
     ap.add_argument("--leaf-size", type=int, default=0)
     ap.add_argument("--piece-factor", type=float, default=0.0)
     ap.add_argument("--waves-per-block", type=int, default=0)
+    ap.add_argument("--photon-waves-per-block", type=int, default=0,
+                    help="waves per block of the photon kernel alone (default: --waves-per-block, else tuned)")
     ap.add_argument("--no-tuning", action="store_true",
                     help="ignore smallupbp_amd/tuning_cache.json and use the library defaults")
     ap.add_argument("--batched", type=float, default=1.0,
@@ -712,7 +714,7 @@ def main():
     leaf_bm = args.leaf_size or (tuned["beam_leaf_size"] if tuned else 0)
     piece_factor = args.piece_factor or (tuned["piece_factor"] if tuned else 0.0)
     waves = args.waves_per_block or (tuned["beam_waves_per_block"] if tuned else 0)
-    waves_ph = args.waves_per_block or (tuned["photon_waves_per_block"] if tuned else 0)
+    waves_ph = args.photon_waves_per_block or args.waves_per_block or (tuned["photon_waves_per_block"] if tuned else 0)
     if args.order == "tuned":
         args.order = "as_given" if tuned and tuned["order"] == _abi.ORDER_AS_GIVEN else "sorted"
     ctx = native.Context(0)

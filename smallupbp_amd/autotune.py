@@ -5,7 +5,8 @@ workload-size bucket:
   build-time: LBVH leaf size (items per leaf tile, 1..32) for photons and for
               beam pieces, and the beam piece length (in units of the beam radius);
   launch-time: waves per workgroup (1, 2, 4) and the query order (Morton-sorted
-              bundles or the caller's order).
+              bundles or the caller's order); with both estimators, the photon kernel's
+              waves per workgroup once more on the joint (concurrent) step.
 Times come from the library's HIP events around each traversal kernel (median of
 `reps` runs). Results are cached in a JSON file keyed by
 (log2 photons, log2 beams, log2 queries) so later runs of similar size reuse them.
@@ -74,6 +75,22 @@ def _median_kernel_ms(ctx, bre, beams, qv, reps, which):
     return float(np.median(ts))
 
 
+def _median_step_ms(ctx, bre, beams, qv, reps):
+    """Median wall time of a whole eval call (both kernels, synchronised) over `reps` runs."""
+    import time
+
+    import torch
+
+    ts = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        native.eval_batch(ctx, bre, beams, qv)
+        torch.cuda.synchronize()
+        ts.append((time.perf_counter() - t) * 1e3)
+    return float(np.median(ts))
+
+
 def tune(w: scenes.Workload, reps: int = 3, path: Optional[str] = DEFAULT_CACHE, log=print) -> dict:
     """Sweep the parameter space for workload `w`; returns (and caches) the best configuration."""
     ctx = native.Context(0)
@@ -110,6 +127,27 @@ def tune(w: scenes.Workload, reps: int = 3, path: Optional[str] = DEFAULT_CACHE,
                 if best["beams"] is None or ms < best["beams"]["ms"]:
                     best["beams"] = cfg
             bm.destroy()
+    if best["photons"] is not None and best["beams"] is not None and best["photons"]["order"] == best["beams"]["order"]:
+        # One eval call runs the two kernels concurrently on two streams, and how their blocks
+        # share the CUs depends on both launch shapes: a photon shape that ties with others alone
+        # can lose 10 % of the joint step. Re-choose the photon waves per block on the joint step.
+        bre = native.EmbreeBre(ctx)
+        bre.build(w.photons, leaf_size=best["photons"]["leaf_size"])
+        bm = native.PhotonBeamsEvaluator(ctx)
+        bm.build(w.beams, leaf_size=best["beams"]["leaf_size"], piece_factor=best["beams"]["piece_factor"])
+        qv = qviews[best["beams"]["order"]]
+        joint = []
+        for wpb in WAVES:
+            ctx.set_tuning(best["beams"]["waves_per_block"], 0, wpb)
+            native.eval_batch(ctx, bre, bm, qv)  # warm
+            ms = _median_step_ms(ctx, bre, bm, qv, reps)
+            joint.append({"photon_waves_per_block": wpb, "ms": ms})
+            log("joint step %s" % joint[-1])
+        pick = min(joint, key=lambda j: j["ms"])
+        best["photons"] = dict(best["photons"], waves_per_block=pick["photon_waves_per_block"], joint_ms=pick["ms"])
+        results["joint"] = joint
+        bre.destroy()
+        bm.destroy()
     key = bucket(w.photons.count if w.photons is not None else 0, w.beams.count if w.beams is not None else 0,
                  w.queries.count)
     entry = {"workload": w.name, "best": best, "all": results}
