@@ -297,6 +297,19 @@ upbp_status upbp_beams_selection_pdfs(const upbp_beams* beams, const float* posi
 upbp_status upbp_beams_grid_stats(upbp_ctx* ctx, const upbp_beams* beams, const upbp_query_batch* queries,
                                   uint32_t* out_stats, upbp_memory out_memory);
 
+/* Replaces EmbreeBre::evalBreBruteForce (SmallUPBP/src/Bre/Bre.cxx:980-1029), the reference's
+ * debug estimate. Per query i: the ray origin[i], direction[i] (xyz each), the interval
+ * (min_t[i], max_t[i]) and the medium medium_id[i]; every kept photon of the structure is
+ * tested with the PB2D disc test (Bre.cxx:199-223) and a hit adds
+ * flux exp(-sigma_t t) PhaseFunction::Evaluate(direction, incDir) / r^2; out[i] (RGB) is the
+ * sum times sigma_s / pi. Its kernel is the reference's debug one, not evalBre's: constant over
+ * the disc, attenuation from the ray origin, the stored incoming direction as given, photons
+ * of every medium. All arrays in `memory`; one wave per query walks all photons, so the call is
+ * meant for tests and debugging. UPBP_EINVAL for a medium outside the table. */
+upbp_status upbp_bre_eval_brute_force(upbp_ctx* ctx, const upbp_bre* bre, const float* origin, const float* direction,
+                                      const float* min_t, const float* max_t, const int32_t* medium_id, int64_t count,
+                                      float* out, upbp_memory memory);
+
 /* ---------------------------------------------------------------------------
  * Evaluation. out_photons receives, per query, EmbreeBre::evalBre
  * (Bre.cxx:812-879 full segments, :895-964 lite); out_beams receives

@@ -140,6 +140,18 @@ public:
         check(upbp_eval(ctx_.get(), h_, nullptr, &q, outRgb, nullptr, UPBP_MEM_HOST, stats), ctx_.get());
     }
 
+    // EmbreeBre::evalBreBruteForce (Bre.cxx:980-1029): every photon against [minT, maxT] of one
+    // ray in medium mediumId, with the reference's debug kernel. beamType is unused there too.
+    std::array<float, 3> evalBreBruteForce(uint32_t beamType, const float origin[3], const float direction[3],
+                                           float minT, float maxT, int32_t mediumId) {
+        (void)beamType;
+        std::array<float, 3> out{0.f, 0.f, 0.f};
+        check(upbp_bre_eval_brute_force(ctx_.get(), h_, origin, direction, &minT, &maxT, &mediumId, 1, out.data(),
+                                        UPBP_MEM_HOST),
+              ctx_.get());
+        return out;
+    }
+
     // EmbreeBre::destroy (Bre.cxx:762-777); also run by the destructor.
     void destroy() {
         if (h_) upbp_bre_destroy(h_);

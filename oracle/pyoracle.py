@@ -58,6 +58,8 @@ def lib():
         L.or_beams_count.argtypes = [vp]
         L.or_beams_grid_info.argtypes = [vp, vp]
         L.or_beam_grid_stats.restype = C.c_int
+        L.or_bre_eval_brute_force.restype = C.c_int
+        L.or_bre_eval_brute_force.argtypes = [vp, vp, i32, vp, vp, vp, vp, vp, i64, C.c_int, vp]
         L.or_beams_grid_geometry.argtypes = [vp, vp]
         L.or_beams_grid_cell_counts.restype = C.c_int
         L.or_beams_grid_cell_counts.argtypes = [vp, vp, i64]
@@ -215,6 +217,24 @@ def bre_eval(oph: OraclePhotons, media, q: Queries, accel=1, precision=32, threa
     if r:
         raise RuntimeError("or_bre_eval failed: %s" % _abi.STATUS_NAMES.get(r, r))
     return out, sig, cnt
+
+
+def bre_eval_brute_force(oph: OraclePhotons, media, origin, direction, min_t, max_t, medium_id, threads=None):
+    """EmbreeBre::evalBreBruteForce (Bre.cxx:980-1029) per ray -> [n,3] f32."""
+    threads = threads or _threads()
+    o = np.ascontiguousarray(origin, np.float32).reshape(-1, 3)
+    d = np.ascontiguousarray(direction, np.float32).reshape(-1, 3)
+    n = o.shape[0]
+    t0 = np.ascontiguousarray(min_t, np.float32).reshape(n)
+    t1 = np.ascontiguousarray(max_t, np.float32).reshape(n)
+    m = np.ascontiguousarray(medium_id, np.int32).reshape(n)
+    mm, mp, nm = _media_ptr(media)
+    out = np.zeros((n, 3), np.float32)
+    r = lib().or_bre_eval_brute_force(oph.h, mp, nm, o.ctypes.data, d.ctypes.data, t0.ctypes.data, t1.ctypes.data,
+                                      m.ctypes.data, n, threads, out.ctypes.data)
+    if r:
+        raise RuntimeError("or_bre_eval_brute_force failed: %s" % _abi.STATUS_NAMES.get(r, r))
+    return out
 
 
 def beam_grid_stats(obm: OracleBeams, media, q: Queries, threads=None):

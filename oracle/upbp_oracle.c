@@ -1032,6 +1032,42 @@ int or_beams_eval(const or_beams* bm, const upbp_medium* media, int32_t n_media,
     return r;
 }
 
+/* EmbreeBre::evalBreBruteForce, Bre.cxx:980-1029, in float as the reference: every kept photon
+ * against [min_t, max_t] with testIntersectionBre, each hit adding
+ * ((flux exp(-t sigma_t)) PhaseFunction::Evaluate(d, incDir)) / r^2 (attenuation from the ray
+ * origin, the stored incoming direction as given, no medium check of the photon), the sum times
+ * sigma_s and INV_PI_F of the query's medium. */
+int or_bre_eval_brute_force(const or_photons* ph, const upbp_medium* media, int32_t n_media, const float* origin,
+                            const float* direction, const float* min_t, const float* max_t, const int32_t* medium,
+                            int64_t count, int threads, float* out) {
+    if (count < 0 || n_media <= 0 || (count > 0 && (!origin || !direction || !min_t || !max_t || !medium || !out)))
+        return UPBP_EINVAL;
+    for (int64_t q = 0; q < count; ++q)
+        if (medium[q] < 0 || medium[q] >= n_media) return UPBP_EINVAL;
+    or_medium* m = (or_medium*)malloc(sizeof(or_medium) * (size_t)n_media);
+    or_prepare_media(media, n_media, m);
+    if (threads <= 0) threads = 1;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads)
+    for (int64_t q = 0; q < count; ++q) {
+        const or_medium* md = &m[medium[q]];
+        const float* o = &origin[3 * q];
+        const float* d = &direction[3 * q];
+        float acc[3] = {0.f, 0.f, 0.f};
+        for (int64_t i = 0; i < ph->n; ++i) {
+            float t, s;
+            if (!test_bre_f32(o, d, min_t[q], max_t[q], &ph->pos[3 * i], ph->r2[i], &t, &s)) continue;
+            const float rho = phase_f32(d, &ph->inc[3 * i], md->mean_cosine);
+            for (int c = 0; c < 3; ++c) {
+                const float att = or_exp_cephes(-t * md->sigma_t[c]);
+                acc[c] += ((ph->power[3 * i + c] * att) * rho) / ph->r2[i];
+            }
+        }
+        for (int c = 0; c < 3; ++c) out[3 * q + c] = (acc[c] * md->sigma_s[c]) * INV_PI_F;
+    }
+    free(m);
+    return 0;
+}
+
 /* GridStats per query (GridStats.hxx:37-54): Grid::intersect's counts (Grid.hxx:365,378-382)
  * summed over the query's segments in a scattering medium, with the walk over segments of
  * PhotonBeamsEvaluator::evalBeamBeamEstimate (PhBeams.cxx:177-230): the float attenuation

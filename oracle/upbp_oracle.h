@@ -122,6 +122,14 @@ int or_beams_eval(const or_beams* bm, const upbp_medium* media, int32_t n_media,
                   const upbp_query_batch* q, int accel, int precision, int threads,
                   float* out32, double* out64, uint32_t* sig, uint64_t* counters);
 
+/* EmbreeBre::evalBreBruteForce (Bre.cxx:980-1029): per query [count] (origin, direction, [min_t,
+ * max_t], medium) -> out [count][3], every kept photon tested; its kernel is the reference's
+ * debug one (constant disc kernel 1 / (pi r^2), attenuation from the ray origin, phase of the
+ * stored incoming direction as given, no photon medium check). */
+int or_bre_eval_brute_force(const or_photons* ph, const upbp_medium* media, int32_t n_media, const float* origin,
+                            const float* direction, const float* min_t, const float* max_t, const int32_t* medium,
+                            int64_t count, int threads, float* out);
+
 /* GridStats of Grid::intersect per query over its scattering segments (GridStats.hxx:37-54,
  * Grid.hxx:318-399, PhBeams.cxx:177-230): out[3 q] beams tested (a reduced cell counts
  * max_beams_in_cell), out[3 q + 1] cells crossed, out[3 q + 2] reduced (overfull) cells

@@ -658,6 +658,39 @@ upbp_status upbp_beams_selection_pdfs(const upbp_beams* beams, const float* posi
     });
 }
 
+upbp_status upbp_bre_eval_brute_force(upbp_ctx* ctx, const upbp_bre* bre, const float* origin, const float* direction,
+                                      const float* min_t, const float* max_t, const int32_t* medium_id, int64_t count,
+                                      float* out, upbp_memory memory) {
+    if (!ctx) return fail(nullptr, UPBP_EINVAL, "ctx is NULL");
+    return guarded(ctx, [&] {
+        UPBP_REQUIRE(bre != nullptr, UPBP_EINVAL, "bre is NULL");
+        UPBP_REQUIRE(bre->ctx == ctx, UPBP_EINVAL, "bre belongs to another context");
+        UPBP_REQUIRE(count >= 0 && count < ((int64_t)1 << 31), UPBP_EINVAL, "query count out of range");
+        UPBP_REQUIRE(count == 0 || (origin && direction && min_t && max_t && medium_id && out), UPBP_EINVAL,
+                     "origin, direction, min_t, max_t, medium_id and out are required");
+        UPBP_REQUIRE(memory == UPBP_MEM_HOST || memory == UPBP_MEM_DEVICE, UPBP_EINVAL, "bad memory kind");
+        UPBP_REQUIRE(ctx->n_media > 0, UPBP_EINVAL, "upbp_set_media must be called before evaluation");
+        if (count == 0) return;
+        float* dout = out;
+        DevBuf tmp;
+        if (memory == UPBP_MEM_HOST) {
+            tmp.ensure((size_t)count * 12);
+            dout = tmp.as<float>();
+        }
+        if (bre->n == 0) {
+            launch_clear_f32(*ctx, dout, 3 * count);
+        } else {
+            bre_eval_brute_force(*ctx, *bre, stage_q(*ctx, 0, origin, 3 * count, memory),
+                                 stage_q(*ctx, 1, direction, 3 * count, memory), stage_q(*ctx, 2, min_t, count, memory),
+                                 stage_q(*ctx, 3, max_t, count, memory), stage_q(*ctx, 4, medium_id, count, memory),
+                                 count, dout);
+        }
+        if (memory == UPBP_MEM_HOST)
+            UPBP_HIP(hipMemcpyAsync(out, dout, (size_t)count * 12, hipMemcpyDeviceToHost, ctx->stream));
+        UPBP_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
 upbp_status upbp_beams_grid_stats(upbp_ctx* ctx, const upbp_beams* beams, const upbp_query_batch* queries,
                                   uint32_t* out_stats, upbp_memory out_memory) {
     if (!ctx) return fail(nullptr, UPBP_EINVAL, "ctx is NULL");

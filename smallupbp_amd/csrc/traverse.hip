@@ -1499,4 +1499,69 @@ void launch_combine_media_grads(Ctx& ctx, const float* gm, const float* sign, in
     UPBP_HIP(hipGetLastError());
 }
 
+// EmbreeBre::evalBreBruteForce (SmallUPBP/src/Bre/Bre.cxx:980-1029), the reference's debug
+// estimate: one wave per query, the lanes taking every 64th kept photon; a photon inside the
+// query's [min_t, max_t] disc test adds ((flux exp(-sigma_t t)) rho(d, inc)) / r^2, with the
+// attenuation from the ray origin and the phase of the stored incoming direction as given
+// (PhaseFunction::Evaluate(d, incDir)), and the wave's sum is scaled by sigma_s / pi.
+__global__ __launch_bounds__(256) void k_bre_brute(int64_t nq, int64_t np, const float4* __restrict__ pos_r2,
+                                                   const float4* __restrict__ power, const float4* __restrict__ inc,
+                                                   const float* origin, const float* direction, const float* min_t,
+                                                   const float* max_t, const int32_t* medium, const MediumDev* media,
+                                                   int32_t n_media, float* out, uint32_t* err) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t q = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave;  // wave-uniform
+    if (q >= nq) return;
+    const int32_t mid = medium[q];
+    if (mid < 0 || mid >= n_media) {
+        if (lane == 0) {
+            atomicOr(err, kErrBadMedium);
+            out[3 * q] = out[3 * q + 1] = out[3 * q + 2] = 0.f;
+        }
+        return;
+    }
+    const MediumDev& M = media[mid];
+    const float ox = origin[3 * q], oy = origin[3 * q + 1], oz = origin[3 * q + 2];
+    const float dx = direction[3 * q], dy = direction[3 * q + 1], dz = direction[3 * q + 2];
+    const float t0 = min_t[q], t1 = max_t[q];
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+    for (int64_t i = lane; i < np; i += kWave) {
+        const float4 p = pos_r2[i];
+        float t, sq;
+        if (test_bre(ox, oy, oz, dx, dy, dz, t0, t1, p.x, p.y, p.z, p.w, t, sq)) {
+            const float4 f = power[i];
+            const float4 in = inc[i];
+            const float rho = phase_pdf(M.g, M.isotropic != 0, -dot3(in.x, in.y, in.z, dx, dy, dz));
+            a0 += ((f.x * fast_exp(-t * M.st[0])) * rho) / p.w;
+            a1 += ((f.y * fast_exp(-t * M.st[1])) * rho) / p.w;
+            a2 += ((f.z * fast_exp(-t * M.st[2])) * rho) / p.w;
+        }
+    }
+    a0 = wave_sum_dpp(a0);
+    a1 = wave_sum_dpp(a1);
+    a2 = wave_sum_dpp(a2);
+    if (lane == 0) {
+        out[3 * q] = (a0 * M.ss[0]) * kInvPi;
+        out[3 * q + 1] = (a1 * M.ss[1]) * kInvPi;
+        out[3 * q + 2] = (a2 * M.ss[2]) * kInvPi;
+    }
+}
+
+void bre_eval_brute_force(Ctx& ctx, const Bre& b, const float* origin, const float* direction, const float* min_t,
+                          const float* max_t, const int32_t* medium, int64_t count, float* out) {
+    if (count == 0) return;
+    DevBuf err;
+    err.ensure(4);
+    launch_fill_u32(ctx, err.as<uint32_t>(), 0u, 1);
+    const int64_t threads = count * kWave;
+    hipLaunchKernelGGL(k_bre_brute, dim3(grid_for(threads)), dim3(256), 0, ctx.stream, count, b.n,
+                       b.pos_r2.as<float4>(), b.power.as<float4>(), b.inc.as<float4>(), origin, direction, min_t,
+                       max_t, medium, ctx.media.as<MediumDev>(), (int32_t)ctx.n_media, out, err.as<uint32_t>());
+    UPBP_HIP(hipGetLastError());
+    uint32_t e = 0;
+    UPBP_HIP(hipMemcpyAsync(&e, err.p, 4, hipMemcpyDeviceToHost, ctx.stream));
+    UPBP_HIP(hipStreamSynchronize(ctx.stream));
+    UPBP_REQUIRE(!(e & kErrBadMedium), UPBP_EINVAL, "query medium outside the media table");
+}
+
 }  // namespace upbp

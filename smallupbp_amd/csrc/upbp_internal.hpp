@@ -240,6 +240,10 @@ float beam_selection_pdf(const Beams& b, const float pos[3]);
 void beam_selection_pdfs(const Beams& b, const float* pos, int64_t n, float* out, upbp_memory mem);
 
 // ---- traverse.hip ---------------------------------------------------------
+// EmbreeBre::evalBreBruteForce per query (device arrays; out [count][3]).
+void bre_eval_brute_force(Ctx& ctx, const Bre& b, const float* origin, const float* direction, const float* min_t,
+                          const float* max_t, const int32_t* medium, int64_t count, float* out);
+
 enum class Mode : int { Forward = 0, Signature = 1, Adjoint = 2, Upbp = 3 };
 
 struct PhotonArgs {

@@ -27,7 +27,7 @@ _lib = None
 EXPORTED = [
     "upbp_ctx_create", "upbp_ctx_destroy", "upbp_ctx_last_error", "upbp_ctx_set_stream",
     "upbp_ctx_get_stream", "upbp_ctx_set_sync", "upbp_set_media", "upbp_bre_build", "upbp_bre_destroy",
-    "upbp_bre_set_power", "upbp_bre_count", "upbp_beams_build", "upbp_beams_destroy", "upbp_beams_count",
+    "upbp_bre_set_power", "upbp_bre_count", "upbp_bre_eval_brute_force", "upbp_beams_build", "upbp_beams_destroy", "upbp_beams_count",
     "upbp_beams_piece_count", "upbp_beams_selection_pdf", "upbp_beams_selection_pdfs", "upbp_beams_grid_stats",
     "upbp_eval", "upbp_eval_noalloc", "upbp_hit_signatures",
     "upbp_adjoint", "upbp_batch_plan_create", "upbp_batch_plan_destroy", "upbp_batch_run",
@@ -102,6 +102,8 @@ def load():
     L.upbp_beams_selection_pdfs.restype = C.c_int
     L.upbp_beams_selection_pdfs.argtypes = [vp, vp, C.c_int64, vp, C.c_int]
     L.upbp_beams_grid_stats.restype = C.c_int
+    L.upbp_bre_eval_brute_force.restype = C.c_int
+    L.upbp_bre_eval_brute_force.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_int64, vp, C.c_int]
     L.upbp_beams_grid_stats.argtypes = [vp, vp, C.POINTER(_abi.QueryBatch), vp, C.c_int]
     L.upbp_eval.restype = st
     L.upbp_eval.argtypes = [vp, vp, vp, C.POINTER(_abi.QueryBatch), vp, vp, C.c_int, C.POINTER(_abi.EvalStats)]
@@ -280,6 +282,23 @@ class EmbreeBre:
         p = _arr(power, np.float32)
         mem = _abi.MEM_DEVICE if (_is_torch(p) and p.is_cuda) else _abi.MEM_HOST
         self.ctx.check(self.ctx.lib.upbp_bre_set_power(self.h, _abi.ptr(p), mem))
+
+    def evalBreBruteForce(self, origin, direction, min_t, max_t, medium_id):
+        """EmbreeBre::evalBreBruteForce (Bre.cxx:980-1029) for n rays (host arrays: origin and
+        direction [n,3], min_t, max_t [n], medium_id [n]) -> [n,3]."""
+        o = np.ascontiguousarray(origin, np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(direction, np.float32).reshape(-1, 3)
+        n = o.shape[0]
+        t0 = np.ascontiguousarray(min_t, np.float32).reshape(n)
+        t1 = np.ascontiguousarray(max_t, np.float32).reshape(n)
+        m = np.ascontiguousarray(medium_id, np.int32).reshape(n)
+        if d.shape[0] != n:
+            raise ValueError("origin and direction must hold the same number of rays")
+        out = np.zeros((n, 3), np.float32)
+        self.ctx.check(self.ctx.lib.upbp_bre_eval_brute_force(self.ctx.h, self.h, _abi.ptr(o), _abi.ptr(d),
+                                                              _abi.ptr(t0), _abi.ptr(t1), _abi.ptr(m), n,
+                                                              _abi.ptr(out), _abi.MEM_HOST))
+        return out
 
     def evalBre(self, queries: QueryBatchView, out=None, stats=None):
         """EmbreeBre::evalBre (Bre.cxx:812-964) for every query of the batch -> [Q,3]."""
