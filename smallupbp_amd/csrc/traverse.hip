@@ -515,11 +515,13 @@ __device__ __forceinline__ void upbp_segment_end(const EvalArgs& A, const Ray& r
     g_hacc[w][2][lane] = 0.f;
 }
 
+// Tests the photons start + j of a leaf for the bits j of items (ascending).
 template <int MODE>
 __device__ __forceinline__ void photon_leaf(const EvalArgs& A, const Ray& r, SegState& s, LaneOut<MODE>& lo,
-                                            uint32_t start, uint32_t cnt, bool short_q) {
-    const uint32_t end = start + cnt;
-    for (uint32_t i = start; i < end; ++i) {
+                                            uint32_t start, uint64_t items, bool short_q) {
+    while (items) {
+        const uint32_t i = start + (uint32_t)__builtin_ctzll(items);
+        items &= items - 1;
         const float4 p = A.pos_r2[i];  // scalar load, wave-uniform index
         float t, sq;
         const bool hit = test_bre(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, s.tmin, s.tmax, p.x, p.y, p.z, p.w, t, sq);
@@ -603,13 +605,153 @@ __device__ __forceinline__ void photon_leaf(const EvalArgs& A, const Ray& r, Seg
     }
 }
 
+// ---- wave frustum of rays from one point ------------------------------------
+// When every active lane of a wave starts its ray at the same point (the camera rays of a
+// frame), four planes through that point bound the rays' directions, and every ray lies on
+// the inner side of each. A photon, or a beam piece whose segment, widened by its box margin,
+// that lies beyond one of the planes is then farther than its radius from every ray of the
+// wave: no lane can accept it, and its test is skipped for the whole wave. One lane per item
+// of a leaf does that check, so a leaf costs one pass however many items it holds. The wave
+// keeps its planes in LDS: they are wave-uniform, and VGPRs are what limits the beam kernel's
+// occupancy.
+constexpr int kFrWaves = 4;              // waves per block of the traversal launches (at most 4)
+__shared__ float4 g_fr[kFrWaves][5];     // inner unit normals n0..n3, then the origin
+
+__device__ __forceinline__ float wave_min(float v) {
+    for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+
+// Builds the wave's frustum for the lanes with a segment (s.tmin <= s.tmax); false (no
+// culling) when their origins differ or their directions spread over more than 60 degrees
+// from the mean. The bounds of the direction slopes are widened by 1e-5 relative, far above
+// their rounding, so every ray stays inside its planes.
+__device__ bool make_frustum(const Ray& r, const SegState& s) {
+    const int w = threadIdx.x / kWave;
+    const bool act = s.tmin <= s.tmax;
+    const uint64_t am = __ballot(act);
+    if (am == 0) return false;
+    if (__ballot(act && !(s.tmin >= 0.f)) != 0) return false;  // hits only at t >= 0 lie inside
+    const int fl = __builtin_ctzll(am);
+    const float ox = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.ox), fl));
+    const float oy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.oy), fl));
+    const float oz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.oz), fl));
+    if (__ballot(act && !(r.ox == ox && r.oy == oy && r.oz == oz)) != 0) return false;
+    const float sx = wave_sum_dpp(act ? r.dx : 0.f), sy = wave_sum_dpp(act ? r.dy : 0.f),
+                sz = wave_sum_dpp(act ? r.dz : 0.f);
+    const float sn = __builtin_sqrtf((sx * sx + sy * sy) + sz * sz);
+    if (!(sn > 0.f)) return false;
+    const float mx = sx / sn, my = sy / sn, mz = sz / sn;  // mean direction
+    const float dd = act ? (r.dx * mx + r.dy * my) + r.dz * mz : 1.f;
+    if (__ballot(act && !(dd > 0.5f)) != 0) return false;
+    // U across the mean direction (crossed with the axis it has the least of), V = M x U
+    const float amx = __builtin_fabsf(mx), amy = __builtin_fabsf(my), amz = __builtin_fabsf(mz);
+    float ux, uy, uz;
+    if (amx <= amy && amx <= amz) {
+        ux = 0.f, uy = mz, uz = -my;  // M x (1,0,0)
+    } else if (amy <= amz) {
+        ux = -mz, uy = 0.f, uz = mx;  // M x (0,1,0)
+    } else {
+        ux = my, uy = -mx, uz = 0.f;  // M x (0,0,1)
+    }
+    const float un = __builtin_sqrtf((ux * ux + uy * uy) + uz * uz);
+    ux /= un, uy /= un, uz /= un;
+    const float vx = my * uz - mz * uy, vy = mz * ux - mx * uz, vz = mx * uy - my * ux;
+    const float u = act ? ((r.dx * ux + r.dy * uy) + r.dz * uz) / dd : 0.f;
+    const float v = act ? ((r.dx * vx + r.dy * vy) + r.dz * vz) / dd : 0.f;
+    float u0 = wave_min(act ? u : INFINITY), u1 = wave_max(act ? u : -INFINITY);
+    float v0 = wave_min(act ? v : INFINITY), v1 = wave_max(act ? v : -INFINITY);
+    u0 -= 1e-5f * (1.f + __builtin_fabsf(u0));
+    u1 += 1e-5f * (1.f + __builtin_fabsf(u1));
+    v0 -= 1e-5f * (1.f + __builtin_fabsf(v0));
+    v1 += 1e-5f * (1.f + __builtin_fabsf(v1));
+    // inner normals: d.n >= 0 for every ray direction d inside the slope bounds
+    float n[4][3] = {{ux - u0 * mx, uy - u0 * my, uz - u0 * mz},
+                     {u1 * mx - ux, u1 * my - uy, u1 * mz - uz},
+                     {vx - v0 * mx, vy - v0 * my, vz - v0 * mz},
+                     {v1 * mx - vx, v1 * my - vy, v1 * mz - vz}};
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        for (int k = 0; k < 4; ++k) {
+            const float nn = __builtin_sqrtf((n[k][0] * n[k][0] + n[k][1] * n[k][1]) + n[k][2] * n[k][2]);
+            g_fr[w][k] = make_float4(n[k][0] / nn, n[k][1] / nn, n[k][2] / nn, 0.f);
+        }
+        g_fr[w][4] = make_float4(ox, oy, oz, 0.f);
+    }
+    // the wave's other lanes read the planes next: keep the compiler from moving those LDS
+    // reads above the writes (a wave's own LDS accesses complete in order)
+    __builtin_amdgcn_wave_barrier();
+    return true;
+}
+
+// The pieces [start, start + cnt) of a leaf that some ray of the wave may accept: lane j checks
+// piece start + j against the wave's frustum. An inner piece owns only pairs whose closest
+// point on the beam line lies on its own segment, and that point is within the beam's radius
+// of a ray, so the segment widened by 1.01 of the largest radius (the piece box's margin)
+// cannot lie beyond a plane. The first and last piece of a beam also own the pairs whose
+// closest point lies beyond the beam's ends (which the reference accepts only through the
+// rounding of its t2 for nearly perpendicular lines): they are always kept, so the set of
+// pairs tested for them, and the hit set, stay those of the leaf-box traversal alone.
+__device__ __forceinline__ uint64_t frustum_items(const EvalArgs& A, uint32_t start, uint32_t cnt) {
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    bool keep = false;
+    if ((uint32_t)lane < cnt) {
+        const float4 p0 = A.pieces[start + lane].p0, p1 = A.pieces[start + lane].p1,
+                     p2 = A.pieces[start + lane].p2;
+        const float rb = __builtin_sqrtf(p2.w) * 1.01f;
+        const float ta = p2.x, tb = p2.y;
+        const float4 og = g_fr[w][4];
+        const float ax = (p0.x + p1.x * ta) - og.x, ay = (p0.y + p1.y * ta) - og.y, az = (p0.z + p1.z * ta) - og.z;
+        const float bx = (p0.x + p1.x * tb) - og.x, by = (p0.y + p1.y * tb) - og.y, bz = (p0.z + p1.z * tb) - og.z;
+        // rounding of the endpoints and of the plane distances: 1e-6 of the coordinates involved
+        const float m = rb + 1e-6f * ((((__builtin_fabsf(og.x) + __builtin_fabsf(og.y)) + __builtin_fabsf(og.z)) +
+                                       ((__builtin_fabsf(p0.x) + __builtin_fabsf(p0.y)) + __builtin_fabsf(p0.z))) +
+                                      (p0.w + rb));
+        keep = true;
+        for (int k = 0; k < 4; ++k) {
+            const float4 n = g_fr[w][k];
+            const float da = (n.x * ax + n.y * ay) + n.z * az, db = (n.x * bx + n.y * by) + n.z * bz;
+            keep = keep && !(da < -m && db < -m);
+        }
+        keep = keep || !(__builtin_isfinite(ta) && __builtin_isfinite(tb));  // first or last piece
+    }
+    return __ballot(keep);
+}
+
+// The photons [start, start + cnt) of a leaf that some ray of the wave may accept: a photon is
+// accepted at a point of a ray with t > min_t >= 0, inside the frustum, within its radius, so a
+// photon farther than 1.01 of its radius beyond one of the planes is skipped.
+__device__ __forceinline__ uint64_t frustum_photons(const EvalArgs& A, uint32_t start, uint32_t cnt) {
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    bool keep = false;
+    if ((uint32_t)lane < cnt) {
+        const float4 p = A.pos_r2[start + lane];
+        const float4 og = g_fr[w][4];
+        const float cx = p.x - og.x, cy = p.y - og.y, cz = p.z - og.z;
+        // rounding of the hit test and of the plane distances: 1e-6 of the coordinates involved
+        const float m = __builtin_sqrtf(p.w) * 1.01f +
+                        1e-6f * (((__builtin_fabsf(og.x) + __builtin_fabsf(og.y)) + __builtin_fabsf(og.z)) +
+                                 ((__builtin_fabsf(p.x) + __builtin_fabsf(p.y)) + __builtin_fabsf(p.z)));
+        keep = true;
+        for (int k = 0; k < 4; ++k) {
+            const float4 n = g_fr[w][k];
+            keep = keep && !((n.x * cx + n.y * cy) + n.z * cz < -m);
+        }
+    }
+    return __ballot(keep);
+}
+
 template <int MODE>
 __device__ __forceinline__ void beam_leaf(const EvalArgs& A, const Ray& r, SegState& s, LaneOut<MODE>& lo,
-                                          uint32_t start, uint32_t cnt, bool short_q) {
-    const uint32_t end = start + cnt;
+                                          uint32_t start, uint64_t items, bool short_q) {
     // No software prefetch of the next piece: the rotation of two 64-byte records through
     // SGPRs cost more scalar moves and SGPR pressure than the latency it hid (measured 14 %).
-    for (uint32_t i = start; i < end; ++i) {
+    while (items) {
+        const uint32_t i = start + (uint32_t)__builtin_ctzll(items);
+        items &= items - 1;
         const PieceDev pc = A.pieces[i];
         const uint32_t word = __float_as_uint(pc.p2.z);
         const int32_t bmed = (int32_t)(word & kMediumMask);
@@ -730,6 +872,7 @@ __device__ __forceinline__ void traverse(const EvalArgs& A, const Ray& r, SegSta
     int sp = 0;
     uint32_t node = A.root;
     const uint32_t active = (uint32_t)__popcll(__ballot(s.tmin <= s.tmax));
+    const bool fr = make_frustum(r, s);
     while (true) {
         node = uni(node);
         // One 64-byte scalar load for the whole node, then both slab tests without
@@ -745,18 +888,28 @@ __device__ __forceinline__ void traverse(const EvalArgs& A, const Ray& r, SegSta
         const bool any0 = m0 != 0 && rr.z != kInvalid;
         const bool any1 = m1 != 0 && rr.w != kInvalid;
         if (any0 && rr.z != 0) {
-            tests += (uint64_t)rr.z * active;
-            if (EST == kEstPhotons)
-                photon_leaf<MODE>(A, r, s, lo, rr.x, rr.z, short_q);
-            else
-                beam_leaf<MODE>(A, r, s, lo, rr.x, rr.z, short_q);
+            const uint64_t all = rr.z >= 64u ? ~0ull : ((1ull << rr.z) - 1ull);
+            if (EST == kEstPhotons) {
+                const uint64_t items = fr ? frustum_photons(A, rr.x, rr.z) & all : all;
+                tests += (uint64_t)__popcll(items) * active;
+                photon_leaf<MODE>(A, r, s, lo, rr.x, items, short_q);
+            } else {
+                const uint64_t items = fr ? frustum_items(A, rr.x, rr.z) & all : all;
+                tests += (uint64_t)__popcll(items) * active;
+                beam_leaf<MODE>(A, r, s, lo, rr.x, items, short_q);
+            }
         }
         if (any1 && rr.w != 0) {
-            tests += (uint64_t)rr.w * active;
-            if (EST == kEstPhotons)
-                photon_leaf<MODE>(A, r, s, lo, rr.y, rr.w, short_q);
-            else
-                beam_leaf<MODE>(A, r, s, lo, rr.y, rr.w, short_q);
+            const uint64_t all = rr.w >= 64u ? ~0ull : ((1ull << rr.w) - 1ull);
+            if (EST == kEstPhotons) {
+                const uint64_t items = fr ? frustum_photons(A, rr.y, rr.w) & all : all;
+                tests += (uint64_t)__popcll(items) * active;
+                photon_leaf<MODE>(A, r, s, lo, rr.y, items, short_q);
+            } else {
+                const uint64_t items = fr ? frustum_items(A, rr.y, rr.w) & all : all;
+                tests += (uint64_t)__popcll(items) * active;
+                beam_leaf<MODE>(A, r, s, lo, rr.y, items, short_q);
+            }
         }
         const bool go0 = any0 && rr.z == 0;
         const bool go1 = any1 && rr.w == 0;
