@@ -1,0 +1,159 @@
+"""The traversal's wave frustum culling against the oracle, on items placed where a wrong bound
+would lose them.
+
+When the 64 queries of a wave share their origin and start their segments at t >= 0, the
+kernels skip leaf items that lie beyond one of four planes bounding the wave's rays by more than
+1.01 of the item's radius. The scene here puts every photon, and the closest point of every
+beam, within 0.9-0.999 of its radius of one camera ray, displaced in a random direction across
+the ray: for rays on the rim of a wave's bundle about half of those items lie OUTSIDE the wave's
+frustum, yet are hits. The queries are given in 8x8 pixel tiles with ORDER_AS_GIVEN, so wave w
+holds queries [64 w, 64 w + 64), and the host test counts those outside items with the
+frustum's own construction (float64). Four waves' queries start their first segment behind the
+eye (dist_min -1), with items placed behind the eye too: for them culling must switch off.
+Parity bar: identical per-query hit signatures and radiance within 1e-4 of the FP32 oracle."""
+import numpy as np
+import pytest
+
+from smallupbp_amd import _abi, native, scenes
+
+from gpu_helpers import REL_TOL, gpu_context, oracle_beams, oracle_photons, print_case, rel_err
+
+W = H = 64
+NEG_WAVES = 4  # waves whose first segment starts at -1 (behind the eye)
+
+
+def _tiled_rays():
+    o, d = scenes.pinhole_rays(W, H)
+    y, x = np.divmod(np.arange(W * H), W)
+    key = ((y // 8) * (W // 8) + x // 8) * 64 + (y % 8) * 8 + x % 8  # tile-major order
+    idx = np.argsort(key)
+    return np.ascontiguousarray(o[idx]), np.ascontiguousarray(d[idx])
+
+
+def _perp(rng, d):
+    u = rng.normal(size=d.shape)
+    u -= (u * d).sum(-1, keepdims=True) * d
+    return u / np.linalg.norm(u, axis=-1, keepdims=True)
+
+
+def frustum_scene(seed=91, n_items=6000, radius=0.01):
+    rng = scenes.rng_for(seed)
+    media = scenes.make_media([((0.1, 0.1, 0.1), (0.5, 0.5, 0.5), 0.3, 0.8), ((0, 0, 0), (0, 0, 0), 0.0, 1.0)])
+    o, d = _tiled_rays()
+    nq = o.shape[0]
+    off, seg = scenes.ray_segments(media, o, d, (0, 0, 0), (1, 1, 1), 0, 1)
+    assert np.array_equal(np.diff(off), np.full(nq, 2))  # [0, tin] vacuum, [tin, tout] box
+    tin = seg["dist_min"][1::2].astype(np.float64)
+    tout = seg["dist_max"][1::2].astype(np.float64)
+    neg = np.arange(nq) < NEG_WAVES * 64
+    first = off[:-1]
+    seg["dist_min"][first[neg]] = -1.0
+    seg["medium_id"][first[neg]] = 0
+    seg = scenes.segment_values(media, seg["medium_id"], seg["dist_min"], seg["dist_max"])
+    q = scenes.Queries(o, d, off, seg, scenes.LONG_BEAM)
+
+    # a query per item; items of the behind-the-eye queries go to t in (-0.9, -0.1)
+    j = rng.integers(0, nq, n_items)
+    behind = neg[j] & (rng.uniform(0, 1, n_items) < 0.5)
+    margin = np.minimum(0.05, 0.25 * (tout[j] - tin[j]))
+    t_in = tin[j] + margin + rng.uniform(0, 1, n_items) * (tout[j] - tin[j] - 2 * margin)
+    t = np.where(behind, rng.uniform(-0.9, -0.1, n_items), t_in)
+    d64 = d[j].astype(np.float64)
+    u = _perp(rng, d64)
+    f = rng.uniform(0.9, 0.999, n_items)
+
+    # photons with their own radius
+    pr = rng.uniform(0.5, 2.0, n_items) * radius
+    ppos = o[j] + d64 * t[:, None] + u * (f * pr)[:, None]
+    photons = scenes.Photons(ppos.astype(np.float32), scenes._isotropic_dirs(rng, n_items),
+                             rng.uniform(0, 1e-4, (n_items, 3)).astype(np.float32), pr.astype(np.float32), 0.0)
+
+    # beams: direction across u (so the lines' closest approach is f r, at the placed point),
+    # not nearly perpendicular to the ray, closest point inside the beam
+    b = _perp(rng, u)
+    while True:
+        bad = np.abs((b * d64).sum(-1)) < 0.05
+        if not bad.any():
+            break
+        b[bad] = _perp(rng, u[bad])
+    length = rng.uniform(0.1, 0.6, n_items)
+    s = rng.uniform(0.1, 0.9, n_items) * length
+    P = o[j] + d64 * t[:, None] + u * (f * radius)[:, None]
+    flags = np.where(rng.uniform(0, 1, n_items) < 0.5, scenes.SHORT_BEAM, scenes.LONG_BEAM).astype(np.uint32)
+    beams = scenes.Beams((P - b * s[:, None]).astype(np.float32), b.astype(np.float32), length.astype(np.float32),
+                         rng.uniform(0, 1e-4, (n_items, 3)).astype(np.float32), flags,
+                         np.zeros(n_items, np.int32), float(radius))
+    w = scenes.Workload("frustum_%dx%d" % (W, H), media, photons, beams, q, None, 0, 1, seed)
+    return w, j, behind, ppos, P
+
+
+def wave_planes(o, d):
+    """The four inner unit normals of a wave's frustum, built as the kernel builds them (mean
+    direction, the axis it has least of, slope bounds), in float64 without the widening."""
+    m = d.sum(0)
+    m /= np.linalg.norm(m)
+    a = np.abs(m)
+    if a[0] <= a[1] and a[0] <= a[2]:
+        u = np.array([0.0, m[2], -m[1]])
+    elif a[1] <= a[2]:
+        u = np.array([-m[2], 0.0, m[0]])
+    else:
+        u = np.array([m[1], -m[0], 0.0])
+    u /= np.linalg.norm(u)
+    v = np.cross(m, u)
+    dd = d @ m
+    su, sv = (d @ u) / dd, (d @ v) / dd
+    n = np.array([u - su.min() * m, su.max() * m - u, v - sv.min() * m, sv.max() * m - v])
+    return n / np.linalg.norm(n, axis=1, keepdims=True)
+
+
+def outside_counts(w, j, behind, points):
+    """How many placed points lie beyond a plane of their query's wave frustum (excluding the
+    behind-the-eye waves), and how many of the behind-the-eye points lie beyond by more than
+    2 % of the radius (what culling would drop if it stayed on for those waves)."""
+    o, d = w.queries.origin.astype(np.float64), w.queries.direction.astype(np.float64)
+    wave = j // 64
+    out_front = 0
+    out_behind = 0
+    for k in np.unique(wave):
+        sel = wave == k
+        n = wave_planes(o[64 * k:64 * k + 64], d[64 * k:64 * k + 64])
+        dist = (points[sel] - o[64 * k]) @ n.T
+        if k < NEG_WAVES:
+            out_behind += int((behind[sel] & (dist.min(1) < -0.02 * w.beams.radius)).sum())
+        else:
+            out_front += int((dist.min(1) < 0).sum())
+    return out_front, out_behind
+
+
+def test_scene_puts_hits_outside_the_wave_frustum():
+    """Host check of the construction: the scene exercises the margin and the t >= 0 switch."""
+    w, j, behind, ppos, P = frustum_scene()
+    pf, pb = outside_counts(w, j, behind, ppos)
+    bf, bb = outside_counts(w, j, behind, P)
+    print("outside the wave frustum: photons %d (+%d behind the eye), beams %d (+%d)" % (pf, pb, bf, bb))
+    assert pf > 500 and bf > 500
+    assert pb > 100 and bb > 100
+
+
+@pytest.mark.gpu
+def test_frustum_culling_keeps_every_hit():
+    print_case("frustum_culling", queries=W * H, items=6000)
+    w, j, behind, _, _ = frustum_scene()
+    ctx = gpu_context(w.media)
+    bre = native.EmbreeBre(ctx)
+    bre.build(w.photons)
+    bm = native.PhotonBeamsEvaluator(ctx)
+    bm.build(w.beams)
+    qv = native.QueryBatchView.to_device(w.queries, order=_abi.ORDER_AS_GIVEN)
+    sp, sb = native.hit_signatures(ctx, bre, bm, qv)
+    ph = bre.evalBre(qv).cpu().numpy()
+    be = bm.evalBeamBeamEstimate(qv).cpu().numpy()
+    ref_p, rsig_p, cnt_p = oracle_photons(w, accel=0)
+    ref_b, rsig_b, cnt_b = oracle_beams(w, accel=0)
+    print("oracle hits: %d photon, %d beam of 6000 placed each" % (int(cnt_p[0]), int(cnt_b[0])))
+    assert int(cnt_p[0]) >= 6000 and int(cnt_b[0]) >= 6000
+    assert np.array_equal(sp, rsig_p), "photon hit sets differ"
+    assert np.array_equal(sb, rsig_b), "beam hit sets differ"
+    assert rel_err(ph, ref_p).max() <= REL_TOL
+    assert rel_err(be, ref_b).max() <= REL_TOL
