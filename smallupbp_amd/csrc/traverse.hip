@@ -611,11 +611,18 @@ __device__ __forceinline__ void photon_leaf(const EvalArgs& A, const Ray& r, Seg
 // the inner side of each. A photon, or a beam piece whose segment, widened by its box margin,
 // that lies beyond one of the planes is then farther than its radius from every ray of the
 // wave: no lane can accept it, and its test is skipped for the whole wave. One lane per item
-// of a leaf does that check, so a leaf costs one pass however many items it holds. The wave
-// keeps its planes in LDS: they are wave-uniform, and VGPRs are what limits the beam kernel's
-// occupancy.
-constexpr int kFrWaves = 4;              // waves per block of the traversal launches (at most 4)
-__shared__ float4 g_fr[kFrWaves][5];     // inner unit normals n0..n3, then the origin
+// of a leaf does that check, so a leaf costs one pass however many items it holds. The planes
+// are wave-uniform and live in SGPRs (VGPRs are what limits the beam kernel's occupancy); the
+// vector instructions of the check take them as scalar operands.
+struct Frustum {
+    float n[4][3];     // inner unit normals
+    float ox, oy, oz;  // the common origin
+    float om;          // the origin's part of the items' rounding margin: 1e-6 |o|_1
+};
+
+__device__ __forceinline__ float uni_f(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
 
 __device__ __forceinline__ float wave_min(float v) {
     for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, kWave));
@@ -630,8 +637,7 @@ __device__ __forceinline__ float wave_max(float v) {
 // culling) when their origins differ or their directions spread over more than 60 degrees
 // from the mean. The bounds of the direction slopes are widened by 1e-5 relative, far above
 // their rounding, so every ray stays inside its planes.
-__device__ bool make_frustum(const Ray& r, const SegState& s) {
-    const int w = threadIdx.x / kWave;
+__device__ bool make_frustum(const Ray& r, const SegState& s, Frustum& F) {
     const bool act = s.tmin <= s.tmax;
     const uint64_t am = __ballot(act);
     if (am == 0) return false;
@@ -674,16 +680,15 @@ __device__ bool make_frustum(const Ray& r, const SegState& s) {
                      {u1 * mx - ux, u1 * my - uy, u1 * mz - uz},
                      {vx - v0 * mx, vy - v0 * my, vz - v0 * mz},
                      {v1 * mx - vx, v1 * my - vy, v1 * mz - vz}};
-    if ((threadIdx.x & (kWave - 1)) == 0) {
-        for (int k = 0; k < 4; ++k) {
-            const float nn = __builtin_sqrtf((n[k][0] * n[k][0] + n[k][1] * n[k][1]) + n[k][2] * n[k][2]);
-            g_fr[w][k] = make_float4(n[k][0] / nn, n[k][1] / nn, n[k][2] / nn, 0.f);
-        }
-        g_fr[w][4] = make_float4(ox, oy, oz, 0.f);
+    // every lane holds the same values (wave reductions of the same inputs): take lane 0's
+    for (int k = 0; k < 4; ++k) {
+        const float nn = __builtin_sqrtf((n[k][0] * n[k][0] + n[k][1] * n[k][1]) + n[k][2] * n[k][2]);
+        F.n[k][0] = uni_f(n[k][0] / nn);
+        F.n[k][1] = uni_f(n[k][1] / nn);
+        F.n[k][2] = uni_f(n[k][2] / nn);
     }
-    // the wave's other lanes read the planes next: keep the compiler from moving those LDS
-    // reads above the writes (a wave's own LDS accesses complete in order)
-    __builtin_amdgcn_wave_barrier();
+    F.ox = ox, F.oy = oy, F.oz = oz;
+    F.om = uni_f(1e-6f * ((__builtin_fabsf(ox) + __builtin_fabsf(oy)) + __builtin_fabsf(oz)));
     return true;
 }
 
@@ -695,53 +700,60 @@ __device__ bool make_frustum(const Ray& r, const SegState& s) {
 // closest point lies beyond the beam's ends (which the reference accepts only through the
 // rounding of its t2 for nearly perpendicular lines): they are always kept, so the set of
 // pairs tested for them, and the hit set, stay those of the leaf-box traversal alone.
-__device__ __forceinline__ uint64_t frustum_items(const EvalArgs& A, uint32_t start, uint32_t cnt) {
-    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-    bool keep = false;
+__device__ __forceinline__ uint64_t frustum_items(const EvalArgs& A, const Frustum& F, uint32_t start,
+                                                  uint32_t cnt) {
+    const int lane = threadIdx.x & (kWave - 1);
+    bool out = false;
     if ((uint32_t)lane < cnt) {
         const float4 p0 = A.pieces[start + lane].p0, p1 = A.pieces[start + lane].p1,
                      p2 = A.pieces[start + lane].p2;
-        const float rb = __builtin_sqrtf(p2.w) * 1.01f;
         const float ta = p2.x, tb = p2.y;
-        const float4 og = g_fr[w][4];
-        const float ax = (p0.x + p1.x * ta) - og.x, ay = (p0.y + p1.y * ta) - og.y, az = (p0.z + p1.z * ta) - og.z;
-        const float bx = (p0.x + p1.x * tb) - og.x, by = (p0.y + p1.y * tb) - og.y, bz = (p0.z + p1.z * tb) - og.z;
+        // A margin only, so the hardware square root (within 1 ulp) does, and the plane
+        // distances are fused multiply-adds: the 1e-6 relative margin is far above the error.
+        const float rb = __builtin_amdgcn_sqrtf(p2.w) * 1.01f;
+        const float ox = p0.x - F.ox, oy = p0.y - F.oy, oz = p0.z - F.oz;
+        const float ax = __builtin_fmaf(p1.x, ta, ox), ay = __builtin_fmaf(p1.y, ta, oy),
+                    az = __builtin_fmaf(p1.z, ta, oz);
+        const float bx = __builtin_fmaf(p1.x, tb, ox), by = __builtin_fmaf(p1.y, tb, oy),
+                    bz = __builtin_fmaf(p1.z, tb, oz);
         // rounding of the endpoints and of the plane distances: 1e-6 of the coordinates involved
-        const float m = rb + 1e-6f * ((((__builtin_fabsf(og.x) + __builtin_fabsf(og.y)) + __builtin_fabsf(og.z)) +
-                                       ((__builtin_fabsf(p0.x) + __builtin_fabsf(p0.y)) + __builtin_fabsf(p0.z))) +
-                                      (p0.w + rb));
-        keep = true;
+        const float m = __builtin_fmaf(
+            1e-6f, ((__builtin_fabsf(p0.x) + __builtin_fabsf(p0.y)) + __builtin_fabsf(p0.z)) + (p0.w + rb), rb + F.om);
+        // the smallest over the planes of the larger endpoint distance: the segment lies beyond
+        // some plane iff it is below -m
+        float near = INFINITY;
         for (int k = 0; k < 4; ++k) {
-            const float4 n = g_fr[w][k];
-            const float da = (n.x * ax + n.y * ay) + n.z * az, db = (n.x * bx + n.y * by) + n.z * bz;
-            keep = keep && !(da < -m && db < -m);
+            const float* n = F.n[k];
+            const float da = __builtin_fmaf(n[2], az, __builtin_fmaf(n[1], ay, n[0] * ax));
+            const float db = __builtin_fmaf(n[2], bz, __builtin_fmaf(n[1], by, n[0] * bx));
+            near = fminf(near, fmaxf(da, db));
         }
-        keep = keep || !(__builtin_isfinite(ta) && __builtin_isfinite(tb));  // first or last piece
+        out = near < -m && __builtin_isfinite(ta) && __builtin_isfinite(tb);  // first and last pieces stay
     }
-    return __ballot(keep);
+    return __ballot((uint32_t)lane < cnt && !out);
 }
 
 // The photons [start, start + cnt) of a leaf that some ray of the wave may accept: a photon is
 // accepted at a point of a ray with t > min_t >= 0, inside the frustum, within its radius, so a
 // photon farther than 1.01 of its radius beyond one of the planes is skipped.
-__device__ __forceinline__ uint64_t frustum_photons(const EvalArgs& A, uint32_t start, uint32_t cnt) {
-    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-    bool keep = false;
+__device__ __forceinline__ uint64_t frustum_photons(const EvalArgs& A, const Frustum& F, uint32_t start,
+                                                    uint32_t cnt) {
+    const int lane = threadIdx.x & (kWave - 1);
+    bool out = false;
     if ((uint32_t)lane < cnt) {
         const float4 p = A.pos_r2[start + lane];
-        const float4 og = g_fr[w][4];
-        const float cx = p.x - og.x, cy = p.y - og.y, cz = p.z - og.z;
-        // rounding of the hit test and of the plane distances: 1e-6 of the coordinates involved
-        const float m = __builtin_sqrtf(p.w) * 1.01f +
-                        1e-6f * (((__builtin_fabsf(og.x) + __builtin_fabsf(og.y)) + __builtin_fabsf(og.z)) +
-                                 ((__builtin_fabsf(p.x) + __builtin_fabsf(p.y)) + __builtin_fabsf(p.z)));
-        keep = true;
+        const float cx = p.x - F.ox, cy = p.y - F.oy, cz = p.z - F.oz;
+        // as for beam pieces: hardware square root and fused plane distances, inside the margin
+        const float m = __builtin_fmaf(1e-6f, (__builtin_fabsf(p.x) + __builtin_fabsf(p.y)) + __builtin_fabsf(p.z),
+                                       __builtin_fmaf(__builtin_amdgcn_sqrtf(p.w), 1.01f, F.om));
+        float near = INFINITY;
         for (int k = 0; k < 4; ++k) {
-            const float4 n = g_fr[w][k];
-            keep = keep && !((n.x * cx + n.y * cy) + n.z * cz < -m);
+            const float* n = F.n[k];
+            near = fminf(near, __builtin_fmaf(n[2], cz, __builtin_fmaf(n[1], cy, n[0] * cx)));
         }
+        out = near < -m;
     }
-    return __ballot(keep);
+    return __ballot((uint32_t)lane < cnt && !out);
 }
 
 template <int MODE>
@@ -872,7 +884,8 @@ __device__ __forceinline__ void traverse(const EvalArgs& A, const Ray& r, SegSta
     int sp = 0;
     uint32_t node = A.root;
     const uint32_t active = (uint32_t)__popcll(__ballot(s.tmin <= s.tmax));
-    const bool fr = make_frustum(r, s);
+    Frustum F;
+    const bool fr = make_frustum(r, s, F);
     while (true) {
         node = uni(node);
         // One 64-byte scalar load for the whole node, then both slab tests without
@@ -890,11 +903,11 @@ __device__ __forceinline__ void traverse(const EvalArgs& A, const Ray& r, SegSta
         if (any0 && rr.z != 0) {
             const uint64_t all = rr.z >= 64u ? ~0ull : ((1ull << rr.z) - 1ull);
             if (EST == kEstPhotons) {
-                const uint64_t items = fr ? frustum_photons(A, rr.x, rr.z) & all : all;
+                const uint64_t items = fr ? frustum_photons(A, F, rr.x, rr.z) & all : all;
                 tests += (uint64_t)__popcll(items) * active;
                 photon_leaf<MODE>(A, r, s, lo, rr.x, items, short_q);
             } else {
-                const uint64_t items = fr ? frustum_items(A, rr.x, rr.z) & all : all;
+                const uint64_t items = fr ? frustum_items(A, F, rr.x, rr.z) & all : all;
                 tests += (uint64_t)__popcll(items) * active;
                 beam_leaf<MODE>(A, r, s, lo, rr.x, items, short_q);
             }
@@ -902,11 +915,11 @@ __device__ __forceinline__ void traverse(const EvalArgs& A, const Ray& r, SegSta
         if (any1 && rr.w != 0) {
             const uint64_t all = rr.w >= 64u ? ~0ull : ((1ull << rr.w) - 1ull);
             if (EST == kEstPhotons) {
-                const uint64_t items = fr ? frustum_photons(A, rr.y, rr.w) & all : all;
+                const uint64_t items = fr ? frustum_photons(A, F, rr.y, rr.w) & all : all;
                 tests += (uint64_t)__popcll(items) * active;
                 photon_leaf<MODE>(A, r, s, lo, rr.y, items, short_q);
             } else {
-                const uint64_t items = fr ? frustum_items(A, rr.y, rr.w) & all : all;
+                const uint64_t items = fr ? frustum_items(A, F, rr.y, rr.w) & all : all;
                 tests += (uint64_t)__popcll(items) * active;
                 beam_leaf<MODE>(A, r, s, lo, rr.y, items, short_q);
             }
