@@ -121,7 +121,7 @@ typedef struct upbp_build_params {
                                    with UPBP_KNN_RADIUS the multiplier of 2 sqrt(d2_knn) */
     int32_t knn;                /* UPBP_KNN_RADIUS: which nearest neighbour (>= 1, the item itself
                                    counts as the first) sets the radius */
-    int32_t leaf_size;          /* max items per LBVH leaf, 1..32; 0 = library default (8) */
+    int32_t leaf_size;          /* max items per LBVH leaf, 1..64; 0 = library default (8) */
     float beam_piece_factor;    /* beams only: target piece length in units of the max beam radius; 0 = default (8) */
     int32_t max_pieces_per_beam;/* beams only: cap on pieces per beam, 1..256; 0 = default */
     /* beams only: per-cell beam reduction of the reference's uniform grid (-gridmax / -gridred,
@@ -700,7 +700,9 @@ typedef struct upbp_tuning {
     int32_t waves_per_block;        /* 1..4: waves per workgroup of the traversal kernels */
     int32_t blocks_per_cu;          /* persistent kernels: resident blocks per CU; 0 = occupancy-derived */
     int32_t photon_waves_per_block; /* 1..4: the photon kernel's own count; 0 = waves_per_block */
-    int32_t pad;
+    int32_t frustum_leaf;           /* 1..64: a wave whose rays share their origin tests an inner
+                                       node's subtree of at most this many items as one leaf
+                                       (1 = only the built leaves); 0 = library default (64) */
 } upbp_tuning;
 upbp_status upbp_set_tuning(upbp_ctx* ctx, const upbp_tuning* tuning);
 upbp_status upbp_get_tuning(const upbp_ctx* ctx, upbp_tuning* tuning);

@@ -86,7 +86,7 @@ class Instance(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("waves_per_block", C.c_int32), ("blocks_per_cu", C.c_int32),
-                ("photon_waves_per_block", C.c_int32), ("pad", C.c_int32)]
+                ("photon_waves_per_block", C.c_int32), ("frustum_leaf", C.c_int32)]
 
 
 assert C.sizeof(Medium) == 32

@@ -2,7 +2,7 @@
 
 Searches the parameters that the GPU path exposes and keeps the fastest per
 workload-size bucket:
-  build-time: LBVH leaf size (items per leaf tile, 1..32) for photons and for
+  build-time: LBVH leaf size (items per leaf tile, 4..64) for photons and for
               beam pieces, and the beam piece length (in units of the beam radius);
   launch-time: waves per workgroup (1, 2, 4) and the query order (Morton-sorted
               bundles or the caller's order); with both estimators, the photon kernel's
@@ -25,9 +25,11 @@ from . import _abi, native, scenes
 
 DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_cache.json")
 
-PHOTON_LEAF = (1, 2, 4, 8, 16)
-BEAM_LEAF = (1, 2, 4, 8)
-PIECE_FACTOR = (2.0, 4.0, 8.0, 16.0)
+# Leaves up to 64 items (the library's limit): when a wave's rays share their origin, a leaf's
+# items are checked against the wave's frustum in one pass, so large leaves can pay off.
+PHOTON_LEAF = (4, 8, 16, 32, 64)
+BEAM_LEAF = (4, 8, 16, 32, 64)
+PIECE_FACTOR = (4.0, 8.0, 16.0, 32.0)
 WAVES = (1, 2, 4)
 ORDERS = (_abi.ORDER_SORTED, _abi.ORDER_AS_GIVEN)
 

@@ -231,6 +231,8 @@ void check_err_word(uint32_t e) {
 // 11-17 % slower on each.
 constexpr int kDefaultLeaf = 8;
 constexpr float kDefaultPieceFactor = 8.f;
+// Largest leaf: the traversal checks a leaf's items against the wave's frustum one item per lane.
+constexpr int kMaxLeaf = 64;
 
 void bre_build(Ctx& ctx, const upbp_photon_input& in, const upbp_build_params& params, Bre& out) {
     UPBP_REQUIRE(in.count >= 0 && in.count < (int64_t)0x7FFFFFFF, UPBP_EINVAL, "photon count out of range");
@@ -242,7 +244,7 @@ void bre_build(Ctx& ctx, const upbp_photon_input& in, const upbp_build_params& p
     UPBP_REQUIRE(in.radius || (params.radius > 0.f && std::isfinite(params.radius)), UPBP_EINVAL,
                  "photon radius must be > 0");
     const int leaf = params.leaf_size > 0 ? params.leaf_size : kDefaultLeaf;
-    UPBP_REQUIRE(leaf >= 1 && leaf <= 32, UPBP_EINVAL, "leaf_size must be in 1..32");
+    UPBP_REQUIRE(leaf >= 1 && leaf <= kMaxLeaf, UPBP_EINVAL, "leaf_size must be in 1..64");
     out.ctx = &ctx;
     out.n_in = in.count;
     out.n = 0;
@@ -324,7 +326,7 @@ void beams_build(Ctx& ctx, const upbp_beam_input& in, const upbp_build_params& p
                  "beam radius must be > 0");
     UPBP_REQUIRE(ctx.n_media > 0, UPBP_EINVAL, "upbp_set_media must be called before building beams");
     const int leaf = params.leaf_size > 0 ? params.leaf_size : kDefaultLeaf;
-    UPBP_REQUIRE(leaf >= 1 && leaf <= 32, UPBP_EINVAL, "leaf_size must be in 1..32");
+    UPBP_REQUIRE(leaf >= 1 && leaf <= kMaxLeaf, UPBP_EINVAL, "leaf_size must be in 1..64");
     const float factor = params.beam_piece_factor > 0.f ? params.beam_piece_factor : kDefaultPieceFactor;
     const int maxp = params.max_pieces_per_beam > 0 ? params.max_pieces_per_beam : 64;
     UPBP_REQUIRE(maxp <= 256 && std::isfinite(factor), UPBP_EINVAL, "max_pieces_per_beam must be in 1..256");

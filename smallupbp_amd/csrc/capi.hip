@@ -1186,12 +1186,14 @@ static void merge_structures(Ctx& ctx, upbp_batch_plan* plan, const upbp_instanc
     for (int i = 0; i < plan->n; ++i) {
         const Bre* b = instances[i].bre;
         const Beams* m = instances[i].beams;
+        // every instance's nodes start at the same offset as its items (n_nodes <= items), so
+        // the item range that an inner node's own index gives stays right after rebasing
         if (plan->host[i].has_photons) {
-            pn += b->bvh.n_nodes;
+            pn += std::max<int64_t>(b->bvh.n_nodes, b->n);
             pi += b->n;
         }
         if (plan->host[i].has_beams) {
-            bn += m->bvh.n_nodes;
+            bn += std::max<int64_t>(m->bvh.n_nodes, m->pieces);
             bi += m->pieces;
         }
     }
@@ -1218,7 +1220,7 @@ static void merge_structures(Ctx& ctx, upbp_batch_plan* plan, const upbp_instanc
             UPBP_HIP(hipMemcpyAsync(plan->m_ph_inc.as<float4>() + pi, b->inc.p, b->n * 16, dd, ctx.stream));
             launch_rebase_nodes(ctx, dst, b->bvh.n_nodes, (uint32_t)pn, (uint32_t)pi);
             d.ph_root = (uint32_t)pn;
-            pn += b->bvh.n_nodes;
+            pn += std::max<int64_t>(b->bvh.n_nodes, b->n);
             pi += b->n;
         }
         if (d.has_beams) {
@@ -1229,7 +1231,7 @@ static void merge_structures(Ctx& ctx, upbp_batch_plan* plan, const upbp_instanc
             UPBP_HIP(hipMemcpyAsync(plan->m_bm_flux.as<float4>() + bi, m->piece_flux.p, m->pieces * 16, dd, ctx.stream));
             launch_rebase_nodes(ctx, dst, m->bvh.n_nodes, (uint32_t)bn, (uint32_t)bi);
             d.bm_root = (uint32_t)bn;
-            bn += m->bvh.n_nodes;
+            bn += std::max<int64_t>(m->bvh.n_nodes, m->pieces);
             bi += m->pieces;
         }
     }
@@ -1432,11 +1434,15 @@ upbp_status upbp_batch_run(upbp_ctx* ctx, upbp_batch_plan* plan, int use_graph, 
 upbp_status upbp_set_tuning(upbp_ctx* ctx, const upbp_tuning* tuning) {
     if (!ctx || !tuning) return fail(ctx, UPBP_EINVAL, "ctx and tuning are required");
     if (tuning->waves_per_block < 1 || tuning->waves_per_block > 4 || tuning->blocks_per_cu < 0 ||
-        tuning->blocks_per_cu > 64 || tuning->photon_waves_per_block < 0 || tuning->photon_waves_per_block > 4)
-        return fail(ctx, UPBP_EINVAL, "waves_per_block must be 1..4, photon_waves_per_block 0..4 and blocks_per_cu 0..64");
+        tuning->blocks_per_cu > 64 || tuning->photon_waves_per_block < 0 || tuning->photon_waves_per_block > 4 ||
+        tuning->frustum_leaf < 0 || tuning->frustum_leaf > 64)
+        return fail(ctx, UPBP_EINVAL,
+                    "waves_per_block must be 1..4, photon_waves_per_block 0..4, blocks_per_cu 0..64 and "
+                    "frustum_leaf 0..64");
     ctx->tuning.waves_per_block = tuning->waves_per_block;
     ctx->tuning.blocks_per_cu = tuning->blocks_per_cu;
     ctx->tuning.photon_waves_per_block = tuning->photon_waves_per_block;
+    ctx->tuning.frustum_leaf = tuning->frustum_leaf > 0 ? tuning->frustum_leaf : Tuning{}.frustum_leaf;
     return UPBP_OK;
 }
 
@@ -1445,7 +1451,7 @@ upbp_status upbp_get_tuning(const upbp_ctx* ctx, upbp_tuning* tuning) {
     tuning->waves_per_block = ctx->tuning.waves_per_block;
     tuning->blocks_per_cu = ctx->tuning.blocks_per_cu;
     tuning->photon_waves_per_block = ctx->tuning.photon_waves_per_block;
-    tuning->pad = 0;
+    tuning->frustum_leaf = ctx->tuning.frustum_leaf;
     return UPBP_OK;
 }
 
@@ -1485,7 +1491,7 @@ upbp_status upbp_autotune(upbp_ctx* ctx, const upbp_bre* bre, const upbp_beams* 
             best->waves_per_block = best_cfg.waves_per_block;
             best->blocks_per_cu = best_cfg.blocks_per_cu;
             best->photon_waves_per_block = best_cfg.photon_waves_per_block;
-            best->pad = 0;
+            best->frustum_leaf = best_cfg.frustum_leaf;
         }
         if (best_ms) *best_ms = best_t / 3.f;
     });

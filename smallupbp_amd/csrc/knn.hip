@@ -136,11 +136,11 @@ __device__ __forceinline__ bool walk(const NodeDev* nodes, float qx, float qy, f
         const bool h0 = rr.z != kInvalid && want(d0);
         const bool h1 = rr.w != kInvalid && want(d1);
         const uint64_t b0 = __ballot(h0), b1 = __ballot(h1);
-        if (b0 && rr.z != 0)
+        if (b0 && !(rr.z & kInner))
             for (uint32_t i = rr.x; i < rr.x + rr.z; ++i) visit(i);
-        if (b1 && rr.w != 0)
+        if (b1 && !(rr.w & kInner))
             for (uint32_t i = rr.y; i < rr.y + rr.w; ++i) visit(i);
-        const bool go0 = b0 && rr.z == 0, go1 = b1 && rr.w == 0;
+        const bool go0 = b0 && (rr.z & kInner), go1 = b1 && (rr.w & kInner);
         if (go0 && go1) {
             // nearer child first, by a vote of the lanes that want both
             const uint64_t both = b0 & b1;

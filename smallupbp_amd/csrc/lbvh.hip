@@ -247,8 +247,10 @@ __global__ __launch_bounds__(kBlock) void k_emit(int64_t n, int leaf_size, float
                 ref[c] = first[m];
                 cnt[c] = count;
             } else {
+                // the subtree's size and where its items start, for traversals that take a
+                // small subtree as one leaf
                 ref[c] = m;
-                cnt[c] = 0;
+                cnt[c] = kInner | (first[m] == m ? 0u : kSpanEnds) | min(count, kSpanMask - 1u);
             }
         }
         bl[c] = make_float4(bl[c].x - eps, bl[c].y - eps, bl[c].z - eps, 0.f);

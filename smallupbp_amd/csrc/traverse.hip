@@ -67,6 +67,8 @@ struct EvalArgs {
     MisGlobals mis;
     // traversal starts here (0, or an instance's root in a batched plan's merged tree array)
     uint32_t root;
+    // a wave with a frustum tests an inner node's subtree of at most this many items as one leaf
+    uint32_t frustum_leaf;
     // per-cell beam reduction of the beam estimators (red.max == 0: none)
     RedDev red;
 };
@@ -875,6 +877,24 @@ __device__ __forceinline__ void beam_leaf(const EvalArgs& A, const Ray& r, SegSt
 }
 
 // ---- wave-cooperative traversal ------------------------------------------
+// The items [start, start + cnt) (cnt <= 64) of a leaf, culled against the wave's frustum first
+// when it has one.
+template <int EST, int MODE>
+__device__ __forceinline__ void visit_leaf(const EvalArgs& A, const Frustum& F, bool fr, const Ray& r, SegState& s,
+                                           LaneOut<MODE>& lo, uint32_t start, uint32_t cnt, bool short_q,
+                                           uint32_t active, uint64_t& tests) {
+    const uint64_t all = cnt >= 64u ? ~0ull : ((1ull << cnt) - 1ull);
+    if (EST == kEstPhotons) {
+        const uint64_t items = fr ? frustum_photons(A, F, start, cnt) & all : all;
+        tests += (uint64_t)__popcll(items) * active;
+        photon_leaf<MODE>(A, r, s, lo, start, items, short_q);
+    } else {
+        const uint64_t items = fr ? frustum_items(A, F, start, cnt) & all : all;
+        tests += (uint64_t)__popcll(items) * active;
+        beam_leaf<MODE>(A, r, s, lo, start, items, short_q);
+    }
+}
+
 template <int EST, int MODE>
 __device__ __forceinline__ void traverse(const EvalArgs& A, const Ray& r, SegState& s, LaneOut<MODE>& lo,
                                          bool short_q, uint64_t& nodes_visited, uint64_t& tests,
@@ -900,32 +920,20 @@ __device__ __forceinline__ void traverse(const EvalArgs& A, const Ray& r, SegSta
         // the wave-uniform child references are tested on the scalar side
         const bool any0 = m0 != 0 && rr.z != kInvalid;
         const bool any1 = m1 != 0 && rr.w != kInvalid;
-        if (any0 && rr.z != 0) {
-            const uint64_t all = rr.z >= 64u ? ~0ull : ((1ull << rr.z) - 1ull);
-            if (EST == kEstPhotons) {
-                const uint64_t items = fr ? frustum_photons(A, F, rr.x, rr.z) & all : all;
-                tests += (uint64_t)__popcll(items) * active;
-                photon_leaf<MODE>(A, r, s, lo, rr.x, items, short_q);
-            } else {
-                const uint64_t items = fr ? frustum_items(A, F, rr.x, rr.z) & all : all;
-                tests += (uint64_t)__popcll(items) * active;
-                beam_leaf<MODE>(A, r, s, lo, rr.x, items, short_q);
-            }
-        }
-        if (any1 && rr.w != 0) {
-            const uint64_t all = rr.w >= 64u ? ~0ull : ((1ull << rr.w) - 1ull);
-            if (EST == kEstPhotons) {
-                const uint64_t items = fr ? frustum_photons(A, F, rr.y, rr.w) & all : all;
-                tests += (uint64_t)__popcll(items) * active;
-                photon_leaf<MODE>(A, r, s, lo, rr.y, items, short_q);
-            } else {
-                const uint64_t items = fr ? frustum_items(A, F, rr.y, rr.w) & all : all;
-                tests += (uint64_t)__popcll(items) * active;
-                beam_leaf<MODE>(A, r, s, lo, rr.y, items, short_q);
-            }
-        }
-        const bool go0 = any0 && rr.z == 0;
-        const bool go1 = any1 && rr.w == 0;
+        // A leaf; or, for a wave with a frustum, an inner node whose subtree holds at most
+        // frustum_leaf items: the frustum check takes a whole leaf in one pass, so a small
+        // subtree costs less as one leaf than as nodes (leaves stay small for the other waves).
+        const uint32_t n0 = rr.z & kSpanMask, n1 = rr.w & kSpanMask;
+        const bool leaf0 = any0 && (!(rr.z & kInner) || (fr && n0 <= A.frustum_leaf));
+        const bool leaf1 = any1 && (!(rr.w & kInner) || (fr && n1 <= A.frustum_leaf));
+        if (leaf0)
+            visit_leaf<EST, MODE>(A, F, fr, r, s, lo, (rr.z & kSpanEnds) ? rr.x + 1u - n0 : rr.x, n0, short_q,
+                                  active, tests);
+        if (leaf1)
+            visit_leaf<EST, MODE>(A, F, fr, r, s, lo, (rr.w & kSpanEnds) ? rr.y + 1u - n1 : rr.y, n1, short_q,
+                                  active, tests);
+        const bool go0 = any0 && !leaf0;
+        const bool go1 = any1 && !leaf1;
         if (go0 && go1) {
             if (sp >= kStackMax) {
                 overflow++;
@@ -1343,8 +1351,8 @@ __global__ void k_rebase_nodes(NodeDev* nodes, int64_t n, uint32_t node_off, uin
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint4 r = nodes[i].r;
-    if (r.z != kInvalid) r.x += r.z == 0 ? node_off : item_off;
-    if (r.w != kInvalid) r.y += r.w == 0 ? node_off : item_off;
+    if (r.z != kInvalid) r.x += (r.z & kInner) ? node_off : item_off;
+    if (r.w != kInvalid) r.y += (r.w & kInner) ? node_off : item_off;
     nodes[i].r = r;
 }
 
@@ -1471,6 +1479,7 @@ EvalArgs base_args(Ctx& ctx, const QueryView& q) {
     A.media_base = 0;
     A.lds_media_n = std::max(1, std::min(ctx.n_media, kMaxLdsMedia));
     A.stats = ctx.stats.as<unsigned long long>();
+    A.frustum_leaf = (uint32_t)ctx.tuning.frustum_leaf;
     return A;
 }
 
@@ -1604,6 +1613,7 @@ void launch_batched(Ctx& ctx, const InstanceDev* inst, int n_inst, int64_t total
     A.media = ctx.media.as<MediumDev>();
     A.n_media = ctx.n_media;
     A.stats = ctx.stats.as<unsigned long long>();
+    A.frustum_leaf = (uint32_t)ctx.tuning.frustum_leaf;  // merged nodes keep node index == item index offsets
     if (which == kEstPhotons) {
         A.nodes = merged.ph_nodes;
         A.pos_r2 = merged.ph_pos_r2;

@@ -38,13 +38,16 @@ static_assert(sizeof(MediumDev) == 64, "MediumDev layout");
 
 // LBVH node: two child boxes and references, 64 bytes, read with scalar loads.
 //   a = (c0.lo.xyz, c0.hi.x)  b = (c0.hi.yz, c1.lo.xy)  c = (c1.lo.z, c1.hi.xyz)
-//   r = (ref0, ref1, cnt0, cnt1): cnt 0 = inner node ref, cnt > 0 = leaf of cnt items
-//       starting at ref, cnt == kInvalid = no child.
+//   r = (ref0, ref1, cnt0, cnt1): cnt == kInvalid = no child; cnt without kInner = leaf of cnt
+//       items starting at ref; cnt with kInner = inner node ref whose subtree holds the
+//       cnt & kSpanMask items (saturated below kSpanMask) that start at ref, or that end at ref
+//       when kSpanEnds is set: the item range of Karras node i starts or ends at item i.
 struct alignas(16) NodeDev {
     float4 a, b, c;
     uint4 r;
 };
 static_assert(sizeof(NodeDev) == 64, "NodeDev layout");
+constexpr uint32_t kInner = 0x80000000u, kSpanEnds = 0x40000000u, kSpanMask = 0x3FFFFFFFu;
 
 // Photon record in leaf order: pos_r2 = (pos.xyz, r^2), power = (rgb, 1 / (r^2 pi / 2)),
 // inc = (WorldDirFix.xyz, unused) read only for anisotropic media.

@@ -75,6 +75,7 @@ struct Tuning {
     int waves_per_block = 4;
     int blocks_per_cu = 0;
     int photon_waves_per_block = 0;  // 0: waves_per_block
+    int frustum_leaf = 64;           // largest subtree a frustum wave tests as one leaf
 };
 
 struct Ctx {

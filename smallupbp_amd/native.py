@@ -182,16 +182,18 @@ class Context:
     def set_sync(self, sync: bool):
         self.check(self.lib.upbp_ctx_set_sync(self.h, 1 if sync else 0))
 
-    def set_tuning(self, waves_per_block: int, blocks_per_cu: int = 0, photon_waves_per_block: int = 0):
+    def set_tuning(self, waves_per_block: int, blocks_per_cu: int = 0, photon_waves_per_block: int = 0,
+                   frustum_leaf: int = 0):
         """Waves per workgroup of the traversal kernels (the photon kernel's own count when
-        photon_waves_per_block > 0) and resident blocks per CU of the persistent kernels."""
-        t = _abi.Tuning(waves_per_block, blocks_per_cu, photon_waves_per_block, 0)
+        photon_waves_per_block > 0), resident blocks per CU of the persistent kernels, and the
+        largest subtree a wave whose rays share their origin tests as one leaf (0 = default)."""
+        t = _abi.Tuning(waves_per_block, blocks_per_cu, photon_waves_per_block, frustum_leaf)
         self.check(self.lib.upbp_set_tuning(self.h, C.byref(t)))
 
     def get_tuning(self):
         t = _abi.Tuning()
         self.check(self.lib.upbp_get_tuning(self.h, C.byref(t)))
-        return t.waves_per_block, t.blocks_per_cu, t.photon_waves_per_block
+        return t.waves_per_block, t.blocks_per_cu, t.photon_waves_per_block, t.frustum_leaf
 
     def close(self):
         if getattr(self, "h", None):

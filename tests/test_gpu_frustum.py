@@ -137,14 +137,15 @@ def test_scene_puts_hits_outside_the_wave_frustum():
 
 
 @pytest.mark.gpu
-def test_frustum_culling_keeps_every_hit():
-    print_case("frustum_culling", queries=W * H, items=6000)
+@pytest.mark.parametrize("leaf_size", [8, 64])
+def test_frustum_culling_keeps_every_hit(leaf_size):
+    print_case("frustum_culling", queries=W * H, items=6000, leaf_size=leaf_size)
     w, j, behind, _, _ = frustum_scene()
     ctx = gpu_context(w.media)
     bre = native.EmbreeBre(ctx)
-    bre.build(w.photons)
+    bre.build(w.photons, leaf_size=leaf_size)
     bm = native.PhotonBeamsEvaluator(ctx)
-    bm.build(w.beams)
+    bm.build(w.beams, leaf_size=leaf_size)
     qv = native.QueryBatchView.to_device(w.queries, order=_abi.ORDER_AS_GIVEN)
     sp, sb = native.hit_signatures(ctx, bre, bm, qv)
     ph = bre.evalBre(qv).cpu().numpy()
@@ -157,3 +158,36 @@ def test_frustum_culling_keeps_every_hit():
     assert np.array_equal(sb, rsig_b), "beam hit sets differ"
     assert rel_err(ph, ref_p).max() <= REL_TOL
     assert rel_err(be, ref_b).max() <= REL_TOL
+
+
+@pytest.mark.gpu
+def test_subtree_leaf_setting_keeps_hit_sets():
+    """A wave with a frustum tests an inner node's whole subtree of at most frustum_leaf items as
+    one leaf (the subtree's item range derived from the node's own index). Built with leaves of
+    4, every setting must give the oracle's hit sets; out-of-range settings are rejected."""
+    print_case("frustum_leaf_setting", queries=W * H, items=6000, leaf_size=4)
+    w, _, _, _, _ = frustum_scene(seed=93)
+    ctx = gpu_context(w.media)
+    bre = native.EmbreeBre(ctx)
+    bre.build(w.photons, leaf_size=4)
+    bm = native.PhotonBeamsEvaluator(ctx)
+    bm.build(w.beams, leaf_size=4)
+    qv = native.QueryBatchView.to_device(w.queries, order=_abi.ORDER_AS_GIVEN)
+    _, rsig_p, _ = oracle_photons(w, accel=0, queries=w.queries)
+    _, rsig_b, _ = oracle_beams(w, accel=0, queries=w.queries)
+    nodes = {}
+    for fl in (1, 2, 16, 64):
+        ctx.set_tuning(1, 0, 0, fl)
+        assert ctx.get_tuning()[3] == fl
+        st = {}
+        native.eval_batch(ctx, bre, bm, qv, stats=st)
+        nodes[fl] = st["nodes_visited"]
+        sp, sb = native.hit_signatures(ctx, bre, bm, qv)
+        assert np.array_equal(sp, rsig_p), "photon hit sets differ at frustum_leaf %d" % fl
+        assert np.array_equal(sb, rsig_b), "beam hit sets differ at frustum_leaf %d" % fl
+    print("nodes visited per setting:", nodes)
+    assert nodes[64] < nodes[1]
+    ctx.set_tuning(1, 0, 0, 0)
+    assert ctx.get_tuning()[3] == 64  # library default
+    with pytest.raises(native.UpbpError):
+        ctx.set_tuning(1, 0, 0, 65)

@@ -105,6 +105,42 @@ def test_bre_beams_1080p_full_size():
 
 
 @pytest.mark.timeout(600)
+def test_subtree_leaves_add_only_reference_hits():
+    """The 1080p beams with the built leaves alone (frustum_leaf 1) and with frustum waves taking
+    subtrees of up to 64 items as one leaf (the default): the larger subtree boxes reach end-tie
+    pairs that the leaf boxes miss, so the hit sets may differ, but only by pairs the reference
+    accepts. Every differing query is compared with the oracle's brute force: the default's hit
+    set equals the oracle's up to end ties, and it holds every hit of the leaf-only run."""
+    print_case("subtree_leaves_1080p")
+    w = scenes.bre_beams()
+    w = scenes.Workload(w.name, w.media, None, w.beams, w.queries, w.sphere, w.box_medium, w.vacuum_medium, w.seed)
+    ctx, _, bm, qv, p = _tuned_context(w)
+    wpb = p["beam_waves_per_block"] or 4
+    sigs, hits = {}, {}
+    for fl in (1, 64):
+        ctx.set_tuning(wpb, 0, 0, fl)
+        st = {}
+        native.eval_batch(ctx, None, bm, qv, stats=st)
+        hits[fl] = st["beam_hits"]
+        sigs[fl] = native.hit_signatures(ctx, None, bm, qv)[1]
+    diff = np.nonzero(np.any(sigs[1] != sigs[64], axis=1))[0]
+    print("beam hits: leaves only %d, subtrees of 64 %d; %d queries differ" % (hits[1], hits[64], len(diff)),
+          flush=True)
+    assert hits[64] >= hits[1]
+    assert np.all(sigs[64][:, 0] >= sigs[1][:, 0])
+    assert len(diff) < 1000
+    if len(diff):
+        qs = w.queries.subset(diff)
+        _, rsig, _ = po.beams_eval(po.OracleBeams(w.beams, grid_res=0), w.media, qs, accel=0, signatures=True)
+        for fl in (1, 64):
+            n = check_beam_ties(qs, w.beams, sigs[fl][diff], rsig)
+            print("frustum_leaf %d: %d end-tie pairs missed on the differing queries" % (fl, n), flush=True)
+        # the default misses fewer of the reference's pairs there than the leaf-only run
+        assert (sigs[64][diff, 0].astype(np.int64) - rsig[:, 0].astype(np.int64)).sum() > \
+            (sigs[1][diff, 0].astype(np.int64) - rsig[:, 0].astype(np.int64)).sum()
+
+
+@pytest.mark.timeout(600)
 def test_beams_stress_full_size():
     """1M query beams x 2M photon beams (half SHORT, half LONG): 2,000 sampled queries
     against the oracle's brute force; their total hit count equals the oracle's."""

@@ -38,7 +38,7 @@ def _run_beams(w, leaf_size=0, order=_abi.ORDER_SORTED, piece_factor=0.0, max_pi
     return out, sig, st
 
 
-@pytest.mark.parametrize("leaf_size", [1, 4, 8, 32])
+@pytest.mark.parametrize("leaf_size", [1, 4, 8, 32, 64])
 def test_bre_small_matches_oracle(leaf_size):
     print_case("bre_small", leaf_size=leaf_size)
     w = scenes.plain_bre(64, 64, 20_000, 0.01, seed=11)
@@ -93,7 +93,8 @@ def test_bre_nested_media_lite_segments():
     assert rel_err(out, ref).max() <= REL_TOL
 
 
-@pytest.mark.parametrize("leaf_size,piece_factor,max_pieces", [(4, 0.0, 0), (1, 2.0, 128), (16, 16.0, 8)])
+@pytest.mark.parametrize("leaf_size,piece_factor,max_pieces",
+                         [(4, 0.0, 0), (1, 2.0, 128), (16, 16.0, 8), (64, 8.0, 0)])
 def test_beams_small_matches_oracle(leaf_size, piece_factor, max_pieces):
     print_case("beams_small", leaf_size=leaf_size, piece_factor=piece_factor, max_pieces=max_pieces)
     w = scenes.bre_beams(48, 27, 10, 3_000, seed=15)
