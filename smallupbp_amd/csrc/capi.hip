@@ -329,6 +329,7 @@ void run_eval(Ctx& ctx, const Bre* bre, const Beams* bm, const upbp_query_batch*
     if (mis && mode == Mode::Upbp && Q > 0) {
         mis_t = *mis;
         launch_light_tails(ctx, mis_t);
+        if (bm) launch_beam_records(ctx, mis_t, bm->n);
         mis = &mis_t;
     }
     // device outputs (temporary for host outputs)

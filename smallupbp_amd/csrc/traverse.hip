@@ -296,13 +296,13 @@ __device__ void beam_hit_mis(const EvalArgs& A, const Ray& r, SegState& s, const
                              bool short_q, uint32_t& err) {
     const MisGlobals& G = A.mis;
     const upbp_mis_params& P = G.p;
-    const upbp_beam_mis bm = G.beams[beam_idx];
-    if (bm.light_vertex < 0 || (int64_t)bm.light_vertex >= G.lv_count) {
+    // the beam's record: its MIS fields, origin light vertex and light tail in one 64-byte read
+    const BeamRec rec = G.beam_recs[beam_idx];
+    if (rec.info & kRecBadVertex) {
         err |= kBadLightIndex;
         return;
     }
-    const upbp_light_vertex* __restrict__ lv = G.lv + bm.light_vertex;
-    const uint32_t path_len = lv->path_length;
+    const uint32_t path_len = rec.info & kRecLengthMask;
     const uint32_t cam_len = (uint32_t)ml.cam_len;
     if ((path_len + 1 + cam_len > P.max_path_length) || (path_len + 1 + cam_len < P.min_path_length)) return;
     if (cam_len == 1 && t1 < P.min_dist_to_med) return;
@@ -318,13 +318,13 @@ __device__ void beam_hit_mis(const EvalArgs& A, const Ray& r, SegState& s, const
     float rsp_q = s.min_comp * pq;
     float rsr_q = (ml.seg_flags & UPBP_ORIGIN_IN_MEDIUM) ? rsp_q : pq;
     float rsp_b = s.min_comp * pb;
-    float rsr_b = (bm.ray_sampling_flags & UPBP_ORIGIN_IN_MEDIUM) ? rsp_b : pb;
+    float rsr_b = (rec.info & kRecOriginInMedium) ? rsp_b : pb;
     const float ratio = __builtin_amdgcn_rcpf(s.min_comp);  // ratio_q = ratio_b
     if (!(a0 > 1e-20f && a1 > 1e-20f && a2 > 1e-20f)) return;
     rsp_q *= ml.pref_pdf;
     rsr_q *= ml.pref_rev;
-    rsp_b *= bm.ray_sample_pdf;
-    rsr_b *= bm.ray_sample_rev_pdf;
+    rsp_b *= rec.ray_sample_pdf;
+    rsr_b *= rec.ray_sample_rev_pdf;
     // PhaseFunction::Evaluate(-ray.direction, -beam.direction): cos = -dot(-d2, -d1)
     const float cos_t = -dot3(-d2x, -d2y, -d2z, -r.dx, -r.dy, -r.dz);
     const float bsdf = phase_pdf(s.g, s.iso != 0, cos_t);
@@ -342,9 +342,9 @@ __device__ void beam_hit_mis(const EvalArgs& A, const Ray& r, SegState& s, const
     const float rsp_inv_q = __builtin_amdgcn_rcpf(rsp_q);
     const MisF mv = merge_vertex_mis(P, ml.last_pdf_w_inv * dist_sq_q * rsp_inv_q, rsp_inv_q, ratio, sel_pdf,
                                      UPBP_MIS_IN_MEDIUM_WITH_BEAMS);
-    const float dist_sq_b = lv->is_finite ? t2 * t2 : 1.0f;
+    const float dist_sq_b = (rec.info & kRecFinite) ? t2 * t2 : 1.0f;
     const float rsp_inv_b = __builtin_amdgcn_rcpf(rsp_b);
-    const MisF bv = merge_vertex_mis(P, bm.last_pdf_w_inv * dist_sq_b * rsp_inv_b, rsp_inv_b, ratio, sel_pdf,
+    const MisF bv = merge_vertex_mis(P, rec.last_pdf_w_inv * dist_sq_b * rsp_inv_b, rsp_inv_b, ratio, sel_pdf,
                                      UPBP_MIS_IN_MEDIUM_WITH_BEAMS);
     const uint32_t tech = A.short_query ? (UPBP_SHORT_BEAM | P.techniques) : (UPBP_LONG_BEAM | P.techniques);
     const float sin_t = (tech & UPBP_NO_SINE_IN_WEIGHTS) ? 1.0f : sin_theta;
@@ -358,17 +358,16 @@ __device__ void beam_hit_mis(const EvalArgs& A, const Ray& r, SegState& s, const
     const float w_cam = camera_weight_tail(ml.tail, mv, last, sin_t, rsp_inv_b, ratio,
                                            rev_pdf * rsr_q * __builtin_amdgcn_rcpf(dist_sq_q),
                                            __builtin_amdgcn_rcpf(rsr_q), P.query_beam_type, P.photon_beam_type, tech);
-    const uint32_t lvf = lv->mis.flags;
+    const uint32_t lvf = rec.lv_flags;
     const bool on_light_delta = (lvf & UPBP_MIS_ON_LIGHT_SOURCE) && (lvf & UPBP_MIS_DELTA);
     const float ntl = on_light_delta
                           ? 0.f
-                          : dir_pdf * rsr_b * __builtin_fabsf(lv->mis.cos_theta_out) * __builtin_amdgcn_rcpf(dist_sq_b);
+                          : dir_pdf * rsr_b * __builtin_fabsf(rec.cos_theta_out) * __builtin_amdgcn_rcpf(dist_sq_b);
     const float rsr_inv_b = __builtin_amdgcn_rcpf(rsr_b);
     float w_light;
-    if (!light_weight_tail_bb1d(G, (int64_t)bm.light_vertex, lv->path_idx, (int)path_len, last, bv.pdf_a_inv, ntl,
-                                rsr_inv_b, tech, w_light)) {
-        LightView lview{&G, &bv, (int64_t)bm.light_vertex, rsr_inv_b};
-        w_light = light_path_weight(lview, lv->path_idx, (int)path_len + 1, last, ntl, UPBP_BB1D, P.query_beam_type,
+    if (!light_weight_tail_rec(rec, last, bv.pdf_a_inv, ntl, rsr_inv_b, tech, w_light)) {
+        LightView lview{&G, &bv, (int64_t)rec.light_vertex, rsr_inv_b};
+        w_light = light_path_weight(lview, rec.path_idx, (int)path_len + 1, last, ntl, UPBP_BB1D, P.query_beam_type,
                                     P.photon_beam_type, tech, err);
     }
     const float w = __builtin_amdgcn_rcpf(w_light + w_cam) * (float)mult;
@@ -1512,6 +1511,53 @@ __global__ void k_light_tails(const upbp_light_vertex* __restrict__ lv, int64_t 
     }
 }
 
+__global__ void k_beam_records(MisGlobals G, int64_t n, BeamRec* __restrict__ out) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    const upbp_beam_mis bm = G.beams[b];
+    BeamRec r = {};
+    r.light_vertex = bm.light_vertex;
+    if (bm.light_vertex < 0 || (int64_t)bm.light_vertex >= G.lv_count) {
+        r.info = kRecBadVertex;
+        out[b] = r;
+        return;
+    }
+    const upbp_light_vertex* __restrict__ lv = G.lv + bm.light_vertex;
+    r.tail = G.tails ? G.tails[bm.light_vertex] : make_float4(0.f, 0.f, 0.f, __builtin_nanf(""));
+    r.ray_sample_pdf = bm.ray_sample_pdf;
+    r.ray_sample_rev_pdf = bm.ray_sample_rev_pdf;
+    r.last_pdf_w_inv = bm.last_pdf_w_inv;
+    r.cos_theta_out = lv->mis.cos_theta_out;
+    r.pdf_a_inv = lv->mis.pdf_a_inv;
+    r.lv_flags = lv->mis.flags;
+    r.path_idx = lv->path_idx;
+    const uint32_t len = lv->path_length;
+    // light_weight_tail_bb1d's test that the tails apply: the vertex ends the stored chain
+    const bool chain =
+        G.tails && light_last_index(G, lv->path_idx, (int)len + 1) == (int64_t)bm.light_vertex + 1;
+    r.info = (len < kRecLengthMask ? len : kRecLengthMask) | (lv->is_finite ? kRecFinite : 0u) |
+             ((bm.ray_sampling_flags & UPBP_ORIGIN_IN_MEDIUM) ? kRecOriginInMedium : 0u) | (chain ? kRecTailChain : 0u);
+    out[b] = r;
+}
+
+void launch_beam_records(Ctx& ctx, MisDev& mis, int64_t n_beams) {
+    mis.beam_recs = nullptr;
+    if (n_beams <= 0 || !mis.beams) return;
+    ctx.beam_recs.ensure((size_t)n_beams * sizeof(BeamRec));
+    MisGlobals G = {};
+    G.lv = mis.lv;
+    G.lv_count = mis.lv_count;
+    G.path_ends = mis.path_ends;
+    G.path_count = mis.path_count;
+    G.beams = mis.beams;
+    G.p = mis.params;
+    G.tails = mis.tails;
+    hipLaunchKernelGGL(k_beam_records, dim3(grid_for(n_beams)), dim3(256), 0, ctx.stream, G, n_beams,
+                       ctx.beam_recs.as<BeamRec>());
+    UPBP_HIP(hipGetLastError());
+    mis.beam_recs = ctx.beam_recs.p;
+}
+
 void launch_light_tails(Ctx& ctx, MisDev& mis) {
     mis.tails = nullptr;
     if (mis.lv_count <= 0) return;
@@ -1536,6 +1582,7 @@ static void set_mis(EvalArgs& A, const MisDev* mis) {
     A.mis.path_ends = mis->path_ends;
     A.mis.path_count = mis->path_count;
     A.mis.beams = mis->beams;
+    A.mis.beam_recs = static_cast<const BeamRec*>(mis->beam_recs);
     A.mis.p = mis->params;
 }
 

@@ -102,6 +102,7 @@ struct Ctx {
     DevBuf sort_tmp;    // hipcub temporary storage
     DevBuf qkeys, qkeys2, qperm, qperm2;
     DevBuf light_tails;  // float4 per light vertex of the current UPBP call
+    DevBuf beam_recs;    // 64-byte MIS record per photon beam of the current UPBP call
     DevBuf vm_rec;       // merge records (4 float4 per grid slot) of the current PP3D / SURF call
     DevBuf lt_slots[8];  // light tracing: per-path vertex, position and beam slots, counts, offsets, lights
     DevBuf staging[8];  // host-input staging copies
@@ -283,7 +284,8 @@ struct MisDev {
     int64_t path_count;
     const upbp_beam_mis* beams;
     upbp_mis_params params;
-    const float4* tails = nullptr;  // per light vertex, launch_light_tails
+    const float4* tails = nullptr;      // per light vertex, launch_light_tails
+    const void* beam_recs = nullptr;    // per photon beam, launch_beam_records
 };
 
 // stats: the kStatSlots counters the kernels add to (null: the context's own)
@@ -298,6 +300,9 @@ void launch_query_keys(Ctx& ctx, const QueryView& q, const float* lo, const floa
 // Folds each light vertex's light walk below it into a float4 (the O(1) light walk of the
 // UPBP weights) into ctx.light_tails and points mis.tails at it; nothing for no vertices.
 void launch_light_tails(Ctx& ctx, MisDev& mis);
+// Gathers each photon beam's MIS inputs (its record, origin light vertex and that vertex's tail)
+// into ctx.beam_recs and points mis.beam_recs at it; after launch_light_tails.
+void launch_beam_records(Ctx& ctx, MisDev& mis, int64_t n_beams);
 void launch_clear_u64(Ctx& ctx, uint64_t* p, int64_t n);
 void launch_clear_f32(Ctx& ctx, float* p, int64_t n);
 // gs += sign_s * (direct + d/d sigma_t), ga += sign_a * d/d sigma_t per medium and channel

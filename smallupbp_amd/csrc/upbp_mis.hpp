@@ -49,6 +49,29 @@ __device__ __forceinline__ MisF mis_load(const upbp_mis_data* __restrict__ p) {
     return m;
 }
 
+// Per photon beam, what its hits' MIS weights read of the beam's MIS record, its origin light
+// vertex and that vertex's light-walk tail, gathered once per call (k_beam_records) so that a
+// hit reads one 64-byte record instead of following beam -> light vertex -> path end -> tail.
+struct alignas(16) BeamRec {
+    float4 tail;                 // the origin light vertex's LightTail (NaN .w: none)
+    float ray_sample_pdf;        // upbp_beam_mis.ray_sample_pdf
+    float ray_sample_rev_pdf;    // upbp_beam_mis.ray_sample_rev_pdf
+    float last_pdf_w_inv;        // upbp_beam_mis.last_pdf_w_inv
+    float cos_theta_out;         // light vertex mis.cos_theta_out
+    float pdf_a_inv;             // light vertex mis.pdf_a_inv
+    uint32_t lv_flags;           // light vertex mis.flags
+    uint32_t info;               // path length | kRec* bits
+    int32_t light_vertex;        // upbp_beam_mis.light_vertex
+    int32_t path_idx;            // light vertex path_idx
+    uint32_t pad[3];
+};
+static_assert(sizeof(BeamRec) == 64, "BeamRec layout");
+constexpr uint32_t kRecLengthMask = 0x00FFFFFFu;
+constexpr uint32_t kRecFinite = 1u << 24;          // light vertex is_finite
+constexpr uint32_t kRecOriginInMedium = 1u << 25;  // beam ray_sampling_flags & UPBP_ORIGIN_IN_MEDIUM
+constexpr uint32_t kRecTailChain = 1u << 26;       // the vertex ends its path's stored chain (tail usable)
+constexpr uint32_t kRecBadVertex = 1u << 27;       // light_vertex outside the light vertices
+
 struct MisGlobals {
     const upbp_mis_data* __restrict__ camera;
     int64_t camera_count;
@@ -60,6 +83,8 @@ struct MisGlobals {
     upbp_mis_params p;
     // per light vertex, the light walk below it folded into four numbers (LightTail), or null
     const float4* __restrict__ tails;
+    // per photon beam, its hits' MIS inputs (k_beam_records), or null
+    const BeamRec* __restrict__ beam_recs;
 };
 
 struct LightView {
@@ -196,6 +221,22 @@ __device__ __forceinline__ bool light_weight_tail(const MisGlobals& G, int64_t v
     const float r1 = nx->mis.rev_pdf_a_without_bsdf * ntl;
     if (r1 == 0.f) return true;
     w += p0 * r1 * ((t.x + nx->mis.ray_sample_rev_pdf_inv * t.y) + nx->mis.pdf_a_inv * (t.z + t.w));
+    return true;
+}
+
+// light_weight_tail_bb1d from the beam's record: the same tests and terms, without loads.
+__device__ __forceinline__ bool light_weight_tail_rec(const BeamRec& rec, float last, float bv_pdf_a_inv, float ntl,
+                                                      float ray_rev_b, uint32_t tech, float& w) {
+    if (!(rec.info & kRecTailChain)) return false;
+    w = 0.f;
+    if (last == 0.f) return true;
+    const float p0 = last * bv_pdf_a_inv;
+    if ((tech & UPBP_BPT) && !(rec.lv_flags & UPBP_MIS_SPECULAR)) w = p0;
+    if (tech & UPBP_PREVIOUS) return true;
+    const float4 t = rec.tail;
+    if (__builtin_isnan(t.w)) return false;
+    if (ntl == 0.f) return true;
+    w += p0 * ntl * ((t.x + ray_rev_b * t.y) + rec.pdf_a_inv * (t.z + t.w));
     return true;
 }
 
