@@ -330,6 +330,7 @@ void run_eval(Ctx& ctx, const Bre* bre, const Beams* bm, const upbp_query_batch*
         mis_t = *mis;
         launch_light_tails(ctx, mis_t);
         if (bm) launch_beam_records(ctx, mis_t, bm->n);
+        if (bre) launch_photon_records(ctx, mis_t);
         mis = &mis_t;
     }
     // device outputs (temporary for host outputs)

@@ -601,6 +601,7 @@ void vm_eval(Ctx& ctx, const HashGrid& grid, const Query* q, int64_t count, cons
     A.mis.path_count = mis.path_count;
     A.mis.beams = nullptr;
     A.mis.beam_recs = nullptr;
+    A.mis.photon_recs = nullptr;
     A.mis.p = mis.params;
     A.mis.tails = mis.tails;
     A.stats = ctx.stats.as<unsigned long long>();

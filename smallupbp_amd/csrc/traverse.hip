@@ -222,8 +222,9 @@ __device__ void photon_hit_mis(const EvalArgs& A, const Ray& r, SegState& s, con
         err |= kBadLightIndex;
         return;
     }
-    const upbp_light_vertex* __restrict__ lv = G.lv + lv_idx;
-    const uint32_t path_len = lv->path_length;
+    // the vertex's record: its fields, and those of the walk's index 1 with its tail, in one read
+    const PhotonRec rec = G.photon_recs[lv_idx];
+    const uint32_t path_len = rec.d.x & kRecLengthMask;
     const uint32_t cam_len = (uint32_t)ml.cam_len;
     if ((path_len + cam_len > P.max_path_length) || (path_len + cam_len < P.min_path_length)) return;
     if (cam_len == 1 && t < P.min_dist_to_med) return;
@@ -246,18 +247,18 @@ __device__ void photon_hit_mis(const EvalArgs& A, const Ray& r, SegState& s, con
     ray_pdf *= ml.pref_pdf;
     ray_rev *= ml.pref_rev;
     // PhaseFunction::Evaluate(-ray.dir, lightDirection): cos = -dot(gen, fix) = dot(dir_fix, d)
-    const float lx = lv->dir_fix[0], ly = lv->dir_fix[1], lz = lv->dir_fix[2];
+    const float lx = rec.b.x, ly = rec.b.y, lz = rec.b.z;
     const float cos_t = -dot3(lx, ly, lz, -r.dx, -r.dy, -r.dz);
     const float bsdf = phase_pdf(s.g, s.iso != 0, cos_t);
     const float sin_t = __builtin_amdgcn_sqrtf(fmaxf(0.f, 1.f - cos_t * cos_t));
     if (bsdf <= 0.f) return;
     const float dir_pdf = bsdf * s.cont;
-    const float rev_pdf = bsdf * lv->continuation_prob;
+    const float rev_pdf = bsdf * rec.a.w;
     const float kernel = radial_kernel(sq, r2) * __builtin_amdgcn_rcpf((r2 * kPi) * 0.5f);
     if (!(kernel > 1e-20f)) return;
-    const float u0 = (((lv->throughput[0] * a0) * s.ss0) * bsdf) * kernel;
-    const float u1 = (((lv->throughput[1] * a1) * s.ss1) * bsdf) * kernel;
-    const float u2 = (((lv->throughput[2] * a2) * s.ss2) * bsdf) * kernel;
+    const float u0 = (((rec.a.x * a0) * s.ss0) * bsdf) * kernel;
+    const float u1 = (((rec.a.y * a1) * s.ss1) * bsdf) * kernel;
+    const float u2 = (((rec.a.z * a2) * s.ss2) * bsdf) * kernel;
     if (u0 <= 0.f && u1 <= 0.f && u2 <= 0.f) return;
     const float dist_sq = t * t;
     const float ray_pdf_inv = __builtin_amdgcn_rcpf(ray_pdf);
@@ -269,15 +270,14 @@ __device__ void photon_hit_mis(const EvalArgs& A, const Ray& r, SegState& s, con
                                      s.mfp > P.bb1d_min_mfp ? UPBP_MIS_IN_MEDIUM_WITH_BEAMS : 0u);
     const float last = short_q ? __builtin_amdgcn_rcpf(ratio * mv.pb2d) : ray_pdf * __builtin_amdgcn_rcpf(mv.pb2d);
     const uint32_t tech = A.short_query ? (UPBP_SHORT_BEAM | P.techniques) : (UPBP_LONG_BEAM | P.techniques);
-    const float w_cam = camera_weight_tail(ml.tail, mv, last, sin_t, lv->mis.ray_sample_pdf_inv,
-                                           lv->mis.ray_sample_pdfs_ratio,
+    const float w_cam = camera_weight_tail(ml.tail, mv, last, sin_t, rec.b.w, rec.c.x,
                                            rev_pdf * ray_rev * __builtin_amdgcn_rcpf(dist_sq),
                                            __builtin_amdgcn_rcpf(ray_rev), P.query_beam_type, P.photon_beam_type,
                                            tech);
     float w_light;
-    if (!light_weight_tail(G, (int64_t)lv_idx, lv->path_idx, (int)path_len, last, dir_pdf, tech, w_light)) {
+    if (!light_weight_tail_prec(rec, last, dir_pdf, tech, w_light)) {
         LightView lview{&G, nullptr, -1, 0.f};
-        w_light = light_path_weight(lview, lv->path_idx, (int)path_len, last, dir_pdf, UPBP_PB2D, P.query_beam_type,
+        w_light = light_path_weight(lview, (int)rec.d.y, (int)path_len, last, dir_pdf, UPBP_PB2D, P.query_beam_type,
                                     P.photon_beam_type, tech, err);
     }
     const float w = __builtin_amdgcn_rcpf(w_light + w_cam);
@@ -1540,6 +1540,54 @@ __global__ void k_beam_records(MisGlobals G, int64_t n, BeamRec* __restrict__ ou
     out[b] = r;
 }
 
+// light_weight_tail's inputs per light vertex v: everything but `last` and `ntl`.
+__global__ void k_photon_records(MisGlobals G, PhotonRec* __restrict__ out) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= G.lv_count) return;
+    const upbp_light_vertex* __restrict__ lv = G.lv + v;
+    PhotonRec r;
+    r.a = make_float4(lv->throughput[0], lv->throughput[1], lv->throughput[2], lv->continuation_prob);
+    r.b = make_float4(lv->dir_fix[0], lv->dir_fix[1], lv->dir_fix[2], lv->mis.ray_sample_pdf_inv);
+    const uint32_t len = lv->path_length;
+    const bool has1 = len > 0;
+    uint32_t info = len < kRecLengthMask ? len : kRecLengthMask;
+    if (G.tails && light_last_index(G, lv->path_idx, (int)len) == v && v >= (has1 ? 1 : 0)) info |= kPRecChain;
+    if (has1) info |= kPRecHas1;
+    float cs = 0.f, tt = 0.f;
+    if (info & kPRecChain) {
+        const upbp_light_vertex* __restrict__ nx = has1 ? lv - 1 : lv;
+        if (!(nx->mis.flags & UPBP_MIS_SPECULAR)) info |= kPRecNextPlain;
+        if (has1) {
+            const float4 t = G.tails[v - 1];
+            if (!__builtin_isnan(t.w) && (int)nx->path_length == (int)len - 1) {
+                info |= kPRecNextTail;
+                cs = nx->mis.rev_pdf_a_without_bsdf;
+                tt = (t.x + nx->mis.ray_sample_rev_pdf_inv * t.y) + nx->mis.pdf_a_inv * (t.z + t.w);
+            }
+        }
+    }
+    r.c = make_float4(lv->mis.ray_sample_pdfs_ratio, lv->mis.pdf_a_inv, cs, tt);
+    r.d = make_uint4(info, (uint32_t)lv->path_idx, (uint32_t)v, 0u);
+    out[v] = r;
+}
+
+void launch_photon_records(Ctx& ctx, MisDev& mis) {
+    mis.photon_recs = nullptr;
+    if (mis.lv_count <= 0) return;
+    ctx.photon_recs.ensure((size_t)mis.lv_count * sizeof(PhotonRec));
+    MisGlobals G = {};
+    G.lv = mis.lv;
+    G.lv_count = mis.lv_count;
+    G.path_ends = mis.path_ends;
+    G.path_count = mis.path_count;
+    G.p = mis.params;
+    G.tails = mis.tails;
+    hipLaunchKernelGGL(k_photon_records, dim3(grid_for(mis.lv_count)), dim3(256), 0, ctx.stream, G,
+                       ctx.photon_recs.as<PhotonRec>());
+    UPBP_HIP(hipGetLastError());
+    mis.photon_recs = ctx.photon_recs.p;
+}
+
 void launch_beam_records(Ctx& ctx, MisDev& mis, int64_t n_beams) {
     mis.beam_recs = nullptr;
     if (n_beams <= 0 || !mis.beams) return;
@@ -1583,6 +1631,7 @@ static void set_mis(EvalArgs& A, const MisDev* mis) {
     A.mis.path_count = mis->path_count;
     A.mis.beams = mis->beams;
     A.mis.beam_recs = static_cast<const BeamRec*>(mis->beam_recs);
+    A.mis.photon_recs = static_cast<const PhotonRec*>(mis->photon_recs);
     A.mis.p = mis->params;
 }
 

@@ -103,6 +103,7 @@ struct Ctx {
     DevBuf qkeys, qkeys2, qperm, qperm2;
     DevBuf light_tails;  // float4 per light vertex of the current UPBP call
     DevBuf beam_recs;    // 64-byte MIS record per photon beam of the current UPBP call
+    DevBuf photon_recs;  // 64-byte MIS record per light vertex of the current UPBP call
     DevBuf vm_rec;       // merge records (4 float4 per grid slot) of the current PP3D / SURF call
     DevBuf lt_slots[8];  // light tracing: per-path vertex, position and beam slots, counts, offsets, lights
     DevBuf staging[8];  // host-input staging copies
@@ -286,6 +287,7 @@ struct MisDev {
     upbp_mis_params params;
     const float4* tails = nullptr;      // per light vertex, launch_light_tails
     const void* beam_recs = nullptr;    // per photon beam, launch_beam_records
+    const void* photon_recs = nullptr;  // per light vertex, launch_photon_records
 };
 
 // stats: the kStatSlots counters the kernels add to (null: the context's own)
@@ -303,6 +305,8 @@ void launch_light_tails(Ctx& ctx, MisDev& mis);
 // Gathers each photon beam's MIS inputs (its record, origin light vertex and that vertex's tail)
 // into ctx.beam_recs and points mis.beam_recs at it; after launch_light_tails.
 void launch_beam_records(Ctx& ctx, MisDev& mis, int64_t n_beams);
+// The same per light vertex for the PB2D hits into ctx.photon_recs (mis.photon_recs).
+void launch_photon_records(Ctx& ctx, MisDev& mis);
 void launch_clear_u64(Ctx& ctx, uint64_t* p, int64_t n);
 void launch_clear_f32(Ctx& ctx, float* p, int64_t n);
 // gs += sign_s * (direct + d/d sigma_t), ga += sign_a * d/d sigma_t per medium and channel

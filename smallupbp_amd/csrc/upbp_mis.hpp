@@ -72,6 +72,22 @@ constexpr uint32_t kRecOriginInMedium = 1u << 25;  // beam ray_sampling_flags & 
 constexpr uint32_t kRecTailChain = 1u << 26;       // the vertex ends its path's stored chain (tail usable)
 constexpr uint32_t kRecBadVertex = 1u << 27;       // light_vertex outside the light vertices
 
+// Per light vertex used as a photon, what its PB2D hits' weights read of the vertex, of the
+// vertex below it on its path and of that vertex's light tail, gathered once per call
+// (k_photon_records): one 64-byte record per hit instead of the 112-byte vertex, the path end
+// and the neighbour's fields and tail.
+struct alignas(16) PhotonRec {
+    float4 a;     // throughput.xyz, continuation_prob
+    float4 b;     // dir_fix.xyz, mis.ray_sample_pdf_inv
+    float4 c;     // mis.ray_sample_pdfs_ratio, mis.pdf_a_inv, next.rev_pdf_a_without_bsdf, next tail term
+    uint4 d;      // path length | kPRec* bits, path_idx, the vertex index, unused
+};
+static_assert(sizeof(PhotonRec) == 64, "PhotonRec layout");
+constexpr uint32_t kPRecChain = 1u << 24;       // the vertex ends its path's stored chain (tails apply)
+constexpr uint32_t kPRecHas1 = 1u << 25;        // path length > 0: the walk has an index 1
+constexpr uint32_t kPRecNextPlain = 1u << 26;   // the index-1 vertex is not specular
+constexpr uint32_t kPRecNextTail = 1u << 27;    // the index-1 vertex's tail is usable
+
 struct MisGlobals {
     const upbp_mis_data* __restrict__ camera;
     int64_t camera_count;
@@ -85,6 +101,8 @@ struct MisGlobals {
     const float4* __restrict__ tails;
     // per photon beam, its hits' MIS inputs (k_beam_records), or null
     const BeamRec* __restrict__ beam_recs;
+    // per light vertex, its PB2D hits' MIS inputs (k_photon_records), or null
+    const PhotonRec* __restrict__ photon_recs;
 };
 
 struct LightView {
@@ -221,6 +239,23 @@ __device__ __forceinline__ bool light_weight_tail(const MisGlobals& G, int64_t v
     const float r1 = nx->mis.rev_pdf_a_without_bsdf * ntl;
     if (r1 == 0.f) return true;
     w += p0 * r1 * ((t.x + nx->mis.ray_sample_rev_pdf_inv * t.y) + nx->mis.pdf_a_inv * (t.z + t.w));
+    return true;
+}
+
+// light_weight_tail from the photon's record: the same tests and terms, without loads.
+__device__ __forceinline__ bool light_weight_tail_prec(const PhotonRec& rec, float last, float ntl, uint32_t tech,
+                                                       float& w) {
+    const uint32_t info = rec.d.x;
+    if (!(info & kPRecChain)) return false;
+    w = 0.f;
+    if (last == 0.f) return true;  // the walk breaks at index 0
+    const float p0 = last * rec.c.y;
+    if ((tech & UPBP_BPT) && (info & kPRecNextPlain)) w = p0;
+    if ((tech & UPBP_PREVIOUS) || !(info & kPRecHas1)) return true;
+    if (!(info & kPRecNextTail)) return false;
+    const float r1 = rec.c.z * ntl;
+    if (r1 == 0.f) return true;
+    w += p0 * r1 * rec.c.w;
     return true;
 }
 
