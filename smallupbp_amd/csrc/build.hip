@@ -251,7 +251,7 @@ void bre_build(Ctx& ctx, const upbp_photon_input& in, const upbp_build_params& p
     if (in.count == 0) return;
     UPBP_REQUIRE(in.position && in.power, UPBP_EINVAL, "photon position and power are required");
     const int64_t n_in = in.count;
-    DevBuf s_pos, s_inc, s_pow, s_rad, s_med;
+    TmpBuf s_pos(ctx.tmp), s_inc(ctx.tmp), s_pow(ctx.tmp), s_rad(ctx.tmp), s_med(ctx.tmp);
     const float* pos = stage(ctx, s_pos, in.position, 3 * n_in, in.memory);
     const float* inc = stage(ctx, s_inc, in.inc_dir, 3 * n_in, in.memory);
     const float* pw = stage(ctx, s_pow, in.power, 3 * n_in, in.memory);
@@ -259,11 +259,11 @@ void bre_build(Ctx& ctx, const upbp_photon_input& in, const upbp_build_params& p
     const uint8_t* med = stage(ctx, s_med, in.in_medium, n_in, in.memory);
     hipStream_t st = ctx.stream;
 
-    DevBuf keep_buf;
+    TmpBuf keep_buf(ctx.tmp);
     const uint32_t* keep = nullptr;
     int64_t n = n_in;
     if (med) {
-        DevBuf flags, pos_scan;
+        TmpBuf flags(ctx.tmp), pos_scan(ctx.tmp);
         flags.ensure(n_in * 4);
         pos_scan.ensure(n_in * 4);
         hipLaunchKernelGGL(k_flags_u8, dim3(grid_for(n_in)), dim3(kBlock), 0, st, n_in, med, flags.as<uint32_t>());
@@ -278,13 +278,13 @@ void bre_build(Ctx& ctx, const upbp_photon_input& in, const upbp_build_params& p
     }
     out.n = n;
     if (n == 0) return;
-    DevBuf knn_radius;
+    TmpBuf knn_radius(ctx.tmp);
     if (knn) {
         // radius of each kept photon from its knn-th nearest kept photon (Bre.cxx:562-602)
         photon_knn_radius(ctx, n, pos, keep, params.radius, params.knn, knn_radius, n_in);
         rad = knn_radius.as<float>();
     }
-    DevBuf lo, hi, err, order;
+    TmpBuf lo(ctx.tmp), hi(ctx.tmp), err(ctx.tmp), order(ctx.tmp);
     lo.ensure(n * 16);
     hi.ensure(n * 16);
     err.ensure(4);
@@ -341,17 +341,17 @@ void beams_build(Ctx& ctx, const upbp_beam_input& in, const upbp_build_params& p
     UPBP_REQUIRE(in.origin && in.direction && in.length && in.throughput && in.flags && in.medium_id,
                  UPBP_EINVAL, "beam origin, direction, length, throughput, flags and medium_id are required");
     const int64_t n = in.count;
-    DevBuf s[8];
+    TmpBuf s0(ctx.tmp), s1(ctx.tmp), s2(ctx.tmp), s3(ctx.tmp), s4(ctx.tmp), s5(ctx.tmp), s6(ctx.tmp), s7(ctx.tmp);
     BeamIn b;
-    b.origin = stage(ctx, s[0], in.origin, 3 * n, in.memory);
-    b.direction = stage(ctx, s[1], in.direction, 3 * n, in.memory);
-    b.length = stage(ctx, s[2], in.length, n, in.memory);
-    b.throughput = stage(ctx, s[3], in.throughput, 3 * n, in.memory);
-    b.flags = stage(ctx, s[4], in.flags, n, in.memory);
-    b.medium = stage(ctx, s[5], in.medium_id, n, in.memory);
-    b.r0 = stage(ctx, s[6], in.start_radius, n, in.memory);
-    b.r1 = stage(ctx, s[7], in.end_radius, n, in.memory);
-    DevBuf knn_r0, knn_r1;
+    b.origin = stage(ctx, s0, in.origin, 3 * n, in.memory);
+    b.direction = stage(ctx, s1, in.direction, 3 * n, in.memory);
+    b.length = stage(ctx, s2, in.length, n, in.memory);
+    b.throughput = stage(ctx, s3, in.throughput, 3 * n, in.memory);
+    b.flags = stage(ctx, s4, in.flags, n, in.memory);
+    b.medium = stage(ctx, s5, in.medium_id, n, in.memory);
+    b.r0 = stage(ctx, s6, in.start_radius, n, in.memory);
+    b.r1 = stage(ctx, s7, in.end_radius, n, in.memory);
+    TmpBuf knn_r0(ctx.tmp), knn_r1(ctx.tmp);
     if (knn) {
         // start / end radius from the knn-th nearest beam origin (PhBeams.cxx:77-117)
         beam_knn_radius(ctx, n, b.origin, b.direction, b.length, params.radius, params.knn, knn_r0, knn_r1);
@@ -363,7 +363,7 @@ void beams_build(Ctx& ctx, const upbp_beam_input& in, const upbp_build_params& p
     b.max_pieces = maxp;
     b.n_media = ctx.n_media;
     hipStream_t st = ctx.stream;
-    DevBuf count, offset, err;
+    TmpBuf count(ctx.tmp), offset(ctx.tmp), err(ctx.tmp);
     count.ensure(n * 4);
     offset.ensure(n * 4);
     err.ensure(4);
@@ -376,7 +376,7 @@ void beams_build(Ctx& ctx, const upbp_beam_input& in, const upbp_build_params& p
     const int64_t total = (int64_t)read_u32(ctx, offset.as<uint32_t>() + (n - 1)) +
                           read_u32(ctx, count.as<uint32_t>() + (n - 1));
     UPBP_REQUIRE(total < (int64_t)0x7FFFFFFF, UPBP_EINVAL, "too many beam pieces");
-    DevBuf pieces, flux, piece_beam, lo, hi, order;
+    TmpBuf pieces(ctx.tmp), flux(ctx.tmp), piece_beam(ctx.tmp), lo(ctx.tmp), hi(ctx.tmp), order(ctx.tmp);
     pieces.ensure(total * sizeof(PieceDev));
     flux.ensure(total * 16);
     piece_beam.ensure(total * 4);

@@ -326,7 +326,7 @@ void build_lbvh(Ctx& ctx, int64_t n, const float4* box_lo, const float4* box_hi,
     UPBP_REQUIRE(n > 0 && n < (int64_t)0x7FFFFFFF, UPBP_EINVAL, "LBVH item count out of range");
     hipStream_t st = ctx.stream;
     out.n_items = n;
-    DevBuf bounds, partial;
+    TmpBuf bounds(ctx.tmp), partial(ctx.tmp);
     bounds.ensure(12 * 4);
     const unsigned gb = std::min<unsigned>(grid_for(n), 1024u);
     partial.ensure((size_t)gb * 12 * 4);
@@ -364,7 +364,7 @@ void build_lbvh(Ctx& ctx, int64_t n, const float4* box_lo, const float4* box_hi,
         UPBP_HIP(hipStreamSynchronize(st));
         return;
     }
-    DevBuf keys, keys2, vals;
+    TmpBuf keys(ctx.tmp), keys2(ctx.tmp), vals(ctx.tmp);
     keys.ensure((size_t)n * 8);
     keys2.ensure((size_t)n * 8);
     vals.ensure((size_t)n * 4);
@@ -374,7 +374,8 @@ void build_lbvh(Ctx& ctx, int64_t n, const float4* box_lo, const float4* box_hi,
     sort_pairs_u64(ctx, keys.as<uint64_t>(), keys2.as<uint64_t>(), vals.as<uint32_t>(),
                    out_order.as<uint32_t>(), n, 63);
     const int64_t ni = n - 1;
-    DevBuf left, right, first, last, leaf_parent, int_parent, ilo, ihi;
+    TmpBuf left(ctx.tmp), right(ctx.tmp), first(ctx.tmp), last(ctx.tmp), leaf_parent(ctx.tmp),
+        int_parent(ctx.tmp), ilo(ctx.tmp), ihi(ctx.tmp);
     left.ensure(ni * 4);
     right.ensure(ni * 4);
     first.ensure(ni * 4);
@@ -389,7 +390,8 @@ void build_lbvh(Ctx& ctx, int64_t n, const float4* box_lo, const float4* box_hi,
     UPBP_HIP(hipGetLastError());
     {
         // group internal nodes by depth, then merge bottom-up one level per launch
-        DevBuf depth, depth_sorted, ids, ids_sorted, starts, err;
+        TmpBuf depth(ctx.tmp), depth_sorted(ctx.tmp), ids(ctx.tmp), ids_sorted(ctx.tmp), starts(ctx.tmp),
+            err(ctx.tmp);
         depth.ensure(ni * 4);
         depth_sorted.ensure(ni * 4);
         ids.ensure(ni * 4);

@@ -58,6 +58,40 @@ struct DevBuf {
     T* as() const { return static_cast<T*>(p); }
 };
 
+// Build temporaries: grow-only device buffers that a context keeps and lends out for one scope
+// at a time, last borrowed first returned (every call on a context runs on its one stream, and
+// a context is used by one host thread at a time). A renderer that rebuilds its structures every
+// iteration then allocates and frees no temporary device memory once the sizes are reached; the
+// buffers are freed with the context.
+struct TmpPool {
+    static constexpr int kSlots = 48;
+    DevBuf slot[kSlots];
+    int top = 0;
+};
+
+class TmpBuf {
+  public:
+    explicit TmpBuf(TmpPool& pool) : pool_(pool), b_(take(pool)), p(b_.p) {}
+    ~TmpBuf() { --pool_.top; }
+    TmpBuf(const TmpBuf&) = delete;
+    TmpBuf& operator=(const TmpBuf&) = delete;
+    void ensure(size_t n) { b_.ensure(n); }
+    template <class T>
+    T* as() const { return b_.as<T>(); }
+    operator DevBuf&() { return b_; }
+
+  private:
+    static DevBuf& take(TmpPool& pool) {
+        UPBP_REQUIRE(pool.top < TmpPool::kSlots, UPBP_EDEVICE, "build temporaries exhausted");
+        return pool.slot[pool.top++];
+    }
+    TmpPool& pool_;
+    DevBuf& b_;
+
+  public:
+    void*& p;  // the lent buffer's pointer (follows ensure)
+};
+
 // Counters written by the traversal kernels (device, uint64 each).
 enum StatSlot : int {
     kStatPhotonHits = 0,
@@ -100,6 +134,7 @@ struct Ctx {
     uint64_t media_gen = 0;
     DevBuf stats;       // kStatSlots x uint64
     DevBuf sort_tmp;    // hipcub temporary storage
+    TmpPool tmp;        // build temporaries
     DevBuf qkeys, qkeys2, qperm, qperm2;
     DevBuf light_tails;  // float4 per light vertex of the current UPBP call
     DevBuf beam_recs;    // 64-byte MIS record per photon beam of the current UPBP call
