@@ -69,13 +69,22 @@ __global__ __launch_bounds__(kBlock) void k_bounds(int64_t n, const float4* lo, 
     }
 }
 
+// One wave per bound component: its lanes stride over the per-block partials, then a wave
+// reduction (a single thread per component took ~125 us over 1024 partials).
 __global__ void k_bounds_final(int nb, const uint32_t* partial, uint32_t* b) {
-    const int k = threadIdx.x;
-    if (k >= 12) return;
+    const int k = threadIdx.x / 64, lane = threadIdx.x & 63;
+    if (k >= 12) return;  // whole waves
     const bool is_min = (k / 3) % 2 == 0;
     uint32_t r = is_min ? 0xFFFFFFFFu : 0u;
-    for (int i = 0; i < nb; ++i) r = is_min ? min(r, partial[(size_t)i * 12 + k]) : max(r, partial[(size_t)i * 12 + k]);
-    b[k] = r;
+    for (int i = lane; i < nb; i += 64) {
+        const uint32_t v = partial[(size_t)i * 12 + k];
+        r = is_min ? min(r, v) : max(r, v);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t o = (uint32_t)__shfl_xor((int)r, off, 64);
+        r = is_min ? min(r, o) : max(r, o);
+    }
+    if (lane == 0) b[k] = r;
 }
 
 __device__ __forceinline__ uint64_t spread21(uint64_t x) {
@@ -332,7 +341,8 @@ void build_lbvh(Ctx& ctx, int64_t n, const float4* box_lo, const float4* box_hi,
     partial.ensure((size_t)gb * 12 * 4);
     hipLaunchKernelGGL(k_bounds, dim3(gb), dim3(kBlock), 0, st, n, box_lo, box_hi, partial.as<uint32_t>());
     UPBP_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(64), 0, st, (int)gb, partial.as<uint32_t>(), bounds.as<uint32_t>());
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(12 * 64), 0, st, (int)gb, partial.as<uint32_t>(),
+                       bounds.as<uint32_t>());
     UPBP_HIP(hipGetLastError());
     uint32_t hb[12];
     UPBP_HIP(hipMemcpyAsync(hb, bounds.p, sizeof(hb), hipMemcpyDeviceToHost, st));
