@@ -609,10 +609,11 @@ __device__ __forceinline__ void photon_leaf(const EvalArgs& A, const Ray& r, Seg
 // ---- wave frustum of rays from one point ------------------------------------
 // When every active lane of a wave starts its ray at the same point (the camera rays of a
 // frame), four planes through that point bound the rays' directions, and every ray lies on
-// the inner side of each. A photon, or a beam piece whose segment, widened by its box margin,
-// that lies beyond one of the planes is then farther than its radius from every ray of the
-// wave: no lane can accept it, and its test is skipped for the whole wave. One lane per item
-// of a leaf does that check, so a leaf costs one pass however many items it holds. The planes
+// the inner side of each. A photon lying beyond one of the planes by more than its radius, or a
+// beam piece whose segment does (widened by its box margin), is then farther than its radius
+// from every ray of the wave: no lane can accept it, and its test is skipped for the whole
+// wave. One lane per item of a leaf does that check, so a leaf costs one pass however many
+// items it holds (which is why such a wave also takes small subtrees as one leaf). The planes
 // are wave-uniform and live in SGPRs (VGPRs are what limits the beam kernel's occupancy); the
 // vector instructions of the check take them as scalar operands.
 struct Frustum {
