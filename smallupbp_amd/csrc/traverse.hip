@@ -700,8 +700,8 @@ __device__ bool make_frustum(const Ray& r, const SegState& s, Frustum& F) {
 // of a ray, so the segment widened by 1.01 of the largest radius (the piece box's margin)
 // cannot lie beyond a plane. The first and last piece of a beam also own the pairs whose
 // closest point lies beyond the beam's ends (which the reference accepts only through the
-// rounding of its t2 for nearly perpendicular lines): they are always kept, so the set of
-// pairs tested for them, and the hit set, stay those of the leaf-box traversal alone.
+// rounding of its t2 for nearly perpendicular lines): they are always kept, so culling drops
+// no pair that the visited leaf (or subtree) boxes reach.
 __device__ __forceinline__ uint64_t frustum_items(const EvalArgs& A, const Frustum& F, uint32_t start,
                                                   uint32_t cnt) {
     const int lane = threadIdx.x & (kWave - 1);
