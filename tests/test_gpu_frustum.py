@@ -191,3 +191,37 @@ def test_subtree_leaf_setting_keeps_hit_sets():
     assert ctx.get_tuning()[3] == 64  # library default
     with pytest.raises(native.UpbpError):
         ctx.set_tuning(1, 0, 0, 65)
+
+
+@pytest.mark.gpu
+def test_queries_without_common_origin():
+    """Rays from random points in random directions: no wave has a frustum, so the traversal
+    tests the built leaves alone (the node's subtree fields unused); hit sets and values against
+    the oracle at leaf sizes 4 and 64."""
+    print_case("incoherent_queries", queries=3000)
+    rng = scenes.rng_for(97)
+    base = scenes.bre_beams(16, 9, 20_000, 2_000, seed=98)
+    n = 3000
+    o = rng.uniform(0, 1, (n, 3)).astype(np.float32)
+    d = scenes._isotropic_dirs(rng, n)
+    off, seg = scenes.ray_segments(base.media, o, d, (0, 0, 0), (1, 1, 1), 0, 1, scenes.SPHERE,
+                                   leading_vacuum=False)
+    q = scenes.Queries(o, d, off, seg, scenes.LONG_BEAM)
+    w = scenes.Workload("incoherent", base.media, base.photons, base.beams, q, scenes.SPHERE, 0, 1, 98)
+    ref_p, rsig_p, cnt_p = oracle_photons(w, accel=0)
+    ref_b, rsig_b, cnt_b = oracle_beams(w, accel=0)
+    assert int(cnt_p[0]) > 1000 and int(cnt_b[0]) > 1000
+    for leaf in (4, 64):
+        ctx = gpu_context(w.media)
+        bre = native.EmbreeBre(ctx)
+        bre.build(w.photons, leaf_size=leaf)
+        bm = native.PhotonBeamsEvaluator(ctx)
+        bm.build(w.beams, leaf_size=leaf)
+        qv = native.QueryBatchView.to_device(w.queries)
+        sp, sb = native.hit_signatures(ctx, bre, bm, qv)
+        ph = bre.evalBre(qv).cpu().numpy()
+        be = bm.evalBeamBeamEstimate(qv).cpu().numpy()
+        assert np.array_equal(sp, rsig_p), "photon hit sets differ at leaf %d" % leaf
+        assert np.array_equal(sb, rsig_b), "beam hit sets differ at leaf %d" % leaf
+        assert rel_err(ph, ref_p).max() <= REL_TOL
+        assert rel_err(be, ref_b).max() <= REL_TOL
